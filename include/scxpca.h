@@ -1,0 +1,184 @@
+/*
+ * scxpca — MI355X-native randomized sparse PCA (C ABI).
+ *
+ * The entry points below replace, for device="gpu", the reference's GPU PCA
+ * dispatch:
+ *   scx_accel::randomized_pca_gpu(device_id, &source, n_components,
+ *       n_oversamples, n_power_iterations, zero_center, seed, QrMethod,
+ *       GpuPcaTuning) -> Result<PcaResult, AccelError>
+ *   (scx-accel/src/pca/gpu.rs:32-78, which calls scx_gpu::gpu_randomized_pca,
+ *    scx-gpu/src/gpu_pca.rs:706-763, and the resident operator
+ *    scx-gpu/src/gpu_pca_resident.rs:87-347).
+ *
+ * Plain pointers and sizes only; no torch or HIP types in the signatures. A
+ * Rust scx-gpu would bind these through `extern "C"` blocks; Python binds them
+ * through ctypes (scx_amd/_lib.py).
+ *
+ * Threading: every call runs on the library's own stream of the chosen device
+ * and returns after the device work completes. Calls on different matrices may
+ * run from different threads. Errors never abort the process: every entry point
+ * returns a status and leaves a thread-local message for scxpca_last_error.
+ */
+#ifndef SCXPCA_H
+#define SCXPCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    SCXPCA_OK = 0,
+    SCXPCA_ERR_INVALID = 1,   /* AccelError::InvalidInput (scx-accel/src/error.rs) */
+    SCXPCA_ERR_LINALG = 2,    /* AccelError::LinAlg, e.g. CholeskyQR2 non-SPD     */
+    SCXPCA_ERR_OOM = 3,       /* GPU memory preflight (scx-gpu/src/gpu_pca.rs:450-462) */
+    SCXPCA_ERR_DEVICE = 4,    /* HIP runtime failure                              */
+    SCXPCA_ERR_NO_DEVICE = 5  /* no GPU / ordinal out of range                    */
+} scxpca_status;
+
+/* QR used for the power-iteration orthonormalization
+ * (QrMethod, scx-gpu/src/cusolver.rs:256-262). */
+typedef enum {
+    SCXPCA_QR_HOUSEHOLDER = 0, /* stable: shifted CholeskyQR3 (no failure on ill-conditioned Y) */
+    SCXPCA_QR_CHOLESKY = 1     /* CholeskyQR2; non-SPD Gram is an error, as in cusolver.rs:381-385 */
+} scxpca_qr_method;
+
+typedef enum {
+    SCXPCA_PRECISION_FP32 = 0,        /* X~ streamed as f32, f32 accumulate               */
+    SCXPCA_PRECISION_BF16_SKETCH = 1  /* sketch passes stream X~ as bf16, final B pass f32 */
+} scxpca_precision;
+
+/* CSR matrix: int64 indptr (n_obs+1), uint32 indices (nnz, non-decreasing per
+ * row on device input; host input is sorted if needed), float32 values.
+ * Raw counts; the normalize/log1p transform is applied on the fly. */
+typedef struct {
+    int64_t n_obs;
+    int64_t n_vars;
+    const int64_t *indptr;
+    const uint32_t *indices;
+    const float *data;
+    int32_t on_device; /* 1: device pointers on the call's device (borrowed), 0: host */
+} scxpca_csr;
+
+/* Arguments of randomized_pca_gpu (scx-accel/src/pca/gpu.rs:32) plus the lazy
+ * transform chain that pyscx.accel.normalize_total/log1p record
+ * (pyscx/src/lazy_transform/transforms.rs:15-72). */
+typedef struct {
+    int64_t n_comps;
+    int64_t n_oversamples;
+    int64_t n_power_iterations;
+    int32_t zero_center;
+    int32_t qr_method;      /* scxpca_qr_method */
+    uint64_t seed;          /* ChaCha8Rng::seed_from_u64 stream, as cpu.rs:1174-1178 */
+    double target_sum;      /* > 0: normalize_total to this target; <= 0: none   */
+    int32_t log1p;          /* apply log1p after normalization                  */
+    int32_t precision;      /* scxpca_precision                                   */
+} scxpca_params;
+
+/* Caller-allocated outputs (PcaResult, scx-accel/src/pca/cpu.rs:84-110).
+ * embeddings: n_obs x n_comps row-major f32; components: n_comps x n_vars
+ * row-major f32; variance / variance_ratio: n_comps f64; mean: n_vars f64 or
+ * NULL; sigma: k f64 (all singular values of B) or NULL. */
+typedef struct {
+    float *embeddings;
+    float *components;
+    double *variance;
+    double *variance_ratio;
+    double *mean;
+    double *sigma;
+    int32_t on_device;       /* 1: embeddings/components are device pointers */
+    /* filled by the call */
+    int64_t k;               /* min(n_comps + n_oversamples, n_vars, n_obs) */
+    double total_variance;
+    int32_t tv_recomputed;   /* 1 if the cancellation guard re-ran the stable pass */
+    double seconds_device;   /* device time of the PCA (events on the library stream) */
+} scxpca_result;
+
+typedef struct scxpca_matrix scxpca_matrix;
+
+/* One-shot: upload (or borrow) the CSR, run the PCA, free device memory.
+ * Replaces randomized_pca_gpu (scx-accel/src/pca/gpu.rs:32-78). */
+scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p,
+                                    scxpca_result *out, int device);
+
+/* Resident matrix: keep the CSR (and its per-row transform factors) in HBM
+ * across calls. Replaces try_build_resident_csr / ResidentPcaOperator
+ * (scx-gpu/src/gpu_pca_resident.rs:87-283). target_sum/log1p fix X~. */
+scxpca_status scxpca_matrix_create(const scxpca_csr *x, double target_sum, int32_t log1p,
+                                   int device, scxpca_matrix **out);
+scxpca_status scxpca_matrix_destroy(scxpca_matrix *m);
+/* The borrowed device buffers of m changed in place: recompute the row factors
+ * and chunk offsets and drop cached statistics (the next PCA re-runs the stats
+ * pass). No reference counterpart: the reference re-decodes shards per call. */
+scxpca_status scxpca_matrix_refresh(scxpca_matrix *m);
+/* PCA on a resident matrix (the transform fields of p are ignored: the
+ * matrix's own are used). Replaces run_resident_power_loop + host SVD
+ * (gpu_pca_resident.rs:295-347, gpu_pca.rs:592-697). */
+scxpca_status scxpca_matrix_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out);
+
+/* Stage profiler (the reference's SCX_GPU_PROFILE stage timers,
+ * scx-gpu/src/profile.rs:77-208): while enabled, every stage launch on the
+ * matrix stream is bracketed by HIP events. scxpca_matrix_profile returns the
+ * per-stage totals since the last call (ms_total / launches indexed by
+ * scxpca_stage, either may be NULL), clears them, and sets the enable flag. */
+typedef enum {
+    SCXPCA_STAGE_STATS = 0,
+    SCXPCA_STAGE_SPMM_FWD = 1,
+    SCXPCA_STAGE_SPMM_T = 2,
+    SCXPCA_STAGE_SPMM_T_REDUCE = 3,
+    SCXPCA_STAGE_GRAM = 4,
+    SCXPCA_STAGE_APPLY = 5,
+    SCXPCA_STAGE_SPMM_FWD_BF16 = 6,
+    SCXPCA_STAGE_SPMM_T_BF16 = 7,
+    SCXPCA_N_STAGES = 8
+} scxpca_stage;
+scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms_total,
+                                    int64_t *launches, int32_t n_stages);
+
+/* Stage probes on a resident matrix (host buffers in and out), used by the
+ * parity tests. Each mirrors one reference function:
+ *  stats: col_means_and_sum_sq (scx-format-io/src/prefetch.rs:616-650) and
+ *         row sums (scx-sparse/src/csr.rs:543-552): sums, sum_sq, col_nnz
+ *         (any may be NULL), row_factor = target/rowsum (1 when rowsum <= 0).
+ *  spmm:  transpose == 0: Y = (X~ - 1 mu^T) M, M n_vars x k, Y n_obs x k
+ *         (cpu.rs:1185-1229); transpose == 1: Z = (X~ - 1 mu^T)^T Q, Q n_obs x
+ *         k, Z n_vars x k (cpu.rs:1397-1423). Row-major f64 in and out, f32 on
+ *         the device. means NULL: no centering. bf16 != 0 streams X~ as bf16.
+ *  qr:    orthonormal basis of row-major A (m x k) by the GPU CholeskyQR
+ *         (cusolver.rs:297-409); replaces gpu_qr_q / gpu_cholesky_qr2. */
+scxpca_status scxpca_matrix_stats(scxpca_matrix *m, double *sums, double *sum_sq,
+                                  uint32_t *col_nnz, double *row_factor);
+scxpca_status scxpca_matrix_spmm(scxpca_matrix *m, int32_t transpose, int64_t k,
+                                 const double *in, const double *means, int32_t bf16,
+                                 double *out);
+scxpca_status scxpca_qr(const double *a, int64_t m, int64_t k, int32_t qr_method, int device,
+                        double *q_out);
+
+/* Projection Z = (X~ - 1 mu^T) W and its gradient dW = (X~ - 1 mu^T)^T dZ on a
+ * resident matrix, device pointers, on the caller's HIP stream (NULL: the
+ * library stream). W/dW: n_vars x k f32 row-major, Z/dZ: n_obs x k f32
+ * row-major, mean: n_vars f32 or NULL. No reference counterpart (projection
+ * autograd is new in this build); the kernels are the PCA's SpMMs. */
+scxpca_status scxpca_project_forward(scxpca_matrix *m, const float *w, int64_t k,
+                                     const float *mean, float *z, void *stream);
+scxpca_status scxpca_project_backward(scxpca_matrix *m, const float *dz, int64_t k,
+                                      const float *mean, float *dw, void *stream);
+
+/* Ω: ChaCha8Rng::seed_from_u64(seed) + StandardNormal, row-major rows x cols
+ * f64, draw i -> element i (random_gaussian, cpu.rs:1174-1178). Host only. */
+scxpca_status scxpca_gaussian(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
+
+/* Device discovery (scx-accel/src/pca/gpu.rs:81-133, device.rs:96-162). */
+int scxpca_device_count(void);
+scxpca_status scxpca_device_info(int device, char *name, size_t name_len, int64_t *total_mem,
+                                 int32_t *n_cu);
+
+/* Message of the last failed call on this thread; returns its length. */
+size_t scxpca_last_error(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

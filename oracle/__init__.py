@@ -1,0 +1,252 @@
+"""CPU oracle for the randomized sparse PCA path — test infrastructure only.
+
+ctypes wrapper over ``oracle/liborc.so`` (built from ``scxpca_oracle.c`` by
+``oracle/Makefile``). The C file restates scx-accel's CPU randomized PCA
+(``scx-accel/src/pca/cpu.rs``) and the lazy normalize/log1p transform
+(``pyscx/src/lazy_transform/transforms.rs``); each C function cites the
+reference lines it follows.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this package. The product (``scx_amd``) never does.
+
+Parity status:
+  * ChaCha8 stream, ``seed_from_u64``, ``gen_range``/``shuffle`` — pinned by the
+    reference KAT ``seeded_permutation(16, 42)``.
+  * ``next_u64`` pairing, ``Standard``/``Open01`` f64, ziggurat ``StandardNormal``
+    — partially pinned (table invariants, published leading table entries,
+    distribution tests); no reference fixture pins the sample bits.
+  * X~ transform, column stats, SpMMs, QR, SVD, assembly, total variance —
+    pinned against closed forms, brute-force dense f64 and numpy/LAPACK.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+_i64 = ctypes.c_int64
+_u64 = ctypes.c_uint64
+_p = ctypes.c_void_p
+
+
+class _Csr(ctypes.Structure):
+    _fields_ = [("n_obs", _i64), ("n_vars", _i64), ("indptr", _p), ("indices", _p), ("data", _p)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [
+        ("n_comps", _i64),
+        ("n_oversamples", _i64),
+        ("n_power_iterations", _i64),
+        ("zero_center", ctypes.c_int32),
+        ("seed", _u64),
+        ("target_sum", ctypes.c_double),
+        ("log1p", ctypes.c_int32),
+        ("nthreads", ctypes.c_int32),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liborc.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-B" if force else "liborc.so"], cwd=_HERE)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(_LIB_PATH)
+        _lib.orc_splitmix64.restype = _u64
+        _lib.orc_splitmix64.argtypes = [_u64]
+        _lib.orc_total_variance.restype = ctypes.c_double
+        _lib.orc_randomized_pca.restype = ctypes.c_int
+        _lib.orc_thin_svd.restype = ctypes.c_int
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class CsrView:
+    """Keeps the numpy buffers alive for a ctypes ``orc_csr``."""
+
+    def __init__(self, indptr, indices, data, shape):
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.data = np.ascontiguousarray(data, dtype=np.float32)
+        self.shape = (int(shape[0]), int(shape[1]))
+        self.c = _Csr(self.shape[0], self.shape[1], _ptr(self.indptr), _ptr(self.indices), _ptr(self.data))
+
+    @classmethod
+    def from_scipy(cls, m):
+        return cls(m.indptr, m.indices, m.data, m.shape)
+
+    def with_data(self, data):
+        return CsrView(self.indptr, self.indices, data, self.shape)
+
+
+def _as_view(x) -> CsrView:
+    return x if isinstance(x, CsrView) else CsrView.from_scipy(x)
+
+
+# ---------------------------------------------------------------- RNG
+def splitmix64(x: int) -> int:
+    return int(lib().orc_splitmix64(_u64(x)))
+
+
+def seeded_permutation(n: int, seed: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_seeded_permutation(_u64(n), _u64(seed), _ptr(out))
+    return out
+
+
+def chacha8_block(key_words, counter: int = 0, nonce: int = 0) -> np.ndarray:
+    key = np.ascontiguousarray(key_words, dtype=np.uint32)
+    out = np.empty(16, dtype=np.uint32)
+    lib().orc_chacha8_block(_ptr(key), _u64(counter), _u64(nonce), _ptr(out))
+    return out
+
+
+def chacha8_u32(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint32)
+    lib().orc_chacha8_u32_stream(_u64(seed), _u64(n), _ptr(out))
+    return out
+
+
+def chacha8_u64(seed: int, n: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().orc_chacha8_u64_stream(_u64(seed), _u64(n), _ptr(out))
+    return out
+
+
+def zig_norm_tables():
+    x = np.empty(257)
+    f = np.empty(257)
+    lib().orc_zig_norm_tables(_ptr(x), _ptr(f))
+    return x, f
+
+
+def random_gaussian(rows: int, cols: int, seed: int) -> np.ndarray:
+    out = np.empty((rows, cols), dtype=np.float64)
+    lib().orc_random_gaussian(_u64(rows), _u64(cols), _u64(seed), _ptr(out))
+    return out
+
+
+# ---------------------------------------------------------------- X~ / stats
+def row_sums(x) -> np.ndarray:
+    v = _as_view(x)
+    out = np.empty(v.shape[0])
+    lib().orc_row_sums(ctypes.byref(v.c), _ptr(out))
+    return out
+
+
+def transform(x, target_sum: float | None, log1p: bool) -> np.ndarray:
+    v = _as_view(x)
+    out = np.empty_like(v.data)
+    lib().orc_transform(ctypes.byref(v.c), ctypes.c_double(target_sum or 0.0), ctypes.c_int(int(log1p)), _ptr(out))
+    return out
+
+
+def col_sums_and_sum_sq(x):
+    v = _as_view(x)
+    s = np.empty(v.shape[1])
+    ss = np.empty(v.shape[1])
+    lib().orc_col_sums_and_sum_sq(ctypes.byref(v.c), _ptr(s), _ptr(ss))
+    return s, ss
+
+
+# ---------------------------------------------------------------- SpMM / dense
+def spmm_forward(x, m: np.ndarray, means=None, nthreads: int = 0) -> np.ndarray:
+    v = _as_view(x)
+    m = np.ascontiguousarray(m, dtype=np.float64)
+    k = m.shape[1]
+    y = np.empty((v.shape[0], k))
+    mu = None if means is None else np.ascontiguousarray(means, dtype=np.float64)
+    lib().orc_spmm_forward(ctypes.byref(v.c), _ptr(m), _i64(k), None if mu is None else _ptr(mu), _ptr(y), ctypes.c_int(nthreads))
+    return y
+
+
+def spmm_transpose(x, q: np.ndarray, means=None, nthreads: int = 0) -> np.ndarray:
+    v = _as_view(x)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    k = q.shape[1]
+    z = np.empty((v.shape[1], k))
+    mu = None if means is None else np.ascontiguousarray(means, dtype=np.float64)
+    lib().orc_spmm_transpose(ctypes.byref(v.c), _ptr(q), _i64(k), None if mu is None else _ptr(mu), _ptr(z), ctypes.c_int(nthreads))
+    return z
+
+
+def qr_thin_q(a: np.ndarray, nthreads: int = 0) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    q = np.empty_like(a)
+    lib().orc_qr_thin_q(_ptr(a), _i64(a.shape[0]), _i64(a.shape[1]), _ptr(q), ctypes.c_int(nthreads))
+    return q
+
+
+def thin_svd(b: np.ndarray):
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    m, n = b.shape
+    u = np.empty((m, n))
+    s = np.empty(n)
+    vt = np.empty((n, n))
+    rc = lib().orc_thin_svd(_ptr(b), _i64(m), _i64(n), _ptr(u), _ptr(s), _ptr(vt))
+    if rc != 0:
+        raise RuntimeError("oracle Jacobi SVD did not converge")
+    return u, s, vt
+
+
+def total_variance(xt, col_sum_sq, means=None):
+    v = _as_view(xt)
+    css = np.ascontiguousarray(col_sum_sq, dtype=np.float64)
+    mu = None if means is None else np.ascontiguousarray(means, dtype=np.float64)
+    rec = ctypes.c_int(0)
+    tv = lib().orc_total_variance(ctypes.byref(v.c), _ptr(css), None if mu is None else _ptr(mu), ctypes.byref(rec))
+    return float(tv), bool(rec.value)
+
+
+_PCA_ERRORS = {
+    1: "n_components must be > 0",
+    2: "matrix must be non-empty",
+    3: "n_components ({nc}) exceeds matrix rank bound min(n_obs, n_vars) = {mn}",
+    4: "SVD failed",
+}
+
+
+def randomized_pca(x, n_comps=50, n_oversamples=10, n_power_iterations=2, zero_center=True,
+                   seed=0, target_sum=None, log1p=False, nthreads=0):
+    """``randomized_pca_inmemory`` on the raw CSR ``x`` with the lazy transform
+    (normalize_total to ``target_sum`` when given, then log1p) applied once.
+    Returns a dict of f64 arrays."""
+    v = _as_view(x)
+    n, nv = v.shape
+    k = max(1, min(n_comps + n_oversamples, nv, n))
+    emb = np.zeros((n, max(n_comps, 1)))
+    comps = np.zeros((max(n_comps, 1), nv))
+    var = np.zeros(max(n_comps, 1))
+    ratio = np.zeros(max(n_comps, 1))
+    mean = np.zeros(max(nv, 1))
+    sig = np.zeros(k)
+    prm = _Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), seed,
+                  float(target_sum or 0.0), int(bool(log1p)), int(nthreads))
+    rc = lib().orc_randomized_pca(ctypes.byref(v.c), ctypes.byref(prm), _ptr(emb), _ptr(comps), _ptr(var),
+                                  _ptr(ratio), _ptr(mean), _ptr(sig))
+    if rc != 0:
+        raise ValueError(_PCA_ERRORS[rc].format(nc=n_comps, mn=min(n, nv)))
+    return {
+        "embeddings": emb,
+        "components": comps,
+        "variance": var,
+        "variance_ratio": ratio,
+        "mean": mean if zero_center else None,
+        "sigma_all": sig,
+    }

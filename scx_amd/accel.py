@@ -1,0 +1,307 @@
+"""``pyscx.accel``-compatible surface for the randomized sparse PCA on MI355X.
+
+Mirrors the reference binding so ``device="gpu"`` is a drop-in:
+
+* ``pca(adata, n_comps=50, zero_center=True, random_state=0, n_oversamples=10,
+  n_power_iterations=2, device="auto", method="auto", qr_method="householder",
+  prefer_format="csr", allow_tf32=False, spmm_policy="default",
+  memory_budget=None, mask_var=None)`` — pyscx/src/accel/pca.rs:573-1219,
+  stub pyscx/python/pyscx/accel.pyi:318-333. Same defaults, same ValueError
+  texts for bad arguments, same output slots: ``obsm["X_pca"]`` (f32),
+  ``varm["PCs"]`` (f32, full var axis, masked-out genes 0), ``uns["pca"]``
+  (variance, variance_ratio, backend, params) and the route stamp
+  ``uns["scx_accel"]["pca"]`` (scx-accel/src/route.rs:325-372).
+* ``normalize_total(adata, target_sum=None)`` / ``log1p(adata)`` record the
+  transform lazily (pyscx/src/accel/preprocessing.rs:95-342,
+  pyscx/src/lazy_transform/transforms.rs:15-88): ``adata.X`` becomes a
+  :class:`LazyCsr` and the GPU PCA applies normalize -> log1p inside its SpMM
+  kernels, so X~ is never materialized.
+
+Only the GPU route is implemented here. ``device="cpu"`` (and ``"auto"`` with no
+GPU) raises: the CPU path of the reference is not part of this package.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+
+__all__ = ["pca", "normalize_total", "log1p", "LazyCsr", "resolve_device", "gpu_available", "gpu_info"]
+
+BACKEND = "scx-gpu-hip"
+
+
+# ---------------------------------------------------------------------------
+# device grammar (scx-accel/src/device.rs:96-162)
+# ---------------------------------------------------------------------------
+
+def gpu_available() -> bool:
+    try:
+        return _lib.device_count() > 0
+    except ImportError:
+        return False
+
+
+def gpu_info(device: int = 0) -> dict:
+    return _lib.device_info(device)
+
+
+def resolve_device(device: str):
+    """Returns ("cpu", None) or ("gpu", ordinal); raises ValueError /
+    RuntimeError with the reference's messages."""
+    if device == "cpu":
+        return ("cpu", None)
+    if device == "auto":
+        return ("gpu", 0) if gpu_available() else ("cpu", None)
+    if device.startswith("gpu"):
+        suffix = device[3:]
+        if suffix == "":
+            ordinal = 0
+        elif suffix.startswith(":"):
+            try:
+                ordinal = int(suffix[1:])
+                if ordinal < 0 or not suffix[1:].isdigit():
+                    raise ValueError
+            except ValueError:
+                raise ValueError(f"device='{device}' has a non-integer suffix; expected 'gpu:N' with N a "
+                                 f"non-negative integer.") from None
+        else:
+            raise ValueError(f"unknown device: '{device}'. Use 'auto', 'cpu', 'gpu', or 'gpu:N'.")
+        if not gpu_available():
+            raise RuntimeError(f"device='{device}' requested but no ROCm GPU found")
+        count = _lib.device_count()
+        if ordinal >= count:
+            raise RuntimeError(f"device='{device}' but only {count} ROCm device{' is' if count == 1 else 's are'} "
+                               f"visible")
+        return ("gpu", ordinal)
+    raise ValueError(f"unknown device: '{device}'. Use 'auto', 'cpu', 'gpu', or 'gpu:N'.")
+
+
+# ---------------------------------------------------------------------------
+# lazy normalize/log1p view
+# ---------------------------------------------------------------------------
+
+class LazyCsr:
+    """An in-memory CSR with a pending transform chain, the counterpart of the
+    reference's ScxLazyTransformedDataset (pyscx/src/lazy_transform/). Only the
+    fused chain the PCA path consumes is modelled: NormalizeTotal(target) then
+    Log1p, or either alone."""
+
+    def __init__(self, csr, target_sum=None, log1p=False):
+        import scipy.sparse as sp
+
+        if not sp.issparse(csr):
+            csr = sp.csr_matrix(np.asarray(csr))
+        csr = csr.tocsr()
+        self.raw = csr
+        self.target_sum = target_sum
+        self.log1p = bool(log1p)
+
+    @property
+    def shape(self):
+        return self.raw.shape
+
+    def materialize(self):
+        """The transformed CSR (host, f32), with the reference rounding rule
+        ((x as f64 * factor) as f32).ln_1p() (transforms.rs:21-39)."""
+        import scipy.sparse as sp
+
+        m = self.raw
+        data = m.data.astype(np.float32, copy=True)
+        if self.target_sum is not None:
+            rs = np.add.reduceat(m.data.astype(np.float64), m.indptr[:-1]) if m.nnz else np.zeros(m.shape[0])
+            rs = np.where(np.diff(m.indptr) > 0, rs, 0.0)
+            f = np.where(rs > 0, self.target_sum / np.where(rs > 0, rs, 1.0), 1.0)
+            rows = np.repeat(np.arange(m.shape[0]), np.diff(m.indptr))
+            data = (data.astype(np.float64) * f[rows]).astype(np.float32)
+        if self.log1p:
+            data = np.log1p(data)
+        return sp.csr_matrix((data, m.indices, m.indptr), shape=m.shape)
+
+
+def _median_positive_total(csr) -> float:
+    rs = np.asarray(csr.sum(axis=1, dtype=np.float64)).ravel()
+    pos = np.sort(rs[rs > 0])
+    if pos.size == 0:
+        return 0.0
+    n = pos.size
+    return float(pos[n // 2]) if n % 2 else float(0.5 * (pos[n // 2 - 1] + pos[n // 2]))
+
+
+def normalize_total(adata, target_sum=None, device="auto") -> None:
+    """Record normalize_total lazily. target_sum=None -> median of positive
+    per-cell totals (scanpy semantics, preprocessing.rs:89-92)."""
+    resolve_device(device)
+    x = adata.X
+    if isinstance(x, LazyCsr):
+        if x.target_sum is not None or x.log1p:
+            # a second normalization over an already-transformed view is not a
+            # chain the fused kernels model; materialize first
+            x = LazyCsr(x.materialize())
+        base = x.raw
+    else:
+        base = LazyCsr(x).raw
+    t = _median_positive_total(base) if target_sum is None else float(target_sum)
+    adata.X = LazyCsr(base, target_sum=t, log1p=False)
+
+
+def log1p(adata, device="auto") -> None:
+    resolve_device(device)
+    x = adata.X
+    if isinstance(x, LazyCsr):
+        if x.log1p:
+            x = LazyCsr(x.materialize())
+        adata.X = LazyCsr(x.raw, target_sum=x.target_sum, log1p=True)
+    else:
+        adata.X = LazyCsr(x, log1p=True)
+
+
+# ---------------------------------------------------------------------------
+# pca
+# ---------------------------------------------------------------------------
+
+def _bool_vec(obj, n_vars):
+    try:
+        arr = np.ascontiguousarray(np.asarray(obj), dtype=bool)
+    except Exception:
+        raise ValueError("mask_var must be a boolean array (or a var-column name resolving to one)") from None
+    if arr.ndim != 1:
+        raise ValueError("mask_var must be 1-D and boolean (or a var-column name resolving to one)")
+    return arr
+
+
+def _resolve_mask_var(adata, mask_var, n_vars):
+    """resolve_mask_var (pyscx/src/accel/pca.rs:470-526)."""
+    if mask_var is not None:
+        if isinstance(mask_var, str):
+            var = getattr(adata, "var", None)
+            try:
+                col = var[mask_var]
+            except Exception:
+                raise ValueError(f"mask_var column '{mask_var}' not found in adata.var") from None
+            mask, name = _bool_vec(col, n_vars), mask_var
+        else:
+            mask, name = _bool_vec(mask_var, n_vars), None
+    else:
+        var = getattr(adata, "var", None)
+        has = False
+        try:
+            has = var is not None and "highly_variable" in var
+        except Exception:
+            has = False
+        if not has:
+            return None
+        mask, name = _bool_vec(var["highly_variable"], n_vars), "highly_variable"
+    if mask.shape[0] != n_vars:
+        raise ValueError(f"mask_var length {mask.shape[0]} != n_vars {n_vars}")
+    cols = np.nonzero(mask)[0].astype(np.int64)
+    if cols.size == 0:
+        raise ValueError("mask_var selects zero genes (all-false mask)")
+    return cols, name
+
+
+def _route_stamp(device_id, spmm_policy, allow_tf32, resident):
+    """AccelExecutionInfo fields (scx-accel/src/route.rs:325-372)."""
+    return {
+        "route": "gpu_csr",
+        "fallback_reason": "none",
+        "chunk_size": None,
+        "graph_replay": False,
+        "csc_available": None,
+        "shards_decoded": None,
+        "shards_uploaded": None,
+        "math_mode": "allow_tf32" if allow_tf32 else "strict_fp32",
+        "spmm_policy": spmm_policy,
+        "rapids_version": None,
+        "cuml_version": None,
+        "cupy_version": None,
+        "transfer_mode": None,
+        "device_id": device_id,
+        "bytes_uploaded": None,
+        "n_shards_shufdelta_gpu": None,
+        "resident_csr": resident,
+        "reduction": "deterministic_partials",
+    }
+
+
+def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 0, n_oversamples: int = 10,
+        n_power_iterations: int = 2, device: str = "auto", method: str = "auto", qr_method: str = "householder",
+        prefer_format: str = "csr", allow_tf32: bool = False, spmm_policy: str = "default", memory_budget=None,
+        mask_var=None, *, precision: str = "fp32") -> None:
+    """Randomized PCA of ``adata.X`` on the GPU; results written into ``adata``.
+
+    ``precision`` (keyword only, not in the reference): ``"fp32"`` (default)
+    or ``"bf16"`` — the BF16-sketch form streams X~ as bf16 in the sketch
+    passes and runs the final B pass and Rayleigh-Ritz in FP32/FP64.
+    """
+    kind, device_id = resolve_device(device)
+    if method not in ("auto", "covariance", "randomized"):
+        raise ValueError(f"Invalid method={method!r}; expected 'auto', 'covariance', or 'randomized'")
+    if qr_method not in ("householder", "cholesky"):
+        raise ValueError(f"Invalid qr_method={qr_method!r}; expected 'householder' or 'cholesky'")
+    if spmm_policy not in ("default", "deterministic", "benchmark_once"):
+        raise ValueError(f"Invalid spmm_policy={spmm_policy!r}; expected 'default', 'deterministic', "
+                         f"or 'benchmark_once'")
+    if prefer_format != "csr":
+        raise ValueError(f"pyscx.accel.pca only supports prefer_format='csr' (got {prefer_format!r}). The "
+                         f"covariance build and randomized SpMM paths are row-major; CSC offers no measurable "
+                         f"speed-up and is not implemented.")
+    if precision not in ("fp32", "bf16"):
+        raise ValueError(f"Invalid precision={precision!r}; expected 'fp32' or 'bf16'")
+    if kind != "gpu":
+        raise RuntimeError("scx_amd implements the GPU route of pca only; device='cpu' (or 'auto' without a "
+                           "visible ROCm GPU) is served by the reference CPU implementation")
+
+    import scipy.sparse as sp
+
+    x = adata.X
+    lazy = x if isinstance(x, LazyCsr) else LazyCsr(x)
+    raw = lazy.raw
+    if not isinstance(raw, sp.csr_matrix):
+        raw = sp.csr_matrix(raw)
+    full_n_vars = raw.shape[1]
+    masked = _resolve_mask_var(adata, mask_var, full_n_vars)
+    if masked is not None:
+        raw = raw[:, masked[0]]
+    if not raw.has_sorted_indices:
+        raw = raw.sorted_indices()
+    n_obs, n_vars = raw.shape
+
+    m = _lib.Matrix(raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
+                    raw.data.astype(np.float32, copy=False), raw.shape,
+                    target_sum=lazy.target_sum if lazy.target_sum is not None else None, log1p=lazy.log1p,
+                    device=device_id)
+    try:
+        res = m.pca(n_comps=n_comps, n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
+                    zero_center=zero_center, seed=int(random_state) & ((1 << 64) - 1),
+                    qr_method=_lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER,
+                    precision=_lib.PRECISION_BF16_SKETCH if precision == "bf16" else _lib.PRECISION_FP32)
+    except ValueError as e:
+        # AccelError::InvalidInput surfaces as RuntimeError (pca.rs:845)
+        raise RuntimeError(str(e)) from None
+    except _lib.ScxPcaError as e:
+        raise RuntimeError(f"GPU PCA failed: {e}") from None
+    finally:
+        m.close()
+
+    pcs = np.zeros((full_n_vars, n_comps), dtype=np.float32)
+    if masked is not None:
+        pcs[masked[0], :] = res["components"].T
+    else:
+        pcs[:, :] = res["components"].T
+    adata.obsm["X_pca"] = res["embeddings"]
+    adata.varm["PCs"] = pcs
+    adata.uns["pca"] = {
+        "variance": np.asarray(res["variance"], dtype=np.float64),
+        "variance_ratio": np.asarray(res["variance_ratio"], dtype=np.float64),
+        "backend": BACKEND,
+        "params": {
+            "zero_center": bool(zero_center),
+            "use_highly_variable": masked is not None and masked[1] == "highly_variable",
+            "mask_var": None if masked is None else masked[1],
+            "n_comps": int(n_comps),
+        },
+    }
+    container = dict(adata.uns.get("scx_accel", {}) or {})
+    container["pca"] = _route_stamp(device_id, spmm_policy, allow_tf32, True)
+    adata.uns["scx_accel"] = container

@@ -1,0 +1,999 @@
+// Host side of the MI355X randomized sparse PCA: device memory, the power loop
+// and the C ABI declared in include/scxpca.h.
+//
+// Pipeline for one call (reference order: scx-accel/src/pca/cpu.rs:535-605,
+// GPU reference scx-gpu/src/gpu_pca.rs:419-697):
+//   stats pass   -> mu, sum_sq, col_nnz            (k_col_stats)
+//   Omega (host, overlapped with the stats pass)   (omega.cpp)
+//   Y = X~ Omega - 1 (mu^T Omega)                  (k_spmm_fwd)
+//   Q = CholeskyQR(Y)                              (k_gram, k_chol, k_apply)
+//   repeat q times:
+//     B = X~^T Q - mu (1^T Q)                      (k_spmm_t + k_spmm_t_reduce)
+//     Q_B = CholeskyQR(B) in f64                   (k_gram64, k_chol, k_apply64)
+//     Y = X~ Q_B - 1 (mu^T Q_B); Q = CholeskyQR(Y)
+//   B = X~^T Q - mu (1^T Q)
+//   Rayleigh-Ritz: B = Q_B R (f64 CholeskyQR3), R = U_R S V^T (Jacobi, f64),
+//   components = (Q_B U_R)^T, embeddings = Q (V S)
+// Every kernel runs on the matrix's stream; the call synchronizes once at the end.
+//
+// Deviation from the CPU loop, recorded because it changes rounding only:
+// the CPU skips the QR of B when n_power_iterations <= 2 (cpu.rs:574-584);
+// here B is always orthonormalized before the next forward pass (as the
+// reference GPU loop does, pca_operator.rs:100-123). In exact arithmetic both
+// span the same subspace; the orthonormal panel keeps the FP32 Gram of the
+// next Y well conditioned.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/scxpca.h"
+#include "kernels.h"
+
+namespace scxpca {
+void gaussian_matrix(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
+}
+
+using namespace scxpca;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error : std::runtime_error {
+    scxpca_status status;
+    Error(scxpca_status s, const std::string &m) : std::runtime_error(m), status(s) {}
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation)
+        throw Error(SCXPCA_ERR_OOM, std::string("GPU out of memory: ") + what);
+    throw Error(SCXPCA_ERR_DEVICE, std::string("HIP error '") + hipGetErrorString(e) + "' in " + what);
+}
+#define HIPC(x) hip_check((x), #x)
+
+template <typename F>
+scxpca_status guarded(F &&f) {
+    try {
+        f();
+        return SCXPCA_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.status;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host out of memory";
+        return SCXPCA_ERR_OOM;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return SCXPCA_ERR_DEVICE;
+    }
+}
+
+// Owned device allocation.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) n = 16;
+        HIPC(hipMalloc(&p, n));
+        bytes = n;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
+inline unsigned blocks_for(int64_t n, int per = 256) {
+    const int64_t b = (n + per - 1) / per;
+    return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(b, 1 << 30)));
+}
+
+constexpr int kCB = 256;       // gene width of the row split offsets
+constexpr int kStatsCG = 2048; // genes per column-stats block
+constexpr int kMaxColBlocks = 1024;  // partial rows of the column-sum kernels
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+            throw Error(SCXPCA_ERR_NO_DEVICE, "no ROCm GPU visible");
+        if (dev < 0 || dev >= n)
+            throw Error(SCXPCA_ERR_NO_DEVICE, "device=" + std::to_string(dev) + " but only " + std::to_string(n) +
+                                                  " GPU" + (n == 1 ? " is" : "s are") + " visible");
+        HIPC(hipGetDevice(&prev));
+        HIPC(hipSetDevice(dev));
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Resident matrix
+// ---------------------------------------------------------------------------
+
+struct scxpca_matrix {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n_obs = 0, n_vars = 0, nnz = 0;
+    DevBuf own_indptr, own_idx, own_val;
+    const int64_t *indptr = nullptr;
+    const uint32_t *idx = nullptr;
+    const float *val = nullptr;
+    double target = 0.0;
+    int flags = 0;
+    DevBuf rowf;
+    DevBuf spl;
+    int nsplit = 0;
+    DevBuf xbf;
+    bool xbf_ready = false;
+    // cached column statistics of X~
+    bool stats_ready = false;
+    DevBuf sum, sq, cnt, psum, psq, pcnt;
+    // PCA workspace
+    DevBuf y, q, panel, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
+    std::vector<double> host_omega;
+    // stage profiler (events on the matrix stream; off unless enabled)
+    bool profiling = false;
+    struct Mark { int stage; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+
+    void clear_marks() {
+        for (auto &mk : marks) { (void)hipEventDestroy(mk.a); (void)hipEventDestroy(mk.b); }
+        marks.clear();
+    }
+    ~scxpca_matrix() {
+        clear_marks();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+// Records a start/stop event pair around one stage launch when profiling.
+struct StageScope {
+    scxpca_matrix *m;
+    hipStream_t s;
+    int idx = -1;
+    StageScope(scxpca_matrix *mm, hipStream_t ss, int stage) : m(mm), s(ss) {
+        if (!m->profiling) return;
+        scxpca_matrix::Mark mk{stage, nullptr, nullptr};
+        HIPC(hipEventCreate(&mk.a));
+        HIPC(hipEventCreate(&mk.b));
+        HIPC(hipEventRecord(mk.a, s));
+        m->marks.push_back(mk);
+        idx = static_cast<int>(m->marks.size()) - 1;
+    }
+    ~StageScope() {
+        if (idx >= 0) (void)hipEventRecord(m->marks[idx].b, s);
+    }
+};
+
+void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1p) {
+    if (x->n_obs < 0 || x->n_vars < 0) throw Error(SCXPCA_ERR_INVALID, "matrix shape must be non-negative");
+    if (x->n_vars > 0xffffffffll) throw Error(SCXPCA_ERR_INVALID, "n_vars exceeds the u32 index range");
+    m->n_obs = x->n_obs;
+    m->n_vars = x->n_vars;
+    m->target = target;
+    m->flags = (target > 0.0 ? kXfNorm : 0) | (log1p ? kXfLog1p : 0);
+    HIPC(hipStreamCreate(&m->stream));
+    if (x->on_device) {
+        m->indptr = x->indptr;
+        m->idx = x->indices;
+        m->val = x->data;
+        HIPC(hipMemcpy(&m->nnz, x->indptr + x->n_obs, sizeof(int64_t), hipMemcpyDeviceToHost));
+    } else {
+        m->nnz = x->indptr[x->n_obs];
+        // canonicalize host rows: the chunked kernels need non-decreasing indices
+        std::vector<uint32_t> sidx;
+        std::vector<float> sval;
+        const uint32_t *hidx = x->indices;
+        const float *hval = x->data;
+        bool sorted = true;
+        for (int64_t r = 0; r < x->n_obs && sorted; ++r)
+            for (int64_t p = x->indptr[r] + 1; p < x->indptr[r + 1]; ++p)
+                if (x->indices[p] < x->indices[p - 1]) { sorted = false; break; }
+        for (int64_t p = 0; p < m->nnz; ++p)
+            if (x->indices[p] >= static_cast<uint64_t>(x->n_vars))
+                throw Error(SCXPCA_ERR_INVALID, "column index out of range");
+        if (!sorted) {
+            sidx.assign(x->indices, x->indices + m->nnz);
+            sval.assign(x->data, x->data + m->nnz);
+            std::vector<int64_t> perm;
+            for (int64_t r = 0; r < x->n_obs; ++r) {
+                const int64_t a = x->indptr[r], b = x->indptr[r + 1];
+                perm.resize(b - a);
+                for (int64_t i = 0; i < b - a; ++i) perm[i] = a + i;
+                std::stable_sort(perm.begin(), perm.end(),
+                                 [&](int64_t u, int64_t v) { return x->indices[u] < x->indices[v]; });
+                for (int64_t i = 0; i < b - a; ++i) {
+                    sidx[a + i] = x->indices[perm[i]];
+                    sval[a + i] = x->data[perm[i]];
+                }
+            }
+            hidx = sidx.data();
+            hval = sval.data();
+        }
+        m->own_indptr.ensure(sizeof(int64_t) * (x->n_obs + 1));
+        m->own_idx.ensure(sizeof(uint32_t) * std::max<int64_t>(m->nnz, 1));
+        m->own_val.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+        HIPC(hipMemcpyAsync(m->own_indptr.p, x->indptr, sizeof(int64_t) * (x->n_obs + 1), hipMemcpyHostToDevice, m->stream));
+        if (m->nnz) {
+            HIPC(hipMemcpyAsync(m->own_idx.p, hidx, sizeof(uint32_t) * m->nnz, hipMemcpyHostToDevice, m->stream));
+            HIPC(hipMemcpyAsync(m->own_val.p, hval, sizeof(float) * m->nnz, hipMemcpyHostToDevice, m->stream));
+        }
+        HIPC(hipStreamSynchronize(m->stream));  // host staging vectors go out of scope
+        m->indptr = m->own_indptr.as<int64_t>();
+        m->idx = m->own_idx.as<uint32_t>();
+        m->val = m->own_val.as<float>();
+    }
+    if (m->n_obs == 0) return;
+    if (x->on_device) {
+        DevBuf flag;
+        flag.ensure(sizeof(int));
+        HIPC(hipMemsetAsync(flag.p, 0, sizeof(int), m->stream));
+        k_check_sorted<<<blocks_for(std::max<int64_t>(m->n_obs, m->nnz), 256), 256, 0, m->stream>>>(
+            m->indptr, m->idx, m->n_obs, m->n_vars, flag.as<int>());
+        HIPC(hipGetLastError());
+        int h = 0;
+        HIPC(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+        HIPC(hipStreamSynchronize(m->stream));
+        if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
+        if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
+        if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
+    }
+    const unsigned row_blocks = blocks_for(m->n_obs, 4);  // 4 waves per block, one row per wave
+    if (m->flags & kXfNorm) {
+        m->rowf.ensure(sizeof(double) * m->n_obs);
+        k_row_factor<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, target,
+                                                                         m->rowf.as<double>());
+        HIPC(hipGetLastError());
+    }
+    m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
+    m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
+    k_splits<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
+                                                                  m->spl.as<uint32_t>());
+    HIPC(hipGetLastError());
+    HIPC(hipStreamSynchronize(m->stream));
+}
+
+const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr; }
+
+void ensure_xbf(scxpca_matrix *m) {
+    if (m->xbf_ready) return;
+    m->xbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    if (m->n_obs > 0)
+        k_xtilde_bf16<<<std::min(blocks_for(m->n_obs, 4), 65536u), 256, 0, m->stream>>>(
+            m->indptr, m->val, rowf_ptr(m), m->flags, m->n_obs, m->xbf.as<uint16_t>());
+    HIPC(hipGetLastError());
+    m->xbf_ready = true;
+}
+
+// Column statistics of X~ (sum, sum_sq, nnz per gene), cached on the matrix.
+void ensure_stats(scxpca_matrix *m) {
+    if (m->stats_ready) return;
+    const int nch = static_cast<int>((m->n_vars + kStatsCG - 1) / kStatsCG);
+    const int nrb = std::max(1, std::min<int>(static_cast<int>((m->n_obs + 63) / 64), std::max(8, 1024 / nch)));
+    const int64_t rows_per_rb = (m->n_obs + nrb - 1) / nrb;
+    m->sum.ensure(sizeof(double) * m->n_vars);
+    m->sq.ensure(sizeof(double) * m->n_vars);
+    m->cnt.ensure(sizeof(uint32_t) * m->n_vars);
+    m->psum.ensure(sizeof(double) * nrb * m->n_vars);
+    m->psq.ensure(sizeof(double) * nrb * m->n_vars);
+    m->pcnt.ensure(sizeof(uint32_t) * nrb * m->n_vars);
+    StageScope prof(m, m->stream, SCXPCA_STAGE_STATS);
+    k_col_stats<kStatsCG><<<nrb * nch, 256, 0, m->stream>>>(
+        m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, m->spl.as<uint32_t>(), m->nsplit, kStatsCG / kCB, nch,
+        m->n_obs, m->n_vars, rows_per_rb, m->psum.as<double>(), m->psq.as<double>(), m->pcnt.as<uint32_t>());
+    HIPC(hipGetLastError());
+    k_col_stats_reduce<<<blocks_for(m->n_vars), 256, 0, m->stream>>>(m->psum.as<double>(), m->psq.as<double>(),
+                                                                     m->pcnt.as<uint32_t>(), nrb, m->n_vars,
+                                                                     m->sum.as<double>(), m->sq.as<double>(),
+                                                                     m->cnt.as<uint32_t>());
+    HIPC(hipGetLastError());
+    m->stats_ready = true;
+}
+
+__global__ void k_mean(const double *sum, int64_t n_vars, double inv_n, double *mu) {
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v < n_vars) mu[v] = sum[v] * inv_n;
+}
+
+// ---------------------------------------------------------------------------
+// The PCA engine, specialised on the padded panel width KP.
+// ---------------------------------------------------------------------------
+
+template <int KP>
+struct Engine {
+    static constexpr int CG = (KP == 64) ? 512 : 256;  // genes per k_spmm_t block: CG*KP*4 = 128 KB
+    scxpca_matrix *m;
+    hipStream_t s;
+    int k = 0;
+    // small KP x KP f64 scratch layout inside m->small
+    double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *partc;
+    float *W32, *mc32;
+    int *status;
+    int nrb_t = 0, nch_t = 0, ncol_blk = 0, gram_waves = 0;
+    int64_t rows_per_rb_t = 0, rows_per_wave = 0, rows_per_colblk = 0;
+
+    explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        nch_t = static_cast<int>((nv + CG - 1) / CG);
+        int nrb = std::max(1, 512 / nch_t);
+        nrb = std::min<int64_t>(nrb, std::max<int64_t>(1, (n + 31) / 32));
+        nrb_t = (nrb + 7) / 8 * 8;
+        rows_per_rb_t = (n + nrb_t - 1) / nrb_t;
+        gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
+        gram_waves = (gram_waves + 3) / 4 * 4;
+        rows_per_wave = (n + gram_waves - 1) / gram_waves;
+        rows_per_wave += rows_per_wave & 1;
+        ncol_blk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((std::max(n, nv) + 1023) / 1024, 1024)));
+        rows_per_colblk = (std::max(n, nv) + ncol_blk - 1) / ncol_blk;
+
+        const size_t panel_rows = static_cast<size_t>(std::max<int64_t>(n, 1));
+        m->y.ensure(sizeof(float) * panel_rows * KP);
+        m->q.ensure(sizeof(float) * panel_rows * KP);
+        m->panel.ensure(sizeof(float) * std::max<int64_t>(nv, 1) * KP);
+        m->panel64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
+        m->b64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
+        m->t64a.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
+        m->t64b.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
+        m->tpart.ensure(sizeof(float) * static_cast<size_t>(nrb_t) * std::max<int64_t>(nv, 1) * KP);
+        const int nb64 = gram64_blocks();
+        m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(gram_waves, nb64)) * KP * KP);
+        m->cpart.ensure(sizeof(double) * static_cast<size_t>(kMaxColBlocks) * KP);
+        constexpr size_t KK = static_cast<size_t>(KP) * KP;
+        m->small.ensure(sizeof(double) * (KK * 11 + 4 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + KP));
+        double *d = m->small.as<double>();
+        G = d; R1 = G + KK; R2 = R1 + KK; R3 = R2 + KK; Rt = R3 + KK; W64 = Rt + KK; U = W64 + KK; V = U + KK;
+        Acol = V + KK; Vcol = Acol + KK; partc = Vcol + KK;  // partc: KK doubles spare
+        sig = partc + KK; sumq = sig + KP; mc64 = sumq + KP; mu = mc64 + KP + KP;
+        W32 = reinterpret_cast<float *>(mu + std::max<int64_t>(nv, 1));
+        mc32 = W32 + KK;
+        m->status.ensure(sizeof(int));
+        status = m->status.as<int>();
+    }
+
+    int gram64_blocks() const {
+        return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((m->n_vars + 255) / 256, 256)));
+    }
+
+    void check() { HIPC(hipGetLastError()); }
+
+    // mc = mu^T panel (f32 for the forward epilogue)
+    void mean_correction(const float *panel) {
+        const int64_t nv = m->n_vars;
+        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nv + 255) / 256, kMaxColBlocks)));
+        const int64_t rpb = (nv + nb - 1) / nb;
+        k_colsum<KP, float><<<nb, 256, 0, s>>>(panel, mu, nv, rpb, m->cpart.as<double>());
+        check();
+        k_colsum_reduce<KP><<<1, KP, 0, s>>>(m->cpart.as<double>(), nb, mc64, mc32);
+        check();
+    }
+
+    // sumq = 1^T Q (f64)
+    void column_sums(const float *Q) {
+        const int64_t n = m->n_obs;
+        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 1023) / 1024, kMaxColBlocks)));
+        const int64_t rpb = (n + nb - 1) / nb;
+        k_colsum<KP, float><<<nb, 256, 0, s>>>(Q, nullptr, n, rpb, m->cpart.as<double>());
+        check();
+        k_colsum_reduce<KP><<<1, KP, 0, s>>>(m->cpart.as<double>(), nb, sumq, nullptr);
+        check();
+    }
+
+    void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
+        if (m->n_obs == 0) return;
+        const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
+        StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
+        if (bf16)
+            k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
+                                                      m->flags, panel, centered ? mc32 : nullptr, Y, m->n_obs);
+        else
+            k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
+                                                       panel, centered ? mc32 : nullptr, Y, m->n_obs);
+        check();
+    }
+
+    // B64 (and/or B32) = X~^T Q - mu sumq^T
+    void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
+        const int64_t nv = m->n_vars;
+        if (centered) column_sums(Q);
+        if (m->n_obs > 0) {
+            const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
+            StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
+            if (bf16)
+                k_spmm_t<KP, CG, true><<<grid, 512, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
+                                                            m->flags, Q, m->spl.as<uint32_t>(), m->nsplit, CG / kCB,
+                                                            nch_t, nrb_t, m->n_obs, nv, rows_per_rb_t,
+                                                            m->tpart.as<float>());
+            else
+                k_spmm_t<KP, CG, false><<<grid, 512, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
+                                                             Q, m->spl.as<uint32_t>(), m->nsplit, CG / kCB, nch_t,
+                                                             nrb_t, m->n_obs, nv, rows_per_rb_t, m->tpart.as<float>());
+            check();
+        } else {
+            HIPC(hipMemsetAsync(m->tpart.p, 0, sizeof(float) * nrb_t * nv * KP, s));
+        }
+        StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
+        k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb_t, nv,
+                                                                 centered ? mu : nullptr, sumq, B64, B32);
+        check();
+    }
+
+    // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
+    void chol(double shift, double *Rdst) {
+        k_chol<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+        check();
+    }
+
+    // Tall f32 panel CholeskyQR: CQR2, or shifted CQR3 (stable) for Householder mode.
+    // Returns the buffer that holds Q (one of src / tmp).
+    float *cqr_tall(float *src, float *tmp, int qr_method) {
+        const int64_t n = m->n_obs;
+        const int passes = qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2;
+        const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
+        for (int pass = 0; pass < passes; ++pass) {
+            {
+                StageScope prof(m, s, SCXPCA_STAGE_GRAM);
+                k_gram<KP><<<gram_waves / 4, 256, 0, s>>>(src, n, rows_per_wave, m->gpart.as<double>());
+            }
+            check();
+            k_gram_reduce<KP><<<blocks_for(KP * KP), 256, 0, s>>>(m->gpart.as<double>(), gram_waves, G);
+            check();
+            const double shift = (qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0;
+            chol(shift, R1);
+            {
+                StageScope prof(m, s, SCXPCA_STAGE_APPLY);
+                k_apply<KP><<<apply_grid, 256, 0, s>>>(src, W32, tmp, n, KP, KP);
+            }
+            check();
+            std::swap(src, tmp);
+        }
+        return src;
+    }
+
+    // Small f64 CholeskyQR3 (shifted first pass) of A (n_vars x KP) in place
+    // into Qout; R = R3 R2 R1 written to Rt.
+    void cqr_small64(const double *A, double *Qout) {
+        const int64_t nv = m->n_vars;
+        const int nb = gram64_blocks();
+        const int64_t rpb = (nv + nb - 1) / nb;
+        const double *src = A;
+        double *bufs[3] = {m->t64a.as<double>(), m->t64b.as<double>(), Qout};
+        double *Rs[3] = {R1, R2, R3};
+        for (int pass = 0; pass < 3; ++pass) {
+            k_gram64<KP><<<nb, 256, 0, s>>>(src, nv, rpb, m->gpart.as<double>());
+            check();
+            k_sum_parts<KP><<<blocks_for(KP * KP), 256, 0, s>>>(m->gpart.as<double>(), nb, G);
+            check();
+            chol(pass == 0 ? 1e-13 : 0.0, Rs[pass]);
+            k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(src, W64, bufs[pass], nullptr, nv);
+            check();
+            src = bufs[pass];
+        }
+        // Rt = R3 * (R2 * R1)
+        k_small_gemm64<KP><<<1, 256, 0, s>>>(R2, R1, Acol);
+        check();
+        k_small_gemm64<KP><<<1, 256, 0, s>>>(R3, Acol, Rt);
+        check();
+    }
+
+    void run(const scxpca_params *p, scxpca_result *out, const double *host_omega) {
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        const bool centered = p->zero_center != 0;
+        const bool sketch_bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
+        const int nc = static_cast<int>(p->n_comps);
+        HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
+        if (sketch_bf16) ensure_xbf(m);
+        ensure_stats(m);
+        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, 1.0 / static_cast<double>(n), mu);
+        check();
+        // Omega -> f32 panel
+        HIPC(hipMemcpyAsync(m->panel64.p, host_omega, sizeof(double) * nv * k, hipMemcpyHostToDevice, s));
+        k_pack_panel<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), nv, k, m->panel.as<float>());
+        check();
+        float *Y = m->y.as<float>(), *T = m->q.as<float>();
+        if (centered) mean_correction(m->panel.as<float>());
+        spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
+        float *Q = cqr_tall(Y, T, p->qr_method);
+        float *Qother = (Q == Y) ? T : Y;
+        for (int64_t it = 0; it < p->n_power_iterations; ++it) {
+            spmm_t(Q, centered, sketch_bf16, m->b64.as<double>(), nullptr);
+            cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
+            k_f64_to_f32<<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), m->panel.as<float>(), nv * KP);
+            check();
+            if (centered) mean_correction(m->panel.as<float>());
+            spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Qother);
+            float *Qn = cqr_tall(Qother, Q, p->qr_method);
+            Qother = (Qn == Q) ? Qother : Q;
+            Q = Qn;
+        }
+        // Final B pass in FP32 (also in the BF16 sketch form).
+        spmm_t(Q, centered, false, m->b64.as<double>(), nullptr);
+        // Rayleigh-Ritz: B = Q_B Rt, Rt = U_R S V^T
+        cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
+        k_jacobi_svd<KP><<<1, 256, 0, s>>>(Rt, k, Acol, Vcol, U, V, sig, status);
+        check();
+        // Uhat = Q_B U_R (n_vars x KP, f64) -> components
+        k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), U, m->t64a.as<double>(), nullptr, nv);
+        check();
+        float *comps_dev = nullptr;
+        DevBuf comps_tmp, emb_tmp;
+        if (out->on_device) {
+            comps_dev = out->components;
+        } else {
+            comps_tmp.ensure(sizeof(float) * std::max<int64_t>(nv * nc, 1));
+            comps_dev = comps_tmp.as<float>();
+        }
+        k_components<KP><<<blocks_for(nv * nc), 256, 0, s>>>(m->t64a.as<double>(), nc, nv, comps_dev);
+        check();
+        k_emb_weights<KP><<<1, 256, 0, s>>>(V, sig, nc, W32);
+        check();
+        float *emb_dev = nullptr;
+        if (out->on_device) {
+            emb_dev = out->embeddings;
+        } else {
+            emb_dev = Qother;  // n_obs x KP scratch, embeddings written with ld = nc
+        }
+        const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
+        k_apply<KP><<<apply_grid, 256, 0, s>>>(Q, W32, emb_dev, n, nc, nc);
+        check();
+        // results back
+        std::vector<double> hsig(KP), hsum(nv), hsq(nv);
+        int hstatus = 0;
+        HIPC(hipMemcpyAsync(hsig.data(), sig, sizeof(double) * KP, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsum.data(), m->sum.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsq.data(), m->sq.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(&hstatus, status, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!out->on_device) {
+            HIPC(hipMemcpyAsync(out->embeddings, emb_dev, sizeof(float) * n * nc, hipMemcpyDeviceToHost, s));
+            HIPC(hipMemcpyAsync(out->components, comps_dev, sizeof(float) * nv * nc, hipMemcpyDeviceToHost, s));
+        }
+        HIPC(hipStreamSynchronize(s));
+        if (hstatus & 1) {
+            if (p->qr_method == SCXPCA_QR_CHOLESKY)
+                throw Error(SCXPCA_ERR_LINALG,
+                            "CholeskyQR2 failed: non-SPD; retry with qr_method='householder'");
+            throw Error(SCXPCA_ERR_LINALG, "shifted CholeskyQR failed: Gram matrix not positive definite");
+        }
+        if (hstatus & 2) throw Error(SCXPCA_ERR_LINALG, "SVD failed: Jacobi sweeps did not converge");
+        // total variance with the cancellation guard (cpu.rs:1029-1092)
+        std::vector<double> hmu(nv);
+        for (int64_t v = 0; v < nv; ++v) hmu[v] = hsum[v] / static_cast<double>(n);
+        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
+        double tv = 0.0;
+        int recomputed = 0;
+        if (!centered) {
+            for (int64_t v = 0; v < nv; ++v) tv += hsq[v];
+            tv /= denom;
+        } else {
+            double tot = 0.0, sumsq = 0.0, msq = 0.0;
+            for (int64_t v = 0; v < nv; ++v) tot += hsq[v] - static_cast<double>(n) * hmu[v] * hmu[v];
+            tv = tot / denom;
+            for (int64_t v = 0; v < nv; ++v) sumsq += hsq[v];
+            for (int64_t v = 0; v < nv; ++v) msq += hmu[v] * hmu[v];
+            msq *= static_cast<double>(n);
+            if (sumsq > 0.0 && (sumsq - msq) <= 1e-7 * sumsq) {
+                // cancelled: stable centered recompute on the device
+                recomputed = 1;
+                const int nb = 1024;
+                DevBuf part;
+                part.ensure(sizeof(double) * nb);
+                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, mu, n,
+                                                 part.as<double>());
+                check();
+                std::vector<double> hp(nb);
+                std::vector<uint32_t> hcnt(nv);
+                HIPC(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+                HIPC(hipMemcpyAsync(hcnt.data(), m->cnt.p, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, s));
+                HIPC(hipStreamSynchronize(s));
+                double t = 0.0;
+                for (int i = 0; i < nb; ++i) t += hp[i];
+                for (int64_t v = 0; v < nv; ++v) {
+                    const double zeros = static_cast<double>(std::max<int64_t>(n - static_cast<int64_t>(hcnt[v]), 0));
+                    t += zeros * hmu[v] * hmu[v];
+                }
+                tv = t / denom;
+            }
+        }
+        for (int j = 0; j < nc; ++j) {
+            const double var = hsig[j] * hsig[j] / denom;
+            if (out->variance) out->variance[j] = var;
+            if (out->variance_ratio) out->variance_ratio[j] = tv > 0.0 ? var / tv : 0.0;
+        }
+        if (out->sigma)
+            for (int j = 0; j < k; ++j) out->sigma[j] = hsig[j];
+        if (out->mean && centered) std::memcpy(out->mean, hmu.data(), sizeof(double) * nv);
+        out->k = k;
+        out->total_variance = tv;
+        out->tv_recomputed = recomputed;
+    }
+};
+
+void validate(const scxpca_matrix *m, const scxpca_params *p) {
+    // validate_inputs (cpu.rs:979-997)
+    if (p->n_comps <= 0) throw Error(SCXPCA_ERR_INVALID, "n_components must be > 0");
+    if (m->n_obs == 0 || m->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "matrix must be non-empty");
+    const int64_t mn = std::min(m->n_obs, m->n_vars);
+    if (p->n_comps > mn)
+        throw Error(SCXPCA_ERR_INVALID, "n_components (" + std::to_string(p->n_comps) +
+                                            ") exceeds matrix rank bound min(n_obs, n_vars) = " + std::to_string(mn));
+    if (p->n_oversamples < 0 || p->n_power_iterations < 0)
+        throw Error(SCXPCA_ERR_INVALID, "n_oversamples and n_power_iterations must be >= 0");
+    if (p->qr_method != SCXPCA_QR_HOUSEHOLDER && p->qr_method != SCXPCA_QR_CHOLESKY)
+        throw Error(SCXPCA_ERR_INVALID, "Invalid qr_method; expected householder or cholesky");
+    if (p->precision != SCXPCA_PRECISION_FP32 && p->precision != SCXPCA_PRECISION_BF16_SKETCH)
+        throw Error(SCXPCA_ERR_INVALID, "Invalid precision");
+}
+
+int64_t pca_k(const scxpca_matrix *m, const scxpca_params *p) {
+    return std::min<int64_t>(std::min<int64_t>(p->n_comps + p->n_oversamples, m->n_vars), m->n_obs);
+}
+
+void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
+    validate(m, p);
+    const int64_t k = pca_k(m, p);
+    if (k > 128)
+        throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples = " + std::to_string(k) +
+                                            " exceeds the GPU panel limit of 128");
+    DeviceGuard dg(m->device);
+    m->host_omega.resize(static_cast<size_t>(m->n_vars * k));
+    // Launch the stats pass first so Omega generation overlaps it.
+    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_xbf(m);
+    ensure_stats(m);
+    gaussian_matrix(static_cast<uint64_t>(m->n_vars), static_cast<uint64_t>(k), p->seed, m->host_omega.data());
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, m->stream));
+    if (k <= 64) {
+        Engine<64> eng(m, static_cast<int>(k));
+        eng.run(p, out, m->host_omega.data());
+    } else {
+        Engine<128> eng(m, static_cast<int>(k));
+        eng.run(p, out, m->host_omega.data());
+    }
+    HIPC(hipEventRecord(e1, m->stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out->seconds_device = ms * 1e-3;
+}
+
+template <int KP>
+void stage_spmm(scxpca_matrix *m, int transpose, int64_t k, const double *in, const double *means, int bf16,
+                double *out) {
+    Engine<KP> eng(m, static_cast<int>(k));
+    const int64_t n = m->n_obs, nv = m->n_vars;
+    hipStream_t s = m->stream;
+    if (bf16) ensure_xbf(m);
+    const bool centered = means != nullptr;
+    if (centered) HIPC(hipMemcpyAsync(eng.mu, means, sizeof(double) * nv, hipMemcpyHostToDevice, s));
+    if (!transpose) {
+        HIPC(hipMemcpyAsync(m->panel64.p, in, sizeof(double) * nv * k, hipMemcpyHostToDevice, s));
+        k_pack_panel<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), nv, static_cast<int>(k),
+                                                             m->panel.as<float>());
+        HIPC(hipGetLastError());
+        if (centered) eng.mean_correction(m->panel.as<float>());
+        eng.spmm_fwd(m->panel.as<float>(), centered, bf16 != 0, m->y.as<float>());
+        std::vector<float> h(static_cast<size_t>(n) * KP);
+        HIPC(hipMemcpyAsync(h.data(), m->y.p, sizeof(float) * n * KP, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int64_t r = 0; r < n; ++r)
+            for (int64_t j = 0; j < k; ++j) out[r * k + j] = h[r * KP + j];
+    } else {
+        std::vector<float> h(static_cast<size_t>(std::max<int64_t>(n, 1)) * KP, 0.f);
+        for (int64_t r = 0; r < n; ++r)
+            for (int64_t j = 0; j < k; ++j) h[r * KP + j] = static_cast<float>(in[r * k + j]);
+        HIPC(hipMemcpyAsync(m->q.p, h.data(), sizeof(float) * n * KP, hipMemcpyHostToDevice, s));
+        eng.spmm_t(m->q.as<float>(), centered, bf16 != 0, m->b64.as<double>(), nullptr);
+        std::vector<double> hb(static_cast<size_t>(nv) * KP);
+        HIPC(hipMemcpyAsync(hb.data(), m->b64.p, sizeof(double) * nv * KP, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        for (int64_t v = 0; v < nv; ++v)
+            for (int64_t j = 0; j < k; ++j) out[v * k + j] = hb[v * KP + j];
+    }
+}
+
+template <int KP>
+void stage_qr(int64_t rows, int64_t k, const double *a, int qr_method, double *qout) {
+    // A throwaway matrix shell provides the stream and workspace.
+    scxpca_matrix shell;
+    HIPC(hipStreamCreate(&shell.stream));
+    shell.n_obs = rows;
+    shell.n_vars = 1;
+    Engine<KP> eng(&shell, static_cast<int>(k));
+    hipStream_t s = shell.stream;
+    std::vector<float> h(static_cast<size_t>(rows) * KP, 0.f);
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t j = 0; j < k; ++j) h[r * KP + j] = static_cast<float>(a[r * k + j]);
+    HIPC(hipMemsetAsync(eng.status, 0, sizeof(int), s));
+    HIPC(hipMemcpyAsync(shell.y.p, h.data(), sizeof(float) * rows * KP, hipMemcpyHostToDevice, s));
+    float *Q = eng.cqr_tall(shell.y.as<float>(), shell.q.as<float>(), qr_method);
+    int st = 0;
+    HIPC(hipMemcpyAsync(h.data(), Q, sizeof(float) * rows * KP, hipMemcpyDeviceToHost, s));
+    HIPC(hipMemcpyAsync(&st, eng.status, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPC(hipStreamSynchronize(s));
+    if (st & 1) {
+        if (qr_method == SCXPCA_QR_CHOLESKY)
+            throw Error(SCXPCA_ERR_LINALG, "CholeskyQR2 failed: non-SPD; retry with qr_method='householder'");
+        throw Error(SCXPCA_ERR_LINALG, "shifted CholeskyQR failed: Gram matrix not positive definite");
+    }
+    for (int64_t r = 0; r < rows; ++r)
+        for (int64_t j = 0; j < k; ++j) qout[r * k + j] = h[r * KP + j];
+}
+
+
+// Projection Z = (X~ - 1 mu^T) W (forward) and dW = (X~ - 1 mu^T)^T dZ
+// (backward) on a caller stream; W / dW are n_vars x k, Z / dZ n_obs x k, f32
+// row-major device buffers.
+template <int KP>
+void project(scxpca_matrix *m, bool backward, const float *in, int64_t k, const float *mean, float *out,
+             hipStream_t user) {
+    hipStream_t saved = m->stream;
+    if (user) m->stream = user;
+    struct Restore {
+        scxpca_matrix *m;
+        hipStream_t s;
+        ~Restore() { m->stream = s; }
+    } restore{m, saved};
+    Engine<KP> eng(m, static_cast<int>(k));
+    hipStream_t s = m->stream;
+    const int64_t n = m->n_obs, nv = m->n_vars;
+    const bool centered = mean != nullptr;
+    if (centered) {
+        k_f32_to_f64<<<blocks_for(nv), 256, 0, s>>>(mean, eng.mu, nv);
+        HIPC(hipGetLastError());
+    }
+    if (!backward) {
+        const float *panel = in;
+        if (k != KP) {
+            k_pack_panel32<KP><<<blocks_for(nv * KP), 256, 0, s>>>(in, nv, static_cast<int>(k), m->panel.as<float>());
+            HIPC(hipGetLastError());
+            panel = m->panel.as<float>();
+        }
+        if (centered) eng.mean_correction(panel);
+        float *dst = (k == KP) ? out : m->y.as<float>();
+        eng.spmm_fwd(panel, centered, false, dst);
+        if (k != KP) {
+            k_unpack_panel32<KP><<<blocks_for(n * k), 256, 0, s>>>(dst, n, static_cast<int>(k), out);
+            HIPC(hipGetLastError());
+        }
+    } else {
+        const float *qp = in;
+        if (k != KP) {
+            k_pack_panel32<KP><<<blocks_for(n * KP), 256, 0, s>>>(in, n, static_cast<int>(k), m->q.as<float>());
+            HIPC(hipGetLastError());
+            qp = m->q.as<float>();
+        }
+        float *dst = (k == KP) ? out : m->panel.as<float>();
+        eng.spmm_t(qp, centered, false, nullptr, dst);
+        if (k != KP) {
+            k_unpack_panel32<KP><<<blocks_for(nv * k), 256, 0, s>>>(dst, nv, static_cast<int>(k), out);
+            HIPC(hipGetLastError());
+        }
+    }
+    if (!user) HIPC(hipStreamSynchronize(s));
+}
+
+void refresh(scxpca_matrix *m) {
+    m->stats_ready = false;
+    m->xbf_ready = false;
+    if (m->n_obs == 0) return;
+    const unsigned row_blocks = blocks_for(m->n_obs, 4);
+    if (m->flags & kXfNorm) {
+        k_row_factor<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, m->target,
+                                                                         m->rowf.as<double>());
+        HIPC(hipGetLastError());
+    }
+    k_splits<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
+                                                                  m->spl.as<uint32_t>());
+    HIPC(hipGetLastError());
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+scxpca_status scxpca_matrix_create(const scxpca_csr *x, double target_sum, int32_t log1p, int device,
+                                   scxpca_matrix **out) {
+    return guarded([&] {
+        if (!x || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        *out = nullptr;
+        DeviceGuard dg(device);
+        std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
+        m->device = device;
+        build_matrix(m.get(), x, target_sum, log1p);
+        *out = m.release();
+    });
+}
+
+scxpca_status scxpca_matrix_destroy(scxpca_matrix *m) {
+    return guarded([&] {
+        if (!m) return;
+        DeviceGuard dg(m->device);
+        (void)hipStreamSynchronize(m->stream);
+        delete m;
+    });
+}
+
+scxpca_status scxpca_matrix_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
+    return guarded([&] {
+        if (!m || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        run_pca(m, p, out);
+    });
+}
+
+scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p, scxpca_result *out,
+                                    int device) {
+    return guarded([&] {
+        if (!x || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (p->n_comps <= 0) throw Error(SCXPCA_ERR_INVALID, "n_components must be > 0");
+        if (x->n_obs == 0 || x->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "matrix must be non-empty");
+        DeviceGuard dg(device);
+        std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
+        m->device = device;
+        build_matrix(m.get(), x, p->target_sum, p->log1p);
+        run_pca(m.get(), p, out);
+    });
+}
+
+scxpca_status scxpca_matrix_stats(scxpca_matrix *m, double *sums, double *sum_sq, uint32_t *col_nnz,
+                                  double *row_factor) {
+    return guarded([&] {
+        if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
+        DeviceGuard dg(m->device);
+        if (m->n_obs == 0 || m->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "matrix must be non-empty");
+        ensure_stats(m);
+        if (sums) HIPC(hipMemcpyAsync(sums, m->sum.p, sizeof(double) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
+        if (sum_sq) HIPC(hipMemcpyAsync(sum_sq, m->sq.p, sizeof(double) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
+        if (col_nnz) HIPC(hipMemcpyAsync(col_nnz, m->cnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
+        if (row_factor) {
+            if (m->flags & kXfNorm)
+                HIPC(hipMemcpyAsync(row_factor, m->rowf.p, sizeof(double) * m->n_obs, hipMemcpyDeviceToHost, m->stream));
+            else
+                for (int64_t r = 0; r < m->n_obs; ++r) row_factor[r] = 1.0;
+        }
+        HIPC(hipStreamSynchronize(m->stream));
+    });
+}
+
+scxpca_status scxpca_matrix_spmm(scxpca_matrix *m, int32_t transpose, int64_t k, const double *in,
+                                 const double *means, int32_t bf16, double *out) {
+    return guarded([&] {
+        if (!m || !in || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (k <= 0 || k > 128) throw Error(SCXPCA_ERR_INVALID, "k must be in 1..128");
+        if (m->n_obs == 0 || m->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "matrix must be non-empty");
+        DeviceGuard dg(m->device);
+        if (k <= 64) stage_spmm<64>(m, transpose, k, in, means, bf16, out);
+        else stage_spmm<128>(m, transpose, k, in, means, bf16, out);
+    });
+}
+
+scxpca_status scxpca_qr(const double *a, int64_t rows, int64_t k, int32_t qr_method, int device,
+                        double *q_out) {
+    return guarded([&] {
+        if (!a || !q_out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (k <= 0 || k > 128 || rows < k) throw Error(SCXPCA_ERR_INVALID, "need rows >= k and 1 <= k <= 128");
+        DeviceGuard dg(device);
+        if (k <= 64) stage_qr<64>(rows, k, a, qr_method, q_out);
+        else stage_qr<128>(rows, k, a, qr_method, q_out);
+    });
+}
+
+scxpca_status scxpca_gaussian(uint64_t rows, uint64_t cols, uint64_t seed, double *out) {
+    return guarded([&] {
+        if (!out && rows * cols) throw Error(SCXPCA_ERR_INVALID, "null output");
+        gaussian_matrix(rows, cols, seed, out);
+    });
+}
+
+scxpca_status scxpca_matrix_refresh(scxpca_matrix *m) {
+    return guarded([&] {
+        if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
+        DeviceGuard dg(m->device);
+        refresh(m);
+    });
+}
+
+scxpca_status scxpca_project_forward(scxpca_matrix *m, const float *w, int64_t k, const float *mean, float *z,
+                                     void *stream) {
+    return guarded([&] {
+        if (!m || !w || !z) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (k <= 0 || k > 128) throw Error(SCXPCA_ERR_INVALID, "k must be in 1..128");
+        DeviceGuard dg(m->device);
+        if (k <= 64) project<64>(m, false, w, k, mean, z, static_cast<hipStream_t>(stream));
+        else project<128>(m, false, w, k, mean, z, static_cast<hipStream_t>(stream));
+    });
+}
+
+scxpca_status scxpca_project_backward(scxpca_matrix *m, const float *dz, int64_t k, const float *mean, float *dw,
+                                      void *stream) {
+    return guarded([&] {
+        if (!m || !dz || !dw) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (k <= 0 || k > 128) throw Error(SCXPCA_ERR_INVALID, "k must be in 1..128");
+        DeviceGuard dg(m->device);
+        if (k <= 64) project<64>(m, true, dz, k, mean, dw, static_cast<hipStream_t>(stream));
+        else project<128>(m, true, dz, k, mean, dw, static_cast<hipStream_t>(stream));
+    });
+}
+
+scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms_total, int64_t *launches,
+                                    int32_t n_stages) {
+    return guarded([&] {
+        if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
+        DeviceGuard dg(m->device);
+        if (ms_total || launches) {
+            HIPC(hipStreamSynchronize(m->stream));
+            for (int i = 0; i < n_stages; ++i) {
+                if (ms_total) ms_total[i] = 0.0;
+                if (launches) launches[i] = 0;
+            }
+            for (auto &mk : m->marks) {
+                if (mk.stage < 0 || mk.stage >= n_stages) continue;
+                float ms = 0.f;
+                HIPC(hipEventElapsedTime(&ms, mk.a, mk.b));
+                if (ms_total) ms_total[mk.stage] += ms;
+                if (launches) launches[mk.stage] += 1;
+            }
+        }
+        m->clear_marks();
+        m->profiling = enable != 0;
+    });
+}
+
+int scxpca_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+scxpca_status scxpca_device_info(int device, char *name, size_t name_len, int64_t *total_mem, int32_t *n_cu) {
+    return guarded([&] {
+        DeviceGuard dg(device);
+        hipDeviceProp_t prop;
+        HIPC(hipGetDeviceProperties(&prop, device));
+        if (name && name_len) {
+            std::snprintf(name, name_len, "%s (%s)", prop.name, prop.gcnArchName);
+        }
+        if (total_mem) *total_mem = static_cast<int64_t>(prop.totalGlobalMem);
+        if (n_cu) *n_cu = prop.multiProcessorCount;
+    });
+}
+
+size_t scxpca_last_error(char *buf, size_t len) {
+    const size_t n = g_last_error.size();
+    if (buf && len) {
+        const size_t c = std::min(n, len - 1);
+        std::memcpy(buf, g_last_error.data(), c);
+        buf[c] = '\0';
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Tolerances: FP32 device arithmetic against the f64 oracle. Stage outputs are
+compared relative to the magnitude of the f64 result (2e-5 for SpMM panels, a
+few f32 ulps for the X~ column statistics); full PCA results to the contract
+of the path: singular values <= 1e-4 relative, largest principal angle <= 1e-3
+rad, per-component scores <= 1e-3 relative after sign alignment.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from scx_amd import synth
+from tests.pca_metrics import max_principal_angle, score_rel_err, sigma_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small():
+    # the first 3000 rows of the 10k x 2k workload: real NB counts, ~10% density
+    return synth.generate(synth.CFG1, n_obs=3000, workers=1)
+
+
+@pytest.fixture(scope="module")
+def small_mat(gpu, small):
+    say("case: Matrix(cfg1 head 3000)")
+    return gpu.Matrix(small.indptr, small.indices, small.data, small.shape, target_sum=1e4, log1p=True)
+
+
+def test_stats_parity(small, small_mat):
+    s, ss, cnt, rf = small_mat.stats()
+    view = oracle.CsrView(small.indptr, small.indices, small.data, small.shape)
+    rs = oracle.row_sums(view)
+    np.testing.assert_allclose(rf, np.where(rs > 0, 1e4 / rs, 1.0), rtol=1e-15)
+    xt = oracle.transform(view, 1e4, True)
+    os_, oss = oracle.col_sums_and_sum_sq(view.with_data(xt))
+    np.testing.assert_allclose(s, os_, rtol=5e-7, atol=1e-9)
+    np.testing.assert_allclose(ss, oss, rtol=5e-7, atol=1e-9)
+    np.testing.assert_array_equal(cnt, np.bincount(small.indices, minlength=small.shape[1]))
+
+
+@pytest.mark.parametrize("k,centered", [(60, True), (60, False), (100, True), (7, True)])
+def test_spmm_forward_parity(small, small_mat, k, centered):
+    say(f"case: spmm forward k={k} centered={centered}")
+    rng = np.random.default_rng(k)
+    m = rng.standard_normal((small.shape[1], k))
+    view = oracle.CsrView(small.indptr, small.indices, small.data, small.shape)
+    xt = view.with_data(oracle.transform(view, 1e4, True))
+    mu = oracle.col_sums_and_sum_sq(xt)[0] / small.shape[0] if centered else None
+    ref = oracle.spmm_forward(xt, m, mu)
+    got = small_mat.spmm(m, transpose=False, means=mu)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() / scale < 2e-5
+
+
+@pytest.mark.parametrize("k,centered", [(60, True), (60, False), (120, True), (3, True)])
+def test_spmm_transpose_parity(small, small_mat, k, centered):
+    say(f"case: spmm transpose k={k} centered={centered}")
+    rng = np.random.default_rng(100 + k)
+    q = rng.standard_normal((small.shape[0], k)) / np.sqrt(small.shape[0])
+    view = oracle.CsrView(small.indptr, small.indices, small.data, small.shape)
+    xt = view.with_data(oracle.transform(view, 1e4, True))
+    mu = oracle.col_sums_and_sum_sq(xt)[0] / small.shape[0] if centered else None
+    ref = oracle.spmm_transpose(xt, q, mu)
+    got = small_mat.spmm(q, transpose=True, means=mu)
+    scale = np.abs(ref).max()
+    assert np.abs(got - ref).max() / scale < 2e-5
+
+
+@pytest.mark.parametrize("method", [0, 1])
+@pytest.mark.parametrize("rows,k", [(5000, 60), (20000, 120), (300, 5)])
+def test_qr(gpu, method, rows, k):
+    say(f"case: qr rows={rows} k={k} method={method}")
+    rng = np.random.default_rng(rows + k)
+    a = rng.standard_normal((rows, k)) @ np.diag(np.logspace(0, 2, k))
+    q = gpu.qr(a, qr_method=method)
+    assert np.abs(q.T @ q - np.eye(k)).max() < 1e-4
+    assert max_principal_angle(a, q) < 1e-4
+
+
+def _oracle_pca(csr, w, n_comps, n_over, n_iter, seed=0):
+    return oracle.randomized_pca(oracle.CsrView(csr.indptr, csr.indices, csr.data, csr.shape), n_comps=n_comps,
+                                 n_oversamples=n_over, n_power_iterations=n_iter, zero_center=True, seed=seed,
+                                 target_sum=w.target_sum, log1p=True)
+
+
+@pytest.mark.parametrize("n_iter,qr_method", [(2, 0), (3, 1), (3, 0)])
+def test_pca_parity_small(small, small_mat, n_iter, qr_method):
+    say(f"case: pca small n_iter={n_iter} qr={qr_method}")
+    ref = _oracle_pca(small, synth.CFG1, 50, 10, n_iter)
+    got = small_mat.pca(n_comps=50, n_oversamples=10, n_power_iterations=n_iter, seed=0, qr_method=qr_method)
+    se = sigma_rel_err(ref["sigma_all"][:50], got["sigma"][:50])
+    say("sigma rel err max", se.max())
+    assert se.max() <= 1e-4
+    ang = max_principal_angle(ref["components"].T[:, :10], got["components"].T[:, :10].astype(np.float64))
+    say("angle top10", ang)
+    assert ang <= 1e-3
+    np.testing.assert_allclose(got["mean"], ref["mean"], rtol=1e-6, atol=1e-12)
+    np.testing.assert_allclose(got["variance_ratio"], ref["variance_ratio"], rtol=2e-4)
+    assert abs(got["total_variance"] - (ref["variance"] / ref["variance_ratio"])[0]) <= 1e-6 * got["total_variance"]

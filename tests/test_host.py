@@ -1,0 +1,108 @@
+"""Host-side tests (no GPU): the C ABI library loads and exports every declared
+symbol, the pca(...) surface validates like the reference, the lazy transform
+chain matches the oracle's X~, and the synthetic generator is reproducible."""
+from __future__ import annotations
+
+import os
+import re
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from scx_amd import _lib, accel, synth
+from scx_amd.adata import AnnDataLite
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "scxpca.h")).read()
+    declared = set(re.findall(r"\b(scxpca_[a-z0-9_]+)\s*\(", header))
+    assert declared, "no declarations parsed"
+    assert declared == set(_lib.EXPORTS), declared ^ set(_lib.EXPORTS)
+    lib = _lib.lib()
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_device_grammar():
+    assert accel.resolve_device("cpu") == ("cpu", None)
+    with pytest.raises(ValueError, match="unknown device: 'tpu'"):
+        accel.resolve_device("tpu")
+    with pytest.raises(ValueError, match="non-integer suffix"):
+        accel.resolve_device("gpu:x")
+    with pytest.raises(ValueError, match="unknown device"):
+        accel.resolve_device("gpu0")
+    if not accel.gpu_available():
+        assert accel.resolve_device("auto") == ("cpu", None)
+        with pytest.raises(RuntimeError, match="no ROCm GPU found"):
+            accel.resolve_device("gpu")
+
+
+def _adata(n=50, nv=20):
+    c = synth.random_csr(n, nv, 0.3, seed=1)
+    return AnnDataLite(c.to_scipy())
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(method="pca"), "Invalid method='pca'; expected 'auto', 'covariance', or 'randomized'"),
+    (dict(qr_method="qr"), "Invalid qr_method='qr'; expected 'householder' or 'cholesky'"),
+    (dict(spmm_policy="x"), "Invalid spmm_policy='x'"),
+    (dict(prefer_format="csc"), "only supports prefer_format='csr'"),
+])
+def test_pca_argument_validation(kw, msg):
+    with pytest.raises(ValueError, match=re.escape(msg)):
+        accel.pca(_adata(), device="cpu", **kw)
+
+
+def test_pca_cpu_route_is_not_silently_served():
+    with pytest.raises(RuntimeError, match="GPU route of pca only"):
+        accel.pca(_adata(), device="cpu")
+
+
+def test_lazy_chain_matches_oracle_transform():
+    c = synth.random_csr(200, 60, 0.2, seed=2)
+    ad = AnnDataLite(c.to_scipy())
+    accel.normalize_total(ad, target_sum=1e4, device="cpu")
+    accel.log1p(ad, device="cpu")
+    assert isinstance(ad.X, accel.LazyCsr) and ad.X.target_sum == 1e4 and ad.X.log1p
+    got = ad.X.materialize().data
+    ref = oracle.transform(oracle.CsrView(c.indptr, c.indices, c.data, c.shape), 1e4, True)
+    # numpy float32 log1p vs glibc log1pf: within one f32 ulp
+    assert np.all(np.abs(got - ref) <= np.spacing(np.abs(ref)))
+
+
+def test_normalize_total_default_target_is_median_positive_total():
+    m = sp.csr_matrix(np.array([[1.0, 1.0], [0.0, 0.0], [3.0, 1.0], [5.0, 5.0]], np.float32))
+    ad = AnnDataLite(m)
+    accel.normalize_total(ad, device="cpu")
+    assert ad.X.target_sum == 4.0  # median of (2, 4, 10)
+
+
+def test_mask_var_resolution():
+    ad = _adata()
+    ad.var["highly_variable"] = np.arange(20) % 2 == 0
+    cols, name = accel._resolve_mask_var(ad, None, 20)
+    assert name == "highly_variable" and cols.tolist() == list(range(0, 20, 2))
+    with pytest.raises(ValueError, match="mask_var length 3 != n_vars 20"):
+        accel._resolve_mask_var(ad, np.array([True, False, True]), 20)
+    with pytest.raises(ValueError, match="all-false mask"):
+        accel._resolve_mask_var(ad, np.zeros(20, bool), 20)
+    with pytest.raises(ValueError, match="mask_var column 'nope' not found"):
+        accel._resolve_mask_var(ad, "nope", 20)
+
+
+def test_generator_reproducible_and_prefix_stable():
+    a = synth.generate(synth.CFG1, n_obs=5000, workers=1)
+    b = synth.generate(synth.CFG1, n_obs=5000, workers=2)
+    np.testing.assert_array_equal(a.indptr, b.indptr)
+    np.testing.assert_array_equal(a.indices, b.indices)
+    np.testing.assert_array_equal(a.data, b.data)
+    head = synth.generate(synth.CFG1, n_obs=3000, workers=1)
+    np.testing.assert_array_equal(head.indices, a.indices[: head.nnz])
+    # canonical rows and integer counts
+    m = a.to_scipy()
+    assert m.has_sorted_indices and np.all(a.data >= 1) and np.all(a.data == np.round(a.data))
+    assert abs(a.nnz / (5000 * 2000) - 0.10) < 0.01
