@@ -132,7 +132,10 @@ typedef enum {
     SCXPCA_STAGE_APPLY = 5,
     SCXPCA_STAGE_SPMM_FWD_BF16 = 6,
     SCXPCA_STAGE_SPMM_T_BF16 = 7,
-    SCXPCA_N_STAGES = 8
+    SCXPCA_STAGE_PREPARE = 8,   /* row factors, split offsets, tile-CSC build */
+    SCXPCA_STAGE_SOLVE = 9,     /* single-workgroup Cholesky / Jacobi SVD      */
+    SCXPCA_STAGE_SMALL = 10,    /* f64 CholeskyQR of the n_vars x k panels     */
+    SCXPCA_N_STAGES = 11
 } scxpca_stage;
 scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms_total,
                                     int64_t *launches, int32_t n_stages);

@@ -22,7 +22,8 @@ _vp = ctypes.c_void_p
 OK, ERR_INVALID, ERR_LINALG, ERR_OOM, ERR_DEVICE, ERR_NO_DEVICE = range(6)
 QR_HOUSEHOLDER, QR_CHOLESKY = 0, 1
 PRECISION_FP32, PRECISION_BF16_SKETCH = 0, 1
-STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm_fwd_bf16", "spmm_t_bf16")
+STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm_fwd_bf16", "spmm_t_bf16",
+          "prepare", "solve", "small")
 
 # Every symbol include/scxpca.h declares.
 EXPORTS = (

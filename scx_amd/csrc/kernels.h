@@ -30,11 +30,44 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// log1p(x) in f32 without the double-float arithmetic of the library routine:
+// u = 1 + x rounded, the rounding error folded back as c/u, u reduced to
+// [sqrt(2)/2, sqrt(2)) by exponent arithmetic, and log(1+f) = 2 atanh(f/(2+f))
+// by the 4-term polynomial of FreeBSD/musl logf (Sun fdlibm, public domain;
+// |error of the polynomial| < 2^-34.24). Non-finite inputs and x <= -1 take
+// the library routine.
+__host__ __device__ __forceinline__ float rcp_approx(float v) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(v);
+#else
+    return 1.0f / v;
+#endif
+}
+
+__host__ __device__ __forceinline__ float log1p_fast(float x) {
+    if (!(x > -1.0f && x < 3.0e37f)) return log1pf(x);
+    const float ln2_hi = 6.9313812256e-01f, ln2_lo = 9.0580006145e-06f;
+    const float Lg1 = 0.66666662693f, Lg2 = 0.40000972152f, Lg3 = 0.28498786688f, Lg4 = 0.24279078841f;
+    const float u = 1.0f + x;
+    uint32_t iu = __builtin_bit_cast(uint32_t, u) + (0x3f800000u - 0x3f3504f3u);
+    const int k = static_cast<int>(iu >> 23) - 0x7f;
+    float c = (k >= 2) ? 1.0f - (u - x) : x - (u - 1.0f);
+    c = (k < 25) ? c * rcp_approx(u) : 0.0f;
+    iu = (iu & 0x007fffffu) + 0x3f3504f3u;
+    const float f = __builtin_bit_cast(float, iu) - 1.0f;
+    const float s = f * rcp_approx(2.0f + f);
+    const float z = s * s, w = z * z;
+    const float R = z * (Lg1 + w * Lg3) + w * (Lg2 + w * Lg4);
+    const float hfsq = 0.5f * f * f;
+    const float dk = static_cast<float>(k);
+    return s * (hfsq + R) + (dk * ln2_lo + c) - hfsq + f + dk * ln2_hi;
+}
+
 // transforms.rs:21-39: ((x as f64 * factor) as f32).ln_1p(); the factor is 1
 // for rows whose total is not positive, which reproduces the :40-72 branch.
 __device__ __forceinline__ float xtilde(float v, double f, int flags) {
     const float s = (flags & kXfNorm) ? static_cast<float>(static_cast<double>(v) * f) : v;
-    return (flags & kXfLog1p) ? log1pf(s) : s;
+    return (flags & kXfLog1p) ? log1p_fast(s) : s;
 }
 
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) {
@@ -138,53 +171,7 @@ __global__ void __launch_bounds__(256) k_xtilde_bf16(const int64_t *__restrict__
     }
 }
 
-// ---------------------------------------------------------------------------
-// Column statistics of X~: sum, sum of squares (f64) and nonzero count per
-// gene. Block (rb, gc) owns rows of row block rb and genes [gc*CG, gc*CG+CG);
-// LDS accumulators, partials per row block, then a fixed-order reduction.
-// ---------------------------------------------------------------------------
-
-template <int CG>
-__global__ void __launch_bounds__(256) k_col_stats(const int64_t *__restrict__ indptr,
-                                                   const uint32_t *__restrict__ idx,
-                                                   const float *__restrict__ val,
-                                                   const double *__restrict__ rowf, int flags,
-                                                   const uint32_t *__restrict__ spl, int nsplit, int cpc,
-                                                   int nch, int64_t n_obs, int64_t n_vars,
-                                                   int64_t rows_per_rb, double *__restrict__ psum,
-                                                   double *__restrict__ psq, uint32_t *__restrict__ pcnt) {
-    __shared__ double s_sum[CG];
-    __shared__ double s_sq[CG];
-    __shared__ uint32_t s_cnt[CG];
-    const int rb = blockIdx.x / nch, gc = blockIdx.x % nch;
-    const int64_t g0 = static_cast<int64_t>(gc) * CG;
-    for (int i = threadIdx.x; i < CG; i += blockDim.x) { s_sum[i] = 0.0; s_sq[i] = 0.0; s_cnt[i] = 0u; }
-    __syncthreads();
-    const int lane = threadIdx.x & 63, wave = threadIdx.x / kWave, nw = blockDim.x / kWave;
-    const int64_t r0 = static_cast<int64_t>(rb) * rows_per_rb;
-    const int64_t r1 = r0 + rows_per_rb < n_obs ? r0 + rows_per_rb : n_obs;
-    const int s_lo = gc * cpc, s_hi = (gc + 1) * cpc < nsplit ? (gc + 1) * cpc : nsplit;
-    for (int64_t r = r0 + wave; r < r1; r += nw) {
-        const int64_t a = indptr[r];
-        const int64_t lo = a + spl[r * (nsplit + 1) + s_lo], hi = a + spl[r * (nsplit + 1) + s_hi];
-        const double f = rowf ? rowf[r] : 1.0;
-        for (int64_t p = lo + lane; p < hi; p += kWave) {
-            const double x = static_cast<double>(xtilde(val[p], f, flags));
-            const int c = static_cast<int>(idx[p] - g0);
-            atomicAdd(&s_sum[c], x);
-            atomicAdd(&s_sq[c], x * x);
-            atomicAdd(&s_cnt[c], 1u);
-        }
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < CG && g0 + i < n_vars; i += blockDim.x) {
-        const int64_t o = static_cast<int64_t>(rb) * n_vars + g0 + i;
-        psum[o] = s_sum[i];
-        psq[o] = s_sq[i];
-        pcnt[o] = s_cnt[i];
-    }
-}
-
+// Column statistics partials -> totals, fixed order over row blocks.
 __global__ void __launch_bounds__(256) k_col_stats_reduce(const double *__restrict__ psum,
                                                           const double *__restrict__ psq,
                                                           const uint32_t *__restrict__ pcnt, int nrb,
@@ -264,15 +251,25 @@ __global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ A, const d
     }
 }
 
+// Column j of the partials summed by block j: thread t takes partials t,
+// t+256, ...; then a fixed-order tree over the 256 thread sums.
 template <int KP>
-__global__ void k_colsum_reduce(const double *__restrict__ part, int nblk, double *__restrict__ out64,
-                                float *__restrict__ out32) {
-    const int j = threadIdx.x;
-    if (j >= KP) return;
-    double s = 0.0;
-    for (int b = 0; b < nblk; ++b) s += part[static_cast<int64_t>(b) * KP + j];
-    if (out64) out64[j] = s;
-    if (out32) out32[j] = static_cast<float>(s);
+__global__ void __launch_bounds__(256) k_colsum_reduce(const double *__restrict__ part, int nblk,
+                                                       double *__restrict__ out64, float *__restrict__ out32) {
+    __shared__ double red[256];
+    const int j = blockIdx.x;
+    double acc = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += 256) acc += part[static_cast<int64_t>(b) * KP + j];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (out64) out64[j] = red[0];
+        if (out32) out32[j] = static_cast<float>(red[0]);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -347,83 +344,6 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
     }
 }
 
-// ---------------------------------------------------------------------------
-// Transposed SpMM partials: part[rb][v][:] = sum_{r in rb} x~_rv * Q[r][:]
-// (cpu.rs:1282-1389). Block (rb, gc) keeps the f32 accumulator of genes
-// [gc*CG, gc*CG+CG) in LDS (CG*KP*4 = 128 KB); rows are split at gene-chunk
-// boundaries by the precomputed spl offsets, so each nonzero is read once.
-// Blocks of one row block are mapped to one XCD, dispatched back to back, so
-// they share Q rows and row offsets through that XCD's L2.
-// ---------------------------------------------------------------------------
-
-template <int KP, int CG, bool BF16>
-__global__ void __launch_bounds__(512) k_spmm_t(const int64_t *__restrict__ indptr,
-                                                const uint32_t *__restrict__ idx,
-                                                const float *__restrict__ val,
-                                                const uint16_t *__restrict__ xbf,
-                                                const double *__restrict__ rowf, int flags,
-                                                const float *__restrict__ Q, const uint32_t *__restrict__ spl,
-                                                int nsplit, int cpc, int nch, int nrb, int64_t n_obs,
-                                                int64_t n_vars, int64_t rows_per_rb,
-                                                float *__restrict__ part) {
-    constexpr int E = KP / 16;
-    __shared__ float zs[CG * KP];
-    __shared__ uint2 stage[8][kWave];
-    // XCD-aware block mapping: blockIdx % 8 is the XCD; consecutive slots on one
-    // XCD walk the gene chunks of one row block.
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int rb = (slot / nch) * 8 + xcd, gc = slot % nch;
-    if (rb >= nrb) return;
-    const int64_t g0 = static_cast<int64_t>(gc) * CG;
-    for (int i = threadIdx.x; i < CG * KP; i += blockDim.x) zs[i] = 0.f;
-    __syncthreads();
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int nw = blockDim.x / kWave;
-    const int64_t r0 = static_cast<int64_t>(rb) * rows_per_rb;
-    const int64_t r1 = r0 + rows_per_rb < n_obs ? r0 + rows_per_rb : n_obs;
-    const int s_lo = gc * cpc, s_hi = (gc + 1) * cpc < nsplit ? (gc + 1) * cpc : nsplit;
-    for (int64_t r = r0 + wave; r < r1; r += nw) {
-        const int64_t a = indptr[r];
-        const int64_t lo = a + spl[r * (nsplit + 1) + s_lo], hi = a + spl[r * (nsplit + 1) + s_hi];
-        if (lo >= hi) continue;
-        const double f = (!BF16 && rowf) ? rowf[r] : 1.0;
-        float qv[E];
-        const float4 *qr = reinterpret_cast<const float4 *>(Q + r * KP + l * E);
-#pragma unroll
-        for (int e = 0; e < E / 4; ++e) {
-            const float4 q4 = qr[e];
-            qv[4 * e] = q4.x; qv[4 * e + 1] = q4.y; qv[4 * e + 2] = q4.z; qv[4 * e + 3] = q4.w;
-        }
-        for (int64_t base = lo; base < hi; base += kWave) {
-            const int cnt = (hi - base) < kWave ? static_cast<int>(hi - base) : kWave;
-            uint32_t c = 0u;
-            float x = 0.f;
-            if (lane < cnt) {
-                c = idx[base + lane] - static_cast<uint32_t>(g0);
-                x = BF16 ? bf16_to_f32(xbf[base + lane]) : xtilde(val[base + lane], f, flags);
-            }
-            stage[wave][lane] = make_uint2(c, __float_as_uint(x));
-            wave_lds_sync();
-            const int steps = (cnt + 3) >> 2;
-            for (int t = 0; t < steps; ++t) {
-                const int s = 4 * t + g;
-                if (s < cnt) {
-                    const uint2 ent = stage[wave][s];
-                    const float xv = __uint_as_float(ent.y);
-                    float *zr = zs + ent.x * KP + l * E;
-#pragma unroll
-                    for (int e = 0; e < E; ++e) atomicAdd(zr + e, xv * qv[e]);
-                }
-            }
-            wave_lds_sync();
-        }
-    }
-    __syncthreads();
-    const int64_t ng = (g0 + CG <= n_vars) ? CG : (n_vars - g0);
-    float *dst = part + (static_cast<int64_t>(rb) * n_vars + g0) * KP;
-    for (int64_t i = threadIdx.x; i < ng * KP; i += blockDim.x) dst[i] = zs[i];
-}
-
 // B[v][j] = sum_rb part[rb][v][j] - mu[v] * sumq[j]  (f64; cpu.rs:1253-1267)
 template <int KP>
 __global__ void __launch_bounds__(256) k_spmm_t_reduce(const float *__restrict__ part, int nrb,
@@ -450,60 +370,23 @@ __global__ void __launch_bounds__(256) k_spmm_t_reduce(const float *__restrict__
 //   accumulator reg q of lane l holds D[8(q/4) + 4(l/32) + q%4][l%32].
 // ---------------------------------------------------------------------------
 
-template <int KP>
-__global__ void __launch_bounds__(256) k_gram(const float *__restrict__ Y, int64_t nrows,
-                                              int64_t rows_per_wave, double *__restrict__ part) {
-    constexpr int NT = KP / 32;
-    constexpr int NTILE = NT * (NT + 1) / 2;
-    const int lane = threadIdx.x & 63;
-    const int64_t w = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int64_t r0 = w * rows_per_wave;
-    const int64_t r1 = r0 + rows_per_wave < nrows ? r0 + rows_per_wave : nrows;
-    v16f acc[NTILE];
-#pragma unroll
-    for (int t = 0; t < NTILE; ++t) acc[t] = v16f{};
-    const int kr = lane >> 5, cl = lane & 31;
-    for (int64_t r = r0; r < r1; r += 2) {
-        const int64_t rr = r + kr;
-        float v[NT];
-#pragma unroll
-        for (int t = 0; t < NT; ++t) v[t] = rr < r1 ? Y[rr * KP + t * 32 + cl] : 0.f;
-        int tt = 0;
-#pragma unroll
-        for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-            for (int tj = ti; tj < NT; ++tj) {
-                acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(v[ti], v[tj], acc[tt], 0, 0, 0);
-                ++tt;
-            }
-    }
-    double *dst = part + w * KP * KP;
-    int tt = 0;
-#pragma unroll
-    for (int ti = 0; ti < NT; ++ti)
-#pragma unroll
-        for (int tj = ti; tj < NT; ++tj) {
-#pragma unroll
-            for (int q = 0; q < 16; ++q) {
-                const int i = ti * 32 + 8 * (q / 4) + 4 * kr + (q % 4);
-                const int j = tj * 32 + cl;
-                dst[i * KP + j] = static_cast<double>(acc[tt][q]);
-            }
-            ++tt;
-        }
-}
-
-// G[i][j] (full symmetric) from the upper-tile partials.
+// G[i][j] (full symmetric) from the upper-tile partials. Block b reduces the
+// 64 entries [64b, 64b+64): 4 thread groups take every 4th partial, then the 4
+// group sums are added in a fixed order (deterministic).
 template <int KP>
 __global__ void __launch_bounds__(256) k_gram_reduce(const double *__restrict__ part, int nparts,
                                                      double *__restrict__ G) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= KP * KP) return;
+    __shared__ double red[4][64];
+    const int t = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int grp = threadIdx.x >> 6;
     int i = t / KP, j = t % KP;
     if (i / 32 > j / 32) { const int s = i; i = j; j = s; }
-    double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += part[static_cast<int64_t>(p) * KP * KP + i * KP + j];
-    G[t] = s;
+    double acc = 0.0;
+    if (t < KP * KP)
+        for (int p = grp; p < nparts; p += 4) acc += part[static_cast<int64_t>(p) * KP * KP + i * KP + j];
+    red[grp][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (grp == 0 && t < KP * KP) G[t] = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
 }
 
 // Gram of a small f64 matrix (B: n_vars x KP) on the VALU in f64.
@@ -546,121 +429,6 @@ __global__ void __launch_bounds__(256) k_sum_parts(const double *__restrict__ pa
     G[t] = s;
 }
 
-// ---------------------------------------------------------------------------
-// Cholesky (f64, one workgroup) of the leading k x k block of G (+ shift*trace
-// on the diagonal when shift > 0): G = R^T R, R upper. Writes R (f64), its
-// inverse W = R^-1 (f64 and f32, zero outside k x k) and a status word
-// (1 = not SPD, reported like cusolver.rs:381-385).
-// ---------------------------------------------------------------------------
-
-template <int KP>
-__global__ void __launch_bounds__(256) k_chol(const double *__restrict__ G, int k, double shift,
-                                              double *__restrict__ R64, double *__restrict__ W64,
-                                              float *__restrict__ W32, int *__restrict__ status) {
-    __shared__ double a[KP * (KP + 1)];
-    __shared__ int bad;
-    constexpr int LD = KP + 1;
-    const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int t = tid; t < KP * KP; t += blockDim.x) a[(t / KP) * LD + t % KP] = G[t];
-    __syncthreads();
-    if (shift > 0.0) {
-        __shared__ double tr;
-        if (tid == 0) {
-            double s = 0.0;
-            for (int i = 0; i < k; ++i) s += a[i * LD + i];
-            tr = s;
-        }
-        __syncthreads();
-        for (int i = tid; i < k; i += blockDim.x) a[i * LD + i] += shift * tr;
-        __syncthreads();
-    }
-    for (int j = 0; j < k; ++j) {
-        if (tid == 0) {
-            const double d = a[j * LD + j];
-            if (!(d > 0.0) || !isfinite(d)) bad = 1;
-            a[j * LD + j] = (d > 0.0 && isfinite(d)) ? sqrt(d) : 1.0;
-        }
-        __syncthreads();
-        const double rjj = a[j * LD + j];
-        for (int i = j + 1 + tid; i < k; i += blockDim.x) a[j * LD + i] /= rjj;
-        __syncthreads();
-        const int m = k - j - 1;
-        for (int t = tid; t < m * m; t += blockDim.x) {
-            const int i = j + 1 + t / m, c = j + 1 + t % m;
-            if (c >= i) a[i * LD + c] -= a[j * LD + i] * a[j * LD + c];
-        }
-        __syncthreads();
-    }
-    // R (upper), zero elsewhere
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, c = t % KP;
-        R64[t] = (i < k && c < k && c >= i) ? a[i * LD + c] : 0.0;
-    }
-    __syncthreads();
-    // W = R^-1 by back substitution, one column per thread.
-    for (int c = tid; c < KP; c += blockDim.x) {
-        for (int i = KP - 1; i >= 0; --i) {
-            double wv = 0.0;
-            if (c < k && i <= c) {
-                double s = (i == c) ? 1.0 : 0.0;
-                for (int q = i + 1; q <= c; ++q) s -= a[i * LD + q] * W64[q * KP + c];
-                wv = s / a[i * LD + i];
-            }
-            W64[i * KP + c] = wv;
-        }
-    }
-    __syncthreads();
-    for (int t = tid; t < KP * KP; t += blockDim.x) W32[t] = static_cast<float>(W64[t]);
-    if (tid == 0 && bad) atomicOr(status, 1);
-}
-
-// Y <- Y W for a tall panel (KP x KP W in LDS) on MFMA; writes out_cols
-// columns of the product with row stride out_ld. The 32-row tile of Y is staged
-// in LDS with a padded stride so the A operand reads are conflict free.
-template <int KP>
-__global__ void __launch_bounds__(256) k_apply(const float *__restrict__ Y, const float *__restrict__ W,
-                                               float *__restrict__ out, int64_t nrows, int out_cols,
-                                               int64_t out_ld) {
-    constexpr int NT = KP / 32;
-    constexpr int LDY = KP + 1;
-    __shared__ float ws[KP * KP];
-    __shared__ float ys[4][32 * LDY];
-    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) ws[t] = W[t];
-    __syncthreads();
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63;
-    const int kr = lane >> 5, cl = lane & 31;
-    const int64_t ntiles = (nrows + 31) / 32;
-    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave; tile < ntiles;
-         tile += static_cast<int64_t>(gridDim.x) * 4) {
-        const int64_t r0 = tile * 32;
-        float *yt = ys[wave];
-        for (int i = lane; i < 32 * KP; i += kWave) {
-            const int rr = i / KP, cc = i % KP;
-            yt[rr * LDY + cc] = (r0 + rr < nrows) ? Y[(r0 + rr) * KP + cc] : 0.f;
-        }
-        wave_lds_sync();
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-            v16f acc = v16f{};
-            for (int s = 0; s < KP / 2; ++s) {
-                const float av = yt[cl * LDY + 2 * s + kr];
-                const float bv = ws[(2 * s + kr) * KP + nt * 32 + cl];
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
-            }
-            const int col = nt * 32 + cl;
-            if (col < out_cols) {
-#pragma unroll
-                for (int q = 0; q < 16; ++q) {
-                    const int64_t row = r0 + 8 * (q / 4) + 4 * kr + (q % 4);
-                    if (row < nrows) out[row * out_ld + col] = acc[q];
-                }
-            }
-        }
-        wave_lds_sync();
-    }
-}
-
 // out = A W for a small f64 A (rows x KP), W f64 KP x KP; thread per element.
 template <int KP>
 __global__ void __launch_bounds__(256) k_apply64(const double *__restrict__ A, const double *__restrict__ W,
@@ -687,108 +455,6 @@ __global__ void __launch_bounds__(256) k_small_gemm64(const double *__restrict__
         for (int q = 0; q < KP; ++q) s += A[i * KP + q] * B[q * KP + j];
         C[t] = s;
     }
-}
-
-// ---------------------------------------------------------------------------
-// One-sided (Hestenes) Jacobi SVD of the k x k upper-triangular R in f64,
-// one workgroup, round-robin (circle method) pair ordering. Outputs sigma
-// (descending), U (columns A_j / sigma_j) and V, all KP x KP row-major with
-// columns permuted to the descending order.
-// ---------------------------------------------------------------------------
-
-template <int KP>
-__global__ void __launch_bounds__(256) k_jacobi_svd(const double *__restrict__ R, int k,
-                                                    double *__restrict__ Acol, double *__restrict__ Vcol,
-                                                    double *__restrict__ U, double *__restrict__ V,
-                                                    double *__restrict__ sigma, int *__restrict__ status) {
-    // Acol / Vcol: column-major KP x KP scratch in global memory (L2 resident).
-    __shared__ int changed;
-    __shared__ double nrm[KP];
-    __shared__ int order[KP];
-    const int tid = threadIdx.x;
-    const int n = (k + 1) & ~1;  // even count; the extra column (if any) is zero
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t % KP, c = t / KP;  // column-major index t = c*KP + i
-        Acol[t] = (i < k && c < k) ? R[i * KP + c] : 0.0;
-        Vcol[t] = (i == c && c < n) ? 1.0 : 0.0;
-    }
-    __syncthreads();
-    constexpr int TPP = 8;  // threads per pair
-    const int npairs = n / 2;
-    for (int sweep = 0; sweep < 60 && n >= 2; ++sweep) {
-        if (tid == 0) changed = 0;
-        __syncthreads();
-        for (int round = 0; round < n - 1; ++round) {
-            for (int pb = tid / TPP; pb < npairs; pb += blockDim.x / TPP) {
-                int p, q;
-                if (pb == 0) {
-                    p = 0;
-                    q = 1 + (round % (n - 1));
-                } else {
-                    p = 1 + (round + pb) % (n - 1);
-                    q = 1 + (round - pb + (n - 1)) % (n - 1);
-                }
-                if (p > q) { const int s = p; p = q; q = s; }
-                double *ap = Acol + static_cast<int64_t>(p) * KP, *aq = Acol + static_cast<int64_t>(q) * KP;
-                const int sub = tid % TPP;
-                double al = 0.0, be = 0.0, ga = 0.0;
-                for (int i = sub; i < n; i += TPP) {
-                    const double x = ap[i], y = aq[i];
-                    al += x * x; be += y * y; ga += x * y;
-                }
-                for (int o = TPP / 2; o > 0; o >>= 1) {
-                    al += __shfl_xor(al, o, TPP);
-                    be += __shfl_xor(be, o, TPP);
-                    ga += __shfl_xor(ga, o, TPP);
-                }
-                if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
-                    const double zeta = (be - al) / (2.0 * ga);
-                    const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                    const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
-                    for (int i = sub; i < n; i += TPP) {
-                        const double x = ap[i], y = aq[i];
-                        ap[i] = c * x - s * y;
-                        aq[i] = s * x + c * y;
-                    }
-                    double *vp = Vcol + static_cast<int64_t>(p) * KP, *vq = Vcol + static_cast<int64_t>(q) * KP;
-                    for (int i = sub; i < n; i += TPP) {
-                        const double x = vp[i], y = vq[i];
-                        vp[i] = c * x - s * y;
-                        vq[i] = s * x + c * y;
-                    }
-                    if (sub == 0) changed = 1;
-                }
-            }
-            __threadfence_block();
-            __syncthreads();
-        }
-        if (!changed) break;
-        if (sweep == 59 && tid == 0) atomicOr(status, 2);
-        __syncthreads();
-    }
-    for (int c = tid; c < KP; c += blockDim.x) {
-        double s = 0.0;
-        if (c < k)
-            for (int i = 0; i < n; ++i) s += Acol[static_cast<int64_t>(c) * KP + i] * Acol[static_cast<int64_t>(c) * KP + i];
-        nrm[c] = c < k ? sqrt(s) : -1.0;
-    }
-    __syncthreads();
-    // rank by descending norm (ties by index) -> stable order
-    for (int c = tid; c < KP; c += blockDim.x) {
-        int rank = 0;
-        for (int d = 0; d < KP; ++d)
-            if (nrm[d] > nrm[c] || (nrm[d] == nrm[c] && d < c)) ++rank;
-        order[rank] = c;
-    }
-    __syncthreads();
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, jj = t % KP;  // output row-major (i, jj)
-        const int c = order[jj];
-        const double sg = nrm[c] > 0.0 ? nrm[c] : 0.0;
-        U[t] = (jj < k && sg > 0.0) ? Acol[static_cast<int64_t>(c) * KP + i] / sg : 0.0;
-        V[t] = jj < k ? Vcol[static_cast<int64_t>(c) * KP + i] : 0.0;
-    }
-    for (int jj = tid; jj < KP; jj += blockDim.x) sigma[jj] = jj < k ? (nrm[order[jj]] > 0 ? nrm[order[jj]] : 0.0) : 0.0;
 }
 
 // Embedding weights W_emb = (V Sigma)[:, :nc] as f32 KP x KP (zero padded).
@@ -852,4 +518,817 @@ __global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d
     if (t < n) d[t] = static_cast<float>(s[t]);
 }
 
+
+// ===========================================================================
+// Second-generation SpMM kernels.
+// ===========================================================================
+
+// Lane t of each 16-lane row broadcast to the whole row (DPP row_newbcast, gfx90a+).
+template <int T>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
+    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x150 + T, 0xf, 0xf, true));
+}
+template <int T>
+__device__ __forceinline__ float row_bcast(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + T, 0xf, 0xf, true));
+}
+
+// Group-gather loop. The four 16-lane groups take the entries held by lane T
+// of their row (entries 4T+g of the batch, by the interleaved load), read E
+// floats of an LDS panel row at byte offset ofs + lane_ofs and accumulate
+// x * panel. The LDS reads run two steps ahead of the FMAs (three rotating
+// buffers), and the step count is wave-uniform so the exits are scalar
+// branches. Slots past cnt carry x = 0 and offset 0 (a valid address), so the
+// look-ahead reads are unconditional: up to two reads past the last step.
+template <int T, int E>
+__device__ __forceinline__ void gather_load(float4 (&m)[E / 4], float &xb, uint32_t ofs, float x, const char *lds,
+                                            uint32_t lane_ofs) {
+    const uint32_t a = row_bcast<T>(ofs) + lane_ofs;
+#pragma unroll
+    for (int e = 0; e < E / 4; ++e) m[e] = *reinterpret_cast<const float4 *>(lds + a + 16 * e);
+    xb = row_bcast<T>(x);
+}
+
+template <int E>
+__device__ __forceinline__ void gather_fma(float (&acc)[E], const float4 (&m)[E / 4], float xb) {
+#pragma unroll
+    for (int e = 0; e < E / 4; ++e) {
+        acc[4 * e + 0] = __builtin_fmaf(xb, m[e].x, acc[4 * e + 0]);
+        acc[4 * e + 1] = __builtin_fmaf(xb, m[e].y, acc[4 * e + 1]);
+        acc[4 * e + 2] = __builtin_fmaf(xb, m[e].z, acc[4 * e + 2]);
+        acc[4 * e + 3] = __builtin_fmaf(xb, m[e].w, acc[4 * e + 3]);
+    }
+}
+
+template <int E>
+__device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, float x, int cnt, const char *lds,
+                                             uint32_t lane_ofs) {
+    const int steps = __builtin_amdgcn_readfirstlane((cnt + 3) >> 2);
+    float4 b0[E / 4], b1[E / 4], b2[E / 4];
+    float x0 = 0.f, x1 = 0.f, x2 = 0.f;
+    gather_load<0, E>(b0, x0, ofs, x, lds, lane_ofs);
+    gather_load<1, E>(b1, x1, ofs, x, lds, lane_ofs);
+#define SCX_STEP(T, CUR, XC, NXT, XN)                                                   \
+    if ((T) >= steps) return;                                                           \
+    if ((T) + 2 < 16) gather_load<((T) + 2) & 15, E>(NXT, XN, ofs, x, lds, lane_ofs);   \
+    gather_fma<E>(acc, CUR, XC);
+    SCX_STEP(0, b0, x0, b2, x2) SCX_STEP(1, b1, x1, b0, x0) SCX_STEP(2, b2, x2, b1, x1)
+    SCX_STEP(3, b0, x0, b2, x2) SCX_STEP(4, b1, x1, b0, x0) SCX_STEP(5, b2, x2, b1, x1)
+    SCX_STEP(6, b0, x0, b2, x2) SCX_STEP(7, b1, x1, b0, x0) SCX_STEP(8, b2, x2, b1, x1)
+    SCX_STEP(9, b0, x0, b2, x2) SCX_STEP(10, b1, x1, b0, x0) SCX_STEP(11, b2, x2, b1, x1)
+    SCX_STEP(12, b0, x0, b2, x2) SCX_STEP(13, b1, x1, b0, x0) SCX_STEP(14, b2, x2, b1, x1)
+    SCX_STEP(15, b0, x0, b2, x2)
+#undef SCX_STEP
+}
+
+// ---------------------------------------------------------------------------
+// Forward SpMM, gene-chunked panel in LDS: Y[r][:] = sum x~ M[c][:] - mc.
+// A block (8 waves) owns 8*R consecutive rows; each wave keeps R rows' partial
+// outputs in registers (per 16-lane group, E columns per lane). For each chunk
+// of CG genes the block stages M[chunk] (CG*KP*4 = 64 KB) into LDS, then every
+// wave walks its rows' windows of that chunk (precomputed split offsets), 64
+// nonzeros per batch: lane-parallel load + transform, then 16 broadcast steps
+// that feed the four groups with entries 4t+g. After the last chunk the four
+// group partials of a row are summed and the row is written once.
+// ---------------------------------------------------------------------------
+
+template <int KP, bool BF16, int R, int NW, int CG>
+__global__ void __launch_bounds__(NW * 64) k_spmm_fwd2(const int64_t *__restrict__ indptr,
+                                                       const uint32_t *__restrict__ idx,
+                                                       const float *__restrict__ val,
+                                                       const uint16_t *__restrict__ xbf,
+                                                       const double *__restrict__ rowf, int flags,
+                                                       const float *__restrict__ M, const float *__restrict__ mc,
+                                                       float *__restrict__ Y, const uint32_t *__restrict__ spl,
+                                                       int nsplit, int cpc, int nch, int64_t n_obs,
+                                                       int64_t n_vars) {
+    constexpr int E = KP / 16;
+    static_assert(R <= 16, "row metadata is held one row per lane of a 16-lane group");
+    __shared__ float4 ms[CG * KP / 4];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * NW + wave) * R;
+    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
+    const int esel = 4 * l + g;  // entry of a 64-batch this lane loads
+    float acc[R][E];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[r][e] = 0.f;
+    // per-row metadata, lane l of every group holds row row0 + l
+    const int64_t mrow = row0 + l;
+    const bool mvalid = (l < R) && (mrow < n_obs);
+    const int64_t rbase = mvalid ? indptr[mrow] : 0;
+    const double rfac = (!BF16 && rowf && mvalid) ? rowf[mrow] : 1.0;
+    const float4 *M4 = reinterpret_cast<const float4 *>(M);
+    for (int ch = 0; ch < nch; ++ch) {
+        const int64_t g0 = static_cast<int64_t>(ch) * CG;
+        const int s_lo = ch * cpc, s_hi = (ch + 1) * cpc < nsplit ? (ch + 1) * cpc : nsplit;
+        // window bounds of all R rows for this chunk, one row per lane
+        const uint32_t wlo = mvalid ? spl[mrow * (nsplit + 1) + s_lo] : 0u;
+        const uint32_t whi = mvalid ? spl[mrow * (nsplit + 1) + s_hi] : 0u;
+        __syncthreads();
+        for (int i = threadIdx.x; i < CG * KP / 4; i += blockDim.x) {
+            const int64_t gene = g0 + i / (KP / 4);
+            ms[i] = gene < n_vars ? M4[gene * (KP / 4) + i % (KP / 4)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        __syncthreads();
+        // First 64-entry batch of each row's window, loaded PF rows ahead.
+        constexpr int PF = 4;
+        uint32_t pidx[PF];
+        float pval[PF];
+        auto issue = [&](int r, uint32_t &pi, float &pv) {
+            const int64_t base_r = static_cast<int64_t>(__shfl(static_cast<long long>(rbase), r));
+            const uint32_t lo_r = static_cast<uint32_t>(__shfl(static_cast<int>(wlo), r));
+            const uint32_t hi_r = static_cast<uint32_t>(__shfl(static_cast<int>(whi), r));
+            pi = 0u;
+            pv = 0.f;
+            if (lo_r + esel < hi_r) {
+                const int64_t p = base_r + lo_r + esel;
+                pi = idx[p];
+                pv = BF16 ? bf16_to_f32(xbf[p]) : val[p];
+            }
+        };
+#pragma unroll
+        for (int r = 0; r < PF && r < R; ++r) issue(r, pidx[r], pval[r]);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            // uniform row metadata via cross-lane reads from lane r
+            const int64_t base_r = static_cast<int64_t>(__shfl(static_cast<long long>(rbase), r));
+            const uint32_t lo_r = static_cast<uint32_t>(__shfl(static_cast<int>(wlo), r));
+            const uint32_t hi_r = static_cast<uint32_t>(__shfl(static_cast<int>(whi), r));
+            const double f = BF16 ? 1.0 : __shfl(rfac, r);
+            const uint32_t ci = pidx[r % PF];
+            const float cv = pval[r % PF];
+            if (r + PF < R) issue(r + PF, pidx[r % PF], pval[r % PF]);
+            for (uint32_t o = lo_r; o < hi_r; o += kWave) {
+                const int cnt = (hi_r - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi_r - o) : kWave;
+                uint32_t ofs = 0u;
+                float x = 0.f;
+                if (esel < cnt) {
+                    uint32_t c;
+                    float v;
+                    if (o == lo_r) {
+                        c = ci;
+                        v = cv;
+                    } else {
+                        const int64_t p = base_r + o + esel;
+                        c = idx[p];
+                        v = BF16 ? bf16_to_f32(xbf[p]) : val[p];
+                    }
+                    ofs = (c - static_cast<uint32_t>(g0)) * (KP * 4);
+                    x = BF16 ? v : xtilde(v, f, flags);
+                }
+                gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
+            }
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int64_t row = row0 + r;
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            acc[r][e] += __shfl_xor(acc[r][e], 16);
+            acc[r][e] += __shfl_xor(acc[r][e], 32);
+        }
+        if (row < n_obs && g == 0) {
+            float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
+#pragma unroll
+            for (int e = 0; e < E / 4; ++e) {
+                float4 o;
+                o.x = acc[r][4 * e + 0] - (mc ? mc[l * E + 4 * e + 0] : 0.f);
+                o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
+                o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
+                o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
+                yr[e] = o;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Row-tiled CSC ("tile-CSC"): for each tile of RT = 256 consecutive rows, the
+// tile's nonzeros ordered by gene, then by row. tptr[t][c] = offset of gene c's
+// run inside tile t (relative to indptr[t*RT]); trow = row within the tile
+// (u8); tval = raw value. Built once per matrix; the transposed SpMM and the
+// column statistics read it, so each gene's contributions in a tile are one
+// contiguous run. Rows are scattered in order, so runs are row-ascending.
+// ---------------------------------------------------------------------------
+
+constexpr int kTileRows = 256;
+
+constexpr int kTcscGenes = 4096;   // genes per mask pass: 8 words each (256-row tile)
+constexpr int kTcscStage = 12288;  // staged output entries (5 B each: 60 KB)
+
+// Deterministic build. Per pass over a gene range, each gene gets a 256-bit
+// row-membership mask; wave w owns rows [32w, 32w+32) of the tile, i.e. mask
+// word w of every gene, so the masks are built without atomics. A nonzero
+// (r, c) lands at start[c] + popcount(mask[c] below bit r): runs come out in
+// row order whatever the schedule. When the pass's output fits kTcscStage
+// entries it is assembled in LDS and written out contiguously.
+__global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ indptr,
+                                                    const uint32_t *__restrict__ idx,
+                                                    const float *__restrict__ val, int64_t n_obs,
+                                                    int n_vars, uint32_t *__restrict__ tptr,
+                                                    uint8_t *__restrict__ trow, float *__restrict__ tval) {
+    constexpr int GP = (kTcscGenes < 2048) ? kTcscGenes : 2048;  // genes per pass with staging room
+    __shared__ uint32_t mask[GP * 8];
+    __shared__ uint32_t start[GP];
+    __shared__ float sval[kTcscStage];
+    __shared__ uint8_t srow[kTcscStage];
+    __shared__ uint32_t wsum[80];
+    __shared__ uint32_t carry;
+    const int t = blockIdx.x;
+    const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+    const int64_t r1 = r0 + kTileRows < n_obs ? r0 + kTileRows : n_obs;
+    const int64_t base = indptr[r0];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, nw = blockDim.x / kWave;  // nw = 8
+    uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+    if (threadIdx.x == 0) carry = 0u;
+    for (int G0 = 0; G0 < n_vars; G0 += GP) {
+        const int G1 = G0 + GP < n_vars ? G0 + GP : n_vars;
+        const int ng = G1 - G0;
+        for (int i = threadIdx.x; i < ng * 8; i += blockDim.x) mask[i] = 0u;
+        __syncthreads();
+        // wave w: rows 32w .. 32w+31 of the tile (mask word w), in order
+        for (int rr = 32 * wave; rr < 32 * wave + 32; ++rr) {
+            const int64_t r = r0 + rr;
+            if (r >= r1) break;
+            const uint32_t bit = 1u << (rr & 31);
+            for (int64_t p = indptr[r] + lane; p < indptr[r + 1]; p += kWave) {
+                const int c = static_cast<int>(idx[p]) - G0;
+                if (c >= 0 && c < ng) mask[c * 8 + wave] |= bit;
+            }
+            wave_lds_sync();
+        }
+        __syncthreads();
+        // exclusive scan of per-gene counts (contiguous segment per thread)
+        const int per = (ng + blockDim.x - 1) / blockDim.x;
+        const int c0 = threadIdx.x * per, c1 = c0 + per < ng ? c0 + per : ng;
+        uint32_t local = 0;
+        for (int c = c0; c < c1; ++c)
+            for (int w = 0; w < 8; ++w) local += __popc(mask[c * 8 + w]);
+        uint32_t incl = local;
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        if (lane == kWave - 1) wsum[wave] = incl;
+        __syncthreads();
+        if (wave == 0) {
+            const uint32_t wt = lane < nw ? wsum[lane] : 0u;
+            uint32_t wincl = wt;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t v = __shfl_up(wincl, o);
+                if (lane >= o) wincl += v;
+            }
+            if (lane < nw) wsum[32 + lane] = wincl - wt;
+            if (lane == nw - 1) wsum[64] = wincl;
+        }
+        __syncthreads();
+        const uint32_t pass_base = carry;
+        const uint32_t pass_total = wsum[64];
+        uint32_t off = wsum[32 + wave] + (incl - local);  // relative to the pass
+        for (int c = c0; c < c1; ++c) {
+            start[c] = off;
+            tp[G0 + c] = pass_base + off;
+            for (int w = 0; w < 8; ++w) off += __popc(mask[c * 8 + w]);
+        }
+        __syncthreads();
+        const bool staged = pass_total <= static_cast<uint32_t>(kTcscStage);
+        for (int rr = wave; rr < kTileRows; rr += nw) {
+            const int64_t r = r0 + rr;
+            if (r >= r1) break;
+            const int word = rr >> 5;
+            const uint32_t below = (rr & 31) ? ((1u << (rr & 31)) - 1u) : 0u;
+            for (int64_t p = indptr[r] + lane; p < indptr[r + 1]; p += kWave) {
+                const int c = static_cast<int>(idx[p]) - G0;
+                if (c >= 0 && c < ng) {
+                    uint32_t rank = __popc(mask[c * 8 + word] & below);
+                    for (int w = 0; w < word; ++w) rank += __popc(mask[c * 8 + w]);
+                    const uint32_t pos = start[c] + rank;
+                    if (staged) {
+                        srow[pos] = static_cast<uint8_t>(rr);
+                        sval[pos] = val[p];
+                    } else {
+                        trow[base + pass_base + pos] = static_cast<uint8_t>(rr);
+                        tval[base + pass_base + pos] = val[p];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (staged) {
+            for (uint32_t i = threadIdx.x; i < pass_total; i += blockDim.x) {
+                trow[base + pass_base + i] = srow[i];
+                tval[base + pass_base + i] = sval[i];
+            }
+        }
+        if (threadIdx.x == 0) carry = pass_base + pass_total;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) tp[n_vars] = carry;
+}
+
+// ---------------------------------------------------------------------------
+// Transposed SpMM on the tile-CSC: part[tr][v][:] = sum_{tiles of tr} sum_run x~ Q[row][:].
+// Block (tile range tr, gene chunk gc), 16 waves, each owning GW consecutive
+// genes whose (per-group partial) outputs stay in registers across all tiles.
+// Per tile: Q tile (256 x KP) and the row factors are staged in LDS; the
+// chunk's entries (contiguous in gene-major order) are transformed once in
+// full 64-wide batches into an LDS entry array (Q row offset, x~); then each
+// wave walks its genes' runs: groups take entries 4t+g, gather E columns of
+// the Q row from LDS and accumulate. No atomics: every output element is owned
+// by one lane, so the sum order is fixed and the result is deterministic.
+// ---------------------------------------------------------------------------
+
+template <int KP, int GW, bool BF16>
+__global__ void __launch_bounds__(1024) k_spmm_t2(const int64_t *__restrict__ indptr,
+                                                  const uint32_t *__restrict__ tptr,
+                                                  const uint8_t *__restrict__ trow,
+                                                  const float *__restrict__ tval,
+                                                  const uint16_t *__restrict__ txbf,
+                                                  const double *__restrict__ rowf, int flags,
+                                                  const float *__restrict__ Q, int nch, int ntr,
+                                                  int tiles_per_tr, int64_t n_obs, int64_t n_vars, int n_tiles,
+                                                  float *__restrict__ part) {
+    constexpr int E = KP / 16;
+    constexpr int NW = 16;
+    constexpr int CG = NW * GW;
+    constexpr int ENT = (KP == 64) ? 8192 : 2048;  // entry slots in LDS
+    __shared__ float4 qs[kTileRows * KP / 4];        // 64 KB (KP=64) / 128 KB (KP=128)
+    __shared__ uint2 ent[ENT];                        // (Q row byte offset, x~)
+    __shared__ double rf[kTileRows];
+    __shared__ uint32_t bnd[CG + 1];                  // the chunk's run bounds in this tile
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
+    const int64_t cb = static_cast<int64_t>(gc) * CG;               // block's first gene
+    const int64_t cw = cb + static_cast<int64_t>(wave) * GW;        // wave's first gene
+    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
+    const int esel = 4 * l + g;
+    float z[GW][E];
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi)
+#pragma unroll
+        for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
+    const int t_begin = tr * tiles_per_tr;
+    const int t_end = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
+    const int64_t cend = (cb + CG) < n_vars ? (cb + CG) : n_vars;
+    constexpr int QPT = kTileRows * KP / 4 / 1024;  // float4 of the Q tile per thread
+    float4 qpre[QPT];
+    auto load_q = [&](int tt) {
+        const int64_t rq = static_cast<int64_t>(tt) * kTileRows;
+        const int nr = (n_obs - rq) < kTileRows ? static_cast<int>(n_obs - rq) : kTileRows;
+        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + rq * KP);
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) {
+            const int i = threadIdx.x + j * 1024;
+            qpre[j] = (i / (KP / 4) < nr) ? Q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    const int t_first = (tr < ntr) ? t_begin : t_end;
+    if (t_first < t_end) load_q(t_first);
+    for (int t = t_first; t < t_end; ++t) {
+        const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+        const int nrows = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+        const int64_t tb = indptr[r0];
+        const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+        __syncthreads();
+        for (int i = threadIdx.x; i <= CG; i += blockDim.x) bnd[i] = (cb + i <= cend) ? tp[cb + i] : 0u;
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) qs[threadIdx.x + j * 1024] = qpre[j];
+        if (!BF16)
+            for (int i = threadIdx.x; i < kTileRows; i += blockDim.x) rf[i] = (rowf && i < nrows) ? rowf[r0 + i] : 1.0;
+        __syncthreads();
+        if (t + 1 < t_end) load_q(t + 1);
+        const uint32_t e_lo = (cb < n_vars) ? bnd[0] : 0u, e_hi = (cb < n_vars) ? bnd[cend - cb] : 0u;
+        for (uint32_t w0 = e_lo; w0 < e_hi; w0 += ENT) {
+            const uint32_t w1 = (w0 + ENT) < e_hi ? (w0 + ENT) : e_hi;
+            for (uint32_t i = w0 + threadIdx.x; i < w1; i += blockDim.x) {
+                const uint32_t rr = trow[tb + i];
+                const float x = BF16 ? bf16_to_f32(txbf[tb + i]) : xtilde(tval[tb + i], rf[rr], flags);
+                ent[i - w0] = make_uint2(rr * (KP * 4), __float_as_uint(x));
+            }
+            __syncthreads();
+#pragma unroll
+            for (int gi = 0; gi < GW; ++gi) {
+                const int64_t c = cw + gi;
+                if (c < cend) {
+                    uint32_t s0 = bnd[c - cb], s1 = bnd[c - cb + 1];
+                    s0 = s0 > w0 ? s0 : w0;
+                    s1 = s1 < w1 ? s1 : w1;
+                    for (uint32_t bse = s0; bse < s1; bse += kWave) {
+                        const int cnt = (s1 - bse) < kWave ? static_cast<int>(s1 - bse) : kWave;
+                        uint32_t ofs = 0u;
+                        float x = 0.f;
+                        if (esel < cnt) {
+                            const uint2 en = ent[bse - w0 + esel];
+                            ofs = en.x;
+                            x = __uint_as_float(en.y);
+                        }
+                        gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // reduce the four group partials and write this block's partial of Z
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            z[gi][e] += __shfl_xor(z[gi][e], 16);
+            z[gi][e] += __shfl_xor(z[gi][e], 32);
+        }
+        const int64_t c = cw + gi;
+        if (tr < ntr && c < cend && g == 0) {
+            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
+#pragma unroll
+            for (int e = 0; e < E / 4; ++e)
+                dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
+        }
+    }
+}
+
+// X~ values of the tile-CSC as bf16 for the sketch form; one block per tile.
+__global__ void __launch_bounds__(256) k_tcsc_bf16(const int64_t *__restrict__ indptr,
+                                                   const uint8_t *__restrict__ trow,
+                                                   const float *__restrict__ tval,
+                                                   const double *__restrict__ rowf, int flags, int64_t n_obs,
+                                                   uint16_t *__restrict__ out) {
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kTileRows;
+    const int64_t r1 = r0 + kTileRows < n_obs ? r0 + kTileRows : n_obs;
+    const int64_t a = indptr[r0], b = indptr[r1];
+    for (int64_t p = a + threadIdx.x; p < b; p += blockDim.x) {
+        const double f = rowf ? rowf[r0 + trow[p]] : 1.0;
+        out[p] = f32_to_bf16_rne(xtilde(tval[p], f, flags));
+    }
+}
+
+// Gram with in-block reduction: each wave accumulates its upper tiles over its
+// rows (FP32 MFMA), the four waves of the block add into an LDS tile in a fixed
+// order, and the block writes one f64 partial. Two row pairs per iteration
+// keep more loads in flight.
+template <int KP>
+__global__ void __launch_bounds__(256) k_gram2(const float *__restrict__ Y, int64_t nrows,
+                                               int64_t rows_per_blk, double *__restrict__ part) {
+    constexpr int NT = KP / 32;
+    constexpr int NTILE = NT * (NT + 1) / 2;
+    __shared__ float gs[KP * KP];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63;
+    const int kr = lane >> 5, cl = lane & 31;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * rows_per_blk;
+    const int64_t b1 = b0 + rows_per_blk < nrows ? b0 + rows_per_blk : nrows;
+    v16f acc[NTILE];
+#pragma unroll
+    for (int t = 0; t < NTILE; ++t) acc[t] = v16f{};
+    // waves interleave row pairs: wave w takes pairs w, w+4, ...
+    for (int64_t r = b0 + 2 * wave; r < b1; r += 16) {
+        float v0[NT], v1[NT];
+        const int64_t ra = r + kr, rb = r + 8 + kr;
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            v0[t] = ra < b1 ? Y[ra * KP + t * 32 + cl] : 0.f;
+            v1[t] = rb < b1 ? Y[rb * KP + t * 32 + cl] : 0.f;
+        }
+        int tt = 0;
+#pragma unroll
+        for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+            for (int tj = ti; tj < NT; ++tj) {
+                acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(v0[ti], v0[tj], acc[tt], 0, 0, 0);
+                acc[tt] = __builtin_amdgcn_mfma_f32_32x32x2f32(v1[ti], v1[tj], acc[tt], 0, 0, 0);
+                ++tt;
+            }
+    }
+    for (int w = 0; w < 4; ++w) {
+        if (wave == w) {
+            int tt = 0;
+#pragma unroll
+            for (int ti = 0; ti < NT; ++ti)
+#pragma unroll
+                for (int tj = ti; tj < NT; ++tj) {
+#pragma unroll
+                    for (int q = 0; q < 16; ++q) {
+                        const int i = ti * 32 + 8 * (q / 4) + 4 * kr + (q % 4);
+                        const int j = tj * 32 + cl;
+                        gs[i * KP + j] = (w == 0 ? 0.f : gs[i * KP + j]) + acc[tt][q];
+                    }
+                    ++tt;
+                }
+        }
+        __syncthreads();
+    }
+    double *dst = part + static_cast<int64_t>(blockIdx.x) * KP * KP;
+    for (int i = threadIdx.x; i < KP * KP; i += blockDim.x) dst[i] = static_cast<double>(gs[i]);
+}
+
+
+// One-sided Jacobi SVD of the k x k R (f64), columns held in LDS for KP <= 64
+// (A and V) and A in LDS / V in global scratch for KP = 128.
+template <int KP>
+__global__ void __launch_bounds__(256) k_jacobi_lds(const double *__restrict__ R, int k, double *__restrict__ Vg,
+                                                    double *__restrict__ U, double *__restrict__ V,
+                                                    double *__restrict__ sigma, int *__restrict__ status) {
+    constexpr bool VLDS = KP <= 64;
+    __shared__ double A[KP * KP];                 // column-major: A[c*KP + i]
+    __shared__ double Vs[VLDS ? KP * KP : 1];
+    __shared__ int changed;
+    __shared__ double nrm[KP];
+    __shared__ int order[KP];
+    double *Vc = VLDS ? Vs : Vg;
+    const int tid = threadIdx.x;
+    const int n = (k + 1) & ~1;
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t % KP, c = t / KP;
+        A[t] = (i < k && c < k) ? R[i * KP + c] : 0.0;
+        Vc[t] = (i == c && c < n) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    constexpr int TPP = 8;
+    const int npairs = n / 2;
+    for (int sweep = 0; sweep < 60 && n >= 2; ++sweep) {
+        if (tid == 0) changed = 0;
+        __syncthreads();
+        for (int round = 0; round < n - 1; ++round) {
+            for (int pb = tid / TPP; pb < npairs; pb += blockDim.x / TPP) {
+                int p, q;
+                if (pb == 0) {
+                    p = 0;
+                    q = 1 + (round % (n - 1));
+                } else {
+                    p = 1 + (round + pb) % (n - 1);
+                    q = 1 + (round - pb + (n - 1)) % (n - 1);
+                }
+                if (p > q) { const int sw = p; p = q; q = sw; }
+                double *ap = A + p * KP, *aq = A + q * KP;
+                const int sub = tid % TPP;
+                double al = 0.0, be = 0.0, ga = 0.0;
+                for (int i = sub; i < n; i += TPP) {
+                    const double x = ap[i], y = aq[i];
+                    al += x * x; be += y * y; ga += x * y;
+                }
+                for (int o = TPP / 2; o > 0; o >>= 1) {
+                    al += __shfl_xor(al, o, TPP);
+                    be += __shfl_xor(be, o, TPP);
+                    ga += __shfl_xor(ga, o, TPP);
+                }
+                if (ga != 0.0 && fabs(ga) > 1e-15 * sqrt(al * be)) {
+                    const double zeta = (be - al) / (2.0 * ga);
+                    const double tt = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double c = 1.0 / sqrt(1.0 + tt * tt), sn = c * tt;
+                    for (int i = sub; i < n; i += TPP) {
+                        const double x = ap[i], y = aq[i];
+                        ap[i] = c * x - sn * y;
+                        aq[i] = sn * x + c * y;
+                    }
+                    double *vp = Vc + p * KP, *vq = Vc + q * KP;
+                    for (int i = sub; i < n; i += TPP) {
+                        const double x = vp[i], y = vq[i];
+                        vp[i] = c * x - sn * y;
+                        vq[i] = sn * x + c * y;
+                    }
+                    if (sub == 0) changed = 1;
+                }
+            }
+            __syncthreads();
+        }
+        if (!changed) break;
+        if (sweep == 59 && tid == 0) atomicOr(status, 2);
+        __syncthreads();
+    }
+    for (int c = tid; c < KP; c += blockDim.x) {
+        double s = 0.0;
+        if (c < k)
+            for (int i = 0; i < n; ++i) s += A[c * KP + i] * A[c * KP + i];
+        nrm[c] = c < k ? sqrt(s) : -1.0;
+    }
+    __syncthreads();
+    for (int c = tid; c < KP; c += blockDim.x) {
+        int rank = 0;
+        for (int d = 0; d < KP; ++d)
+            if (nrm[d] > nrm[c] || (nrm[d] == nrm[c] && d < c)) ++rank;
+        order[rank] = c;
+    }
+    __syncthreads();
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, jj = t % KP;
+        const int c = order[jj];
+        const double sg = nrm[c] > 0.0 ? nrm[c] : 0.0;
+        U[t] = (jj < k && sg > 0.0) ? A[c * KP + i] / sg : 0.0;
+        V[t] = jj < k ? Vc[c * KP + i] : 0.0;
+    }
+    for (int jj = tid; jj < KP; jj += blockDim.x) sigma[jj] = jj < k ? (nrm[order[jj]] > 0 ? nrm[order[jj]] : 0.0) : 0.0;
+}
+
+// Q = Y W on MFMA with the A operand taken from registers: lane (kr, cl) loads
+// row cl of its 32-row tile once and uses columns 2s+kr; W in LDS.
+template <int KP>
+__global__ void __launch_bounds__(256) k_apply2(const float *__restrict__ Y, const float *__restrict__ W,
+                                                float *__restrict__ out, int64_t nrows, int out_cols,
+                                                int64_t out_ld) {
+    constexpr int NT = KP / 32;
+    __shared__ float ws[KP * KP];
+    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) ws[t] = W[t];
+    __syncthreads();
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63;
+    const int kr = lane >> 5, cl = lane & 31;
+    const int64_t ntiles = (nrows + 31) / 32;
+    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave; tile < ntiles;
+         tile += static_cast<int64_t>(gridDim.x) * 4) {
+        const int64_t r0 = tile * 32;
+        const int64_t row = r0 + cl;
+        float yr[KP / 2];
+        if (row < nrows) {
+            const float2 *src = reinterpret_cast<const float2 *>(Y + row * KP);
+#pragma unroll
+            for (int s2 = 0; s2 < KP / 2; ++s2) {
+                const float2 v = src[s2];
+                yr[s2] = kr ? v.y : v.x;
+            }
+        } else {
+#pragma unroll
+            for (int s2 = 0; s2 < KP / 2; ++s2) yr[s2] = 0.f;
+        }
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+            v16f acc = v16f{};
+#pragma unroll
+            for (int s2 = 0; s2 < KP / 2; ++s2)
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(yr[s2], ws[(2 * s2 + kr) * KP + nt * 32 + cl], acc, 0, 0, 0);
+            const int col = nt * 32 + cl;
+            if (col < out_cols) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int64_t orow = r0 + 8 * (q / 4) + 4 * kr + (q % 4);
+                    if (orow < nrows) out[orow * out_ld + col] = acc[q];
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Cholesky (f64, LDS, 256 threads: the trailing update is parallel over the
+// (i, c) pairs, one barrier per step) and W = R^-1 row by row from the bottom:
+// W[i][c] = -(sum_{l=i+1..c} R[i][l] W[l][c]) / R[i][i], each entry's dot
+// product split over 4 threads and combined in a fixed order. Both triangles
+// are stored packed (upper, row-major) so KP = 128 fits in LDS.
+// ---------------------------------------------------------------------------
+template <int KP>
+__device__ __forceinline__ int upk(int i, int c) {  // c >= i
+    return i * KP - (i * (i - 1)) / 2 + (c - i);
+}
+
+template <int KP>
+__global__ void __launch_bounds__(256) k_chol_blk(const double *__restrict__ G, int k, double shift,
+                                                  double *__restrict__ R64, double *__restrict__ W64,
+                                                  float *__restrict__ W32, int *__restrict__ status) {
+    constexpr int NP = KP * (KP + 1) / 2;
+    __shared__ double a[NP];
+    __shared__ double w[NP];
+    __shared__ int bad;
+    __shared__ double tr;
+    const int tid = threadIdx.x;
+    if (tid == 0) bad = 0;
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, c = t % KP;
+        if (c >= i) a[upk<KP>(i, c)] = G[t];
+    }
+    __syncthreads();
+    if (shift > 0.0) {
+        if (tid == 0) {
+            double s = 0.0;
+            for (int i = 0; i < k; ++i) s += a[upk<KP>(i, i)];
+            tr = s;
+        }
+        __syncthreads();
+        for (int i = tid; i < k; i += blockDim.x) a[upk<KP>(i, i)] += shift * tr;
+        __syncthreads();
+    }
+    for (int j = 0; j < k; ++j) {
+        const double d = a[upk<KP>(j, j)];
+        const bool ok = (d > 0.0) && isfinite(d);
+        const double rjj = ok ? sqrt(d) : 1.0;
+        __syncthreads();  // everyone read the pivot before it is overwritten
+        if (tid == 0) { a[upk<KP>(j, j)] = rjj; if (!ok) bad = 1; }
+        for (int c = j + 1 + tid; c < k; c += blockDim.x) a[upk<KP>(j, c)] /= rjj;
+        __syncthreads();
+        const int m = k - j - 1;
+        for (int t = tid; t < m * m; t += blockDim.x) {
+            const int i = j + 1 + t / m, c = j + 1 + t % m;
+            if (c >= i) a[upk<KP>(i, c)] -= a[upk<KP>(j, i)] * a[upk<KP>(j, c)];
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, c = t % KP;
+        R64[t] = (i < k && c < k && c >= i) ? a[upk<KP>(i, c)] : 0.0;
+    }
+    for (int t = tid; t < NP; t += blockDim.x) w[t] = 0.0;
+    __syncthreads();
+    const int q = tid & 3, cc = tid >> 2;  // 64 columns x 4 parts per pass
+    for (int i = k - 1; i >= 0; --i) {
+        const double inv = 1.0 / a[upk<KP>(i, i)];
+        for (int c0 = 0; c0 < KP; c0 += 64) {
+            const int c = c0 + cc;
+            double part = 0.0;
+            if (c > i && c < k)
+                for (int l = i + 1 + q; l <= c; l += 4) part += a[upk<KP>(i, l)] * w[upk<KP>(l, c)];
+            part += __shfl_xor(part, 1);
+            part += __shfl_xor(part, 2);
+            if (q == 0 && c < k) {
+                if (c > i) w[upk<KP>(i, c)] = -part * inv;
+                else if (c == i) w[upk<KP>(i, c)] = inv;
+            }
+        }
+        __syncthreads();
+    }
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, c = t % KP;
+        const double v = (i < k && c < k && c >= i) ? w[upk<KP>(i, c)] : 0.0;
+        W64[t] = v;
+        W32[t] = static_cast<float>(v);
+    }
+    if (tid == 0 && bad) atomicOr(status, 1);
+}
+
+// Column statistics on the tile-CSC with the transform batched in LDS: block
+// (tile range, 16 waves x 16 genes); per tile the chunk's entries are loaded
+// and transformed by all threads into LDS, then lane l of a gene's wave takes
+// run entries l, l+64, ... into register partial sums (f64) kept across the
+// tile range; one wave reduction per gene at the end (deterministic).
+__global__ void __launch_bounds__(1024) k_col_stats4(const int64_t *__restrict__ indptr,
+                                                     const uint32_t *__restrict__ tptr,
+                                                     const uint8_t *__restrict__ trow,
+                                                     const float *__restrict__ tval,
+                                                     const double *__restrict__ rowf, int flags, int nch,
+                                                     int tiles_per_tr, int n_tiles, int64_t n_obs,
+                                                     int64_t n_vars, double *__restrict__ psum,
+                                                     double *__restrict__ psq, uint32_t *__restrict__ pcnt) {
+    constexpr int GWS = 8, NW = 16, CGS = NW * GWS, ENT = 16384;
+    __shared__ float xs[ENT];
+    __shared__ double rf[kTileRows];
+    __shared__ uint32_t bnd[CGS + 1];
+    const int tr = blockIdx.x / nch, gc = blockIdx.x % nch;
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63;
+    const int64_t cb = static_cast<int64_t>(gc) * CGS;
+    const int64_t cw = cb + wave * GWS;
+    const int64_t cend = (cb + CGS) < n_vars ? (cb + CGS) : n_vars;
+    double s[GWS], q[GWS];
+    uint32_t n[GWS];
+#pragma unroll
+    for (int gi = 0; gi < GWS; ++gi) { s[gi] = 0.0; q[gi] = 0.0; n[gi] = 0u; }
+    const int t_begin = tr * tiles_per_tr;
+    const int t_end = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
+    for (int t = t_begin; t < t_end; ++t) {
+        const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+        const int64_t tb = indptr[r0];
+        const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+        __syncthreads();
+        for (int i = threadIdx.x; i < kTileRows; i += blockDim.x) rf[i] = (rowf && r0 + i < n_obs) ? rowf[r0 + i] : 1.0;
+        for (int i = threadIdx.x; i <= CGS; i += blockDim.x) bnd[i] = (cb + i <= cend) ? tp[cb + i] : 0u;
+        __syncthreads();
+        const uint32_t e_lo = (cb < n_vars) ? bnd[0] : 0u, e_hi = (cb < n_vars) ? bnd[cend - cb] : 0u;
+        for (uint32_t w0 = e_lo; w0 < e_hi; w0 += ENT) {
+            const uint32_t w1 = (w0 + ENT) < e_hi ? (w0 + ENT) : e_hi;
+            for (uint32_t i = w0 + threadIdx.x; i < w1; i += blockDim.x)
+                xs[i - w0] = xtilde(tval[tb + i], rf[trow[tb + i]], flags);
+            __syncthreads();
+#pragma unroll
+            for (int gi = 0; gi < GWS; ++gi) {
+                const int64_t c = cw + gi;
+                if (c < cend) {
+                    uint32_t a = bnd[c - cb], b = bnd[c - cb + 1];
+                    if (w0 == e_lo) n[gi] += b - a;
+                    a = a > w0 ? a : w0;
+                    b = b < w1 ? b : w1;
+                    for (uint32_t i = a + lane; i < b; i += kWave) {
+                        const double x = static_cast<double>(xs[i - w0]);
+                        s[gi] += x;
+                        q[gi] += x * x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int gi = 0; gi < GWS; ++gi) {
+        const double ts = wave_sum(s[gi]), tq = wave_sum(q[gi]);
+        const int64_t c = cw + gi;
+        if (lane == 0 && c < cend) {
+            const int64_t o = static_cast<int64_t>(tr) * n_vars + c;
+            psum[o] = ts;
+            psq[o] = tq;
+            pcnt[o] = n[gi];
+        }
+    }
+}
+
 }  // namespace scxpca
+
+
+
+

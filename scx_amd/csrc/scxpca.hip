@@ -106,7 +106,8 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
     return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>(b, 1 << 30)));
 }
 
-constexpr int kCB = 256;       // gene width of the row split offsets
+constexpr int kCB = 128;       // gene width of the row split offsets
+constexpr int64_t kMaxSplitVars = 8192;  // split offsets (and the chunked forward) up to this many genes
 constexpr int kStatsCG = 2048; // genes per column-stats block
 constexpr int kMaxColBlocks = 1024;  // partial rows of the column-sum kernels
 
@@ -146,11 +147,15 @@ struct scxpca_matrix {
     int nsplit = 0;
     DevBuf xbf;
     bool xbf_ready = false;
+    // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
+    DevBuf tptr, trow, tval, txbf;
+    int n_tiles = 0;
+    bool use_split = false;
     // cached column statistics of X~
     bool stats_ready = false;
     DevBuf sum, sq, cnt, psum, psq, pcnt;
     // PCA workspace
-    DevBuf y, q, panel, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
+    DevBuf y, q, panel, panel2, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
     std::vector<double> host_omega;
     // stage profiler (events on the matrix stream; off unless enabled)
     bool profiling = false;
@@ -187,6 +192,29 @@ struct StageScope {
         if (idx >= 0) (void)hipEventRecord(m->marks[idx].b, s);
     }
 };
+
+// Derived per-matrix data: row factors, split offsets (small n_vars), the
+// tile-CSC copy. Runs at creation and on refresh; invalidates cached stats.
+void prepare(scxpca_matrix *m) {
+    m->stats_ready = false;
+    m->xbf_ready = false;
+    if (m->n_obs == 0) return;
+    StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
+    const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
+    if (m->flags & kXfNorm) {
+        k_row_factor<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, m->target, m->rowf.as<double>());
+        HIPC(hipGetLastError());
+    }
+    if (m->use_split) {
+        k_splits<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
+                                                    m->spl.as<uint32_t>());
+        HIPC(hipGetLastError());
+    }
+    k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs,
+                                                     static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
+                                                     m->trow.as<uint8_t>(), m->tval.as<float>());
+    HIPC(hipGetLastError());
+}
 
 void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1p) {
     if (x->n_obs < 0 || x->n_vars < 0) throw Error(SCXPCA_ERR_INVALID, "matrix shape must be non-negative");
@@ -261,18 +289,15 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
         if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
         if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
     }
-    const unsigned row_blocks = blocks_for(m->n_obs, 4);  // 4 waves per block, one row per wave
-    if (m->flags & kXfNorm) {
-        m->rowf.ensure(sizeof(double) * m->n_obs);
-        k_row_factor<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, target,
-                                                                         m->rowf.as<double>());
-        HIPC(hipGetLastError());
-    }
     m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
-    m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
-    k_splits<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
-                                                                  m->spl.as<uint32_t>());
-    HIPC(hipGetLastError());
+    m->use_split = m->n_vars <= kMaxSplitVars;
+    if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
+    if (m->use_split) m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
+    m->n_tiles = static_cast<int>((m->n_obs + kTileRows - 1) / kTileRows);
+    m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
+    m->trow.ensure(std::max<int64_t>(m->nnz, 1));
+    m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+    prepare(m);
     HIPC(hipStreamSynchronize(m->stream));
 }
 
@@ -281,32 +306,41 @@ const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m
 void ensure_xbf(scxpca_matrix *m) {
     if (m->xbf_ready) return;
     m->xbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
-    if (m->n_obs > 0)
+    m->txbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    if (m->n_obs > 0) {
         k_xtilde_bf16<<<std::min(blocks_for(m->n_obs, 4), 65536u), 256, 0, m->stream>>>(
             m->indptr, m->val, rowf_ptr(m), m->flags, m->n_obs, m->xbf.as<uint16_t>());
+        HIPC(hipGetLastError());
+        k_tcsc_bf16<<<m->n_tiles, 256, 0, m->stream>>>(m->indptr, m->trow.as<uint8_t>(), m->tval.as<float>(),
+                                                       rowf_ptr(m), m->flags, m->n_obs, m->txbf.as<uint16_t>());
+    }
     HIPC(hipGetLastError());
     m->xbf_ready = true;
 }
 
-// Column statistics of X~ (sum, sum_sq, nnz per gene), cached on the matrix.
+// Column statistics of X~ (sum, sum_sq, nnz per gene) from the tile-CSC,
+// cached on the matrix.
 void ensure_stats(scxpca_matrix *m) {
     if (m->stats_ready) return;
-    const int nch = static_cast<int>((m->n_vars + kStatsCG - 1) / kStatsCG);
-    const int nrb = std::max(1, std::min<int>(static_cast<int>((m->n_obs + 63) / 64), std::max(8, 1024 / nch)));
-    const int64_t rows_per_rb = (m->n_obs + nrb - 1) / nrb;
+    const int nch = static_cast<int>((m->n_vars + 127) / 128);  // k_col_stats4: 16 waves x 8 genes
+    const int ntr = std::max(1, std::min(m->n_tiles, std::max(1, 512 / nch)));
+    const int tiles_per_tr = (m->n_tiles + ntr - 1) / ntr;
     m->sum.ensure(sizeof(double) * m->n_vars);
     m->sq.ensure(sizeof(double) * m->n_vars);
     m->cnt.ensure(sizeof(uint32_t) * m->n_vars);
-    m->psum.ensure(sizeof(double) * nrb * m->n_vars);
-    m->psq.ensure(sizeof(double) * nrb * m->n_vars);
-    m->pcnt.ensure(sizeof(uint32_t) * nrb * m->n_vars);
-    StageScope prof(m, m->stream, SCXPCA_STAGE_STATS);
-    k_col_stats<kStatsCG><<<nrb * nch, 256, 0, m->stream>>>(
-        m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, m->spl.as<uint32_t>(), m->nsplit, kStatsCG / kCB, nch,
-        m->n_obs, m->n_vars, rows_per_rb, m->psum.as<double>(), m->psq.as<double>(), m->pcnt.as<uint32_t>());
+    m->psum.ensure(sizeof(double) * ntr * m->n_vars);
+    m->psq.ensure(sizeof(double) * ntr * m->n_vars);
+    m->pcnt.ensure(sizeof(uint32_t) * ntr * m->n_vars);
+    {
+        StageScope prof(m, m->stream, SCXPCA_STAGE_STATS);
+        k_col_stats4<<<ntr * nch, 1024, 0, m->stream>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+                                                       m->tval.as<float>(), rowf_ptr(m), m->flags, nch, tiles_per_tr,
+                                                       m->n_tiles, m->n_obs, m->n_vars, m->psum.as<double>(),
+                                                       m->psq.as<double>(), m->pcnt.as<uint32_t>());
+    }
     HIPC(hipGetLastError());
     k_col_stats_reduce<<<blocks_for(m->n_vars), 256, 0, m->stream>>>(m->psum.as<double>(), m->psq.as<double>(),
-                                                                     m->pcnt.as<uint32_t>(), nrb, m->n_vars,
+                                                                     m->pcnt.as<uint32_t>(), ntr, m->n_vars,
                                                                      m->sum.as<double>(), m->sq.as<double>(),
                                                                      m->cnt.as<uint32_t>());
     HIPC(hipGetLastError());
@@ -324,7 +358,9 @@ __global__ void k_mean(const double *sum, int64_t n_vars, double inv_n, double *
 
 template <int KP>
 struct Engine {
-    static constexpr int CG = (KP == 64) ? 512 : 256;  // genes per k_spmm_t block: CG*KP*4 = 128 KB
+    static constexpr int GW = (KP == 64) ? 12 : 6;  // genes per wave in k_spmm_t2 (Z in registers)
+    static constexpr int CG = 16 * GW;              // genes per k_spmm_t2 block
+    static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd2
     scxpca_matrix *m;
     hipStream_t s;
     int k = 0;
@@ -332,20 +368,23 @@ struct Engine {
     double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *partc;
     float *W32, *mc32;
     int *status;
-    int nrb_t = 0, nch_t = 0, ncol_blk = 0, gram_waves = 0;
-    int64_t rows_per_rb_t = 0, rows_per_wave = 0, rows_per_colblk = 0;
+    int nrb_t = 0, nch_t = 0, ncol_blk = 0, gram_waves = 0, gram_blocks = 0, tiles_per_tr = 0;
+    int64_t rows_per_wave = 0, rows_per_colblk = 0, rows_per_gblk = 0;
 
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         nch_t = static_cast<int>((nv + CG - 1) / CG);
-        int nrb = std::max(1, 512 / nch_t);
-        nrb = std::min<int64_t>(nrb, std::max<int64_t>(1, (n + 31) / 32));
-        nrb_t = (nrb + 7) / 8 * 8;
-        rows_per_rb_t = (n + nrb_t - 1) / nrb_t;
+        int ntr = std::max(1, 512 / nch_t);
+        ntr = std::min(ntr, std::max(1, m->n_tiles));
+        nrb_t = (ntr + 7) / 8 * 8;
+        tiles_per_tr = (m->n_tiles + nrb_t - 1) / nrb_t;
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
         rows_per_wave += rows_per_wave & 1;
+        gram_blocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, KP == 64 ? 1024 : 512)));
+        rows_per_gblk = (n + gram_blocks - 1) / gram_blocks;
+        rows_per_gblk = (rows_per_gblk + 15) / 16 * 16;
         ncol_blk = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((std::max(n, nv) + 1023) / 1024, 1024)));
         rows_per_colblk = (std::max(n, nv) + ncol_blk - 1) / ncol_blk;
 
@@ -353,13 +392,14 @@ struct Engine {
         m->y.ensure(sizeof(float) * panel_rows * KP);
         m->q.ensure(sizeof(float) * panel_rows * KP);
         m->panel.ensure(sizeof(float) * std::max<int64_t>(nv, 1) * KP);
+        m->panel2.ensure(sizeof(float) * std::max<int64_t>(nv, 1) * KP);
         m->panel64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->b64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64a.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64b.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->tpart.ensure(sizeof(float) * static_cast<size_t>(nrb_t) * std::max<int64_t>(nv, 1) * KP);
         const int nb64 = gram64_blocks();
-        m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(gram_waves, nb64)) * KP * KP);
+        m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(std::max(gram_waves, gram_blocks), nb64)) * KP * KP);
         m->cpart.ensure(sizeof(double) * static_cast<size_t>(kMaxColBlocks) * KP);
         constexpr size_t KK = static_cast<size_t>(KP) * KP;
         m->small.ensure(sizeof(double) * (KK * 11 + 4 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + KP));
@@ -386,7 +426,7 @@ struct Engine {
         const int64_t rpb = (nv + nb - 1) / nb;
         k_colsum<KP, float><<<nb, 256, 0, s>>>(panel, mu, nv, rpb, m->cpart.as<double>());
         check();
-        k_colsum_reduce<KP><<<1, KP, 0, s>>>(m->cpart.as<double>(), nb, mc64, mc32);
+        k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, mc64, mc32);
         check();
     }
 
@@ -397,20 +437,36 @@ struct Engine {
         const int64_t rpb = (n + nb - 1) / nb;
         k_colsum<KP, float><<<nb, 256, 0, s>>>(Q, nullptr, n, rpb, m->cpart.as<double>());
         check();
-        k_colsum_reduce<KP><<<1, KP, 0, s>>>(m->cpart.as<double>(), nb, sumq, nullptr);
+        k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, sumq, nullptr);
         check();
     }
 
     void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
         if (m->n_obs == 0) return;
-        const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
-        if (bf16)
-            k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
-                                                      m->flags, panel, centered ? mc32 : nullptr, Y, m->n_obs);
-        else
-            k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
-                                                       panel, centered ? mc32 : nullptr, Y, m->n_obs);
+        const float *mcp = centered ? mc32 : nullptr;
+        if (m->use_split) {
+            constexpr int NWF = 16;
+            constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
+            const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
+            const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
+            if (bf16)
+                k_spmm_fwd2<KP, true, FR, NWF, CGF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr, m->flags, panel, mcp, Y,
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars);
+            else
+                k_spmm_fwd2<KP, false, FR, NWF, CGF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags, panel, mcp, Y, m->spl.as<uint32_t>(),
+                    m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars);
+        } else {
+            const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
+            if (bf16)
+                k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
+                                                          m->flags, panel, mcp, Y, m->n_obs);
+            else
+                k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
+                                                           panel, mcp, Y, m->n_obs);
+        }
         check();
     }
 
@@ -422,14 +478,15 @@ struct Engine {
             const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
             if (bf16)
-                k_spmm_t<KP, CG, true><<<grid, 512, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
-                                                            m->flags, Q, m->spl.as<uint32_t>(), m->nsplit, CG / kCB,
-                                                            nch_t, nrb_t, m->n_obs, nv, rows_per_rb_t,
-                                                            m->tpart.as<float>());
+                k_spmm_t2<KP, GW, true><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+                                                              m->tval.as<float>(), m->txbf.as<uint16_t>(), nullptr,
+                                                              m->flags, Q, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
+                                                              m->n_tiles, m->tpart.as<float>());
             else
-                k_spmm_t<KP, CG, false><<<grid, 512, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
-                                                             Q, m->spl.as<uint32_t>(), m->nsplit, CG / kCB, nch_t,
-                                                             nrb_t, m->n_obs, nv, rows_per_rb_t, m->tpart.as<float>());
+                k_spmm_t2<KP, GW, false><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+                                                               m->tval.as<float>(), nullptr, rowf_ptr(m), m->flags, Q,
+                                                               nch_t, nrb_t, tiles_per_tr, m->n_obs, nv, m->n_tiles,
+                                                               m->tpart.as<float>());
             check();
         } else {
             HIPC(hipMemsetAsync(m->tpart.p, 0, sizeof(float) * nrb_t * nv * KP, s));
@@ -442,29 +499,33 @@ struct Engine {
 
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
     void chol(double shift, double *Rdst) {
-        k_chol<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+        StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
+        k_chol_blk<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
         check();
     }
 
     // Tall f32 panel CholeskyQR: CQR2, or shifted CQR3 (stable) for Householder mode.
     // Returns the buffer that holds Q (one of src / tmp).
-    float *cqr_tall(float *src, float *tmp, int qr_method) {
-        const int64_t n = m->n_obs;
+    float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1) {
+        if (n < 0) n = m->n_obs;
         const int passes = qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2;
         const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
+        const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, gram_blocks)));
+        int64_t rpg = (n + gblocks - 1) / gblocks;
+        rpg = (rpg + 15) / 16 * 16;
         for (int pass = 0; pass < passes; ++pass) {
             {
                 StageScope prof(m, s, SCXPCA_STAGE_GRAM);
-                k_gram<KP><<<gram_waves / 4, 256, 0, s>>>(src, n, rows_per_wave, m->gpart.as<double>());
+                k_gram2<KP><<<gblocks, 256, 0, s>>>(src, n, rpg, m->gpart.as<double>());
             }
             check();
-            k_gram_reduce<KP><<<blocks_for(KP * KP), 256, 0, s>>>(m->gpart.as<double>(), gram_waves, G);
+            k_gram_reduce<KP><<<(KP * KP + 63) / 64, 256, 0, s>>>(m->gpart.as<double>(), gblocks, G);
             check();
             const double shift = (qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0;
             chol(shift, R1);
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
-                k_apply<KP><<<apply_grid, 256, 0, s>>>(src, W32, tmp, n, KP, KP);
+                k_apply2<KP><<<apply_grid, 256, 0, s>>>(src, W32, tmp, n, KP, KP);
             }
             check();
             std::swap(src, tmp);
@@ -481,6 +542,7 @@ struct Engine {
         const double *src = A;
         double *bufs[3] = {m->t64a.as<double>(), m->t64b.as<double>(), Qout};
         double *Rs[3] = {R1, R2, R3};
+        StageScope prof(m, s, SCXPCA_STAGE_SMALL);
         for (int pass = 0; pass < 3; ++pass) {
             k_gram64<KP><<<nb, 256, 0, s>>>(src, nv, rpb, m->gpart.as<double>());
             check();
@@ -518,12 +580,11 @@ struct Engine {
         float *Q = cqr_tall(Y, T, p->qr_method);
         float *Qother = (Q == Y) ? T : Y;
         for (int64_t it = 0; it < p->n_power_iterations; ++it) {
-            spmm_t(Q, centered, sketch_bf16, m->b64.as<double>(), nullptr);
-            cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
-            k_f64_to_f32<<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), m->panel.as<float>(), nv * KP);
-            check();
-            if (centered) mean_correction(m->panel.as<float>());
-            spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Qother);
+            // B (f32 n_vars x KP) -> orthonormal panel by the tall MFMA CholeskyQR
+            spmm_t(Q, centered, sketch_bf16, nullptr, m->panel.as<float>());
+            float *qb = cqr_tall(m->panel.as<float>(), m->panel2.as<float>(), p->qr_method, nv);
+            if (centered) mean_correction(qb);
+            spmm_fwd(qb, centered, sketch_bf16, Qother);
             float *Qn = cqr_tall(Qother, Q, p->qr_method);
             Qother = (Qn == Q) ? Qother : Q;
             Q = Qn;
@@ -532,7 +593,10 @@ struct Engine {
         spmm_t(Q, centered, false, m->b64.as<double>(), nullptr);
         // Rayleigh-Ritz: B = Q_B Rt, Rt = U_R S V^T
         cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
-        k_jacobi_svd<KP><<<1, 256, 0, s>>>(Rt, k, Acol, Vcol, U, V, sig, status);
+        {
+            StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
+            k_jacobi_lds<KP><<<1, 256, 0, s>>>(Rt, k, Vcol, U, V, sig, status);
+        }
         check();
         // Uhat = Q_B U_R (n_vars x KP, f64) -> components
         k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), U, m->t64a.as<double>(), nullptr, nv);
@@ -556,7 +620,7 @@ struct Engine {
             emb_dev = Qother;  // n_obs x KP scratch, embeddings written with ld = nc
         }
         const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
-        k_apply<KP><<<apply_grid, 256, 0, s>>>(Q, W32, emb_dev, n, nc, nc);
+        k_apply2<KP><<<apply_grid, 256, 0, s>>>(Q, W32, emb_dev, n, nc, nc);
         check();
         // results back
         std::vector<double> hsig(KP), hsum(nv), hsq(nv);
@@ -798,20 +862,7 @@ void project(scxpca_matrix *m, bool backward, const float *in, int64_t k, const 
     if (!user) HIPC(hipStreamSynchronize(s));
 }
 
-void refresh(scxpca_matrix *m) {
-    m->stats_ready = false;
-    m->xbf_ready = false;
-    if (m->n_obs == 0) return;
-    const unsigned row_blocks = blocks_for(m->n_obs, 4);
-    if (m->flags & kXfNorm) {
-        k_row_factor<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, m->target,
-                                                                         m->rowf.as<double>());
-        HIPC(hipGetLastError());
-    }
-    k_splits<<<std::min(row_blocks, 65536u), 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
-                                                                  m->spl.as<uint32_t>());
-    HIPC(hipGetLastError());
-}
+void refresh(scxpca_matrix *m) { prepare(m); }
 }  // namespace
 
 // ---------------------------------------------------------------------------
