@@ -10,8 +10,12 @@ embeddings and components written to HBM.
 Prints one JSON line. `value` is wall seconds per rPCA (lower is better);
 `roofline` reports the dominant SpMM kernel's algorithmic HBM bytes per launch
 divided by its average launch time (HIP events on the library stream);
-`cpu_baseline` times the C/OpenMP restatement of the reference CPU path on a
-bounded row sample of the same matrix and scales it linearly to the full rows.
+`roofline.traffic` is the measured HBM bytes per launch of that kernel, read
+from profiles/spmm_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by
+the same counters on a plain copy of known size), when that file names the
+kernel. `cpu_baseline` times the C/OpenMP restatement of the reference CPU path
+(the oracle) on the host cores, on the full matrix by default (--cpu-rows N
+takes the first N rows and scales linearly).
 """
 from __future__ import annotations
 
@@ -45,7 +49,11 @@ def main():
     ap.add_argument("--workload", default="pca_1m_x_2k_k50")
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--qr", choices=["cholesky", "householder"], default="cholesky")
-    ap.add_argument("--cpu-rows", type=int, default=200_000, help="row sample for the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-rows", type=int, default=-1,
+                    help="rows for the CPU baseline: -1 all rows, 0 skip, N the first N rows")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="before the timed region, copy MIB MiB device-to-device once (a plain copy "
+                         "of known size whose FETCH_SIZE/WRITE_SIZE calibrate the traffic counters)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -80,6 +88,15 @@ def main():
     kw = dict(n_comps=w.n_comps, n_oversamples=w.n_oversamples, n_power_iterations=w.n_power_iterations,
               seed=0, qr_method=qr, precision=prec, out={"embeddings": emb, "components": comps})
 
+    if args.copy_calibration > 0:
+        nb = args.copy_calibration << 20
+        src = torch.ones(nb // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"# copy calibration: {nb} bytes read, {nb} bytes written", file=sys.stderr, flush=True)
+        del src, dst
+
     def step():
         m.refresh()
         return m.pca(**kw)
@@ -105,11 +122,19 @@ def main():
     bytes_per = spmm_bytes(x.nnz, n, nv, k, bf16=dom.endswith("bf16"))
     achieved = bytes_per / (dom_ms * 1e-3) / 1e9
 
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "spmm_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as f:
+            rec = json.load(f).get(f"{w.name}:{args.precision}:{dom}")
+        if rec:
+            traffic = rec["traffic_bytes"]
+
     cpu = None
-    if args.cpu_rows > 0:
+    if args.cpu_rows != 0:
         import oracle
 
-        rows = min(args.cpu_rows, n)
+        rows = n if args.cpu_rows < 0 else min(args.cpu_rows, n)
         xs = x.head(rows)
         cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
         cores = min(cores, 16)
@@ -119,9 +144,10 @@ def main():
                               target_sum=w.target_sum, log1p=True, nthreads=cores)
         tcpu = time.perf_counter() - tc
         cpu = {"value": round(tcpu * n / rows, 3), "unit": "s", "cores": cores, "kind": "port",
-               "sample": f"first {rows:,} of {n:,} rows of the same matrix, one run of the C/OpenMP "
-                         f"restatement of randomized_pca_inmemory (f64), {tcpu:.2f}s, scaled linearly to "
-                         f"{n:,} rows"}
+               "sample": (f"the full {n:,}-row matrix" if rows == n else
+                          f"first {rows:,} of {n:,} rows of the same matrix, {tcpu:.2f}s, scaled linearly")
+                         + f"; one run of the C/OpenMP restatement of randomized_pca_inmemory (f64), "
+                           f"{cores} threads"}
 
     stage_ms = {s: round(v[0] / args.steps, 4) for s, v in prof.items() if v[1]}
     out = {
@@ -151,7 +177,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "avg_launch_ms": round(dom_ms, 4),
             "bytes_per_launch": bytes_per,
         },

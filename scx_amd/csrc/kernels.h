@@ -3,15 +3,15 @@
 // Data layout in HBM:
 //   CSR          indptr i64[n_obs+1], indices u32[nnz], values f32[nnz] (raw counts)
 //   rowf         f64[n_obs]   normalize_total factor target/rowsum (1 where rowsum <= 0)
-//   xbf          bf16[nnz]    X~ materialized once for the BF16 sketch form
+//   xt           f32[nnz]     X~ = log1p(f32(x * rowf)) in CSR order, computed once per
+//                             prepared matrix; xbf = bf16(xt) for the BF16 sketch form
 //   spl          u32[n_obs][nsplit+1]  offset of the first nonzero of each CB-gene
 //                chunk inside each row (sorted rows)
+//   tile-CSC     per 256-row tile, gene-major runs: tptr u32[n_tiles][n_vars+1],
+//                trow u8[nnz] (row in tile), tval f32[nnz] (X~), txbf bf16[nnz]
 //   panels       f32 row-major, KP columns (k' padded to 32/64/128, pad columns 0):
 //                M (n_vars x KP), Y and Q (n_obs x KP)
 //   B            f64 n_vars x KP
-//
-// X~ = log1p(f32(x * rowf)) is computed inside every SpMM from the raw value and
-// the row factor; it is never stored in the FP32 form.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -156,19 +156,62 @@ __global__ void __launch_bounds__(256) k_splits(const int64_t *__restrict__ indp
     }
 }
 
-// X~ as bf16 (RNE), materialized once for the sketch form.
-__global__ void __launch_bounds__(256) k_xtilde_bf16(const int64_t *__restrict__ indptr,
-                                                     const float *__restrict__ val,
-                                                     const double *__restrict__ rowf, int flags,
-                                                     int64_t n_obs, uint16_t *__restrict__ out) {
+// X~ values in CSR order, one f32 per stored entry, computed once per
+// prepared matrix (every SpMM pass and the tile-CSC read these instead of
+// re-applying the transform per pass), and the forward SpMM's gene index
+// inside its LDS panel chunk, lg = gene mod kChunkMod (u16; chunks of 512 or
+// 256 genes start at multiples of their size, so lg mod 256 serves both).
+constexpr uint32_t kChunkMod = 512;
+__global__ void __launch_bounds__(256) k_xtilde(const int64_t *__restrict__ indptr,
+                                                const uint32_t *__restrict__ idx,
+                                                const float *__restrict__ val,
+                                                const double *__restrict__ rowf, int flags,
+                                                int64_t n_obs, float *__restrict__ out,
+                                                uint16_t *__restrict__ lg) {
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
          r < n_obs; r += wstride) {
         const int64_t a = indptr[r], b = indptr[r + 1];
         const double f = rowf ? rowf[r] : 1.0;
-        for (int64_t p = a + lane; p < b; p += kWave) out[p] = f32_to_bf16_rne(xtilde(val[p], f, flags));
+        for (int64_t p = a + lane; p < b; p += kWave) {
+            out[p] = xtilde(val[p], f, flags);
+            lg[p] = static_cast<uint16_t>(idx[p] & (kChunkMod - 1));
+        }
     }
+}
+
+// Stored entries per gene (column), for the load-balanced gene order of the
+// transposed SpMM. Block-private LDS histogram when n_vars fits, else global
+// atomics; counts are integers, so the result does not depend on the order.
+__global__ void __launch_bounds__(256) k_gene_counts(const uint32_t *__restrict__ idx, int64_t nnz, int n_vars,
+                                                     uint32_t *__restrict__ cnt) {
+    constexpr int kMaxLds = 16384;
+    __shared__ uint32_t h[kMaxLds];
+    const bool local = n_vars <= kMaxLds;
+    if (local)
+        for (int i = threadIdx.x; i < n_vars; i += blockDim.x) h[i] = 0u;
+    __syncthreads();
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += stride) {
+        const uint32_t c = idx[p];
+        if (local)
+            atomicAdd(&h[c], 1u);
+        else
+            atomicAdd(&cnt[c], 1u);
+    }
+    __syncthreads();
+    if (local)
+        for (int i = threadIdx.x; i < n_vars; i += blockDim.x)
+            if (h[i]) atomicAdd(&cnt[i], h[i]);
+}
+
+// bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
+__global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
+                                                 uint16_t *__restrict__ out) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < n; p += stride)
+        out[p] = f32_to_bf16_rne(src[p]);
 }
 
 // Column statistics partials -> totals, fixed order over row blocks.
@@ -197,8 +240,7 @@ __global__ void __launch_bounds__(256) k_col_stats_reduce(const double *__restri
 // guard (cpu.rs:1130-1171): per-block f64 partials, fixed-order reduction.
 __global__ void __launch_bounds__(256) k_centered_ss(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ idx,
-                                                     const float *__restrict__ val,
-                                                     const double *__restrict__ rowf, int flags,
+                                                     const float *__restrict__ xt,
                                                      const double *__restrict__ mu, int64_t n_obs,
                                                      double *__restrict__ part) {
     __shared__ double red[256];
@@ -208,9 +250,8 @@ __global__ void __launch_bounds__(256) k_centered_ss(const int64_t *__restrict__
     for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
          r < n_obs; r += wstride) {
         const int64_t a = indptr[r], b = indptr[r + 1];
-        const double f = rowf ? rowf[r] : 1.0;
         for (int64_t p = a + lane; p < b; p += kWave) {
-            const double d = static_cast<double>(xtilde(val[p], f, flags)) - mu[idx[p]];
+            const double d = static_cast<double>(xt[p]) - mu[idx[p]];
             acc += d * d;
         }
     }
@@ -283,9 +324,8 @@ __global__ void __launch_bounds__(256) k_colsum_reduce(const double *__restrict_
 template <int KP, bool BF16>
 __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ idx,
-                                                  const float *__restrict__ val,
+                                                  const float *__restrict__ xt,
                                                   const uint16_t *__restrict__ xbf,
-                                                  const double *__restrict__ rowf, int flags,
                                                   const float *__restrict__ M, const float *__restrict__ mc,
                                                   float *__restrict__ Y, int64_t n_obs) {
     constexpr int E = KP / 16;
@@ -294,7 +334,6 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * 4;
     for (int64_t r = static_cast<int64_t>(blockIdx.x) * 4 + wave; r < n_obs; r += wstride) {
         const int64_t a = indptr[r], b = indptr[r + 1];
-        const double f = (!BF16 && rowf) ? rowf[r] : 1.0;
         float acc[E];
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[e] = 0.f;
@@ -304,7 +343,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
             float x = 0.f;
             if (lane < cnt) {
                 c = idx[base + lane];
-                x = BF16 ? bf16_to_f32(xbf[base + lane]) : xtilde(val[base + lane], f, flags);
+                x = BF16 ? bf16_to_f32(xbf[base + lane]) : xt[base + lane];
             }
             stage[wave][lane] = make_uint2(c, __float_as_uint(x));
             wave_lds_sync();
@@ -536,28 +575,47 @@ __device__ __forceinline__ float row_bcast(float v) {
 // Group-gather loop. The four 16-lane groups take the entries held by lane T
 // of their row (entries 4T+g of the batch, by the interleaved load), read E
 // floats of an LDS panel row at byte offset ofs + lane_ofs and accumulate
-// x * panel. The LDS reads run two steps ahead of the FMAs (three rotating
-// buffers), and the step count is wave-uniform so the exits are scalar
-// branches. Slots past cnt carry x = 0 and offset 0 (a valid address), so the
-// look-ahead reads are unconditional: up to two reads past the last step.
-template <int T, int E>
-__device__ __forceinline__ void gather_load(float4 (&m)[E / 4], float &xb, uint32_t ofs, float x, const char *lds,
-                                            uint32_t lane_ofs) {
-    const uint32_t a = row_bcast<T>(ofs) + lane_ofs;
-#pragma unroll
-    for (int e = 0; e < E / 4; ++e) m[e] = *reinterpret_cast<const float4 *>(lds + a + 16 * e);
-    xb = row_bcast<T>(x);
+// x * panel. The broadcast of lane T (DPP row_newbcast) is folded into the
+// consumers: one v_add_u32_dpp forms the LDS address and each FMA takes the
+// broadcast x as its DPP source, so a step costs E/4 LDS reads, E FMAs and
+// one add. The s_nop 1 in each block covers the VALU-write -> DPP-read hazard
+// (two wait states) for operands the compiler may have just moved. The LDS
+// reads run two steps ahead of the FMAs (three rotating buffers), and the step
+// count is wave-uniform so the exits are scalar branches. Slots past cnt carry
+// x = 0 and offset 0 (a valid address), so the look-ahead reads are
+// unconditional: up to two reads past the last step.
+template <int T>
+__device__ __forceinline__ uint32_t bcast_add(uint32_t ofs, uint32_t lane_ofs) {
+    uint32_t a;
+    asm("s_nop 1\n\tv_add_u32_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "=v"(a)
+        : "v"(ofs), "v"(lane_ofs), "i"(T));
+    return a;
 }
 
-template <int E>
-__device__ __forceinline__ void gather_fma(float (&acc)[E], const float4 (&m)[E / 4], float xb) {
+template <int T>
+__device__ __forceinline__ void bcast_fma4(float &a0, float &a1, float &a2, float &a3, float x, const float4 &m) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %4, %5 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %1, %4, %6 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %2, %4, %7 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1\n\t"
+        "v_fmac_f32_dpp %3, %4, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3)
+        : "v"(x), "v"(m.x), "v"(m.y), "v"(m.z), "v"(m.w), "i"(T));
+}
+
+template <int T, int E>
+__device__ __forceinline__ void gather_load(float4 (&m)[E / 4], uint32_t ofs, const char *lds, uint32_t lane_ofs) {
+    const uint32_t a = bcast_add<T>(ofs, lane_ofs);
 #pragma unroll
-    for (int e = 0; e < E / 4; ++e) {
-        acc[4 * e + 0] = __builtin_fmaf(xb, m[e].x, acc[4 * e + 0]);
-        acc[4 * e + 1] = __builtin_fmaf(xb, m[e].y, acc[4 * e + 1]);
-        acc[4 * e + 2] = __builtin_fmaf(xb, m[e].z, acc[4 * e + 2]);
-        acc[4 * e + 3] = __builtin_fmaf(xb, m[e].w, acc[4 * e + 3]);
-    }
+    for (int e = 0; e < E / 4; ++e) m[e] = *reinterpret_cast<const float4 *>(lds + a + 16 * e);
+}
+
+template <int T, int E>
+__device__ __forceinline__ void gather_fma(float (&acc)[E], const float4 (&m)[E / 4], float x) {
+#pragma unroll
+    for (int e = 0; e < E / 4; ++e)
+        bcast_fma4<T>(acc[4 * e + 0], acc[4 * e + 1], acc[4 * e + 2], acc[4 * e + 3], x, m[e]);
 }
 
 template <int E>
@@ -565,45 +623,49 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
                                              uint32_t lane_ofs) {
     const int steps = __builtin_amdgcn_readfirstlane((cnt + 3) >> 2);
     float4 b0[E / 4], b1[E / 4], b2[E / 4];
-    float x0 = 0.f, x1 = 0.f, x2 = 0.f;
-    gather_load<0, E>(b0, x0, ofs, x, lds, lane_ofs);
-    gather_load<1, E>(b1, x1, ofs, x, lds, lane_ofs);
-#define SCX_STEP(T, CUR, XC, NXT, XN)                                                   \
-    if ((T) >= steps) return;                                                           \
-    if ((T) + 2 < 16) gather_load<((T) + 2) & 15, E>(NXT, XN, ofs, x, lds, lane_ofs);   \
-    gather_fma<E>(acc, CUR, XC);
-    SCX_STEP(0, b0, x0, b2, x2) SCX_STEP(1, b1, x1, b0, x0) SCX_STEP(2, b2, x2, b1, x1)
-    SCX_STEP(3, b0, x0, b2, x2) SCX_STEP(4, b1, x1, b0, x0) SCX_STEP(5, b2, x2, b1, x1)
-    SCX_STEP(6, b0, x0, b2, x2) SCX_STEP(7, b1, x1, b0, x0) SCX_STEP(8, b2, x2, b1, x1)
-    SCX_STEP(9, b0, x0, b2, x2) SCX_STEP(10, b1, x1, b0, x0) SCX_STEP(11, b2, x2, b1, x1)
-    SCX_STEP(12, b0, x0, b2, x2) SCX_STEP(13, b1, x1, b0, x0) SCX_STEP(14, b2, x2, b1, x1)
-    SCX_STEP(15, b0, x0, b2, x2)
+    gather_load<0, E>(b0, ofs, lds, lane_ofs);
+    gather_load<1, E>(b1, ofs, lds, lane_ofs);
+#define SCX_STEP(T, CUR, NXT)                                                  \
+    if ((T) >= steps) return;                                                  \
+    if ((T) + 2 < 16) gather_load<((T) + 2) & 15, E>(NXT, ofs, lds, lane_ofs); \
+    gather_fma<T, E>(acc, CUR, x);
+    SCX_STEP(0, b0, b2) SCX_STEP(1, b1, b0) SCX_STEP(2, b2, b1) SCX_STEP(3, b0, b2)
+    SCX_STEP(4, b1, b0) SCX_STEP(5, b2, b1) SCX_STEP(6, b0, b2) SCX_STEP(7, b1, b0)
+    SCX_STEP(8, b2, b1) SCX_STEP(9, b0, b2) SCX_STEP(10, b1, b0) SCX_STEP(11, b2, b1)
+    SCX_STEP(12, b0, b2) SCX_STEP(13, b1, b0) SCX_STEP(14, b2, b1) SCX_STEP(15, b0, b2)
 #undef SCX_STEP
 }
 
 // ---------------------------------------------------------------------------
 // Forward SpMM, gene-chunked panel in LDS: Y[r][:] = sum x~ M[c][:] - mc.
-// A block (8 waves) owns 8*R consecutive rows; each wave keeps R rows' partial
-// outputs in registers (per 16-lane group, E columns per lane). For each chunk
-// of CG genes the block stages M[chunk] (CG*KP*4 = 64 KB) into LDS, then every
-// wave walks its rows' windows of that chunk (precomputed split offsets), 64
-// nonzeros per batch: lane-parallel load + transform, then 16 broadcast steps
-// that feed the four groups with entries 4t+g. After the last chunk the four
-// group partials of a row are summed and the row is written once.
+// A block (NW waves) owns NW*R consecutive rows; each wave keeps R rows'
+// partial outputs in registers (per 16-lane group, E columns per lane). For
+// each chunk of CG genes the block stages M[chunk] (CG*KP*4 = 128 KB) into
+// LDS, then every wave walks its rows' windows of that chunk (precomputed
+// split offsets), 64 nonzeros per batch: lane-parallel load + transform, then
+// broadcast steps that feed the four groups with entries 4t+g. The first batch
+// of each (row, chunk) window is loaded PF rows ahead, across chunk
+// boundaries, so HBM latency overlaps the gather loop. After the last chunk
+// the four group partials of a row are summed and the row is written once.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool BF16, int R, int NW, int CG>
-__global__ void __launch_bounds__(NW * 64) k_spmm_fwd2(const int64_t *__restrict__ indptr,
-                                                       const uint32_t *__restrict__ idx,
-                                                       const float *__restrict__ val,
+template <int KP, bool BF16, int R, int NW, int CG, int PF>
+__global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict__ indptr,
+                                                       const uint16_t *__restrict__ lg,
+                                                       const float *__restrict__ xt,
                                                        const uint16_t *__restrict__ xbf,
-                                                       const double *__restrict__ rowf, int flags,
                                                        const float *__restrict__ M, const float *__restrict__ mc,
                                                        float *__restrict__ Y, const uint32_t *__restrict__ spl,
                                                        int nsplit, int cpc, int nch, int64_t n_obs,
-                                                       int64_t n_vars) {
+                                                       int64_t n_vars, int64_t nnz) {
     constexpr int E = KP / 16;
     static_assert(R <= 16, "row metadata is held one row per lane of a 16-lane group");
+    static_assert(R % PF == 0, "ring slots are static: slot = row % PF in every chunk");
+    static_assert((CG & (CG - 1)) == 0 && CG <= static_cast<int>(kChunkMod), "lg holds gene mod kChunkMod");
+    // Entry loads are unconditional at an index clamped into [0, max(nnz,1)),
+    // with validity applied when the batch is consumed: a load under a
+    // data-dependent branch would make every later wait drain all loads.
+    const int64_t p_max = nnz > 0 ? nnz - 1 : 0;
     __shared__ float4 ms[CG * KP / 4];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
     const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * NW + wave) * R;
@@ -614,73 +676,94 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd2(const int64_t *__restrict
     for (int r = 0; r < R; ++r)
 #pragma unroll
         for (int e = 0; e < E; ++e) acc[r][e] = 0.f;
-    // per-row metadata, lane l of every group holds row row0 + l
+    // per-row metadata, lane l of every group holds row row0 + l; read out
+    // with v_readlane (scalar registers), not through LDS
     const int64_t mrow = row0 + l;
     const bool mvalid = (l < R) && (mrow < n_obs);
-    const int64_t rbase = mvalid ? indptr[mrow] : 0;
-    const double rfac = (!BF16 && rowf && mvalid) ? rowf[mrow] : 1.0;
+    const int64_t mrow_c = mrow < n_obs ? mrow : n_obs - 1;
+    const int64_t rbase = indptr[mrow_c];
+    const uint32_t *splr = spl + mrow_c * (nsplit + 1);
+    auto bounds = [&](int ch, uint32_t &a, uint32_t &b) {
+        const int s_lo = ch * cpc, s_hi = (ch + 1) * cpc < nsplit ? (ch + 1) * cpc : nsplit;
+        const uint32_t a0 = splr[s_lo], b0 = splr[s_hi];
+        a = mvalid ? a0 : 0u;
+        b = mvalid ? b0 : 0u;
+    };
+    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
+    auto rl64 = [](int64_t v, int r) {
+        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffll), r));
+        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), r));
+        return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
+    };
+    // First 64-entry batch of each (row, chunk) window, PF rows ahead; the
+    // ring runs on across chunk boundaries (row r of chunk ch is slot r % PF).
+    uint32_t pidx[PF];
+    float pval[PF];
+    auto issue = [&](int r, uint32_t wl, uint32_t wh, uint32_t &pi, float &pv) {
+        const int64_t base_r = rl64(rbase, r);
+        const uint32_t lo_r = rl32(wl, r);
+        int64_t p = base_r + lo_r + esel;
+        p = p < p_max ? p : p_max;
+        pi = lg[p];
+        pv = BF16 ? bf16_to_f32(xbf[p]) : xt[p];
+    };
     const float4 *M4 = reinterpret_cast<const float4 *>(M);
+    uint32_t wlo, whi, nlo = 0u, nhi = 0u;
+    bounds(0, wlo, whi);
+#pragma unroll
+    for (int r = 0; r < PF; ++r) issue(r, wlo, whi, pidx[r], pval[r]);
     for (int ch = 0; ch < nch; ++ch) {
         const int64_t g0 = static_cast<int64_t>(ch) * CG;
-        const int s_lo = ch * cpc, s_hi = (ch + 1) * cpc < nsplit ? (ch + 1) * cpc : nsplit;
-        // window bounds of all R rows for this chunk, one row per lane
-        const uint32_t wlo = mvalid ? spl[mrow * (nsplit + 1) + s_lo] : 0u;
-        const uint32_t whi = mvalid ? spl[mrow * (nsplit + 1) + s_hi] : 0u;
+        bounds(ch + 1 < nch ? ch + 1 : ch, nlo, nhi);  // unconditional: see the note on loads
         __syncthreads();
         for (int i = threadIdx.x; i < CG * KP / 4; i += blockDim.x) {
             const int64_t gene = g0 + i / (KP / 4);
             ms[i] = gene < n_vars ? M4[gene * (KP / 4) + i % (KP / 4)] : make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
-        // First 64-entry batch of each row's window, loaded PF rows ahead.
-        constexpr int PF = 4;
-        uint32_t pidx[PF];
-        float pval[PF];
-        auto issue = [&](int r, uint32_t &pi, float &pv) {
-            const int64_t base_r = static_cast<int64_t>(__shfl(static_cast<long long>(rbase), r));
-            const uint32_t lo_r = static_cast<uint32_t>(__shfl(static_cast<int>(wlo), r));
-            const uint32_t hi_r = static_cast<uint32_t>(__shfl(static_cast<int>(whi), r));
-            pi = 0u;
-            pv = 0.f;
-            if (lo_r + esel < hi_r) {
-                const int64_t p = base_r + lo_r + esel;
-                pi = idx[p];
-                pv = BF16 ? bf16_to_f32(xbf[p]) : val[p];
-            }
-        };
-#pragma unroll
-        for (int r = 0; r < PF && r < R; ++r) issue(r, pidx[r], pval[r]);
+        // Main flow: the first batch of every row's window, straight-line
+        // code fed by the ring (no loads inside loops, so the waits on the
+        // ring stay partial).
+        bool any_long = false;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            // uniform row metadata via cross-lane reads from lane r
-            const int64_t base_r = static_cast<int64_t>(__shfl(static_cast<long long>(rbase), r));
-            const uint32_t lo_r = static_cast<uint32_t>(__shfl(static_cast<int>(wlo), r));
-            const uint32_t hi_r = static_cast<uint32_t>(__shfl(static_cast<int>(whi), r));
-            const double f = BF16 ? 1.0 : __shfl(rfac, r);
+            const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
             const uint32_t ci = pidx[r % PF];
             const float cv = pval[r % PF];
-            if (r + PF < R) issue(r + PF, pidx[r % PF], pval[r % PF]);
-            for (uint32_t o = lo_r; o < hi_r; o += kWave) {
-                const int cnt = (hi_r - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi_r - o) : kWave;
-                uint32_t ofs = 0u;
-                float x = 0.f;
-                if (esel < cnt) {
-                    uint32_t c;
-                    float v;
-                    if (o == lo_r) {
-                        c = ci;
-                        v = cv;
-                    } else {
-                        const int64_t p = base_r + o + esel;
-                        c = idx[p];
-                        v = BF16 ? bf16_to_f32(xbf[p]) : val[p];
-                    }
-                    ofs = (c - static_cast<uint32_t>(g0)) * (KP * 4);
-                    x = BF16 ? v : xtilde(v, f, flags);
+            if (r + PF < R)
+                issue(r + PF, wlo, whi, pidx[r % PF], pval[r % PF]);
+            else  // next chunk's rows (after the last chunk: harmless reads)
+                issue(r + PF - R, nlo, nhi, pidx[r % PF], pval[r % PF]);
+            const uint32_t len = hi_r - lo_r;
+            any_long |= len > static_cast<uint32_t>(kWave);
+            const int cnt = len < static_cast<uint32_t>(kWave) ? static_cast<int>(len) : kWave;
+            const bool ok = esel < cnt;
+            const uint32_t ofs = ok ? (ci & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+            const float x = ok ? cv : 0.f;
+            gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
+        }
+        // Windows longer than one batch (rare): the remaining batches, read
+        // directly.
+        if (any_long) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                const int64_t base_r = rl64(rbase, r);
+                const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
+                for (uint32_t o = lo_r + kWave; o < hi_r; o += kWave) {
+                    const int cnt = (hi_r - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi_r - o) : kWave;
+                    int64_t p = base_r + o + esel;
+                    p = p < p_max ? p : p_max;
+                    const uint32_t c = lg[p];
+                    const float v = BF16 ? bf16_to_f32(xbf[p]) : xt[p];
+                    const bool ok = esel < cnt;
+                    const uint32_t ofs = ok ? (c & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+                    const float x = ok ? v : 0.f;
+                    gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
                 }
-                gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
             }
         }
+        wlo = nlo;
+        whi = nhi;
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -830,108 +913,136 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// Transposed SpMM on the tile-CSC: part[tr][v][:] = sum_{tiles of tr} sum_run x~ Q[row][:].
-// Block (tile range tr, gene chunk gc), 16 waves, each owning GW consecutive
-// genes whose (per-group partial) outputs stay in registers across all tiles.
-// Per tile: Q tile (256 x KP) and the row factors are staged in LDS; the
-// chunk's entries (contiguous in gene-major order) are transformed once in
-// full 64-wide batches into an LDS entry array (Q row offset, x~); then each
-// wave walks its genes' runs: groups take entries 4t+g, gather E columns of
-// the Q row from LDS and accumulate. No atomics: every output element is owned
-// by one lane, so the sum order is fixed and the result is deterministic.
+// Transposed SpMM over the tile-CSC, load-balanced: part[tr] = X~[rows of tile
+// range tr]^T Q for one (tile range, gene group) per block. The genes are
+// dealt to the 16 x GW wave slots of all blocks through gperm (genes sorted by
+// entry count, snake order), so every wave carries about the same number of
+// entries and the per-tile barriers do not wait on one dense gene. Per tile
+// the block stages Q's 256 x KP tile in LDS (prefetched into registers during
+// the previous tile); each wave reads its genes' run bounds from tptr (one
+// gene per lane, read out with v_readlane) and the run entries (row u8, X~)
+// straight from HBM, the first 64-entry batch of each run PF genes ahead, in a
+// ring that runs on across tiles. Groups take entries 4t+g of a run; every
+// output element is owned by one lane and the sum order over a gene's entries
+// is fixed (tile order, then run order), so results are deterministic and do
+// not depend on the gene order.
+// Grid: blockIdx = slot * 8 + xcd; the nch blocks of one tile range run on one
+// XCD at the same time and share its L2 copy of the Q tiles.
 // ---------------------------------------------------------------------------
-
-template <int KP, int GW, bool BF16>
-__global__ void __launch_bounds__(1024) k_spmm_t2(const int64_t *__restrict__ indptr,
+template <int KP, int GW, int PF, bool BF16>
+__global__ void __launch_bounds__(1024) k_spmm_t4(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ tptr,
                                                   const uint8_t *__restrict__ trow,
                                                   const float *__restrict__ tval,
                                                   const uint16_t *__restrict__ txbf,
-                                                  const double *__restrict__ rowf, int flags,
-                                                  const float *__restrict__ Q, int nch, int ntr,
-                                                  int tiles_per_tr, int64_t n_obs, int64_t n_vars, int n_tiles,
-                                                  float *__restrict__ part) {
+                                                  const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
+                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
+                                                  int64_t n_vars, int n_tiles, float *__restrict__ part) {
     constexpr int E = KP / 16;
     constexpr int NW = 16;
-    constexpr int CG = NW * GW;
-    constexpr int ENT = (KP == 64) ? 8192 : 2048;  // entry slots in LDS
-    __shared__ float4 qs[kTileRows * KP / 4];        // 64 KB (KP=64) / 128 KB (KP=128)
-    __shared__ uint2 ent[ENT];                        // (Q row byte offset, x~)
-    __shared__ double rf[kTileRows];
-    __shared__ uint32_t bnd[CG + 1];                  // the chunk's run bounds in this tile
+    constexpr int NT = NW * kWave;
+    static_assert(GW <= 16 && GW % PF == 0, "one gene per lane of a group; static ring slots");
+    __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
+    const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int64_t cb = static_cast<int64_t>(gc) * CG;               // block's first gene
-    const int64_t cw = cb + static_cast<int64_t>(wave) * GW;        // wave's first gene
+    const int wave = tid / kWave, lane = tid & 63, g = lane >> 4, l = lane & 15;
     const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
     const int esel = 4 * l + g;
+    // lane l (< GW) of every group holds gene slot l of this wave
+    const int64_t gslot = (static_cast<int64_t>(gc) * NW + wave) * GW + l;
+    const uint32_t gene = (l < GW) ? gperm[gslot] : static_cast<uint32_t>(n_vars);
+    const bool gvalid = gene < static_cast<uint64_t>(n_vars);
     float z[GW][E];
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi)
 #pragma unroll
         for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
     const int t_begin = tr * tiles_per_tr;
-    const int t_end = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
-    const int64_t cend = (cb + CG) < n_vars ? (cb + CG) : n_vars;
-    constexpr int QPT = kTileRows * KP / 4 / 1024;  // float4 of the Q tile per thread
+    const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
+    const int t_end = (tr < ntr) ? t_end0 : t_begin;
+
+    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
+    // Loads are unconditional at clamped indices, validity applied at use: a
+    // load under a data-dependent branch would make every later wait drain
+    // all outstanding loads and defeat the prefetch.
+    const uint32_t gene_c = gvalid ? gene : 0u;
+    auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
+        const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
+        const uint32_t a0 = tp[gene_c], b0 = tp[gene_c + 1];
+        a = gvalid ? a0 : 0u;
+        b = gvalid ? b0 : 0u;
+    };
+    constexpr int QPT = kTileRows * KP / 4 / NT;  // float4 of the Q tile per thread
     float4 qpre[QPT];
     auto load_q = [&](int tt) {
-        const int64_t rq = static_cast<int64_t>(tt) * kTileRows;
-        const int nr = (n_obs - rq) < kTileRows ? static_cast<int>(n_obs - rq) : kTileRows;
-        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + rq * KP);
+        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
+        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
 #pragma unroll
         for (int j = 0; j < QPT; ++j) {
-            const int i = threadIdx.x + j * 1024;
-            qpre[j] = (i / (KP / 4) < nr) ? Q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+            const int i = tid + j * NT;
+            const bool ok = i / (KP / 4) < nr;
+            const float4 v = Q4[ok ? i : 0];
+            qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
         }
     };
-    const int t_first = (tr < ntr) ? t_begin : t_end;
-    if (t_first < t_end) load_q(t_first);
-    for (int t = t_first; t < t_end; ++t) {
-        const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
-        const int nrows = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
-        const int64_t tb = indptr[r0];
-        const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
-        __syncthreads();
-        for (int i = threadIdx.x; i <= CG; i += blockDim.x) bnd[i] = (cb + i <= cend) ? tp[cb + i] : 0u;
+    const int64_t p_max = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
+    // first 64-entry batch of gene gi's run in tile tt, PF genes ahead
+    uint32_t prow[PF];
+    float pval[PF];
+    auto issue = [&](int gi, int64_t tb, uint32_t wl, uint32_t &pr, float &pv) {
+        int64_t p = tb + rl32(wl, gi) + esel;
+        p = p < p_max ? p : p_max;
+        pr = trow[p];
+        pv = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
+    };
+    uint32_t wlo = 0u, whi = 0u, nlo = 0u, nhi = 0u;
+    int64_t tb = 0, ntb = 0;
+    if (t_begin < t_end) {
+        bounds(t_begin, wlo, whi);
+        tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
+        load_q(t_begin);
 #pragma unroll
-        for (int j = 0; j < QPT; ++j) qs[threadIdx.x + j * 1024] = qpre[j];
-        if (!BF16)
-            for (int i = threadIdx.x; i < kTileRows; i += blockDim.x) rf[i] = (rowf && i < nrows) ? rowf[r0 + i] : 1.0;
-        __syncthreads();
-        if (t + 1 < t_end) load_q(t + 1);
-        const uint32_t e_lo = (cb < n_vars) ? bnd[0] : 0u, e_hi = (cb < n_vars) ? bnd[cend - cb] : 0u;
-        for (uint32_t w0 = e_lo; w0 < e_hi; w0 += ENT) {
-            const uint32_t w1 = (w0 + ENT) < e_hi ? (w0 + ENT) : e_hi;
-            for (uint32_t i = w0 + threadIdx.x; i < w1; i += blockDim.x) {
-                const uint32_t rr = trow[tb + i];
-                const float x = BF16 ? bf16_to_f32(txbf[tb + i]) : xtilde(tval[tb + i], rf[rr], flags);
-                ent[i - w0] = make_uint2(rr * (KP * 4), __float_as_uint(x));
-            }
-            __syncthreads();
+        for (int gi = 0; gi < PF; ++gi) issue(gi, tb, wlo, prow[gi], pval[gi]);
+    }
+    for (int t = t_begin; t < t_end; ++t) {
+        const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
+        __syncthreads();  // previous tile's gathers are done with qs
 #pragma unroll
-            for (int gi = 0; gi < GW; ++gi) {
-                const int64_t c = cw + gi;
-                if (c < cend) {
-                    uint32_t s0 = bnd[c - cb], s1 = bnd[c - cb + 1];
-                    s0 = s0 > w0 ? s0 : w0;
-                    s1 = s1 < w1 ? s1 : w1;
-                    for (uint32_t bse = s0; bse < s1; bse += kWave) {
-                        const int cnt = (s1 - bse) < kWave ? static_cast<int>(s1 - bse) : kWave;
-                        uint32_t ofs = 0u;
-                        float x = 0.f;
-                        if (esel < cnt) {
-                            const uint2 en = ent[bse - w0 + esel];
-                            ofs = en.x;
-                            x = __uint_as_float(en.y);
-                        }
-                        gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
-                    }
+        for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
+        __syncthreads();
+        load_q(tn);
+        bounds(tn, nlo, nhi);
+        ntb = indptr[static_cast<int64_t>(tn) * kTileRows];
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            const uint32_t lo = rl32(wlo, gi), hi = rl32(whi, gi);
+            const uint32_t cr = prow[gi % PF];
+            const float cv = pval[gi % PF];
+            if (gi + PF < GW)
+                issue(gi + PF, tb, wlo, prow[gi % PF], pval[gi % PF]);
+            else
+                issue(gi + PF - GW, ntb, nlo, prow[gi % PF], pval[gi % PF]);
+            for (uint32_t o = lo; o < hi; o += kWave) {
+                const int cnt = (hi - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi - o) : kWave;
+                uint32_t rr = cr;
+                float v = cv;
+                if (o != lo) {  // later batches of a long run: read directly
+                    int64_t p = tb + o + esel;
+                    p = p < p_max ? p : p_max;
+                    rr = trow[p];
+                    v = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
                 }
+                const bool ok = esel < cnt;
+                const uint32_t ofs = ok ? rr * (KP * 4) : 0u;
+                const float x = ok ? v : 0.f;
+                gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
             }
-            __syncthreads();
         }
+        wlo = nlo;
+        whi = nhi;
+        tb = ntb;
     }
     // reduce the four group partials and write this block's partial of Z
 #pragma unroll
@@ -941,8 +1052,8 @@ __global__ void __launch_bounds__(1024) k_spmm_t2(const int64_t *__restrict__ in
             z[gi][e] += __shfl_xor(z[gi][e], 16);
             z[gi][e] += __shfl_xor(z[gi][e], 32);
         }
-        const int64_t c = cw + gi;
-        if (tr < ntr && c < cend && g == 0) {
+        const uint32_t c = rl32(gene, gi);
+        if (tr < ntr && c < static_cast<uint64_t>(n_vars) && g == 0) {
             float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
 #pragma unroll
             for (int e = 0; e < E / 4; ++e)
@@ -951,18 +1062,181 @@ __global__ void __launch_bounds__(1024) k_spmm_t2(const int64_t *__restrict__ in
     }
 }
 
-// X~ values of the tile-CSC as bf16 for the sketch form; one block per tile.
-__global__ void __launch_bounds__(256) k_tcsc_bf16(const int64_t *__restrict__ indptr,
-                                                   const uint8_t *__restrict__ trow,
-                                                   const float *__restrict__ tval,
-                                                   const double *__restrict__ rowf, int flags, int64_t n_obs,
-                                                   uint16_t *__restrict__ out) {
-    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * kTileRows;
-    const int64_t r1 = r0 + kTileRows < n_obs ? r0 + kTileRows : n_obs;
-    const int64_t a = indptr[r0], b = indptr[r1];
-    for (int64_t p = a + threadIdx.x; p < b; p += blockDim.x) {
-        const double f = rowf ? rowf[r0 + trow[p]] : 1.0;
-        out[p] = f32_to_bf16_rne(xtilde(tval[p], f, flags));
+// ---------------------------------------------------------------------------
+// Transposed SpMM over the tile-CSC with per-wave entry lists: the gene order
+// and block layout of k_spmm_t4, but each wave prefetches its whole next-tile
+// entry list, the runs of its GW genes concatenated (up to CAP entries), into
+// registers while it computes the current tile, and writes it to its own LDS
+// list after that tile; the gathers then read (Q-row offset, x~) from LDS. The
+// list is private to the wave (LDS operations of one wave complete in order),
+// so only the Q tile needs block barriers. Entries past CAP (rare) are read
+// directly from HBM. Entry loads are unconditional at clamped indices with
+// validity applied at use, so the waits on them stay partial.
+// ---------------------------------------------------------------------------
+template <int KP, int GW, int CAP, bool BF16>
+__global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ indptr,
+                                                  const uint32_t *__restrict__ tptr,
+                                                  const uint8_t *__restrict__ trow,
+                                                  const float *__restrict__ tval,
+                                                  const uint16_t *__restrict__ txbf,
+                                                  const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
+                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
+                                                  int64_t n_vars, int n_tiles, float *__restrict__ part) {
+    constexpr int E = KP / 16;
+    constexpr int NW = 16;
+    constexpr int NT = NW * kWave;
+    constexpr int NB = CAP / kWave;  // prefetch slots per lane
+    static_assert(GW <= 16 && CAP % kWave == 0, "one gene per lane of a group");
+    __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
+    __shared__ uint2 ents[NW][CAP];           // per-wave entry lists (Q-row byte offset, x~)
+    const int tid = threadIdx.x;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
+    const int wave = tid / kWave, lane = tid & 63, g = lane >> 4, l = lane & 15;
+    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
+    const int esel = 4 * l + g;
+    uint2 *ent = ents[wave];
+    const int64_t gslot = (static_cast<int64_t>(gc) * NW + wave) * GW + l;
+    const uint32_t gene = (l < GW) ? gperm[gslot] : static_cast<uint32_t>(n_vars);
+    const bool gvalid = gene < static_cast<uint64_t>(n_vars);
+    const uint32_t gene_c = gvalid ? gene : 0u;
+    float z[GW][E];
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi)
+#pragma unroll
+        for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
+    const int t_begin = tr * tiles_per_tr;
+    const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
+    const int t_end = (tr < ntr) ? t_end0 : t_begin;
+    const int64_t p_max = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
+
+    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
+    auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
+        const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
+        const uint32_t a0 = tp[gene_c], b0 = tp[gene_c + 1];
+        a = gvalid ? a0 : 0u;
+        b = gvalid ? b0 : 0u;
+    };
+    constexpr int QPT = kTileRows * KP / 4 / NT;
+    float4 qpre[QPT];
+    auto load_q = [&](int tt) {
+        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
+        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) {
+            const int i = tid + j * NT;
+            const bool ok = i / (KP / 4) < nr;
+            const float4 v = Q4[ok ? i : 0];
+            qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+    };
+    // run layout of one tile: start (tile-relative) and list offset per gene
+    uint32_t lo[GW], pos[GW + 1];
+    uint32_t nlo[GW], npos[GW + 1];
+    auto layout = [&](uint32_t wl, uint32_t wh, uint32_t (&lo_)[GW], uint32_t (&pos_)[GW + 1]) {
+        pos_[0] = 0u;
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            lo_[gi] = rl32(wl, gi);
+            pos_[gi + 1] = pos_[gi] + (rl32(wh, gi) - lo_[gi]);
+        }
+    };
+    uint32_t prow[NB];
+    float pval[NB];
+    // list entry f = 64 b + lane of the tile starting at tb: gene gi with
+    // pos[gi] <= f, at tile offset lo[gi] + f - pos[gi]
+    auto prefetch = [&](int64_t tb_, const uint32_t (&lo_)[GW], const uint32_t (&pos_)[GW + 1]) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t f = static_cast<uint32_t>(b * kWave + lane);
+            uint32_t off = lo_[0] + f;
+#pragma unroll
+            for (int gi = 1; gi < GW; ++gi) off = (f >= pos_[gi]) ? lo_[gi] + (f - pos_[gi]) : off;
+            int64_t p = tb_ + off;
+            p = p < p_max ? p : p_max;
+            prow[b] = trow[p];
+            pval[b] = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
+        }
+    };
+    auto store_list = [&](uint32_t total) {
+#pragma unroll
+        for (int b = 0; b < NB; ++b) {
+            const uint32_t f = static_cast<uint32_t>(b * kWave + lane);
+            if (f < total) ent[f] = make_uint2(prow[b] * (KP * 4), __float_as_uint(pval[b]));
+        }
+    };
+
+    uint32_t wlo = 0u, whi = 0u;
+    int64_t tb = 0;
+    if (t_begin < t_end) {
+        bounds(t_begin, wlo, whi);
+        tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
+        layout(wlo, whi, lo, pos);
+        prefetch(tb, lo, pos);
+        store_list(pos[GW]);
+        load_q(t_begin);
+        const int tn = t_begin + 1 < t_end ? t_begin + 1 : t_begin;
+        bounds(tn, wlo, whi);
+    }
+    for (int t = t_begin; t < t_end; ++t) {
+        const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
+        const int tnn = tn + 1 < t_end ? tn + 1 : tn;
+        __syncthreads();  // previous tile's gathers are done with qs
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
+        __syncthreads();
+        // next tile: Q tile, run layout (bounds loaded one tile ago), entries
+        load_q(tn);
+        const int64_t ntb = indptr[static_cast<int64_t>(tn) * kTileRows];
+        layout(wlo, whi, nlo, npos);
+        prefetch(ntb, nlo, npos);
+        bounds(tnn, wlo, whi);
+        // this tile from the wave's LDS list
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            const uint32_t a = pos[gi], b = pos[gi + 1];
+            const uint32_t bl = b < static_cast<uint32_t>(CAP) ? b : static_cast<uint32_t>(CAP);
+            for (uint32_t o = a; o < bl; o += kWave) {
+                const int cnt = (bl - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(bl - o) : kWave;
+                const bool ok = esel < cnt;
+                const uint2 en = ent[ok ? o + esel : 0];
+                const uint32_t ofs = ok ? en.x : 0u;
+                const float x = ok ? __uint_as_float(en.y) : 0.f;
+                gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
+            }
+            // list overflow (rare): read the rest of the run directly
+            for (uint32_t o = (a > static_cast<uint32_t>(CAP) ? a : static_cast<uint32_t>(CAP)); o < b; o += kWave) {
+                const int cnt = (b - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(b - o) : kWave;
+                const bool ok = esel < cnt;
+                int64_t p = tb + lo[gi] + (o - a) + esel;
+                p = p < p_max ? p : p_max;
+                const uint32_t rr = trow[p];
+                const float v = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
+                gather_batch<E>(z[gi], ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt,
+                                reinterpret_cast<const char *>(qs), lane_ofs);
+            }
+        }
+        store_list(npos[GW]);  // this wave's gathers of tile t are done with its list
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) { lo[gi] = nlo[gi]; pos[gi] = npos[gi]; }
+        pos[GW] = npos[GW];
+        tb = ntb;
+    }
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) {
+            z[gi][e] += __shfl_xor(z[gi][e], 16);
+            z[gi][e] += __shfl_xor(z[gi][e], 32);
+        }
+        const uint32_t c = rl32(gene, gi);
+        if (tr < ntr && c < static_cast<uint64_t>(n_vars) && g == 0) {
+            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
+#pragma unroll
+            for (int e = 0; e < E / 4; ++e)
+                dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
+        }
     }
 }
 
@@ -1254,22 +1528,20 @@ __global__ void __launch_bounds__(256) k_chol_blk(const double *__restrict__ G, 
     if (tid == 0 && bad) atomicOr(status, 1);
 }
 
-// Column statistics on the tile-CSC with the transform batched in LDS: block
-// (tile range, 16 waves x 16 genes); per tile the chunk's entries are loaded
-// and transformed by all threads into LDS, then lane l of a gene's wave takes
+// Column statistics on the tile-CSC (values are X~): block (tile range,
+// 16 waves x 8 genes); per tile the chunk's entries are staged by all threads
+// into LDS, then lane l of a gene's wave takes
 // run entries l, l+64, ... into register partial sums (f64) kept across the
 // tile range; one wave reduction per gene at the end (deterministic).
 __global__ void __launch_bounds__(1024) k_col_stats4(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ tptr,
                                                      const uint8_t *__restrict__ trow,
-                                                     const float *__restrict__ tval,
-                                                     const double *__restrict__ rowf, int flags, int nch,
+                                                     const float *__restrict__ tval, int nch,
                                                      int tiles_per_tr, int n_tiles, int64_t n_obs,
                                                      int64_t n_vars, double *__restrict__ psum,
                                                      double *__restrict__ psq, uint32_t *__restrict__ pcnt) {
     constexpr int GWS = 8, NW = 16, CGS = NW * GWS, ENT = 16384;
     __shared__ float xs[ENT];
-    __shared__ double rf[kTileRows];
     __shared__ uint32_t bnd[CGS + 1];
     const int tr = blockIdx.x / nch, gc = blockIdx.x % nch;
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63;
@@ -1287,14 +1559,13 @@ __global__ void __launch_bounds__(1024) k_col_stats4(const int64_t *__restrict__
         const int64_t tb = indptr[r0];
         const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
         __syncthreads();
-        for (int i = threadIdx.x; i < kTileRows; i += blockDim.x) rf[i] = (rowf && r0 + i < n_obs) ? rowf[r0 + i] : 1.0;
         for (int i = threadIdx.x; i <= CGS; i += blockDim.x) bnd[i] = (cb + i <= cend) ? tp[cb + i] : 0u;
         __syncthreads();
         const uint32_t e_lo = (cb < n_vars) ? bnd[0] : 0u, e_hi = (cb < n_vars) ? bnd[cend - cb] : 0u;
         for (uint32_t w0 = e_lo; w0 < e_hi; w0 += ENT) {
             const uint32_t w1 = (w0 + ENT) < e_hi ? (w0 + ENT) : e_hi;
             for (uint32_t i = w0 + threadIdx.x; i < w1; i += blockDim.x)
-                xs[i - w0] = xtilde(tval[tb + i], rf[trow[tb + i]], flags);
+                xs[i - w0] = tval[tb + i];
             __syncthreads();
 #pragma unroll
             for (int gi = 0; gi < GWS; ++gi) {

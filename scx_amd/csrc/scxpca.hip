@@ -145,11 +145,18 @@ struct scxpca_matrix {
     DevBuf rowf;
     DevBuf spl;
     int nsplit = 0;
+    DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
+    DevBuf lg;  // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
     DevBuf xbf;
     bool xbf_ready = false;
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
     DevBuf tptr, trow, tval, txbf;
     int n_tiles = 0;
+    // entries per gene (host copy) and the load-balanced gene order of the
+    // transposed SpMM for one wave-slot layout (GW genes x 16 waves x nch)
+    DevBuf gcnt, gperm;
+    std::vector<uint32_t> host_gcnt, host_gperm;
+    int gperm_gw = 0, gperm_nch = 0;
     bool use_split = false;
     // cached column statistics of X~
     bool stats_ready = false;
@@ -195,6 +202,8 @@ struct StageScope {
 
 // Derived per-matrix data: row factors, split offsets (small n_vars), the
 // tile-CSC copy. Runs at creation and on refresh; invalidates cached stats.
+const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr; }
+
 void prepare(scxpca_matrix *m) {
     m->stats_ready = false;
     m->xbf_ready = false;
@@ -205,15 +214,58 @@ void prepare(scxpca_matrix *m) {
         k_row_factor<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, m->target, m->rowf.as<double>());
         HIPC(hipGetLastError());
     }
+    k_xtilde<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, m->n_obs,
+                                                m->xt.as<float>(), m->lg.as<uint16_t>());
+    HIPC(hipGetLastError());
     if (m->use_split) {
         k_splits<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
                                                     m->spl.as<uint32_t>());
         HIPC(hipGetLastError());
     }
-    k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs,
+    k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                      static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
                                                      m->trow.as<uint8_t>(), m->tval.as<float>());
     HIPC(hipGetLastError());
+    // entries per gene -> host (the transposed SpMM's gene order is built from it)
+    m->gcnt.ensure(sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1));
+    HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1), m->stream));
+    if (m->nnz > 0) {
+        k_gene_counts<<<std::min(blocks_for(m->nnz, 256 * 64), 2048u), 256, 0, m->stream>>>(
+            m->idx, m->nnz, static_cast<int>(m->n_vars), m->gcnt.as<uint32_t>());
+        HIPC(hipGetLastError());
+    }
+    m->host_gcnt.assign(m->n_vars, 0u);
+    if (m->n_vars > 0)
+        HIPC(hipMemcpyAsync(m->host_gcnt.data(), m->gcnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost,
+                            m->stream));
+    HIPC(hipStreamSynchronize(m->stream));
+    m->gperm_gw = 0;
+}
+
+// Gene order for the transposed SpMM: genes sorted by entry count (descending,
+// ties by index) are dealt to the nch*16 waves in snake order (0..W-1, W-1..0,
+// ...), round r filling slot r of each wave, so per-wave entry totals differ by
+// at most about one gene's count. Unused slots hold n_vars.
+void ensure_gene_order(scxpca_matrix *m, int gw, int nch) {
+    if (m->gperm_gw == gw && m->gperm_nch == nch) return;
+    const int64_t nv = m->n_vars;
+    const int64_t W = static_cast<int64_t>(nch) * 16, S = W * gw;
+    std::vector<uint32_t> order(nv);
+    for (int64_t c = 0; c < nv; ++c) order[c] = static_cast<uint32_t>(c);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return m->host_gcnt[a] > m->host_gcnt[b]; });
+    m->host_gperm.assign(std::max<int64_t>(S, 1), static_cast<uint32_t>(nv));
+    for (int64_t i = 0; i < nv; ++i) {
+        const int64_t r = i / W, pos = i % W;
+        const int64_t w = (r % 2 == 0) ? pos : W - 1 - pos;
+        m->host_gperm[w * gw + r] = order[i];
+    }
+    m->gperm.ensure(sizeof(uint32_t) * m->host_gperm.size());
+    HIPC(hipMemcpyAsync(m->gperm.p, m->host_gperm.data(), sizeof(uint32_t) * m->host_gperm.size(),
+                        hipMemcpyHostToDevice, m->stream));
+    HIPC(hipStreamSynchronize(m->stream));
+    m->gperm_gw = gw;
+    m->gperm_nch = nch;
 }
 
 void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1p) {
@@ -297,22 +349,21 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
     m->trow.ensure(std::max<int64_t>(m->nnz, 1));
     m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+    m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+    m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     prepare(m);
     HIPC(hipStreamSynchronize(m->stream));
 }
-
-const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr; }
 
 void ensure_xbf(scxpca_matrix *m) {
     if (m->xbf_ready) return;
     m->xbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     m->txbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     if (m->n_obs > 0) {
-        k_xtilde_bf16<<<std::min(blocks_for(m->n_obs, 4), 65536u), 256, 0, m->stream>>>(
-            m->indptr, m->val, rowf_ptr(m), m->flags, m->n_obs, m->xbf.as<uint16_t>());
+        const unsigned nb = std::min(blocks_for(m->nnz, 256), 65536u);
+        k_to_bf16<<<nb, 256, 0, m->stream>>>(m->xt.as<float>(), m->nnz, m->xbf.as<uint16_t>());
         HIPC(hipGetLastError());
-        k_tcsc_bf16<<<m->n_tiles, 256, 0, m->stream>>>(m->indptr, m->trow.as<uint8_t>(), m->tval.as<float>(),
-                                                       rowf_ptr(m), m->flags, m->n_obs, m->txbf.as<uint16_t>());
+        k_to_bf16<<<nb, 256, 0, m->stream>>>(m->tval.as<float>(), m->nnz, m->txbf.as<uint16_t>());
     }
     HIPC(hipGetLastError());
     m->xbf_ready = true;
@@ -334,7 +385,7 @@ void ensure_stats(scxpca_matrix *m) {
     {
         StageScope prof(m, m->stream, SCXPCA_STAGE_STATS);
         k_col_stats4<<<ntr * nch, 1024, 0, m->stream>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
-                                                       m->tval.as<float>(), rowf_ptr(m), m->flags, nch, tiles_per_tr,
+                                                       m->tval.as<float>(), nch, tiles_per_tr,
                                                        m->n_tiles, m->n_obs, m->n_vars, m->psum.as<double>(),
                                                        m->psq.as<double>(), m->pcnt.as<uint32_t>());
     }
@@ -358,9 +409,11 @@ __global__ void k_mean(const double *sum, int64_t n_vars, double inv_n, double *
 
 template <int KP>
 struct Engine {
-    static constexpr int GW = (KP == 64) ? 12 : 6;  // genes per wave in k_spmm_t2 (Z in registers)
-    static constexpr int CG = 16 * GW;              // genes per k_spmm_t2 block
-    static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd2
+    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t5 (Z in registers)
+    static constexpr int TCAP = (KP == 64) ? 256 : 128;  // per-wave entry list in k_spmm_t5
+    static constexpr int CG = 16 * GW;              // genes per k_spmm_t5 block
+    static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
+    static constexpr int FPF = 8;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
     hipStream_t s;
     int k = 0;
@@ -374,10 +427,12 @@ struct Engine {
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         nch_t = static_cast<int>((nv + CG - 1) / CG);
-        int ntr = std::max(1, 512 / nch_t);
+        // tile ranges: about one block per CU in total (256 CUs), multiple of 8
+        int ntr = std::max(1, 256 / nch_t);
         ntr = std::min(ntr, std::max(1, m->n_tiles));
         nrb_t = (ntr + 7) / 8 * 8;
         tiles_per_tr = (m->n_tiles + nrb_t - 1) / nrb_t;
+        ensure_gene_order(m, GW, nch_t);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
@@ -451,21 +506,21 @@ struct Engine {
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
             if (bf16)
-                k_spmm_fwd2<KP, true, FR, NWF, CGF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr, m->flags, panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars);
+                k_spmm_fwd3<KP, true, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->lg.as<uint16_t>(), nullptr, m->xbf.as<uint16_t>(), panel, mcp, Y,
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz);
             else
-                k_spmm_fwd2<KP, false, FR, NWF, CGF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags, panel, mcp, Y, m->spl.as<uint32_t>(),
-                    m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars);
+                k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
             if (bf16)
-                k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, m->xbf.as<uint16_t>(), nullptr,
-                                                          m->flags, panel, mcp, Y, m->n_obs);
+                k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, nullptr, m->xbf.as<uint16_t>(), panel,
+                                                          mcp, Y, m->n_obs);
             else
-                k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->val, nullptr, rowf_ptr(m), m->flags,
-                                                           panel, mcp, Y, m->n_obs);
+                k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
+                                                           mcp, Y, m->n_obs);
         }
         check();
     }
@@ -477,16 +532,18 @@ struct Engine {
         if (m->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
+            const uint32_t *gp = m->gperm.as<uint32_t>();
             if (bf16)
-                k_spmm_t2<KP, GW, true><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
-                                                              m->tval.as<float>(), m->txbf.as<uint16_t>(), nullptr,
-                                                              m->flags, Q, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
-                                                              m->n_tiles, m->tpart.as<float>());
+                k_spmm_t5<KP, GW, TCAP, true><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                                                                   m->trow.as<uint8_t>(), m->tval.as<float>(),
+                                                                   m->txbf.as<uint16_t>(), Q, gp, nch_t, nrb_t,
+                                                                   tiles_per_tr, m->n_obs, nv, m->n_tiles,
+                                                                   m->tpart.as<float>());
             else
-                k_spmm_t2<KP, GW, false><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
-                                                               m->tval.as<float>(), nullptr, rowf_ptr(m), m->flags, Q,
-                                                               nch_t, nrb_t, tiles_per_tr, m->n_obs, nv, m->n_tiles,
-                                                               m->tpart.as<float>());
+                k_spmm_t5<KP, GW, TCAP, false><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(), nullptr,
+                                                                    Q, gp, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
+                                                                    m->n_tiles, m->tpart.as<float>());
             check();
         } else {
             HIPC(hipMemsetAsync(m->tpart.p, 0, sizeof(float) * nrb_t * nv * KP, s));
@@ -663,7 +720,7 @@ struct Engine {
                 const int nb = 1024;
                 DevBuf part;
                 part.ensure(sizeof(double) * nb);
-                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, mu, n,
+                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), mu, n,
                                                  part.as<double>());
                 check();
                 std::vector<double> hp(nb);
