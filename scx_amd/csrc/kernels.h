@@ -158,16 +158,17 @@ __global__ void __launch_bounds__(256) k_splits(const int64_t *__restrict__ indp
 
 // X~ values in CSR order, one f32 per stored entry, computed once per
 // prepared matrix (every SpMM pass and the tile-CSC read these instead of
-// re-applying the transform per pass), and the forward SpMM's gene index
-// inside its LDS panel chunk, lg = gene mod kChunkMod (u16; chunks of 512 or
-// 256 genes start at multiples of their size, so lg mod 256 serves both).
+// re-applying the transform per pass), the forward SpMM's gene index inside
+// its LDS panel chunk, lg = gene mod kChunkMod (u16; chunks of 512 or 256
+// genes start at multiples of their size, so lg mod 256 serves both), and the
+// entry's row within its 256-row tile (u8, for the tile-CSC build).
 constexpr uint32_t kChunkMod = 512;
 __global__ void __launch_bounds__(256) k_xtilde(const int64_t *__restrict__ indptr,
                                                 const uint32_t *__restrict__ idx,
                                                 const float *__restrict__ val,
                                                 const double *__restrict__ rowf, int flags,
                                                 int64_t n_obs, float *__restrict__ out,
-                                                uint16_t *__restrict__ lg) {
+                                                uint16_t *__restrict__ lg, uint8_t *__restrict__ erow) {
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
@@ -177,33 +178,9 @@ __global__ void __launch_bounds__(256) k_xtilde(const int64_t *__restrict__ indp
         for (int64_t p = a + lane; p < b; p += kWave) {
             out[p] = xtilde(val[p], f, flags);
             lg[p] = static_cast<uint16_t>(idx[p] & (kChunkMod - 1));
+            erow[p] = static_cast<uint8_t>(r & 255);  // row within its 256-row tile
         }
     }
-}
-
-// Stored entries per gene (column), for the load-balanced gene order of the
-// transposed SpMM. Block-private LDS histogram when n_vars fits, else global
-// atomics; counts are integers, so the result does not depend on the order.
-__global__ void __launch_bounds__(256) k_gene_counts(const uint32_t *__restrict__ idx, int64_t nnz, int n_vars,
-                                                     uint32_t *__restrict__ cnt) {
-    constexpr int kMaxLds = 16384;
-    __shared__ uint32_t h[kMaxLds];
-    const bool local = n_vars <= kMaxLds;
-    if (local)
-        for (int i = threadIdx.x; i < n_vars; i += blockDim.x) h[i] = 0u;
-    __syncthreads();
-    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    for (int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; p < nnz; p += stride) {
-        const uint32_t c = idx[p];
-        if (local)
-            atomicAdd(&h[c], 1u);
-        else
-            atomicAdd(&cnt[c], 1u);
-    }
-    __syncthreads();
-    if (local)
-        for (int i = threadIdx.x; i < n_vars; i += blockDim.x)
-            if (h[i]) atomicAdd(&cnt[i], h[i]);
 }
 
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
@@ -910,6 +887,141 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) tp[n_vars] = carry;
+}
+
+// Tile-CSC build for n_vars <= kBuild2Genes, one block per tile, two flat
+// passes over the tile's entries (contiguous in the CSR, so every load is
+// coalesced; 16 entries per thread in flight):
+//  1. row-membership masks: bit r of mask[r / 32][c] is set for each entry
+//     (row r, gene c) with an LDS atomic OR (word-major layout: the lanes of
+//     a wave hit distinct genes, so distinct banks);
+//  2. an exclusive scan of per-gene counts gives each run's start; an entry
+//     lands at start[c] + popcount(mask bits of gene c in rows < r), so runs
+//     come out in row order whatever the schedule (deterministic).
+// Loads are unconditional at clamped indices so the waits stay partial.
+constexpr int kBuild2Genes = 2048;
+__global__ void __launch_bounds__(1024) k_build_tcsc2(const int64_t *__restrict__ indptr,
+                                                     const uint32_t *__restrict__ idx,
+                                                     const uint8_t *__restrict__ erow,
+                                                     const float *__restrict__ xt, int64_t n_obs, int n_vars,
+                                                     uint32_t *__restrict__ tptr, uint8_t *__restrict__ trow,
+                                                     float *__restrict__ tval) {
+    constexpr int NT = 1024, U = 16;
+    __shared__ uint32_t mask[8][kBuild2Genes];  // 64 KB
+    __shared__ uint32_t start[kBuild2Genes];    // 8 KB
+    __shared__ uint8_t pre[kBuild2Genes][8];    // 16 KB: entries of gene c in mask words < w
+    __shared__ uint32_t wsum[2][16];
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid / kWave;
+    const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+    const int64_t r1 = r0 + kTileRows < n_obs ? r0 + kTileRows : n_obs;
+    const int64_t base = indptr[r0];
+    const int64_t ne = indptr[r1] - base;
+    const int64_t p_last = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
+    uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+    for (int i = tid; i < 8 * kBuild2Genes; i += NT) (&mask[0][0])[i] = 0u;
+    __syncthreads();
+    // pass 1: masks
+    for (int64_t k0 = 0; k0 < ne; k0 += static_cast<int64_t>(NT) * U) {
+        uint32_t c[U];
+        uint32_t rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int64_t p = base + k0 + tid + u * NT;
+            p = p < p_last ? p : p_last;
+            c[u] = idx[p];
+            rr[u] = erow[p];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (k0 + tid + u * NT < ne) atomicOr(&mask[rr[u] >> 5][c[u]], 1u << (rr[u] & 31));
+    }
+    __syncthreads();
+    // per-gene counts and prefix per mask word, then a block scan for starts
+    // (thread j owns genes 2j, 2j+1: n_vars <= 2048)
+    uint32_t cnt2[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int cg = 2 * tid + h;
+        uint32_t acc = 0u;
+        if (cg < n_vars)
+#pragma unroll
+            for (int w = 0; w < 8; ++w) {
+                pre[cg][w] = static_cast<uint8_t>(acc);
+                acc += __popc(mask[w][cg]);
+            }
+        cnt2[h] = acc;
+    }
+    const uint32_t local = cnt2[0] + cnt2[1];
+    uint32_t incl = local;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+    }
+    if (lane == kWave - 1) wsum[0][wave] = incl;
+    __syncthreads();
+    if (tid < 16) {
+        uint32_t pre_w = 0u;
+        for (int w = 0; w < tid; ++w) pre_w += wsum[0][w];
+        wsum[1][tid] = pre_w;
+    }
+    __syncthreads();
+    {
+        const uint32_t s0 = wsum[1][wave] + incl - local;
+        if (2 * tid < n_vars) {
+            start[2 * tid] = s0;
+            tp[2 * tid] = s0;
+        }
+        if (2 * tid + 1 < n_vars) {
+            start[2 * tid + 1] = s0 + cnt2[0];
+            tp[2 * tid + 1] = s0 + cnt2[0];
+        }
+        if (tid == 0) tp[n_vars] = static_cast<uint32_t>(ne);
+    }
+    __syncthreads();
+    // pass 2: placement
+    for (int64_t k0 = 0; k0 < ne; k0 += static_cast<int64_t>(NT) * U) {
+        uint32_t c[U];
+        uint32_t rr[U];
+        float x[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            int64_t p = base + k0 + tid + u * NT;
+            p = p < p_last ? p : p_last;
+            c[u] = idx[p];
+            rr[u] = erow[p];
+            x[u] = xt[p];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (k0 + tid + u * NT < ne) {
+                const uint32_t w = rr[u] >> 5, bit = rr[u] & 31;
+                const uint32_t below = bit ? (mask[w][c[u]] & ((1u << bit) - 1u)) : 0u;
+                const uint32_t pos = start[c[u]] + pre[c[u]][w] + __popc(below);
+                trow[base + pos] = static_cast<uint8_t>(rr[u]);
+                tval[base + pos] = x[u];
+            }
+        }
+    }
+}
+
+// Entries per gene from the tile-CSC run bounds: cnt[c] += sum over the
+// block's tile range of tptr[t][c+1] - tptr[t][c] (coalesced over genes;
+// integer atomics, so the totals do not depend on the order). cnt must start
+// at zero.
+__global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__restrict__ tptr, int n_tiles,
+                                                          int tiles_per_block, int64_t n_vars,
+                                                          uint32_t *__restrict__ cnt) {
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (c >= n_vars) return;
+    const int t0 = blockIdx.y * tiles_per_block;
+    const int t1 = t0 + tiles_per_block < n_tiles ? t0 + tiles_per_block : n_tiles;
+    uint32_t s = 0u;
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+        s += tp[c + 1] - tp[c];
+    }
+    if (s) atomicAdd(&cnt[c], s);
 }
 
 // ---------------------------------------------------------------------------

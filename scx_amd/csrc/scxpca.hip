@@ -146,7 +146,8 @@ struct scxpca_matrix {
     DevBuf spl;
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
-    DevBuf lg;  // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
+    DevBuf lg;    // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
+    DevBuf erow;  // row within its 256-row tile per entry (u8), for the tile-CSC build
     DevBuf xbf;
     bool xbf_ready = false;
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
@@ -215,23 +216,31 @@ void prepare(scxpca_matrix *m) {
         HIPC(hipGetLastError());
     }
     k_xtilde<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, m->n_obs,
-                                                m->xt.as<float>(), m->lg.as<uint16_t>());
+                                                m->xt.as<float>(), m->lg.as<uint16_t>(), m->erow.as<uint8_t>());
     HIPC(hipGetLastError());
     if (m->use_split) {
         k_splits<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
                                                     m->spl.as<uint32_t>());
         HIPC(hipGetLastError());
     }
-    k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
-                                                     static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
-                                                     m->trow.as<uint8_t>(), m->tval.as<float>());
+    if (m->n_vars <= kBuild2Genes)
+        k_build_tcsc2<<<m->n_tiles, 1024, 0, m->stream>>>(m->indptr, m->idx, m->erow.as<uint8_t>(),
+                                                          m->xt.as<float>(), m->n_obs, static_cast<int>(m->n_vars),
+                                                          m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+                                                          m->tval.as<float>());
+    else
+        k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
+                                                         static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
+                                                         m->trow.as<uint8_t>(), m->tval.as<float>());
     HIPC(hipGetLastError());
     // entries per gene -> host (the transposed SpMM's gene order is built from it)
     m->gcnt.ensure(sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1));
     HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1), m->stream));
-    if (m->nnz > 0) {
-        k_gene_counts<<<std::min(blocks_for(m->nnz, 256 * 64), 2048u), 256, 0, m->stream>>>(
-            m->idx, m->nnz, static_cast<int>(m->n_vars), m->gcnt.as<uint32_t>());
+    if (m->n_vars > 0) {
+        const int tpb = 64;
+        const dim3 grid(static_cast<unsigned>((m->n_vars + 255) / 256), static_cast<unsigned>((m->n_tiles + tpb - 1) / tpb));
+        k_gene_counts_tcsc<<<grid, 256, 0, m->stream>>>(m->tptr.as<uint32_t>(), m->n_tiles, tpb, m->n_vars,
+                                                         m->gcnt.as<uint32_t>());
         HIPC(hipGetLastError());
     }
     m->host_gcnt.assign(m->n_vars, 0u);
@@ -351,6 +360,7 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    m->erow.ensure(std::max<int64_t>(m->nnz, 1));
     prepare(m);
     HIPC(hipStreamSynchronize(m->stream));
 }
