@@ -83,6 +83,14 @@ def lib() -> ctypes.CDLL:
         raise ImportError(
             f"native library {LIB_PATH} is not built; run `python -m scx_amd.build` "
             "(the GPU path has no CPU fallback)")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 /
+    # libhsa-runtime64 (same soname as /opt/rocm's). Loaded first, torch's copy
+    # also serves this library; loaded after ours, torch would bring a second
+    # runtime and fail to see the GPU. So torch, when installed, goes first.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     l = ctypes.CDLL(LIB_PATH)
     for name in EXPORTS:
         getattr(l, name)  # raises AttributeError when a declared symbol is missing

@@ -1,0 +1,155 @@
+"""GPU parity on the shapes and edge cases the kernels branch on.
+
+Each matrix goes through the C ABI and is compared with the CPU oracle on the
+same input:
+  * ragged: 1,000 rows (last tile partial) x 300 genes, with empty rows, a row
+    whose stored values are all zero (row total 0: factor 1, the reference's
+    log1p-only branch), genes never expressed, a row holding every gene (a
+    forward window longer than one 64-entry batch) and a gene expressed in
+    every row (a transposed-SpMM wave list longer than its prefetch capacity);
+  * wide3k: 3,000 genes (the general tile-CSC build, many gene chunks);
+  * wide9k: 9,000 genes (the row-parallel forward SpMM without split offsets);
+  * tiny: 40 x 30.
+Tolerances are those of tests/test_gpu_parity.py: SpMM panels to 2e-5 of the
+f64 magnitude, column statistics to 5e-7 relative, PCA singular values to 1e-4
+relative and top principal angles to 1e-3 rad against the f64 oracle run with
+the same Omega.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from tests.pca_metrics import max_principal_angle, sigma_rel_err
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+class Csr:
+    def __init__(self, indptr, indices, data, shape):
+        self.indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+        self.indices = np.ascontiguousarray(indices, dtype=np.uint32)
+        self.data = np.ascontiguousarray(data, dtype=np.float32)
+        self.shape = shape
+
+    def view(self):
+        return oracle.CsrView(self.indptr, self.indices, self.data, self.shape)
+
+
+def structured(n, nv, seed, n_types=4, density=0.15):
+    """Poisson counts from n_types cell programs (a low-rank signal so the top
+    components are well separated), thinned to about `density`."""
+    rng = np.random.default_rng(seed)
+    types = rng.integers(0, n_types, n)
+    prog = rng.lognormal(-1.0, 1.2, (n_types, nv))
+    lib = rng.lognormal(0.0, 0.3, n)[:, None]
+    lam = prog[types] * lib
+    counts = rng.poisson(lam).astype(np.float32)
+    keep = rng.random((n, nv)) < density * 2.0
+    counts *= keep
+    return counts
+
+
+def to_csr(dense):
+    n, nv = dense.shape
+    rows, cols = np.nonzero(dense)
+    indptr = np.zeros(n + 1, np.int64)
+    np.add.at(indptr, rows + 1, 1)
+    indptr = np.cumsum(indptr)
+    return Csr(indptr, cols, dense[rows, cols], (n, nv))
+
+
+def ragged_matrix():
+    d = structured(1000, 300, seed=11)
+    d[:, 250:] = 0.0                 # genes never expressed
+    d[5] = 0.0                       # empty rows
+    d[100:111] = 0.0
+    d[7] = 1.0 + (np.arange(300) % 5)  # a row holding every gene
+    d[7, 250:] = 0.0
+    d[:, 0] = 1.0 + (np.arange(1000) % 7)  # a gene in every row
+    d[5, 0] = 0.0
+    d[100:111, 0] = 0.0
+    c = to_csr(d)
+    # a row whose stored entries are all zero: row total 0 -> factor 1
+    r = 42
+    a, b = c.indptr[r], c.indptr[r + 1]
+    assert b > a
+    c.data[a:b] = 0.0
+    return c
+
+
+MATRICES = {
+    "ragged": ragged_matrix,
+    "wide3k": lambda: to_csr(structured(1500, 3000, seed=12, density=0.05)),
+    "wide9k": lambda: to_csr(structured(1200, 9000, seed=13, density=0.03)),
+    "tiny": lambda: to_csr(structured(40, 30, seed=14, density=0.4)),
+}
+
+
+@pytest.fixture(scope="module", params=list(MATRICES))
+def case(request, gpu):
+    name = request.param
+    csr = MATRICES[name]()
+    say(f"case: Matrix({name}) {csr.shape} nnz={len(csr.data)}")
+    mat = gpu.Matrix(csr.indptr, csr.indices, csr.data, csr.shape, target_sum=1e4, log1p=True)
+    view = csr.view()
+    xt = view.with_data(oracle.transform(view, 1e4, True))
+    mu = oracle.col_sums_and_sum_sq(xt)[0] / csr.shape[0]
+    yield name, csr, mat, xt, mu
+    mat.close()
+
+
+def test_stats(case):
+    name, csr, mat, xt, _ = case
+    s, ss, cnt, rf = mat.stats()
+    rs = oracle.row_sums(csr.view())
+    np.testing.assert_allclose(rf, np.where(rs > 0, 1e4 / np.where(rs > 0, rs, 1.0), 1.0), rtol=1e-15)
+    os_, oss = oracle.col_sums_and_sum_sq(xt)
+    np.testing.assert_allclose(s, os_, rtol=5e-7, atol=1e-9)
+    np.testing.assert_allclose(ss, oss, rtol=5e-7, atol=1e-9)
+    np.testing.assert_array_equal(cnt, np.bincount(csr.indices, minlength=csr.shape[1]))
+
+
+@pytest.mark.parametrize("k", [60, 100, 5])
+def test_spmm_forward(case, k):
+    name, csr, mat, xt, mu = case
+    say(f"case: {name} forward k={k}")
+    m = np.random.default_rng(k).standard_normal((csr.shape[1], k))
+    ref = oracle.spmm_forward(xt, m, mu)
+    got = mat.spmm(m, transpose=False, means=mu)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-5
+
+
+@pytest.mark.parametrize("k", [60, 100, 5])
+def test_spmm_transpose(case, k):
+    name, csr, mat, xt, mu = case
+    say(f"case: {name} transpose k={k}")
+    q = np.random.default_rng(100 + k).standard_normal((csr.shape[0], k))
+    ref = oracle.spmm_transpose(xt, q, mu)
+    got = mat.spmm(q, transpose=True, means=mu)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-5
+
+
+def test_pca(case):
+    name, csr, mat, xt, mu = case
+    n, nv = csr.shape
+    n_comps, p = (5, 3) if name == "tiny" else (20, 10)
+    say(f"case: {name} pca n_comps={n_comps}")
+    ref = oracle.randomized_pca(csr.view(), n_comps=n_comps, n_oversamples=p, n_power_iterations=3, seed=0,
+                                target_sum=1e4, log1p=True)
+    got = mat.pca(n_comps=n_comps, n_oversamples=p, n_power_iterations=3, seed=0, qr_method=1)
+    top = 3
+    se = sigma_rel_err(ref["sigma_all"][:top], got["sigma"][:top])
+    ang = max_principal_angle(ref["components"].T[:, :top], got["components"].T[:, :top].astype(np.float64))
+    say(f"{name}: sigma rel err top{top} {se.max():.2e}, angle {ang:.2e}")
+    assert se.max() <= 1e-4
+    assert ang <= 1e-3
+    np.testing.assert_allclose(got["mean"], ref["mean"], rtol=1e-6, atol=1e-12)
