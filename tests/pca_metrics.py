@@ -15,6 +15,18 @@ def max_principal_angle(a: np.ndarray, b: np.ndarray) -> float:
     return float(np.arcsin(min(1.0, s)))
 
 
+def separated_rank(sigma, max_j: int = 10, gap: float = 0.02) -> int:
+    """Largest j <= max_j whose trailing relative gap (s_j - s_{j+1}) / s_j is at
+    least `gap`: the top-j subspace is then well defined (sin-theta bound), and
+    principal angles against it are meaningful. 0 when no such j exists."""
+    s = np.asarray(sigma, dtype=np.float64)
+    best = 0
+    for j in range(1, min(max_j, len(s) - 1) + 1):
+        if s[j - 1] > 0 and (s[j - 1] - s[j]) / s[j - 1] >= gap:
+            best = j
+    return best
+
+
 def align_signs(ref: np.ndarray, x: np.ndarray, axis: int = 0) -> np.ndarray:
     """Flip components of x (columns when axis=0) to correlate positively with ref."""
     dots = np.sum(ref * x, axis=axis)
