@@ -183,6 +183,100 @@ __global__ void __launch_bounds__(256) k_xtilde(const int64_t *__restrict__ indp
     }
 }
 
+// Per-row preparation in one pass over the CSR (one wave per row): the row
+// total (lane partials, wave reduction; exact for integer counts) and factor
+// t/s (1 where s <= 0; csr.rs:543-552, transforms.rs:21-72), X~ in CSR order,
+// the chunk-local gene index lg, the row-in-tile index, and, when nsplit > 0,
+// the split offsets spl[r][g] = number of entries with gene < g * 128 (rows
+// are sorted: written by the entries that open a chunk, and the row length for
+// chunks after the last entry; spl[r][nsplit] = row length).
+__global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict__ indptr,
+                                                      const uint32_t *__restrict__ idx,
+                                                      const float *__restrict__ val, int64_t n_obs,
+                                                      double target, int flags, int nsplit,
+                                                      double *__restrict__ rowf, float *__restrict__ xt,
+                                                      uint16_t *__restrict__ lg, uint8_t *__restrict__ erow,
+                                                      uint32_t *__restrict__ spl) {
+    constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
+    const int lane = threadIdx.x & 63;
+    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+         r < n_obs; r += wstride) {
+        const int64_t a = indptr[r], b = indptr[r + 1];
+        uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
+        int prev_last = -1;  // chunk of the last entry of the previous batch
+        // split offsets: an entry whose 128-gene chunk differs from its
+        // predecessor's opens chunks (prev, ch]; their offsets are its position
+        auto splits = [&](int64_t p0, bool ok, uint32_t c, int64_t p) {
+            const int ch = ok ? static_cast<int>(c >> 7) : 0x7fffffff;
+            int prev = __shfl_up(ch, 1);
+            if (lane == 0) prev = prev_last;
+            if (ok && ch > prev)
+                for (int g = prev + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p - a);
+            const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;
+            prev_last = __shfl(ch, cnt - 1);
+        };
+        auto emit = [&](int64_t p, uint32_t c, float v, double f) {
+            xt[p] = xtilde(v, f, flags);
+            lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
+            erow[p] = static_cast<uint8_t>(r & 255);
+        };
+        if (b - a <= U * kWave) {
+            // short row: every load issued at once (clamped, unconditional)
+            uint32_t c[U];
+            float v[U];
+            const int64_t last = b > a ? b - 1 : a;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                int64_t p = a + u * kWave + lane;
+                p = p < last ? p : last;
+                c[u] = idx[p];
+                v[u] = val[p];
+            }
+            double f = 1.0;
+            if (flags & kXfNorm) {
+                double sum = 0.0;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (a + u * kWave + lane < b) sum += static_cast<double>(v[u]);
+                sum = wave_sum(sum);
+                f = sum > 0.0 ? target / sum : 1.0;
+                if (lane == 0) rowf[r] = f;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int64_t p0 = a + u * kWave;
+                if (p0 >= b) break;
+                const int64_t p = p0 + lane;
+                const bool ok = p < b;
+                if (ok) emit(p, c[u], v[u], f);
+                if (nsplit > 0) splits(p0, ok, c[u], p);
+            }
+        } else {
+            double f = 1.0;
+            if (flags & kXfNorm) {
+                double sum = 0.0;
+                for (int64_t p = a + lane; p < b; p += kWave) sum += static_cast<double>(val[p]);
+                sum = wave_sum(sum);
+                f = sum > 0.0 ? target / sum : 1.0;
+                if (lane == 0) rowf[r] = f;
+            }
+            for (int64_t p0 = a; p0 < b; p0 += kWave) {
+                const int64_t p = p0 + lane;
+                const bool ok = p < b;
+                const uint32_t c = ok ? idx[p] : 0u;
+                if (ok) emit(p, c, val[p], f);
+                if (nsplit > 0) splits(p0, ok, c, p);
+            }
+        }
+        if (nsplit > 0) {
+            // chunks after the row's last entry start at the row's end
+            for (int g = prev_last + 1 + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
+            if (lane == 0) sr[nsplit] = static_cast<uint32_t>(b - a);
+        }
+    }
+}
+
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
                                                  uint16_t *__restrict__ out) {
@@ -1638,6 +1732,219 @@ __global__ void __launch_bounds__(256) k_chol_blk(const double *__restrict__ G, 
         W32[t] = static_cast<float>(v);
     }
     if (tid == 0 && bad) atomicOr(status, 1);
+}
+
+// Cholesky G (+ shift * trace(G) * I) = R^T R and W = R^-1 (f64, one block of
+// 256 threads, G in LDS as a padded full matrix). Right-looking with one
+// barrier per step: at step j every thread reads the pivot d = a[j][j] (final
+// since the previous barrier) and updates its (i, c) entries of the trailing
+// upper triangle, a[i][c] -= (a[j][i] / r)(a[j][c] / r) with r = sqrt(d), on a
+// 16 x 16 thread grid (no index division); row j itself is scaled in place
+// during step j + 1, when no thread reads it any more. W is then solved column
+// by column, W[i][c] = -(sum_{l=i+1..c} R[i][l] W[l][c]) / R[i][i], LPC lanes
+// per column (partial sums combined by shuffles in a fixed order, within one
+// wave, so no barriers), stored transposed in the free lower triangle.
+// ---------------------------------------------------------------------------
+template <int KP>
+__global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int k, double shift,
+                                               double *__restrict__ R64, double *__restrict__ W64,
+                                               float *__restrict__ W32, int *__restrict__ status) {
+    constexpr int LD = KP + 1;           // padded row; column KP holds W's diagonal
+    constexpr int LPC = 256 / KP;        // lanes per column in the inverse (4 or 2)
+    __shared__ double a[KP * LD];
+    __shared__ double rinv[KP];
+    __shared__ double trs;
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    for (int t = tid; t < KP * KP; t += 256) {
+        const int i = t / KP, c = t % KP;
+        a[i * LD + c] = (i < k && c < k) ? G[t] : 0.0;
+    }
+    __syncthreads();
+    if (shift > 0.0) {
+        if (tid < 64) {
+            double s = 0.0;
+            for (int i = tid; i < k; i += 64) s += a[i * LD + i];
+            s = wave_sum(s);
+            if (tid == 0) trs = s;
+        }
+        __syncthreads();
+        for (int i = tid; i < k; i += 256) a[i * LD + i] += shift * trs;
+        __syncthreads();
+    }
+    bool bad = false;
+    double r_prev = 1.0, inv_prev = 1.0;
+    for (int j = 0; j < k; ++j) {
+        const double d = a[j * LD + j];
+        const bool ok = (d > 0.0) && isfinite(d);
+        bad |= !ok;
+        const double r = ok ? sqrt(d) : 1.0;
+        const double inv = 1.0 / r;
+        if (j > 0) {  // scale row j-1 (no thread reads it in this step)
+            for (int c = j - 1 + tid; c < k; c += 256)
+                a[(j - 1) * LD + c] = (c == j - 1) ? r_prev : a[(j - 1) * LD + c] * inv_prev;
+        }
+        for (int i = j + 1 + ty; i < k; i += 16) {
+            const double ri = a[j * LD + i] * inv;
+            for (int c = i + ((tx - (i - (j + 1))) & 15); c < k; c += 16) {
+                // c runs over the columns >= i congruent to j+1+tx mod 16
+                a[i * LD + c] -= ri * (a[j * LD + c] * inv);
+            }
+        }
+        r_prev = r;
+        inv_prev = inv;
+        __syncthreads();
+    }
+    if (k > 0)
+        for (int c = k - 1 + tid; c < k; c += 256)
+            a[(k - 1) * LD + c] = (c == k - 1) ? r_prev : a[(k - 1) * LD + c] * inv_prev;
+    __syncthreads();
+    for (int i = tid; i < k; i += 256) rinv[i] = 1.0 / a[i * LD + i];
+    __syncthreads();
+    for (int t = tid; t < KP * KP; t += 256) {
+        const int i = t / KP, c = t % KP;
+        R64[t] = (i < k && c < k && c >= i) ? a[i * LD + c] : 0.0;
+    }
+    // inverse: lanes LPC*c .. LPC*c + LPC-1 own column c
+    {
+        const int c = tid / LPC, q = tid % LPC;
+        if (c < k) {
+            const double wcc = rinv[c];
+            if (q == 0) a[c * LD + KP] = wcc;
+            for (int i = c - 1; i >= 0; --i) {
+                double part = 0.0;
+                for (int l = i + 1 + q; l <= c; l += LPC) part += a[i * LD + l] * (l == c ? wcc : a[c * LD + l]);
+#pragma unroll
+                for (int o = 1; o < LPC; o <<= 1) part += __shfl_xor(part, o);
+                if (q == 0) a[c * LD + i] = -part * rinv[i];
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = tid; t < KP * KP; t += 256) {
+        const int i = t / KP, c = t % KP;
+        const double v = (i < k && c < k) ? (c > i ? a[c * LD + i] : (c == i ? a[c * LD + KP] : 0.0)) : 0.0;
+        W64[t] = v;
+        W32[t] = static_cast<float>(v);
+    }
+    if (bad && tid == 0) atomicOr(status, 1);
+}
+
+// Cholesky + inverse for KP = 64 with the matrix in registers: thread
+// (ty, tx) of a 16 x 16 grid owns entries (ty + 16a, tx + 16b), a, b < 4.
+// Factorization, step j: the owners of row j publish it to LDS (double
+// buffered, so one barrier per step), every thread reads the pivot and the
+// row-j values of its rows and columns and updates its trailing entries,
+// a[i][c] -= (a[j][i] / r)(a[j][c] / r), r = sqrt(a[j][j]); row j becomes R's
+// row j. Inverse (right-looking): X = I; for j = k-1 .. 0, W[j][:] = X[j][:] /
+// R[j][j] and X[i][:] -= R[i][j] W[j][:] for i < j, with row j of X and column
+// j of R published per step the same way. Same arithmetic as the textbook
+// algorithms, fixed order per entry.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_chol3(const double *__restrict__ G, int k, double shift,
+                                               double *__restrict__ R64, double *__restrict__ W64,
+                                               float *__restrict__ W32, int *__restrict__ status) {
+    constexpr int KP = 64, NB = KP / 16;
+    __shared__ double row[2][KP];
+    __shared__ double col[2][KP];
+    __shared__ double part[4];
+    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+    double a[NB][NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int i = ty + 16 * p, c = tx + 16 * q;
+            a[p][q] = (i < k && c < k) ? G[i * KP + c] : 0.0;
+        }
+    if (shift > 0.0) {
+        double s = 0.0;
+        if (tx == ty)
+#pragma unroll
+            for (int p = 0; p < NB; ++p) s += a[p][p];  // diagonal entries (i = c) owned when tx == ty
+        s = wave_sum(s);
+        if ((tid & 63) == 0) part[tid >> 6] = s;
+        __syncthreads();
+        const double tr = part[0] + part[1] + part[2] + part[3];
+        if (tx == ty)
+#pragma unroll
+            for (int p = 0; p < NB; ++p)
+                if (ty + 16 * p < k) a[p][p] += shift * tr;
+    }
+    bool bad = false;
+    for (int j = 0; j < k; ++j) {
+        const int b = j & 1, pj = j >> 4;
+        if (ty == (j & 15))
+#pragma unroll
+            for (int q = 0; q < NB; ++q) row[b][tx + 16 * q] = a[pj][q];
+        __syncthreads();
+        const double d = row[b][j];
+        const bool ok = (d > 0.0) && isfinite(d);
+        bad |= !ok;
+        const double r = ok ? sqrt(d) : 1.0;
+        const double inv = 1.0 / r;
+        double ri[NB], rc[NB];
+#pragma unroll
+        for (int p = 0; p < NB; ++p) ri[p] = row[b][ty + 16 * p] * inv;
+#pragma unroll
+        for (int q = 0; q < NB; ++q) rc[q] = row[b][tx + 16 * q] * inv;
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int i = ty + 16 * p, c = tx + 16 * q;
+                if (i > j && c >= i) a[p][q] -= ri[p] * rc[q];
+                if (i == j && c >= j) a[p][q] = (c == j) ? r : rc[q];
+            }
+    }
+    // R (upper) out; the lower part is zero
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int i = ty + 16 * p, c = tx + 16 * q;
+            R64[i * KP + c] = (i < k && c < k && c >= i) ? a[p][q] : 0.0;
+        }
+    // inverse: x starts as I; the R entries stay in a
+    double x[NB][NB];
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) x[p][q] = (ty + 16 * p == tx + 16 * q && ty + 16 * p < k) ? 1.0 : 0.0;
+    for (int j = k - 1; j >= 0; --j) {
+        const int b = j & 1, pj = j >> 4, qj = j >> 4;
+        // owners publish row j of X and column j of R (entries i <= j)
+        if (ty == (j & 15))
+#pragma unroll
+            for (int q = 0; q < NB; ++q) row[b][tx + 16 * q] = x[pj][q];
+        if (tx == (j & 15))
+#pragma unroll
+            for (int p = 0; p < NB; ++p) col[b][ty + 16 * p] = a[p][qj];
+        __syncthreads();
+        const double dinv = 1.0 / col[b][j];
+        double wr[NB], rcj[NB];
+#pragma unroll
+        for (int q = 0; q < NB; ++q) wr[q] = row[b][tx + 16 * q] * dinv;
+#pragma unroll
+        for (int p = 0; p < NB; ++p) rcj[p] = col[b][ty + 16 * p];
+#pragma unroll
+        for (int p = 0; p < NB; ++p)
+#pragma unroll
+            for (int q = 0; q < NB; ++q) {
+                const int i = ty + 16 * p;
+                if (i < j) x[p][q] -= rcj[p] * wr[q];
+                if (i == j) x[p][q] = wr[q];
+            }
+    }
+    if (bad && tid == 0) atomicOr(status, 1);
+#pragma unroll
+    for (int p = 0; p < NB; ++p)
+#pragma unroll
+        for (int q = 0; q < NB; ++q) {
+            const int i = ty + 16 * p, c = tx + 16 * q;
+            const double v = (i < k && c < k && c >= i) ? x[p][q] : 0.0;
+            W64[i * KP + c] = v;
+            W32[i * KP + c] = static_cast<float>(v);
+        }
 }
 
 // Column statistics on the tile-CSC (values are X~): block (tile range,

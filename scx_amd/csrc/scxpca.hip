@@ -211,18 +211,14 @@ void prepare(scxpca_matrix *m) {
     if (m->n_obs == 0) return;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
-    if (m->flags & kXfNorm) {
-        k_row_factor<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->val, m->n_obs, m->target, m->rowf.as<double>());
-        HIPC(hipGetLastError());
-    }
-    k_xtilde<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, rowf_ptr(m), m->flags, m->n_obs,
-                                                m->xt.as<float>(), m->lg.as<uint16_t>(), m->erow.as<uint8_t>());
+    static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
+    k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
+                                                      m->use_split ? m->nsplit : 0,
+                                                      (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
+                                                      m->xt.as<float>(),
+                                                      m->lg.as<uint16_t>(), m->erow.as<uint8_t>(),
+                                                      m->use_split ? m->spl.as<uint32_t>() : nullptr);
     HIPC(hipGetLastError());
-    if (m->use_split) {
-        k_splits<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->n_obs, kCB, m->nsplit,
-                                                    m->spl.as<uint32_t>());
-        HIPC(hipGetLastError());
-    }
     if (m->n_vars <= kBuild2Genes)
         k_build_tcsc2<<<m->n_tiles, 1024, 0, m->stream>>>(m->indptr, m->idx, m->erow.as<uint8_t>(),
                                                           m->xt.as<float>(), m->n_obs, static_cast<int>(m->n_vars),
@@ -567,7 +563,10 @@ struct Engine {
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
     void chol(double shift, double *Rdst) {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
-        k_chol_blk<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+        if constexpr (KP == 64)
+            k_chol3<<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+        else
+            k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
         check();
     }
 
