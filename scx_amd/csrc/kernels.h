@@ -91,21 +91,6 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Per-row preparation
 // ---------------------------------------------------------------------------
 
-// Row sums (f64) and the normalize_total factor; one wavefront per row.
-__global__ void __launch_bounds__(256) k_row_factor(const int64_t *__restrict__ indptr,
-                                                    const float *__restrict__ val, int64_t n_obs,
-                                                    double target, double *__restrict__ rowf) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-         r < n_obs; r += wstride) {
-        const int64_t a = indptr[r], b = indptr[r + 1];
-        double s = 0.0;
-        for (int64_t p = a + lane; p < b; p += kWave) s += static_cast<double>(val[p]);
-        s = wave_sum(s);
-        if (lane == 0) rowf[r] = s > 0.0 ? target / s : 1.0;
-    }
-}
 
 // Flags rows whose indices decrease (the chunked kernels need sorted rows).
 __global__ void __launch_bounds__(256) k_check_sorted(const int64_t *__restrict__ indptr,
@@ -128,65 +113,17 @@ __global__ void __launch_bounds__(256) k_check_sorted(const int64_t *__restrict_
     if (bad) atomicOr(flag, bad);
 }
 
-// spl[r][g] = number of nonzeros of row r with index < g*cb, g = 0..nsplit
-// (spl[r][nsplit] = row length). One wavefront per row, one lane per boundary.
-__global__ void __launch_bounds__(256) k_splits(const int64_t *__restrict__ indptr,
-                                                const uint32_t *__restrict__ idx, int64_t n_obs,
-                                                int cb, int nsplit, uint32_t *__restrict__ spl) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-         r < n_obs; r += wstride) {
-        const int64_t a = indptr[r], len = indptr[r + 1] - a;
-        for (int g = lane; g <= nsplit; g += kWave) {
-            uint32_t pos;
-            if (g == nsplit) {
-                pos = static_cast<uint32_t>(len);
-            } else {
-                const uint32_t bound = static_cast<uint32_t>(g) * static_cast<uint32_t>(cb);
-                int64_t lo = 0, hi = len;
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if (idx[a + mid] < bound) lo = mid + 1; else hi = mid;
-                }
-                pos = static_cast<uint32_t>(lo);
-            }
-            spl[r * (nsplit + 1) + g] = pos;
-        }
-    }
-}
 
-// X~ values in CSR order, one f32 per stored entry, computed once per
-// prepared matrix (every SpMM pass and the tile-CSC read these instead of
-// re-applying the transform per pass), the forward SpMM's gene index inside
-// its LDS panel chunk, lg = gene mod kChunkMod (u16; chunks of 512 or 256
-// genes start at multiples of their size, so lg mod 256 serves both), and the
-// entry's row within its 256-row tile (u8, for the tile-CSC build).
+
+// The forward SpMM's LDS panel chunks hold 512 (KP = 64) or 256 (KP = 128)
+// genes and start at multiples of their size, so gene mod 512 (stored as u16)
+// gives the row inside either chunk (mod 256 for the smaller one).
 constexpr uint32_t kChunkMod = 512;
-__global__ void __launch_bounds__(256) k_xtilde(const int64_t *__restrict__ indptr,
-                                                const uint32_t *__restrict__ idx,
-                                                const float *__restrict__ val,
-                                                const double *__restrict__ rowf, int flags,
-                                                int64_t n_obs, float *__restrict__ out,
-                                                uint16_t *__restrict__ lg, uint8_t *__restrict__ erow) {
-    const int lane = threadIdx.x & 63;
-    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-         r < n_obs; r += wstride) {
-        const int64_t a = indptr[r], b = indptr[r + 1];
-        const double f = rowf ? rowf[r] : 1.0;
-        for (int64_t p = a + lane; p < b; p += kWave) {
-            out[p] = xtilde(val[p], f, flags);
-            lg[p] = static_cast<uint16_t>(idx[p] & (kChunkMod - 1));
-            erow[p] = static_cast<uint8_t>(r & 255);  // row within its 256-row tile
-        }
-    }
-}
 
 // Per-row preparation in one pass over the CSR (one wave per row): the row
 // total (lane partials, wave reduction; exact for integer counts) and factor
 // t/s (1 where s <= 0; csr.rs:543-552, transforms.rs:21-72), X~ in CSR order,
-// the chunk-local gene index lg, the row-in-tile index, and, when nsplit > 0,
+// the chunk-local gene index lg and, when nsplit > 0,
 // the split offsets spl[r][g] = number of entries with gene < g * 128 (rows
 // are sorted: written by the entries that open a chunk, and the row length for
 // chunks after the last entry; spl[r][nsplit] = row length).
@@ -195,8 +132,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                                                       const float *__restrict__ val, int64_t n_obs,
                                                       double target, int flags, int nsplit,
                                                       double *__restrict__ rowf, float *__restrict__ xt,
-                                                      uint16_t *__restrict__ lg, uint8_t *__restrict__ erow,
-                                                      uint32_t *__restrict__ spl) {
+                                                      uint16_t *__restrict__ lg, uint32_t *__restrict__ spl) {
     constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
@@ -219,7 +155,6 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         auto emit = [&](int64_t p, uint32_t c, float v, double f) {
             xt[p] = xtilde(v, f, flags);
             lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
-            erow[p] = static_cast<uint8_t>(r & 255);
         };
         if (b - a <= U * kWave) {
             // short row: every load issued at once (clamped, unconditional)
@@ -634,14 +569,6 @@ __global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d
 // ===========================================================================
 
 // Lane t of each 16-lane row broadcast to the whole row (DPP row_newbcast, gfx90a+).
-template <int T>
-__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
-    return static_cast<uint32_t>(__builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x150 + T, 0xf, 0xf, true));
-}
-template <int T>
-__device__ __forceinline__ float row_bcast(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x150 + T, 0xf, 0xf, true));
-}
 
 // Group-gather loop. The four 16-lane groups take the entries held by lane T
 // of their row (entries 4T+g of the batch, by the interleaved load), read E
@@ -983,118 +910,168 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
     if (threadIdx.x == 0) tp[n_vars] = carry;
 }
 
-// Tile-CSC build for n_vars <= kBuild2Genes, one block per tile, two flat
-// passes over the tile's entries (contiguous in the CSR, so every load is
-// coalesced; 16 entries per thread in flight):
-//  1. row-membership masks: bit r of mask[r / 32][c] is set for each entry
-//     (row r, gene c) with an LDS atomic OR (word-major layout: the lanes of
-//     a wave hit distinct genes, so distinct banks);
-//  2. an exclusive scan of per-gene counts gives each run's start; an entry
-//     lands at start[c] + popcount(mask bits of gene c in rows < r), so runs
-//     come out in row order whatever the schedule (deterministic).
-// Loads are unconditional at clamped indices so the waits stay partial.
-constexpr int kBuild2Genes = 2048;
-__global__ void __launch_bounds__(1024) k_build_tcsc2(const int64_t *__restrict__ indptr,
+// Tile-CSC build from the split offsets (n_vars <= 8192): one block per
+// (tile, 128-gene chunk). Row r's entries of the chunk are CSR positions
+// indptr[r] + [spl[r][s], spl[r][s+1]); the chunk's entries precede all later
+// chunks' in the tile, so its output starts at sum_r spl[r][s] (a block
+// reduction). The block flattens its rows' ranges (row found by binary search
+// over the scanned lengths), builds per-gene row masks with LDS atomic ORs,
+// scans the 128 gene counts, and places each entry at start[c] +
+// popcount(mask bits of gene c in rows < r), so runs are in row order whatever
+// the schedule (deterministic). The chunk's output is assembled in LDS and
+// written contiguously; a chunk with more than kBuild3Stage entries in one
+// tile (only near-dense genes) writes its entries directly instead.
+constexpr int kBuild3Stage = 4096;
+__global__ void __launch_bounds__(256) k_build_tcsc3(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ idx,
-                                                     const uint8_t *__restrict__ erow,
-                                                     const float *__restrict__ xt, int64_t n_obs, int n_vars,
-                                                     uint32_t *__restrict__ tptr, uint8_t *__restrict__ trow,
-                                                     float *__restrict__ tval) {
-    constexpr int NT = 1024, U = 16;
-    __shared__ uint32_t mask[8][kBuild2Genes];  // 64 KB
-    __shared__ uint32_t start[kBuild2Genes];    // 8 KB
-    __shared__ uint8_t pre[kBuild2Genes][8];    // 16 KB: entries of gene c in mask words < w
-    __shared__ uint32_t wsum[2][16];
-    const int t = blockIdx.x, tid = threadIdx.x, lane = tid & 63, wave = tid / kWave;
+                                                     const float *__restrict__ xt,
+                                                     const uint32_t *__restrict__ spl, int nsplit, int64_t n_obs,
+                                                     int64_t n_vars, uint32_t *__restrict__ tptr,
+                                                     uint8_t *__restrict__ trow, float *__restrict__ tval) {
+    constexpr int NT = 256, CBG = 128;
+    __shared__ uint32_t mask[8][CBG];
+    __shared__ uint32_t start[CBG];
+    __shared__ uint8_t pre[CBG][8];
+    __shared__ uint32_t flat[kTileRows + 1];  // scanned range lengths per row
+    __shared__ uint32_t lo_s[kTileRows];
+    __shared__ float sval[kBuild3Stage];
+    __shared__ uint8_t srow[kBuild3Stage];
+    __shared__ uint32_t wred[2][4];
+    const int t = blockIdx.x / nsplit, sc = blockIdx.x % nsplit;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
-    const int64_t r1 = r0 + kTileRows < n_obs ? r0 + kTileRows : n_obs;
+    const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
     const int64_t base = indptr[r0];
-    const int64_t ne = indptr[r1] - base;
-    const int64_t p_last = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
-    uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
-    for (int i = tid; i < 8 * kBuild2Genes; i += NT) (&mask[0][0])[i] = 0u;
-    __syncthreads();
-    // pass 1: masks
-    for (int64_t k0 = 0; k0 < ne; k0 += static_cast<int64_t>(NT) * U) {
-        uint32_t c[U];
-        uint32_t rr[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            int64_t p = base + k0 + tid + u * NT;
-            p = p < p_last ? p : p_last;
-            c[u] = idx[p];
-            rr[u] = erow[p];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (k0 + tid + u * NT < ne) atomicOr(&mask[rr[u] >> 5][c[u]], 1u << (rr[u] & 31));
+    // per row: range of this chunk, and the entries of earlier chunks
+    uint32_t len = 0u, before = 0u;
+    if (tid < nr) {
+        const uint32_t *sr = spl + (r0 + tid) * (nsplit + 1);
+        const uint32_t l0 = sr[sc], l1 = sr[sc + 1];
+        before = l0;
+        len = l1 - l0;
+        lo_s[tid] = static_cast<uint32_t>(indptr[r0 + tid] - base) + l0;  // tile-relative CSR offset
     }
-    __syncthreads();
-    // per-gene counts and prefix per mask word, then a block scan for starts
-    // (thread j owns genes 2j, 2j+1: n_vars <= 2048)
-    uint32_t cnt2[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int cg = 2 * tid + h;
-        uint32_t acc = 0u;
-        if (cg < n_vars)
-#pragma unroll
-            for (int w = 0; w < 8; ++w) {
-                pre[cg][w] = static_cast<uint8_t>(acc);
-                acc += __popc(mask[w][cg]);
-            }
-        cnt2[h] = acc;
-    }
-    const uint32_t local = cnt2[0] + cnt2[1];
-    uint32_t incl = local;
+    for (int i = tid; i < 8 * CBG; i += NT) (&mask[0][0])[i] = 0u;
+    // block scans: exclusive prefix of len (flat) and the sum of before
+    uint32_t incl = len, bsum = before;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
         const uint32_t v = __shfl_up(incl, o);
         if (lane >= o) incl += v;
     }
-    if (lane == kWave - 1) wsum[0][wave] = incl;
+    bsum = static_cast<uint32_t>(wave_sum(static_cast<int>(bsum)));
+    if (lane == kWave - 1) wred[0][wave] = incl;
+    if (lane == 0) wred[1][wave] = bsum;
     __syncthreads();
-    if (tid < 16) {
-        uint32_t pre_w = 0u;
-        for (int w = 0; w < tid; ++w) pre_w += wsum[0][w];
-        wsum[1][tid] = pre_w;
-    }
+    uint32_t wpre = 0u;
+    for (int w = 0; w < wave; ++w) wpre += wred[0][w];
+    const uint32_t total = wred[0][0] + wred[0][1] + wred[0][2] + wred[0][3];
+    const uint32_t cbase = wred[1][0] + wred[1][1] + wred[1][2] + wred[1][3];
+    flat[tid] = wpre + incl - len;
+    if (tid == 0) flat[kTileRows] = total;
     __syncthreads();
-    {
-        const uint32_t s0 = wsum[1][wave] + incl - local;
-        if (2 * tid < n_vars) {
-            start[2 * tid] = s0;
-            tp[2 * tid] = s0;
+    // row of flattened entry f: largest row with flat[row] <= f (rows with
+    // len 0 share their successor's start; upper_bound skips them)
+    auto row_of = [&](uint32_t f) {
+        int lo = 0, hi = nr;  // answer in [0, nr)
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (flat[mid] <= f) lo = mid; else hi = mid;
         }
-        if (2 * tid + 1 < n_vars) {
-            start[2 * tid + 1] = s0 + cnt2[0];
-            tp[2 * tid + 1] = s0 + cnt2[0];
-        }
-        if (tid == 0) tp[n_vars] = static_cast<uint32_t>(ne);
-    }
-    __syncthreads();
-    // pass 2: placement
-    for (int64_t k0 = 0; k0 < ne; k0 += static_cast<int64_t>(NT) * U) {
+        return lo;
+    };
+    const uint32_t g0 = static_cast<uint32_t>(sc) * CBG;
+    const int64_t p_last = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
+    // pass 1: masks. U entries per thread per batch: the row searches (LDS)
+    // first, then all U loads in flight, then the atomics.
+    constexpr int U = 8;
+    for (uint32_t f0 = 0; f0 < total; f0 += NT * U) {
+        int rr[U];
         uint32_t c[U];
-        uint32_t rr[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t f = f0 + tid + u * NT;
+            rr[u] = row_of(f < total ? f : total - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t f = f0 + tid + u * NT;
+            const uint32_t fc = f < total ? f : total - 1;
+            int64_t p = base + lo_s[rr[u]] + (fc - flat[rr[u]]);
+            p = p < p_last ? p : p_last;
+            c[u] = idx[p];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (f0 + tid + u * NT < total) atomicOr(&mask[rr[u] >> 5][c[u] - g0], 1u << (rr[u] & 31));
+    }
+    __syncthreads();
+    // counts and per-word prefix of 128 genes; thread j < 128 owns gene j
+    uint32_t cnt = 0u;
+    if (tid < CBG) {
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+            pre[tid][w] = static_cast<uint8_t>(cnt);
+            cnt += __popc(mask[w][tid]);
+        }
+    }
+    uint32_t ci = cnt;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t v = __shfl_up(ci, o);
+        if (lane >= o) ci += v;
+    }
+    if (lane == kWave - 1 && wave < 2) wred[0][wave] = ci;
+    __syncthreads();
+    if (tid < CBG) {
+        const uint32_t st = (wave == 1 ? wred[0][0] : 0u) + ci - cnt;
+        start[tid] = st;
+        if (g0 + tid < n_vars) tptr[static_cast<int64_t>(t) * (n_vars + 1) + g0 + tid] = cbase + st;
+    }
+    if (tid == 0 && sc == nsplit - 1)
+        tptr[static_cast<int64_t>(t) * (n_vars + 1) + n_vars] = static_cast<uint32_t>(indptr[r0 + nr] - base);
+    __syncthreads();
+    // pass 2: placement (LDS staging when the chunk fits)
+    const bool staged = total <= static_cast<uint32_t>(kBuild3Stage);
+    for (uint32_t f0 = 0; f0 < total; f0 += NT * U) {
+        int rr[U];
+        uint32_t c[U];
         float x[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            int64_t p = base + k0 + tid + u * NT;
+            const uint32_t f = f0 + tid + u * NT;
+            rr[u] = row_of(f < total ? f : total - 1);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t f = f0 + tid + u * NT;
+            const uint32_t fc = f < total ? f : total - 1;
+            int64_t p = base + lo_s[rr[u]] + (fc - flat[rr[u]]);
             p = p < p_last ? p : p_last;
             c[u] = idx[p];
-            rr[u] = erow[p];
             x[u] = xt[p];
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            if (k0 + tid + u * NT < ne) {
-                const uint32_t w = rr[u] >> 5, bit = rr[u] & 31;
-                const uint32_t below = bit ? (mask[w][c[u]] & ((1u << bit) - 1u)) : 0u;
-                const uint32_t pos = start[c[u]] + pre[c[u]][w] + __popc(below);
-                trow[base + pos] = static_cast<uint8_t>(rr[u]);
-                tval[base + pos] = x[u];
+            if (f0 + tid + u * NT >= total) continue;
+            const uint32_t cc = c[u] - g0;
+            const uint32_t w = static_cast<uint32_t>(rr[u]) >> 5, bit = static_cast<uint32_t>(rr[u]) & 31u;
+            const uint32_t below = bit ? (mask[w][cc] & ((1u << bit) - 1u)) : 0u;
+            const uint32_t pos = start[cc] + pre[cc][w] + __popc(below);
+            if (staged) {
+                srow[pos] = static_cast<uint8_t>(rr[u]);
+                sval[pos] = x[u];
+            } else {
+                trow[base + cbase + pos] = static_cast<uint8_t>(rr[u]);
+                tval[base + cbase + pos] = x[u];
             }
+        }
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t q = tid; q < total; q += NT) {
+            trow[base + cbase + q] = srow[q];
+            tval[base + cbase + q] = sval[q];
         }
     }
 }
@@ -1116,156 +1093,6 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
         s += tp[c + 1] - tp[c];
     }
     if (s) atomicAdd(&cnt[c], s);
-}
-
-// ---------------------------------------------------------------------------
-// Transposed SpMM over the tile-CSC, load-balanced: part[tr] = X~[rows of tile
-// range tr]^T Q for one (tile range, gene group) per block. The genes are
-// dealt to the 16 x GW wave slots of all blocks through gperm (genes sorted by
-// entry count, snake order), so every wave carries about the same number of
-// entries and the per-tile barriers do not wait on one dense gene. Per tile
-// the block stages Q's 256 x KP tile in LDS (prefetched into registers during
-// the previous tile); each wave reads its genes' run bounds from tptr (one
-// gene per lane, read out with v_readlane) and the run entries (row u8, X~)
-// straight from HBM, the first 64-entry batch of each run PF genes ahead, in a
-// ring that runs on across tiles. Groups take entries 4t+g of a run; every
-// output element is owned by one lane and the sum order over a gene's entries
-// is fixed (tile order, then run order), so results are deterministic and do
-// not depend on the gene order.
-// Grid: blockIdx = slot * 8 + xcd; the nch blocks of one tile range run on one
-// XCD at the same time and share its L2 copy of the Q tiles.
-// ---------------------------------------------------------------------------
-template <int KP, int GW, int PF, bool BF16>
-__global__ void __launch_bounds__(1024) k_spmm_t4(const int64_t *__restrict__ indptr,
-                                                  const uint32_t *__restrict__ tptr,
-                                                  const uint8_t *__restrict__ trow,
-                                                  const float *__restrict__ tval,
-                                                  const uint16_t *__restrict__ txbf,
-                                                  const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
-                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
-                                                  int64_t n_vars, int n_tiles, float *__restrict__ part) {
-    constexpr int E = KP / 16;
-    constexpr int NW = 16;
-    constexpr int NT = NW * kWave;
-    static_assert(GW <= 16 && GW % PF == 0, "one gene per lane of a group; static ring slots");
-    __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
-    const int tid = threadIdx.x;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
-    const int wave = tid / kWave, lane = tid & 63, g = lane >> 4, l = lane & 15;
-    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
-    const int esel = 4 * l + g;
-    // lane l (< GW) of every group holds gene slot l of this wave
-    const int64_t gslot = (static_cast<int64_t>(gc) * NW + wave) * GW + l;
-    const uint32_t gene = (l < GW) ? gperm[gslot] : static_cast<uint32_t>(n_vars);
-    const bool gvalid = gene < static_cast<uint64_t>(n_vars);
-    float z[GW][E];
-#pragma unroll
-    for (int gi = 0; gi < GW; ++gi)
-#pragma unroll
-        for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
-    const int t_begin = tr * tiles_per_tr;
-    const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
-    const int t_end = (tr < ntr) ? t_end0 : t_begin;
-
-    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
-    // Loads are unconditional at clamped indices, validity applied at use: a
-    // load under a data-dependent branch would make every later wait drain
-    // all outstanding loads and defeat the prefetch.
-    const uint32_t gene_c = gvalid ? gene : 0u;
-    auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
-        const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
-        const uint32_t a0 = tp[gene_c], b0 = tp[gene_c + 1];
-        a = gvalid ? a0 : 0u;
-        b = gvalid ? b0 : 0u;
-    };
-    constexpr int QPT = kTileRows * KP / 4 / NT;  // float4 of the Q tile per thread
-    float4 qpre[QPT];
-    auto load_q = [&](int tt) {
-        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
-        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
-        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
-#pragma unroll
-        for (int j = 0; j < QPT; ++j) {
-            const int i = tid + j * NT;
-            const bool ok = i / (KP / 4) < nr;
-            const float4 v = Q4[ok ? i : 0];
-            qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-    };
-    const int64_t p_max = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
-    // first 64-entry batch of gene gi's run in tile tt, PF genes ahead
-    uint32_t prow[PF];
-    float pval[PF];
-    auto issue = [&](int gi, int64_t tb, uint32_t wl, uint32_t &pr, float &pv) {
-        int64_t p = tb + rl32(wl, gi) + esel;
-        p = p < p_max ? p : p_max;
-        pr = trow[p];
-        pv = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
-    };
-    uint32_t wlo = 0u, whi = 0u, nlo = 0u, nhi = 0u;
-    int64_t tb = 0, ntb = 0;
-    if (t_begin < t_end) {
-        bounds(t_begin, wlo, whi);
-        tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
-        load_q(t_begin);
-#pragma unroll
-        for (int gi = 0; gi < PF; ++gi) issue(gi, tb, wlo, prow[gi], pval[gi]);
-    }
-    for (int t = t_begin; t < t_end; ++t) {
-        const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
-        __syncthreads();  // previous tile's gathers are done with qs
-#pragma unroll
-        for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
-        __syncthreads();
-        load_q(tn);
-        bounds(tn, nlo, nhi);
-        ntb = indptr[static_cast<int64_t>(tn) * kTileRows];
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) {
-            const uint32_t lo = rl32(wlo, gi), hi = rl32(whi, gi);
-            const uint32_t cr = prow[gi % PF];
-            const float cv = pval[gi % PF];
-            if (gi + PF < GW)
-                issue(gi + PF, tb, wlo, prow[gi % PF], pval[gi % PF]);
-            else
-                issue(gi + PF - GW, ntb, nlo, prow[gi % PF], pval[gi % PF]);
-            for (uint32_t o = lo; o < hi; o += kWave) {
-                const int cnt = (hi - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi - o) : kWave;
-                uint32_t rr = cr;
-                float v = cv;
-                if (o != lo) {  // later batches of a long run: read directly
-                    int64_t p = tb + o + esel;
-                    p = p < p_max ? p : p_max;
-                    rr = trow[p];
-                    v = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
-                }
-                const bool ok = esel < cnt;
-                const uint32_t ofs = ok ? rr * (KP * 4) : 0u;
-                const float x = ok ? v : 0.f;
-                gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
-            }
-        }
-        wlo = nlo;
-        whi = nhi;
-        tb = ntb;
-    }
-    // reduce the four group partials and write this block's partial of Z
-#pragma unroll
-    for (int gi = 0; gi < GW; ++gi) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            z[gi][e] += __shfl_xor(z[gi][e], 16);
-            z[gi][e] += __shfl_xor(z[gi][e], 32);
-        }
-        const uint32_t c = rl32(gene, gi);
-        if (tr < ntr && c < static_cast<uint64_t>(n_vars) && g == 0) {
-            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
-#pragma unroll
-            for (int e = 0; e < E / 4; ++e)
-                dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
-        }
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1650,90 +1477,6 @@ __global__ void __launch_bounds__(256) k_apply2(const float *__restrict__ Y, con
 }
 
 // ---------------------------------------------------------------------------
-// Cholesky (f64, LDS, 256 threads: the trailing update is parallel over the
-// (i, c) pairs, one barrier per step) and W = R^-1 row by row from the bottom:
-// W[i][c] = -(sum_{l=i+1..c} R[i][l] W[l][c]) / R[i][i], each entry's dot
-// product split over 4 threads and combined in a fixed order. Both triangles
-// are stored packed (upper, row-major) so KP = 128 fits in LDS.
-// ---------------------------------------------------------------------------
-template <int KP>
-__device__ __forceinline__ int upk(int i, int c) {  // c >= i
-    return i * KP - (i * (i - 1)) / 2 + (c - i);
-}
-
-template <int KP>
-__global__ void __launch_bounds__(256) k_chol_blk(const double *__restrict__ G, int k, double shift,
-                                                  double *__restrict__ R64, double *__restrict__ W64,
-                                                  float *__restrict__ W32, int *__restrict__ status) {
-    constexpr int NP = KP * (KP + 1) / 2;
-    __shared__ double a[NP];
-    __shared__ double w[NP];
-    __shared__ int bad;
-    __shared__ double tr;
-    const int tid = threadIdx.x;
-    if (tid == 0) bad = 0;
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, c = t % KP;
-        if (c >= i) a[upk<KP>(i, c)] = G[t];
-    }
-    __syncthreads();
-    if (shift > 0.0) {
-        if (tid == 0) {
-            double s = 0.0;
-            for (int i = 0; i < k; ++i) s += a[upk<KP>(i, i)];
-            tr = s;
-        }
-        __syncthreads();
-        for (int i = tid; i < k; i += blockDim.x) a[upk<KP>(i, i)] += shift * tr;
-        __syncthreads();
-    }
-    for (int j = 0; j < k; ++j) {
-        const double d = a[upk<KP>(j, j)];
-        const bool ok = (d > 0.0) && isfinite(d);
-        const double rjj = ok ? sqrt(d) : 1.0;
-        __syncthreads();  // everyone read the pivot before it is overwritten
-        if (tid == 0) { a[upk<KP>(j, j)] = rjj; if (!ok) bad = 1; }
-        for (int c = j + 1 + tid; c < k; c += blockDim.x) a[upk<KP>(j, c)] /= rjj;
-        __syncthreads();
-        const int m = k - j - 1;
-        for (int t = tid; t < m * m; t += blockDim.x) {
-            const int i = j + 1 + t / m, c = j + 1 + t % m;
-            if (c >= i) a[upk<KP>(i, c)] -= a[upk<KP>(j, i)] * a[upk<KP>(j, c)];
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, c = t % KP;
-        R64[t] = (i < k && c < k && c >= i) ? a[upk<KP>(i, c)] : 0.0;
-    }
-    for (int t = tid; t < NP; t += blockDim.x) w[t] = 0.0;
-    __syncthreads();
-    const int q = tid & 3, cc = tid >> 2;  // 64 columns x 4 parts per pass
-    for (int i = k - 1; i >= 0; --i) {
-        const double inv = 1.0 / a[upk<KP>(i, i)];
-        for (int c0 = 0; c0 < KP; c0 += 64) {
-            const int c = c0 + cc;
-            double part = 0.0;
-            if (c > i && c < k)
-                for (int l = i + 1 + q; l <= c; l += 4) part += a[upk<KP>(i, l)] * w[upk<KP>(l, c)];
-            part += __shfl_xor(part, 1);
-            part += __shfl_xor(part, 2);
-            if (q == 0 && c < k) {
-                if (c > i) w[upk<KP>(i, c)] = -part * inv;
-                else if (c == i) w[upk<KP>(i, c)] = inv;
-            }
-        }
-        __syncthreads();
-    }
-    for (int t = tid; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, c = t % KP;
-        const double v = (i < k && c < k && c >= i) ? w[upk<KP>(i, c)] : 0.0;
-        W64[t] = v;
-        W32[t] = static_cast<float>(v);
-    }
-    if (tid == 0 && bad) atomicOr(status, 1);
-}
-
 // Cholesky G (+ shift * trace(G) * I) = R^T R and W = R^-1 (f64, one block of
 // 256 threads, G in LDS as a padded full matrix). Right-looking with one
 // barrier per step: at step j every thread reads the pivot d = a[j][j] (final

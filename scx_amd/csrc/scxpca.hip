@@ -147,7 +147,6 @@ struct scxpca_matrix {
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
     DevBuf lg;    // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
-    DevBuf erow;  // row within its 256-row tile per entry (u8), for the tile-CSC build
     DevBuf xbf;
     bool xbf_ready = false;
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
@@ -215,15 +214,13 @@ void prepare(scxpca_matrix *m) {
     k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
                                                       m->use_split ? m->nsplit : 0,
                                                       (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
-                                                      m->xt.as<float>(),
-                                                      m->lg.as<uint16_t>(), m->erow.as<uint8_t>(),
+                                                      m->xt.as<float>(), m->lg.as<uint16_t>(),
                                                       m->use_split ? m->spl.as<uint32_t>() : nullptr);
     HIPC(hipGetLastError());
-    if (m->n_vars <= kBuild2Genes)
-        k_build_tcsc2<<<m->n_tiles, 1024, 0, m->stream>>>(m->indptr, m->idx, m->erow.as<uint8_t>(),
-                                                          m->xt.as<float>(), m->n_obs, static_cast<int>(m->n_vars),
-                                                          m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
-                                                          m->tval.as<float>());
+    if (m->use_split)
+        k_build_tcsc3<<<static_cast<unsigned>(m->n_tiles) * m->nsplit, 256, 0, m->stream>>>(
+            m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, m->n_obs, m->n_vars,
+            m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>());
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                          static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
@@ -356,7 +353,6 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
-    m->erow.ensure(std::max<int64_t>(m->nnz, 1));
     prepare(m);
     HIPC(hipStreamSynchronize(m->stream));
 }
