@@ -581,7 +581,10 @@ __global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d
 // reads run two steps ahead of the FMAs (three rotating buffers), and the step
 // count is wave-uniform so the exits are scalar branches. Slots past cnt carry
 // x = 0 and offset 0 (a valid address), so the look-ahead reads are
-// unconditional: up to two reads past the last step.
+// unconditional: up to two reads past the last step. A scheduling barrier
+// keeps each look-ahead read ahead of the current step's FMAs (otherwise the
+// scheduler sinks it next to its use to save registers and every step waits
+// for its own read).
 template <int T>
 __device__ __forceinline__ uint32_t bcast_add(uint32_t ofs, uint32_t lane_ofs) {
     uint32_t a;
@@ -626,6 +629,7 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
 #define SCX_STEP(T, CUR, NXT)                                                  \
     if ((T) >= steps) return;                                                  \
     if ((T) + 2 < 16) gather_load<((T) + 2) & 15, E>(NXT, ofs, lds, lane_ofs); \
+    __builtin_amdgcn_sched_barrier(0);                                         \
     gather_fma<T, E>(acc, CUR, x);
     SCX_STEP(0, b0, b2) SCX_STEP(1, b1, b0) SCX_STEP(2, b2, b1) SCX_STEP(3, b0, b2)
     SCX_STEP(4, b1, b0) SCX_STEP(5, b2, b1) SCX_STEP(6, b0, b2) SCX_STEP(7, b1, b0)
