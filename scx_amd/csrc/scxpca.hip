@@ -568,9 +568,13 @@ struct Engine {
 
     // Tall f32 panel CholeskyQR: CQR2, or shifted CQR3 (stable) for Householder mode.
     // Returns the buffer that holds Q (one of src / tmp).
-    float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1) {
+    // With final == false (a basis that only feeds the next product of the
+    // power iteration) one shifted pass: Q = Y R^-1 spans Y exactly for any
+    // nonsingular R, and the shift keeps the f32 Gram positive definite; the
+    // final basis gets the requested method.
+    float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true) {
         if (n < 0) n = m->n_obs;
-        const int passes = qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2;
+        const int passes = !final ? 1 : (qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2);
         const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
         const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, gram_blocks)));
         int64_t rpg = (n + gblocks - 1) / gblocks;
@@ -583,7 +587,7 @@ struct Engine {
             check();
             k_gram_reduce<KP><<<(KP * KP + 63) / 64, 256, 0, s>>>(m->gpart.as<double>(), gblocks, G);
             check();
-            const double shift = (qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0;
+            const double shift = !final ? 1e-7 : ((qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0);
             chol(shift, R1);
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
@@ -639,15 +643,16 @@ struct Engine {
         float *Y = m->y.as<float>(), *T = m->q.as<float>();
         if (centered) mean_correction(m->panel.as<float>());
         spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
-        float *Q = cqr_tall(Y, T, p->qr_method);
+        const int64_t n_it = p->n_power_iterations;
+        float *Q = cqr_tall(Y, T, p->qr_method, -1, n_it == 0);
         float *Qother = (Q == Y) ? T : Y;
-        for (int64_t it = 0; it < p->n_power_iterations; ++it) {
-            // B (f32 n_vars x KP) -> orthonormal panel by the tall MFMA CholeskyQR
+        for (int64_t it = 0; it < n_it; ++it) {
+            // B (f32 n_vars x KP) -> basis by the tall MFMA CholeskyQR
             spmm_t(Q, centered, sketch_bf16, nullptr, m->panel.as<float>());
-            float *qb = cqr_tall(m->panel.as<float>(), m->panel2.as<float>(), p->qr_method, nv);
+            float *qb = cqr_tall(m->panel.as<float>(), m->panel2.as<float>(), p->qr_method, nv, false);
             if (centered) mean_correction(qb);
             spmm_fwd(qb, centered, sketch_bf16, Qother);
-            float *Qn = cqr_tall(Qother, Q, p->qr_method);
+            float *Qn = cqr_tall(Qother, Q, p->qr_method, -1, it + 1 == n_it);
             Qother = (Qn == Q) ? Qother : Q;
             Q = Qn;
         }
