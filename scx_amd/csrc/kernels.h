@@ -1118,11 +1118,13 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
                                                   const uint16_t *__restrict__ txbf,
                                                   const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
                                                   int nch, int ntr, int tiles_per_tr, int64_t n_obs,
-                                                  int64_t n_vars, int n_tiles, float *__restrict__ part) {
+                                                  int64_t n_vars, int n_tiles, float *__restrict__ part,
+                                                  double *__restrict__ qsum_part) {
     constexpr int E = KP / 16;
     constexpr int NW = 16;
     constexpr int NT = NW * kWave;
     constexpr int NB = CAP / kWave;  // prefetch slots per lane
+    constexpr int QC = KP / kWave;   // Q columns per lane in the column sums
     static_assert(GW <= 16 && CAP % kWave == 0, "one gene per lane of a group");
     __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
     __shared__ uint2 ents[NW][CAP];           // per-wave entry lists (Q-row byte offset, x~)
@@ -1204,6 +1206,13 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
         }
     };
 
+    // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
+    // with r mod nch == gc (wave w takes the slice's rows w, w + 16, ...), so
+    // the extra work is spread over all blocks of the tile range
+    const bool do_qsum = qsum_part != nullptr;
+    double qsum[QC];
+#pragma unroll
+    for (int e = 0; e < QC; ++e) qsum[e] = 0.0;
     uint32_t wlo = 0u, whi = 0u;
     int64_t tb = 0;
     if (t_begin < t_end) {
@@ -1223,6 +1232,12 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
 #pragma unroll
         for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
         __syncthreads();
+        if (do_qsum) {
+            const float *qf = reinterpret_cast<const float *>(qs);
+            for (int rr = gc + nch * wave; rr < kTileRows; rr += nch * NW)
+#pragma unroll
+                for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
+        }
         // next tile: Q tile, run layout (bounds loaded one tile ago), entries
         load_q(tn);
         const int64_t ntb = indptr[static_cast<int64_t>(tn) * kTileRows];
@@ -1275,6 +1290,28 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
                 dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
         }
     }
+    if (do_qsum) {  // the 16 waves' column sums, added in wave order
+        __syncthreads();
+        double *red = reinterpret_cast<double *>(qs);  // NW x KP f64 fits in the Q tile buffer
+#pragma unroll
+        for (int e = 0; e < QC; ++e) red[wave * KP + lane + kWave * e] = qsum[e];
+        __syncthreads();
+        if (tid < KP) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += red[w * KP + tid];
+            qsum_part[(static_cast<int64_t>(tr) * nch + gc) * KP + tid] = t;
+        }
+    }
+}
+
+// 1^T Q from the transposed SpMM's per-block partials (fixed order).
+template <int KP>
+__global__ void k_sum_cols(const double *__restrict__ part, int nrb, double *__restrict__ out) {
+    const int c = threadIdx.x;
+    if (c >= KP) return;
+    double t = 0.0;
+    for (int r = 0; r < nrb; ++r) t += part[static_cast<int64_t>(r) * KP + c];
+    out[c] = t;
 }
 
 // Gram with in-block reduction: each wave accumulates its upper tiles over its

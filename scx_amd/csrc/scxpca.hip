@@ -457,7 +457,7 @@ struct Engine {
         m->tpart.ensure(sizeof(float) * static_cast<size_t>(nrb_t) * std::max<int64_t>(nv, 1) * KP);
         const int nb64 = gram64_blocks();
         m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(std::max(gram_waves, gram_blocks), nb64)) * KP * KP);
-        m->cpart.ensure(sizeof(double) * static_cast<size_t>(kMaxColBlocks) * KP);
+        m->cpart.ensure(sizeof(double) * static_cast<size_t>(std::max(kMaxColBlocks, nrb_t * nch_t)) * KP);
         constexpr size_t KK = static_cast<size_t>(KP) * KP;
         m->small.ensure(sizeof(double) * (KK * 11 + 4 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + KP));
         double *d = m->small.as<double>();
@@ -488,16 +488,6 @@ struct Engine {
     }
 
     // sumq = 1^T Q (f64)
-    void column_sums(const float *Q) {
-        const int64_t n = m->n_obs;
-        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 1023) / 1024, kMaxColBlocks)));
-        const int64_t rpb = (n + nb - 1) / nb;
-        k_colsum<KP, float><<<nb, 256, 0, s>>>(Q, nullptr, n, rpb, m->cpart.as<double>());
-        check();
-        k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, sumq, nullptr);
-        check();
-    }
-
     void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
         if (m->n_obs == 0) return;
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
@@ -530,7 +520,7 @@ struct Engine {
     // B64 (and/or B32) = X~^T Q - mu sumq^T
     void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
         const int64_t nv = m->n_vars;
-        if (centered) column_sums(Q);
+        double *qsp = centered ? m->cpart.as<double>() : nullptr;  // nrb_t x nch_t x KP partials of 1^T Q
         if (m->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
@@ -540,15 +530,20 @@ struct Engine {
                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(),
                                                                    m->txbf.as<uint16_t>(), Q, gp, nch_t, nrb_t,
                                                                    tiles_per_tr, m->n_obs, nv, m->n_tiles,
-                                                                   m->tpart.as<float>());
+                                                                   m->tpart.as<float>(), qsp);
             else
                 k_spmm_t5<KP, GW, TCAP, false><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                     m->trow.as<uint8_t>(), m->tval.as<float>(), nullptr,
                                                                     Q, gp, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
-                                                                    m->n_tiles, m->tpart.as<float>());
+                                                                    m->n_tiles, m->tpart.as<float>(), qsp);
             check();
         } else {
             HIPC(hipMemsetAsync(m->tpart.p, 0, sizeof(float) * nrb_t * nv * KP, s));
+            if (qsp) HIPC(hipMemsetAsync(qsp, 0, sizeof(double) * nrb_t * nch_t * KP, s));
+        }
+        if (centered) {
+            k_sum_cols<KP><<<1, KP, 0, s>>>(qsp, nrb_t * nch_t, sumq);
+            check();
         }
         StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
         k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb_t, nv,
