@@ -173,6 +173,12 @@ scxpca_status scxpca_project_backward(scxpca_matrix *m, const float *dz, int64_t
  * f64, draw i -> element i (random_gaussian, cpu.rs:1174-1178). Host only. */
 scxpca_status scxpca_gaussian(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
 
+/* The same Omega produced the way the PCA produces it on `device` (keystream
+ * and draw outcomes on the GPU, the chain of draw starts on the host), copied
+ * to host f64 row-major; cols <= 128. Same reference stream as above
+ * (cpu.rs:1174-1178); exists so the device path can be checked bit for bit. */
+scxpca_status scxpca_gaussian_device(uint64_t rows, uint64_t cols, uint64_t seed, int device, double *out);
+
 /* Device discovery (scx-accel/src/pca/gpu.rs:81-133, device.rs:96-162). */
 int scxpca_device_count(void);
 scxpca_status scxpca_device_info(int device, char *name, size_t name_len, int64_t *total_mem,

@@ -39,6 +39,7 @@ EXPORTS = (
     "scxpca_project_forward",
     "scxpca_project_backward",
     "scxpca_gaussian",
+    "scxpca_gaussian_device",
     "scxpca_device_count",
     "scxpca_device_info",
     "scxpca_last_error",
@@ -110,6 +111,7 @@ def lib() -> ctypes.CDLL:
     l.scxpca_project_forward.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
     l.scxpca_project_backward.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
     l.scxpca_gaussian.argtypes = [_u64, _u64, _u64, _vp]
+    l.scxpca_gaussian_device.argtypes = [_u64, _u64, _u64, ctypes.c_int, _vp]
     l.scxpca_device_count.argtypes = []
     l.scxpca_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_i64),
                                      ctypes.POINTER(_i32)]
@@ -160,6 +162,13 @@ def device_info(device: int = 0) -> dict:
 def gaussian(rows: int, cols: int, seed: int) -> np.ndarray:
     out = np.empty((rows, cols), dtype=np.float64)
     check(lib().scxpca_gaussian(_u64(rows), _u64(cols), _u64(seed), ptr(out)))
+    return out
+
+
+def gaussian_device(rows: int, cols: int, seed: int, device: int = 0) -> np.ndarray:
+    """Omega as the PCA generates it on the GPU (cols <= 128)."""
+    out = np.empty((rows, cols), dtype=np.float64)
+    check(lib().scxpca_gaussian_device(_u64(rows), _u64(cols), _u64(seed), device, ptr(out)))
     return out
 
 

@@ -212,6 +212,109 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Omega on the device (the reference stream, scx-accel/src/pca/cpu.rs:1174-1178:
+// ChaCha8Rng::seed_from_u64 + StandardNormal).
+// k_chacha8_u64: u64 draw w = (u32[2w], u32[2w+1]) of the ChaCha8 keystream
+// (8 rounds, 64-bit block counter in words 12-13, zero nonce), one 16-word
+// block per thread.
+// k_zig_outcome: for every draw position w, the outcome of a StandardNormal
+// draw that starts there (rand_distr's 256-box ziggurat): the number of u64
+// draws it consumes and its value. Flagged (bit 7) for the host to recompute
+// with its libm: tail draws (their value goes through log), wedge tests whose
+// exp comparison is within 4 ulp (the device exp may round differently), and
+// draws that would leave the computed words. Everything else is IEEE
+// multiplies and comparisons (no contraction), identical to the host.
+// ---------------------------------------------------------------------------
+struct ChaChaKey { uint32_t k[8]; };
+
+__device__ __forceinline__ uint32_t rotl32(uint32_t v, int r) { return (v << r) | (v >> (32 - r)); }
+
+__global__ void __launch_bounds__(256) k_chacha8_u64(ChaChaKey key, uint64_t n_words, uint64_t *__restrict__ out) {
+    const uint64_t b = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (8 * b >= n_words) return;
+    uint32_t s[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u,
+                      key.k[0], key.k[1], key.k[2], key.k[3], key.k[4], key.k[5], key.k[6], key.k[7],
+                      static_cast<uint32_t>(b), static_cast<uint32_t>(b >> 32), 0u, 0u};
+    uint32_t x[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) x[i] = s[i];
+#define SCX_QR(a, b2, c, d)                      \
+    x[a] += x[b2]; x[d] = rotl32(x[d] ^ x[a], 16); \
+    x[c] += x[d]; x[b2] = rotl32(x[b2] ^ x[c], 12); \
+    x[a] += x[b2]; x[d] = rotl32(x[d] ^ x[a], 8);  \
+    x[c] += x[d]; x[b2] = rotl32(x[b2] ^ x[c], 7);
+#pragma unroll
+    for (int dr = 0; dr < 4; ++dr) {
+        SCX_QR(0, 4, 8, 12) SCX_QR(1, 5, 9, 13) SCX_QR(2, 6, 10, 14) SCX_QR(3, 7, 11, 15)
+        SCX_QR(0, 5, 10, 15) SCX_QR(1, 6, 11, 12) SCX_QR(2, 7, 8, 13) SCX_QR(3, 4, 9, 14)
+    }
+#undef SCX_QR
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint64_t w = 8 * b + i;
+        if (w < n_words)
+            out[w] = static_cast<uint64_t>(x[2 * i] + s[2 * i]) |
+                     (static_cast<uint64_t>(x[2 * i + 1] + s[2 * i + 1]) << 32);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_zig_outcome(const uint64_t *__restrict__ W, uint64_t n_words,
+                                                     const double *__restrict__ X, const double *__restrict__ F,
+                                                     uint8_t *__restrict__ c_out, double *__restrict__ v_out) {
+#pragma clang fp contract(off)
+    constexpr uint64_t kExp2 = static_cast<uint64_t>(1023 + 1) << 52;  // [2,4)
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t w = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; w < n_words; w += stride) {
+        uint32_t c = 0;
+        bool flag = false;
+        double val = 0.0;
+        for (;;) {
+            if (w + c >= n_words || c > 120) { flag = true; break; }
+            const uint64_t bits = W[w + c];
+            ++c;
+            const unsigned i = static_cast<unsigned>(bits & 0xffu);
+            const double u = __builtin_bit_cast(double, (bits >> 12) | kExp2) - 3.0;
+            const double xv = u * X[i];
+            if (fabs(xv) < X[i + 1]) { val = xv; break; }
+            if (i == 0) { flag = true; break; }  // tail
+            if (w + c >= n_words) { flag = true; break; }
+            const double unif = static_cast<double>(W[w + c] >> 11) * 0x1p-53;
+            ++c;
+            const double lhs = F[i + 1] + (F[i] - F[i + 1]) * unif;
+            const double pdf = exp(-xv * xv / 2.0);
+            if (fabs(lhs - pdf) <= 4.0 * 2.220446049250313e-16 * pdf) { flag = true; break; }
+            if (lhs < pdf) { val = xv; break; }
+        }
+        c_out[w] = flag ? static_cast<uint8_t>(0x80u) : static_cast<uint8_t>(c);
+        v_out[w] = val;
+    }
+}
+
+__global__ void k_omega_override(double *__restrict__ v, const uint32_t *__restrict__ pos,
+                                 const double *__restrict__ val, int n) {
+    const int i = static_cast<int>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[pos[i]] = val[i];
+}
+
+// Omega element (r, j) = draw r * k + j, as the f32 panel (KP columns, zero pad)
+// and, when out64 is given, as f64 row-major rows x k.
+template <int KP>
+__global__ void __launch_bounds__(256) k_omega_panel(const double *__restrict__ v, const uint32_t *__restrict__ start,
+                                                     int64_t rows, int k, float *__restrict__ panel,
+                                                     double *__restrict__ out64) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= rows * KP) return;
+    const int64_t r = t / KP;
+    const int j = static_cast<int>(t % KP);
+    double x = 0.0;
+    if (j < k) {
+        x = v[start[r * k + j]];
+        if (out64) out64[r * k + j] = x;
+    }
+    if (panel) panel[t] = static_cast<float>(x);
+}
+
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
                                                  uint16_t *__restrict__ out) {
@@ -924,17 +1027,20 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
 // popcount(mask bits of gene c in rows < r), so runs are in row order whatever
 // the schedule (deterministic). The chunk's output is assembled in LDS and
 // written contiguously; a chunk with more than kBuild3Stage entries in one
-// tile (only near-dense genes) writes its entries directly instead.
+// tile (only near-dense genes) writes its entries directly instead. When psum
+// is given, the block also writes the tile's per-gene sum and sum of squares
+// of X~ (the column statistics' partials).
 constexpr int kBuild3Stage = 4096;
 __global__ void __launch_bounds__(256) k_build_tcsc3(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ idx,
                                                      const float *__restrict__ xt,
                                                      const uint32_t *__restrict__ spl, int nsplit, int64_t n_obs,
                                                      int64_t n_vars, uint32_t *__restrict__ tptr,
-                                                     uint8_t *__restrict__ trow, float *__restrict__ tval) {
+                                                     uint8_t *__restrict__ trow, float *__restrict__ tval,
+                                                     double *__restrict__ psum, double *__restrict__ psq) {
     constexpr int NT = 256, CBG = 128;
     __shared__ uint32_t mask[8][CBG];
-    __shared__ uint32_t start[CBG];
+    __shared__ uint32_t start[CBG + 1];
     __shared__ uint8_t pre[CBG][8];
     __shared__ uint32_t flat[kTileRows + 1];  // scanned range lengths per row
     __shared__ uint32_t lo_s[kTileRows];
@@ -1027,6 +1133,7 @@ __global__ void __launch_bounds__(256) k_build_tcsc3(const int64_t *__restrict__
     }
     if (lane == kWave - 1 && wave < 2) wred[0][wave] = ci;
     __syncthreads();
+    if (tid == 0) start[CBG] = total;
     if (tid < CBG) {
         const uint32_t st = (wave == 1 ? wred[0][0] : 0u) + ci - cnt;
         start[tid] = st;
@@ -1071,13 +1178,44 @@ __global__ void __launch_bounds__(256) k_build_tcsc3(const int64_t *__restrict__
             }
         }
     }
-    if (staged) {
-        __syncthreads();
+    __syncthreads();
+    if (staged)
         for (uint32_t q = tid; q < total; q += NT) {
             trow[base + cbase + q] = srow[q];
             tval[base + cbase + q] = sval[q];
         }
+    // column statistics of this tile: sum and sum of squares of X~ per gene,
+    // in run (row) order, f64
+    if (psum && tid < CBG && g0 + tid < n_vars) {
+        const uint32_t a = start[tid], e = start[tid + 1];
+        double sm = 0.0, sq = 0.0;
+        for (uint32_t q = a; q < e; ++q) {
+            const double v = static_cast<double>(staged ? sval[q] : tval[base + cbase + q]);
+            sm += v;
+            sq += v * v;
+        }
+        const int64_t o = static_cast<int64_t>(t) * n_vars + g0 + tid;
+        psum[o] = sm;
+        psq[o] = sq;
     }
+}
+
+// Column statistics from per-tile partials: out[r][v] = sum over the tiles
+// of range r (fixed order); a second call with one range finishes the sums.
+__global__ void __launch_bounds__(256) k_stats_ranges(const double *__restrict__ psum, const double *__restrict__ psq,
+                                                      int n_rows, int rows_per_range, int64_t n_vars,
+                                                      double *__restrict__ osum, double *__restrict__ osq) {
+    const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (v >= n_vars) return;
+    const int r0 = blockIdx.y * rows_per_range;
+    const int r1 = r0 + rows_per_range < n_rows ? r0 + rows_per_range : n_rows;
+    double s = 0.0, q = 0.0;
+    for (int r = r0; r < r1; ++r) {
+        s += psum[static_cast<int64_t>(r) * n_vars + v];
+        q += psq[static_cast<int64_t>(r) * n_vars + v];
+    }
+    osum[static_cast<int64_t>(blockIdx.y) * n_vars + v] = s;
+    osq[static_cast<int64_t>(blockIdx.y) * n_vars + v] = q;
 }
 
 // Entries per gene from the tile-CSC run bounds: cnt[c] += sum over the
@@ -1304,14 +1442,23 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
     }
 }
 
-// 1^T Q from the transposed SpMM's per-block partials (fixed order).
+// 1^T Q from the transposed SpMM's per-block partials: 256 threads, 256 / KP
+// strided subsets per column, combined in subset order (fixed order).
 template <int KP>
-__global__ void k_sum_cols(const double *__restrict__ part, int nrb, double *__restrict__ out) {
-    const int c = threadIdx.x;
-    if (c >= KP) return;
+__global__ void __launch_bounds__(256) k_sum_cols(const double *__restrict__ part, int nrb,
+                                                  double *__restrict__ out) {
+    constexpr int NS = 256 / KP;
+    __shared__ double red[256];
+    const int c = threadIdx.x % KP, sub = threadIdx.x / KP;
     double t = 0.0;
-    for (int r = 0; r < nrb; ++r) t += part[static_cast<int64_t>(r) * KP + c];
-    out[c] = t;
+    for (int r = sub; r < nrb; r += NS) t += part[static_cast<int64_t>(r) * KP + c];
+    red[threadIdx.x] = t;
+    __syncthreads();
+    if (sub == 0) {
+        double v = 0.0;
+        for (int q = 0; q < NS; ++q) v += red[q * KP + c];
+        out[c] = v;
+    }
 }
 
 // Gram with in-block reduction: each wave accumulates its upper tiles over its

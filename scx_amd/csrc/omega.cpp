@@ -8,14 +8,19 @@
 // rand 0.8.5 (Standard / Open01 f64 conversions), rand_distr 0.4.3
 // (StandardNormal ziggurat, 256 boxes, tables printed with 18 decimals).
 //
-// Ω is drawn on the host: the ziggurat's rejection steps make the position of
-// draw i depend on every earlier draw, so the stream is sequential. It is
-// generated while the GPU runs the column-statistics pass.
+// The ziggurat's rejection steps make the position of draw i depend on every
+// earlier draw. The PCA therefore computes, on the device, the keystream and
+// the outcome of a draw starting at every position, and the host only walks
+// the chain of start positions (plus the rare draws whose value or decision
+// depends on libm's exp/log, recomputed here with the host libm); the whole
+// matrix is also available here (gaussian_matrix) for the C ABI.
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <utility>
+#include <vector>
 
 namespace scxpca {
 
@@ -149,6 +154,61 @@ double standard_normal(ChaCha8Stream &rng, const ZigTables &t) {
 }
 
 }  // namespace
+
+// ---- pieces for the device-side generation ----
+// The u64 draws of a stream that only ever calls next_u64 are the pairs
+// (u32[2w], u32[2w+1]) of the ChaCha8 keystream (refills keep the read
+// position even), so draw w can be computed from its block alone.
+
+void omega_key(uint64_t seed, uint32_t key[8]) {
+    const ChaCha8Stream s(seed);
+    for (int i = 0; i < 8; ++i) key[i] = s.key[i];
+}
+
+void omega_tables(double *x, double *f) {
+    const ZigTables &t = tables();
+    for (int i = 0; i <= kBoxes; ++i) {
+        x[i] = t.x[i];
+        f[i] = t.f[i];
+    }
+}
+
+// One StandardNormal draw starting at u64 draw w of the seed's stream, with
+// the host libm: returns the value and the number of u64 draws it consumed.
+double omega_draw_at(uint64_t seed, uint64_t w, uint32_t *consumed) {
+    ChaCha8Stream rng(seed);
+    rng.block = (2 * w / 64) * 4;
+    rng.refill();
+    rng.pos = static_cast<unsigned>((2 * w) % 64);
+    const uint64_t before = rng.block * 16 - (64 - rng.pos);  // u32 words consumed so far
+    const double v = standard_normal(rng, tables());
+    const uint64_t after = rng.block * 16 - (64 - rng.pos);
+    *consumed = static_cast<uint32_t>((after - before) / 2);
+    return v;
+}
+
+// Walk the chain of draw starts: s_0 = 0, s_{j+1} = s_j + c(s_j), with c from
+// the device (bits 0-6: draws consumed; bit 7: recompute on the host, then
+// the host value overrides the device one). Returns false when the chain
+// leaves the n_words computed on the device.
+bool omega_walk(const uint8_t *c, uint64_t n_words, uint64_t n_samples, uint64_t seed, uint32_t *start,
+                std::vector<std::pair<uint32_t, double>> *overrides) {
+    uint64_t w = 0;
+    for (uint64_t j = 0; j < n_samples; ++j) {
+        if (w >= n_words) return false;
+        start[j] = static_cast<uint32_t>(w);
+        const uint8_t cw = c[w];
+        if (cw & 0x80u) {
+            uint32_t used = 0;
+            const double v = omega_draw_at(seed, w, &used);
+            overrides->emplace_back(static_cast<uint32_t>(w), v);
+            w += used;
+        } else {
+            w += cw;
+        }
+    }
+    return true;
+}
 
 void gaussian_matrix(uint64_t rows, uint64_t cols, uint64_t seed, double *out) {
     const ZigTables &t = tables();

@@ -39,6 +39,10 @@
 
 namespace scxpca {
 void gaussian_matrix(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
+void omega_key(uint64_t seed, uint32_t key[8]);
+void omega_tables(double *x, double *f);
+bool omega_walk(const uint8_t *c, uint64_t n_words, uint64_t n_samples, uint64_t seed, uint32_t *start,
+                std::vector<std::pair<uint32_t, double>> *overrides);
 }
 
 using namespace scxpca;
@@ -132,6 +136,15 @@ struct DeviceGuard {
 // Resident matrix
 // ---------------------------------------------------------------------------
 
+// Buffers of the device Omega generation. The host vectors stay alive with
+// their owner, so the uploads from them need no synchronization.
+struct OmegaScratch {
+    DevBuf w, c, v, start, tab, ov;
+    std::vector<uint8_t> hc;
+    std::vector<uint32_t> hstart, hpos;
+    std::vector<double> htab, hval;
+};
+
 struct scxpca_matrix {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -154,16 +167,22 @@ struct scxpca_matrix {
     int n_tiles = 0;
     // entries per gene (host copy) and the load-balanced gene order of the
     // transposed SpMM for one wave-slot layout (GW genes x 16 waves x nch)
-    DevBuf gcnt, gperm;
-    std::vector<uint32_t> host_gcnt, host_gperm;
-    int gperm_gw = 0, gperm_nch = 0;
+    DevBuf gcnt;
+    std::vector<uint32_t> host_gcnt;
+    struct GeneOrder {
+        int gw = 0, nch = 0;
+        bool valid = false;
+        std::vector<uint32_t> host;  // kept alive for the asynchronous upload
+        DevBuf dev;
+    } gorder[2];  // one per wave-slot layout (KP = 64 and KP = 128)
     bool use_split = false;
     // cached column statistics of X~
     bool stats_ready = false;
+    DevBuf tsum, tsq;  // per-tile column-statistics partials (written by the tile-CSC build)
     DevBuf sum, sq, cnt, psum, psq, pcnt;
     // PCA workspace
     DevBuf y, q, panel, panel2, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
-    std::vector<double> host_omega;
+    OmegaScratch om;  // device Omega scratch (device and host buffers)
     // stage profiler (events on the matrix stream; off unless enabled)
     bool profiling = false;
     struct Mark { int stage; hipEvent_t a, b; };
@@ -220,7 +239,8 @@ void prepare(scxpca_matrix *m) {
     if (m->use_split)
         k_build_tcsc3<<<static_cast<unsigned>(m->n_tiles) * m->nsplit, 256, 0, m->stream>>>(
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, m->n_obs, m->n_vars,
-            m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>());
+            m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
+            m->tsq.as<double>());
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                          static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
@@ -241,33 +261,36 @@ void prepare(scxpca_matrix *m) {
         HIPC(hipMemcpyAsync(m->host_gcnt.data(), m->gcnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost,
                             m->stream));
     HIPC(hipStreamSynchronize(m->stream));
-    m->gperm_gw = 0;
+    for (auto &g : m->gorder) g.valid = false;
 }
 
 // Gene order for the transposed SpMM: genes sorted by entry count (descending,
 // ties by index) are dealt to the nch*16 waves in snake order (0..W-1, W-1..0,
 // ...), round r filling slot r of each wave, so per-wave entry totals differ by
 // at most about one gene's count. Unused slots hold n_vars.
-void ensure_gene_order(scxpca_matrix *m, int gw, int nch) {
-    if (m->gperm_gw == gw && m->gperm_nch == nch) return;
+const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch) {
+    auto &go = m->gorder[gw == 8 ? 0 : 1];
+    if (go.valid && go.gw == gw && go.nch == nch) return go.dev.as<uint32_t>();
     const int64_t nv = m->n_vars;
     const int64_t W = static_cast<int64_t>(nch) * 16, S = W * gw;
     std::vector<uint32_t> order(nv);
     for (int64_t c = 0; c < nv; ++c) order[c] = static_cast<uint32_t>(c);
     std::stable_sort(order.begin(), order.end(),
                      [&](uint32_t a, uint32_t b) { return m->host_gcnt[a] > m->host_gcnt[b]; });
-    m->host_gperm.assign(std::max<int64_t>(S, 1), static_cast<uint32_t>(nv));
+    // the previous upload from go.host (if any) was drained by prepare's sync
+    go.host.assign(std::max<int64_t>(S, 1), static_cast<uint32_t>(nv));
     for (int64_t i = 0; i < nv; ++i) {
         const int64_t r = i / W, pos = i % W;
         const int64_t w = (r % 2 == 0) ? pos : W - 1 - pos;
-        m->host_gperm[w * gw + r] = order[i];
+        go.host[w * gw + r] = order[i];
     }
-    m->gperm.ensure(sizeof(uint32_t) * m->host_gperm.size());
-    HIPC(hipMemcpyAsync(m->gperm.p, m->host_gperm.data(), sizeof(uint32_t) * m->host_gperm.size(),
-                        hipMemcpyHostToDevice, m->stream));
-    HIPC(hipStreamSynchronize(m->stream));
-    m->gperm_gw = gw;
-    m->gperm_nch = nch;
+    go.dev.ensure(sizeof(uint32_t) * go.host.size());
+    HIPC(hipMemcpyAsync(go.dev.p, go.host.data(), sizeof(uint32_t) * go.host.size(), hipMemcpyHostToDevice,
+                        m->stream));
+    go.gw = gw;
+    go.nch = nch;
+    go.valid = true;
+    return go.dev.as<uint32_t>();
 }
 
 void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1p) {
@@ -353,6 +376,10 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    if (m->use_split) {
+        m->tsum.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
+        m->tsq.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
+    }
     prepare(m);
     HIPC(hipStreamSynchronize(m->stream));
 }
@@ -375,12 +402,37 @@ void ensure_xbf(scxpca_matrix *m) {
 // cached on the matrix.
 void ensure_stats(scxpca_matrix *m) {
     if (m->stats_ready) return;
-    const int nch = static_cast<int>((m->n_vars + 127) / 128);  // k_col_stats4: 16 waves x 8 genes
-    const int ntr = std::max(1, std::min(m->n_tiles, std::max(1, 512 / nch)));
-    const int tiles_per_tr = (m->n_tiles + ntr - 1) / ntr;
     m->sum.ensure(sizeof(double) * m->n_vars);
     m->sq.ensure(sizeof(double) * m->n_vars);
     m->cnt.ensure(sizeof(uint32_t) * m->n_vars);
+    if (m->n_obs == 0) {  // no rows: all statistics are zero
+        HIPC(hipMemsetAsync(m->sum.p, 0, sizeof(double) * m->n_vars, m->stream));
+        HIPC(hipMemsetAsync(m->sq.p, 0, sizeof(double) * m->n_vars, m->stream));
+        HIPC(hipMemsetAsync(m->cnt.p, 0, sizeof(uint32_t) * m->n_vars, m->stream));
+        m->stats_ready = true;
+        return;
+    }
+    if (m->use_split) {
+        // the tile-CSC build wrote per-tile partials: tiles -> ranges -> totals
+        StageScope prof(m, m->stream, SCXPCA_STAGE_STATS);
+        const int nrg = std::max(1, std::min(m->n_tiles, 64));
+        const int tpr = (m->n_tiles + nrg - 1) / nrg;
+        m->psum.ensure(sizeof(double) * nrg * m->n_vars);
+        m->psq.ensure(sizeof(double) * nrg * m->n_vars);
+        const unsigned gx = static_cast<unsigned>((m->n_vars + 255) / 256);
+        k_stats_ranges<<<dim3(gx, nrg), 256, 0, m->stream>>>(m->tsum.as<double>(), m->tsq.as<double>(), m->n_tiles,
+                                                             tpr, m->n_vars, m->psum.as<double>(), m->psq.as<double>());
+        HIPC(hipGetLastError());
+        k_stats_ranges<<<dim3(gx, 1), 256, 0, m->stream>>>(m->psum.as<double>(), m->psq.as<double>(), nrg, nrg,
+                                                           m->n_vars, m->sum.as<double>(), m->sq.as<double>());
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(m->cnt.p, m->gcnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToDevice, m->stream));
+        m->stats_ready = true;
+        return;
+    }
+    const int nch = static_cast<int>((m->n_vars + 127) / 128);  // k_col_stats4: 16 waves x 8 genes
+    const int ntr = std::max(1, std::min(m->n_tiles, std::max(1, 512 / nch)));
+    const int tiles_per_tr = (m->n_tiles + ntr - 1) / ntr;
     m->psum.ensure(sizeof(double) * ntr * m->n_vars);
     m->psq.ensure(sizeof(double) * ntr * m->n_vars);
     m->pcnt.ensure(sizeof(uint32_t) * ntr * m->n_vars);
@@ -410,6 +462,9 @@ __global__ void k_mean(const double *sum, int64_t n_vars, double inv_n, double *
 // ---------------------------------------------------------------------------
 
 template <int KP>
+void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc);
+
+template <int KP>
 struct Engine {
     static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t5 (Z in registers)
     static constexpr int TCAP = (KP == 64) ? 256 : 128;  // per-wave entry list in k_spmm_t5
@@ -424,6 +479,7 @@ struct Engine {
     float *W32, *mc32;
     int *status;
     int nrb_t = 0, nch_t = 0, ncol_blk = 0, gram_waves = 0, gram_blocks = 0, tiles_per_tr = 0;
+    const uint32_t *gperm = nullptr;  // wave-slot gene order of the transposed SpMM
     int64_t rows_per_wave = 0, rows_per_colblk = 0, rows_per_gblk = 0;
 
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
@@ -434,7 +490,7 @@ struct Engine {
         ntr = std::min(ntr, std::max(1, m->n_tiles));
         nrb_t = (ntr + 7) / 8 * 8;
         tiles_per_tr = (m->n_tiles + nrb_t - 1) / nrb_t;
-        ensure_gene_order(m, GW, nch_t);
+        gperm = ensure_gene_order(m, GW, nch_t);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
@@ -524,7 +580,7 @@ struct Engine {
         if (m->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
-            const uint32_t *gp = m->gperm.as<uint32_t>();
+            const uint32_t *gp = gperm;
             if (bf16)
                 k_spmm_t5<KP, GW, TCAP, true><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(),
@@ -542,7 +598,7 @@ struct Engine {
             if (qsp) HIPC(hipMemsetAsync(qsp, 0, sizeof(double) * nrb_t * nch_t * KP, s));
         }
         if (centered) {
-            k_sum_cols<KP><<<1, KP, 0, s>>>(qsp, nrb_t * nch_t, sumq);
+            k_sum_cols<KP><<<1, 256, 0, s>>>(qsp, nrb_t * nch_t, sumq);
             check();
         }
         StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
@@ -621,7 +677,7 @@ struct Engine {
         check();
     }
 
-    void run(const scxpca_params *p, scxpca_result *out, const double *host_omega) {
+    void run(const scxpca_params *p, scxpca_result *out) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         const bool centered = p->zero_center != 0;
         const bool sketch_bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
@@ -631,10 +687,8 @@ struct Engine {
         ensure_stats(m);
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, 1.0 / static_cast<double>(n), mu);
         check();
-        // Omega -> f32 panel
-        HIPC(hipMemcpyAsync(m->panel64.p, host_omega, sizeof(double) * nv * k, hipMemcpyHostToDevice, s));
-        k_pack_panel<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->panel64.as<double>(), nv, k, m->panel.as<float>());
-        check();
+        // Omega -> f32 panel, generated on the device
+        device_omega<KP>(s, static_cast<uint64_t>(nv), k, p->seed, m->panel.as<float>(), nullptr, m->om);
         float *Y = m->y.as<float>(), *T = m->q.as<float>();
         if (centered) mean_correction(m->panel.as<float>());
         spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
@@ -776,6 +830,78 @@ int64_t pca_k(const scxpca_matrix *m, const scxpca_params *p) {
     return std::min<int64_t>(std::min<int64_t>(p->n_comps + p->n_oversamples, m->n_vars), m->n_obs);
 }
 
+// Omega (rows x k, the reference stream) on the device: keystream and
+// per-position draw outcomes in parallel, the chain of draw starts walked on
+// the host, values gathered into the f32 panel (KP columns) and/or f64 out64.
+// Scratch comes from the DevBufs given (the matrix's, or the caller's).
+template <int KP>
+void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc) {
+    const uint64_t n = rows * static_cast<uint64_t>(k);
+    if (n == 0) return;
+    uint64_t n_words = n + n / 16 + 4096;
+    // the previous call's uploads from the host vectors were drained by the
+    // synchronization that precedes every walk (same stream)
+    for (int attempt = 0; attempt < 3; ++attempt, n_words *= 2) {
+        if (n_words >= (1ull << 32)) break;
+        sc.w.ensure(sizeof(uint64_t) * n_words);
+        sc.c.ensure(n_words);
+        sc.v.ensure(sizeof(double) * n_words);
+        sc.tab.ensure(sizeof(double) * 2 * 257);
+        ChaChaKey key;
+        omega_key(seed, key.k);
+        sc.htab.resize(2 * 257);
+        omega_tables(sc.htab.data(), sc.htab.data() + 257);
+        HIPC(hipMemcpyAsync(sc.tab.p, sc.htab.data(), sizeof(double) * sc.htab.size(), hipMemcpyHostToDevice, s));
+        k_chacha8_u64<<<blocks_for((n_words + 7) / 8), 256, 0, s>>>(key, n_words, sc.w.as<uint64_t>());
+        HIPC(hipGetLastError());
+        k_zig_outcome<<<std::min(blocks_for(n_words), 65536u), 256, 0, s>>>(
+            sc.w.as<uint64_t>(), n_words, sc.tab.as<double>(), sc.tab.as<double>() + 257, sc.c.as<uint8_t>(),
+            sc.v.as<double>());
+        HIPC(hipGetLastError());
+        sc.hc.resize(n_words);
+        HIPC(hipMemcpyAsync(sc.hc.data(), sc.c.p, n_words, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        sc.hstart.resize(n);
+        std::vector<std::pair<uint32_t, double>> ov;
+        if (!omega_walk(sc.hc.data(), n_words, n, seed, sc.hstart.data(), &ov)) continue;  // more words
+        sc.start.ensure(sizeof(uint32_t) * n);
+        HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        if (!ov.empty()) {
+            sc.hpos.resize(ov.size());
+            sc.hval.resize(ov.size());
+            for (size_t i = 0; i < ov.size(); ++i) {
+                sc.hpos[i] = ov[i].first;
+                sc.hval[i] = ov[i].second;
+            }
+            sc.ov.ensure((sizeof(uint32_t) + sizeof(double)) * ov.size());
+            double *dval = sc.ov.as<double>();
+            uint32_t *dpos = reinterpret_cast<uint32_t *>(dval + ov.size());
+            HIPC(hipMemcpyAsync(dval, sc.hval.data(), sizeof(double) * ov.size(), hipMemcpyHostToDevice, s));
+            HIPC(hipMemcpyAsync(dpos, sc.hpos.data(), sizeof(uint32_t) * ov.size(), hipMemcpyHostToDevice, s));
+            k_omega_override<<<blocks_for(ov.size()), 256, 0, s>>>(sc.v.as<double>(), dpos, dval,
+                                                                    static_cast<int>(ov.size()));
+            HIPC(hipGetLastError());
+        }
+        k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
+                                                                 static_cast<int64_t>(rows), k, panel, out64);
+        HIPC(hipGetLastError());
+        return;
+    }
+    // the chain needed far more draws than expected: the host stream
+    HIPC(hipStreamSynchronize(s));
+    sc.hval.resize(n);
+    gaussian_matrix(rows, static_cast<uint64_t>(k), seed, sc.hval.data());
+    sc.v.ensure(sizeof(double) * n);
+    HIPC(hipMemcpyAsync(sc.v.p, sc.hval.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
+    sc.start.ensure(sizeof(uint32_t) * n);
+    sc.hstart.resize(n);
+    for (uint64_t i = 0; i < n; ++i) sc.hstart[i] = static_cast<uint32_t>(i);
+    HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
+                                                             static_cast<int64_t>(rows), k, panel, out64);
+    HIPC(hipGetLastError());
+}
+
 void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     validate(m, p);
     const int64_t k = pca_k(m, p);
@@ -783,21 +909,18 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
         throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples = " + std::to_string(k) +
                                             " exceeds the GPU panel limit of 128");
     DeviceGuard dg(m->device);
-    m->host_omega.resize(static_cast<size_t>(m->n_vars * k));
-    // Launch the stats pass first so Omega generation overlaps it.
     if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_xbf(m);
     ensure_stats(m);
-    gaussian_matrix(static_cast<uint64_t>(m->n_vars), static_cast<uint64_t>(k), p->seed, m->host_omega.data());
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     HIPC(hipEventRecord(e0, m->stream));
     if (k <= 64) {
         Engine<64> eng(m, static_cast<int>(k));
-        eng.run(p, out, m->host_omega.data());
+        eng.run(p, out);
     } else {
         Engine<128> eng(m, static_cast<int>(k));
-        eng.run(p, out, m->host_omega.data());
+        eng.run(p, out);
     }
     HIPC(hipEventRecord(e1, m->stream));
     HIPC(hipEventSynchronize(e1));
@@ -1031,6 +1154,26 @@ scxpca_status scxpca_matrix_refresh(scxpca_matrix *m) {
         if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
         DeviceGuard dg(m->device);
         refresh(m);
+    });
+}
+
+scxpca_status scxpca_gaussian_device(uint64_t rows, uint64_t cols, uint64_t seed, int device, double *out) {
+    return guarded([&] {
+        if (!out && rows * cols > 0) throw Error(SCXPCA_ERR_INVALID, "null output");
+        if (cols == 0 || cols > 128) throw Error(SCXPCA_ERR_INVALID, "cols must be in 1..128");
+        DeviceGuard dg(device);
+        hipStream_t s;
+        HIPC(hipStreamCreate(&s));
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        OmegaScratch sc;
+        DevBuf b64;
+        b64.ensure(sizeof(double) * std::max<uint64_t>(rows * cols, 1));
+        device_omega<128>(s, rows, static_cast<int>(cols), seed, nullptr, b64.as<double>(), sc);
+        HIPC(hipMemcpyAsync(out, b64.p, sizeof(double) * rows * cols, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
     });
 }
 

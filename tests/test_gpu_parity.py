@@ -37,6 +37,14 @@ def small_mat(gpu, small):
     return gpu.Matrix(small.indptr, small.indices, small.data, small.shape, target_sum=1e4, log1p=True)
 
 
+@pytest.mark.parametrize("rows,cols,seed", [(2000, 60, 0), (300, 7, 42), (33538, 120, 0), (5000, 128, 12345)])
+def test_device_omega_matches_oracle_bit_exactly(gpu, rows, cols, seed):
+    say(f"case: device Omega {rows}x{cols} seed={seed}")
+    got = gpu.gaussian_device(rows, cols, seed)
+    ref = oracle.random_gaussian(rows, cols, seed)
+    np.testing.assert_array_equal(got.view(np.uint64), ref.view(np.uint64))
+
+
 def test_stats_parity(small, small_mat):
     s, ss, cnt, rf = small_mat.stats()
     view = oracle.CsrView(small.indptr, small.indices, small.data, small.shape)
