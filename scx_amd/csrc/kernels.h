@@ -1520,6 +1520,155 @@ __global__ void __launch_bounds__(256) k_gram2(const float *__restrict__ Y, int6
 }
 
 
+// ---------------------------------------------------------------------------
+// Tall f32 panels (KP = 64) on the 16x16x4 f32 MFMA. Operand layout: lane l
+// supplies A[l%16][l/16] and B[l/16][l%16]; accumulator q of lane l holds
+// D[4(l/16) + q][l%16].
+// ---------------------------------------------------------------------------
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Gram partials G_b = Y_b^T Y_b per block of rows. One float4 load per lane
+// covers a 4-row group (lane l: row 4g + l/16, columns 4(l%16) .. +3, 1 KB per
+// wave, contiguous), so component e of lane l is A[i][k] = Y[4g + k][4i + e]
+// (i = l%16, k = l/16); the 10 products of components ea <= eb give
+// D[i][j] = sum_rows Y[row][4i + ea] Y[row][4j + eb] = G[4i + ea][4j + eb],
+// which with symmetry covers all of G. f32 accumulation per wave; the 4 waves
+// are summed in a fixed order and the block's full 64 x 64 partial is stored
+// in f64 (k_sum_parts16 adds the blocks).
+template <int KP>
+__global__ void __launch_bounds__(256) k_gram4(const float *__restrict__ Y, int64_t nrows, int64_t rows_per_blk,
+                                               double *__restrict__ part) {
+    static_assert(KP == 64, "one float4 per lane per row");
+    constexpr int NTL = 10, U = 4;
+    __shared__ float red[4][NTL][256];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int64_t b0 = static_cast<int64_t>(blockIdx.x) * rows_per_blk;
+    const int64_t b1 = b0 + rows_per_blk < nrows ? b0 + rows_per_blk : nrows;
+    const int64_t ngrp = b1 > b0 ? (b1 - b0 + 3) / 4 : 0;
+    v4f acc[NTL];
+#pragma unroll
+    for (int t = 0; t < NTL; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int64_t g0 = wave; g0 < ngrp; g0 += 4 * U) {
+        float4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t row = b0 + 4 * (g0 + 4 * u) + (lane >> 4);
+            const bool ok = row < b1;
+            const int64_t rr = ok ? row : b0;  // in bounds; zeroed below
+            v[u] = *reinterpret_cast<const float4 *>(Y + rr * KP + 4 * (lane & 15));
+            if (!ok) v[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float c[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+            int t = 0;
+#pragma unroll
+            for (int ea = 0; ea < 4; ++ea)
+#pragma unroll
+                for (int eb = ea; eb < 4; ++eb) {
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[ea], c[eb], acc[t], 0, 0, 0);
+                    ++t;
+                }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < NTL; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[wave][t][lane * 4 + q] = acc[t][q];
+    __syncthreads();
+    double *dst = part + static_cast<int64_t>(blockIdx.x) * KP * KP;
+    for (int e = threadIdx.x; e < KP * KP; e += blockDim.x) {
+        const int a = e / KP, b = e % KP;
+        int ea = a & 3, eb = b & 3, i = a >> 2, j = b >> 2;
+        if (ea > eb) {
+            int x = ea; ea = eb; eb = x;
+            x = i; i = j; j = x;
+        }
+        const int t = ea * 4 - ea * (ea - 1) / 2 + (eb - ea);  // index of (ea, eb) in the ea <= eb order
+        const int idx = (16 * (i >> 2) + j) * 4 + (i & 3);     // D[i][j]: lane 16(i/4) + j, reg i%4
+        dst[e] = static_cast<double>(((red[0][t][idx] + red[1][t][idx]) + red[2][t][idx]) + red[3][t][idx]);
+    }
+}
+
+// G = sum of nparts KP x KP f64 partials, in a fixed order: block b sums
+// entries [16b, 16b + 16), 16 lanes per entry each taking every 16th partial,
+// then the 16 lane sums are added in order.
+template <int KP>
+__global__ void __launch_bounds__(256) k_sum_parts16(const double *__restrict__ part, int nparts,
+                                                     double *__restrict__ G) {
+    __shared__ double red[16][17];
+    const int e = blockIdx.x * 16 + (threadIdx.x & 15), g = threadIdx.x >> 4;
+    double acc = 0.0;
+    if (e < KP * KP)
+        for (int p = g; p < nparts; p += 16) acc += part[static_cast<int64_t>(p) * KP * KP + e];
+    red[g][threadIdx.x & 15] = acc;
+    __syncthreads();
+    if (g == 0 && e < KP * KP) {
+        double s = 0.0;
+#pragma unroll
+        for (int h = 0; h < 16; ++h) s += red[h][threadIdx.x & 15];
+        G[e] = s;
+    }
+}
+
+// out = Y W (Y: nrows x 64 f32, W: 64 x 64 f32), the first out_cols columns
+// stored with leading dimension out_ld. A wave takes 16-row tiles: lane l
+// loads 4 float4 of row r0 + l%16 (columns 16m + 4(l/16) .. +3), so component
+// e of load m is A[i][k] for inner index 16m + 4k + e; the matching B values
+// W[16m + 4(l/16) + e][16jt + l%16] stay in 64 registers per lane. 64 MFMAs
+// per tile; lane l then holds out[r0 + 4(l/16) + q][16jt + l%16].
+template <int KP>
+__global__ void __launch_bounds__(256) k_apply4(const float *__restrict__ Y, const float *__restrict__ W,
+                                                float *__restrict__ out, int64_t nrows, int out_cols,
+                                                int64_t out_ld) {
+    static_assert(KP == 64, "four float4 per lane per row");
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i16 = lane & 15, k4 = lane >> 4;
+    float wr[4][4][4];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+#pragma unroll
+            for (int jt = 0; jt < 4; ++jt) wr[m][e][jt] = W[(16 * m + 4 * k4 + e) * KP + 16 * jt + i16];
+    const int64_t ntiles = (nrows + 15) / 16;
+    for (int64_t tile = static_cast<int64_t>(blockIdx.x) * 4 + wave; tile < ntiles;
+         tile += static_cast<int64_t>(gridDim.x) * 4) {
+        const int64_t r0 = tile * 16;
+        const int64_t row = r0 + i16;
+        const bool ok = row < nrows;
+        const int64_t rr = ok ? row : nrows - 1;  // in bounds; zeroed below
+        float4 y[4];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            y[m] = *reinterpret_cast<const float4 *>(Y + rr * KP + 16 * m + 4 * k4);
+            if (!ok) y[m] = make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        v4f acc[4];
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) acc[jt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            const float c[4] = {y[m].x, y[m].y, y[m].z, y[m].w};
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+#pragma unroll
+                for (int jt = 0; jt < 4; ++jt)
+                    acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[e], wr[m][e][jt], acc[jt], 0, 0, 0);
+        }
+#pragma unroll
+        for (int jt = 0; jt < 4; ++jt) {
+            const int col = 16 * jt + i16;
+            if (col < out_cols)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int64_t orow = r0 + 4 * k4 + q;
+                    if (orow < nrows) out[orow * out_ld + col] = acc[jt][q];
+                }
+        }
+    }
+}
+
 // One-sided Jacobi SVD of the k x k R (f64), columns held in LDS for KP <= 64
 // (A and V) and A in LDS / V in global scratch for KP = 128.
 template <int KP>
@@ -1760,122 +1909,264 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
     if (bad && tid == 0) atomicOr(status, 1);
 }
 
-// Cholesky + inverse for KP = 64 with the matrix in registers: thread
-// (ty, tx) of a 16 x 16 grid owns entries (ty + 16a, tx + 16b), a, b < 4.
-// Factorization, step j: the owners of row j publish it to LDS (double
-// buffered, so one barrier per step), every thread reads the pivot and the
-// row-j values of its rows and columns and updates its trailing entries,
-// a[i][c] -= (a[j][i] / r)(a[j][c] / r), r = sqrt(a[j][j]); row j becomes R's
-// row j. Inverse (right-looking): X = I; for j = k-1 .. 0, W[j][:] = X[j][:] /
-// R[j][j] and X[i][:] -= R[i][j] W[j][:] for i < j, with row j of X and column
-// j of R published per step the same way. Same arithmetic as the textbook
-// algorithms, fixed order per entry.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_chol3(const double *__restrict__ G, int k, double shift,
-                                               double *__restrict__ R64, double *__restrict__ W64,
-                                               float *__restrict__ W32, int *__restrict__ status) {
-    constexpr int KP = 64, NB = KP / 16;
-    __shared__ double row[2][KP];
-    __shared__ double col[2][KP];
-    __shared__ double part[4];
-    const int tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
-    double a[NB][NB];
-#pragma unroll
-    for (int p = 0; p < NB; ++p)
-#pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const int i = ty + 16 * p, c = tx + 16 * q;
-            a[p][q] = (i < k && c < k) ? G[i * KP + c] : 0.0;
-        }
+// One-wave f64 helpers: lane broadcast and a 64-lane sum by DPP (quad
+// permutations and row mirrors inside each 16-lane row, then the four row
+// sums read by readlane), the result identical in every lane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), l));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), CTRL, 0xf, 0xf, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), CTRL, 0xf, 0xf, false));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// 1/sqrt(d) for finite d > 0: the hardware estimate (relative error about
+// 2^-23) refined by two Newton-Raphson steps y += y (1/2 - d y^2 / 2), each
+// doubling the correct bits. Branch-free, unlike the library routine.
+__device__ __forceinline__ double rsq_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// Sum over each 16-lane DPP row, the same bits in all 16 lanes (every step
+// adds the same two operands in every lane, in either order).
+__device__ __forceinline__ double row16_sum_dpp(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = row16_sum_dpp(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+// Cholesky factor for KP = 64 in ONE wave, no block barriers: lane c holds
+// column c of the Gram in 64 registers (entries with i >= k or c >= k set to
+// the identity, so every step runs unconditionally and the padded factor is
+// [R 0; 0 I]). Step j: the pivot a[j][j] comes from lane j by readlane,
+// r = d rsqrt(d), lane c forms R[j][c] = a[j][c] / r and publishes it to LDS
+// (double buffered), and every lane updates a[i][c] -= R[j][i] R[j][c] for
+// i > j with R[j][i] read back as a broadcast (measured faster than 63
+// readlanes per step). Lanes c < i keep updating their lower-triangle
+// registers (never read), which keeps the code branch-free. k_trinv_w then
+// inverts R.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int k, double shift,
+                                              double *__restrict__ R64, int *__restrict__ status) {
+    constexpr int KP = 64;
+    __shared__ double rowb[2][KP];
+    __shared__ double Rs[KP * KP];
+    const int c = threadIdx.x;
+    double sh = 0.0;
     if (shift > 0.0) {
-        double s = 0.0;
-        if (tx == ty)
-#pragma unroll
-            for (int p = 0; p < NB; ++p) s += a[p][p];  // diagonal entries (i = c) owned when tx == ty
-        s = wave_sum(s);
-        if ((tid & 63) == 0) part[tid >> 6] = s;
-        __syncthreads();
-        const double tr = part[0] + part[1] + part[2] + part[3];
-        if (tx == ty)
-#pragma unroll
-            for (int p = 0; p < NB; ++p)
-                if (ty + 16 * p < k) a[p][p] += shift * tr;
+        const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
+        sh = c < k ? shift * tr : 0.0;
     }
-    bool bad = false;
-    for (int j = 0; j < k; ++j) {
-        const int b = j & 1, pj = j >> 4;
-        if (ty == (j & 15))
+    // masked, shifted Gram staged row-major in LDS by a rolled loop, then one
+    // unconditional read per register
+#pragma unroll 1
+    for (int i = 0; i < KP; ++i) {
+        const double g = (i < k && c < k) ? G[i * KP + c] : (i == c ? 1.0 : 0.0);
+        Rs[i * KP + c] = (i == c) ? g + sh : g;
+    }
+    wave_lds_sync();
+    double a[KP];
 #pragma unroll
-            for (int q = 0; q < NB; ++q) row[b][tx + 16 * q] = a[pj][q];
-        __syncthreads();
-        const double d = row[b][j];
+    for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c];
+    wave_lds_sync();
+    bool bad = false;
+#pragma unroll
+    for (int j = 0; j < KP; ++j) {
+        const double d = readlane_f64(a[j], j);
         const bool ok = (d > 0.0) && isfinite(d);
         bad |= !ok;
-        const double r = ok ? sqrt(d) : 1.0;
-        const double inv = 1.0 / r;
-        double ri[NB], rc[NB];
+        const double dd = ok ? d : 1.0;  // select the input, not the result: no branch
+        const double ir = rsq_nr(dd);
+        const double r = dd * ir;
+        const double t = (c == j) ? r : a[j] * ir;
+        a[j] = t;
+        rowb[j & 1][c] = t;
+        wave_lds_sync();
 #pragma unroll
-        for (int p = 0; p < NB; ++p) ri[p] = row[b][ty + 16 * p] * inv;
-#pragma unroll
-        for (int q = 0; q < NB; ++q) rc[q] = row[b][tx + 16 * q] * inv;
-#pragma unroll
-        for (int p = 0; p < NB; ++p)
-#pragma unroll
-            for (int q = 0; q < NB; ++q) {
-                const int i = ty + 16 * p, c = tx + 16 * q;
-                if (i > j && c >= i) a[p][q] -= ri[p] * rc[q];
-                if (i == j && c >= j) a[p][q] = (c == j) ? r : rc[q];
-            }
+        for (int i = j + 1; i < KP; ++i) a[i] -= rowb[j & 1][i] * t;
     }
-    // R (upper) out; the lower part is zero
+    if (bad && c == 0) atomicOr(status, 1);
 #pragma unroll
-    for (int p = 0; p < NB; ++p)
+    for (int i = 0; i < KP; ++i) R64[i * KP + c] = (i <= c && c < k) ? a[i] : 0.0;
+}
+
+// W = R^-1 for the upper-triangular k x k R of k_chol_w (KP = 64, zeros
+// outside), one wave, right-looking: lane c holds column c of X; X = I; for
+// j = 63 .. 0, X[j][:] /= R[j][j], X[i][:] -= R[i][j] X[j][:] for i < j, with
+// R[i][j] read from LDS as broadcasts. Pivots j >= k are taken as 1 (the
+// identity block), so padded columns never reach columns c < k. A separate
+// kernel: inlined into the factorization the two 64-register columns spill.
+__global__ void __launch_bounds__(64) k_trinv_w(const double *__restrict__ R64, int k, double *__restrict__ W64,
+                                               float *__restrict__ W32) {
+    constexpr int KP = 64;
+    __shared__ double Rs[KP * KP];
+    __shared__ double rinv[KP];
+    const int c = threadIdx.x;
+#pragma unroll 1
+    for (int i = 0; i < KP; ++i) Rs[i * KP + c] = R64[i * KP + c];
+    rinv[c] = c < k ? 1.0 / R64[c * KP + c] : 1.0;
+    wave_lds_sync();
+    double x[KP];
 #pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const int i = ty + 16 * p, c = tx + 16 * q;
-            R64[i * KP + c] = (i < k && c < k && c >= i) ? a[p][q] : 0.0;
-        }
-    // inverse: x starts as I; the R entries stay in a
-    double x[NB][NB];
+    for (int i = 0; i < KP; ++i) x[i] = (i == c) ? 1.0 : 0.0;
 #pragma unroll
-    for (int p = 0; p < NB; ++p)
+    for (int j = KP - 1; j >= 0; --j) {
+        const double w = x[j] * rinv[j];
+        x[j] = w;
 #pragma unroll
-        for (int q = 0; q < NB; ++q) x[p][q] = (ty + 16 * p == tx + 16 * q && ty + 16 * p < k) ? 1.0 : 0.0;
-    for (int j = k - 1; j >= 0; --j) {
-        const int b = j & 1, pj = j >> 4, qj = j >> 4;
-        // owners publish row j of X and column j of R (entries i <= j)
-        if (ty == (j & 15))
-#pragma unroll
-            for (int q = 0; q < NB; ++q) row[b][tx + 16 * q] = x[pj][q];
-        if (tx == (j & 15))
-#pragma unroll
-            for (int p = 0; p < NB; ++p) col[b][ty + 16 * p] = a[p][qj];
-        __syncthreads();
-        const double dinv = 1.0 / col[b][j];
-        double wr[NB], rcj[NB];
-#pragma unroll
-        for (int q = 0; q < NB; ++q) wr[q] = row[b][tx + 16 * q] * dinv;
-#pragma unroll
-        for (int p = 0; p < NB; ++p) rcj[p] = col[b][ty + 16 * p];
-#pragma unroll
-        for (int p = 0; p < NB; ++p)
-#pragma unroll
-            for (int q = 0; q < NB; ++q) {
-                const int i = ty + 16 * p;
-                if (i < j) x[p][q] -= rcj[p] * wr[q];
-                if (i == j) x[p][q] = wr[q];
-            }
+        for (int i = 0; i < j; ++i) x[i] -= Rs[i * KP + j] * w;
+        // a fence per step keeps every step's broadcast reads from being hoisted
+        // to the top (which would need ~2000 registers)
+        wave_lds_sync();
     }
-    if (bad && tid == 0) atomicOr(status, 1);
 #pragma unroll
-    for (int p = 0; p < NB; ++p)
-#pragma unroll
-        for (int q = 0; q < NB; ++q) {
-            const int i = ty + 16 * p, c = tx + 16 * q;
-            const double v = (i < k && c < k && c >= i) ? x[p][q] : 0.0;
-            W64[i * KP + c] = v;
-            W32[i * KP + c] = static_cast<float>(v);
+    for (int i = 0; i < KP; ++i) {
+        const double v = (i <= c && c < k) ? x[i] : 0.0;
+        W64[i * KP + c] = v;
+        W32[i * KP + c] = static_cast<float>(v);
+    }
+}
+
+// One-sided (Hestenes) Jacobi SVD of the k x k R in f64 for KP = 64: the
+// columns of A (= R, rotated) and V live in LDS, column-major. Each round of
+// the round-robin ordering pairs all columns; pair slot s (0..31) is one
+// 16-lane DPP row, lane l of it holding rows 4l..4l+3 of the pair's columns.
+// The three inner products are summed over the row by DPP (identical bits in
+// all 16 lanes, so they agree on the rotation), which is
+// tan = sign(d) 2g / (|d| + sqrt(d^2 + 4g^2)), d = |y|^2 - |x|^2, g = x.y (the
+// usual zeta formula multiplied through by |2g|), cos = rsqrt(1 + tan^2).
+// A pair is rotated while |x.y| > 1e-10 |x| |y|: converged columns are then
+// orthogonal to 1e-10 (singular values exact to ~1e-20 relative, vectors to
+// ~1e-10, far below the f32 bases this SVD finishes); a tighter threshold
+// only adds sweeps that rotate by rounding noise.
+// One barrier per round; the per-sweep "rotated" flags rotate through three
+// slots so that a reset never races a read. Outputs as k_jacobi_lds: U, V
+// row-major with columns sorted by descending singular value, sigma.
+template <int KP>
+__global__ void __launch_bounds__(512) k_jacobi_w(const double *__restrict__ R, int k, double *__restrict__ U,
+                                                  double *__restrict__ V, double *__restrict__ sigma,
+                                                  int *__restrict__ status) {
+    static_assert(KP == 64, "16 lanes x 4 rows per column");
+    __shared__ __align__(16) double A[KP * KP];
+    __shared__ __align__(16) double Vs[KP * KP];
+    __shared__ int changed[3];
+    __shared__ double nrm[KP];
+    __shared__ int order[KP];
+    const int tid = threadIdx.x, l16 = tid & 15, slot = tid >> 4;
+    const int n = (k + 1) & ~1;
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t % KP, c = t / KP;
+        A[t] = (i < k && c < k) ? R[i * KP + c] : 0.0;
+        Vs[t] = (i == c && c < n) ? 1.0 : 0.0;
+    }
+    if (tid < 3) changed[tid] = 0;
+    __syncthreads();
+    const int npairs = n / 2;
+    const bool act = slot < npairs;
+    int sweep = 0;
+    for (; sweep < 60 && n >= 2; ++sweep) {
+        if (tid == 0) changed[(sweep + 1) % 3] = 0;
+        for (int round = 0; round < n - 1; ++round) {
+            int pp = 0, qq = 0;
+            if (act) {
+                if (slot == 0) {
+                    qq = 1 + (round % (n - 1));
+                } else {
+                    pp = 1 + (round + slot) % (n - 1);
+                    qq = 1 + (round - slot + (n - 1)) % (n - 1);
+                }
+            }
+            const int p = pp < qq ? pp : qq, q = pp < qq ? qq : pp;
+            double2 *ap = reinterpret_cast<double2 *>(A + p * KP + 4 * l16);
+            double2 *aq = reinterpret_cast<double2 *>(A + q * KP + 4 * l16);
+            double2 *vp = reinterpret_cast<double2 *>(Vs + p * KP + 4 * l16);
+            double2 *vq = reinterpret_cast<double2 *>(Vs + q * KP + 4 * l16);
+            const double2 x0 = ap[0], x1 = ap[1], y0 = aq[0], y1 = aq[1];
+            const double2 u0 = vp[0], u1 = vp[1], w0 = vq[0], w1 = vq[1];
+            double al = x0.x * x0.x, be = y0.x * y0.x, ga = x0.x * y0.x;
+            al = fma(x0.y, x0.y, al);
+            be = fma(y0.y, y0.y, be);
+            ga = fma(x0.y, y0.y, ga);
+            al = fma(x1.x, x1.x, al);
+            be = fma(y1.x, y1.x, be);
+            ga = fma(x1.x, y1.x, ga);
+            al = fma(x1.y, x1.y, al);
+            be = fma(y1.y, y1.y, be);
+            ga = fma(x1.y, y1.y, ga);
+            al = row16_sum_dpp(al);
+            be = row16_sum_dpp(be);
+            ga = row16_sum_dpp(ga);
+            if (act && ga != 0.0 && ga * ga > 1e-20 * (al * be)) {
+                const double d = be - al, g2 = 2.0 * ga;
+                const double hh = d * d + g2 * g2;
+                const double h = hh * rsq_nr(hh);
+                const double t = (d >= 0.0 ? g2 : -g2) / (fabs(d) + h);
+                const double cs = rsq_nr(1.0 + t * t), sn = cs * t;
+                ap[0] = make_double2(cs * x0.x - sn * y0.x, cs * x0.y - sn * y0.y);
+                ap[1] = make_double2(cs * x1.x - sn * y1.x, cs * x1.y - sn * y1.y);
+                aq[0] = make_double2(sn * x0.x + cs * y0.x, sn * x0.y + cs * y0.y);
+                aq[1] = make_double2(sn * x1.x + cs * y1.x, sn * x1.y + cs * y1.y);
+                vp[0] = make_double2(cs * u0.x - sn * w0.x, cs * u0.y - sn * w0.y);
+                vp[1] = make_double2(cs * u1.x - sn * w1.x, cs * u1.y - sn * w1.y);
+                vq[0] = make_double2(sn * u0.x + cs * w0.x, sn * u0.y + cs * w0.y);
+                vq[1] = make_double2(sn * u1.x + cs * w1.x, sn * u1.y + cs * w1.y);
+                changed[sweep % 3] = 1;
+            }
+            __syncthreads();
         }
+        if (!changed[sweep % 3]) break;
+    }
+    if (sweep == 60 && tid == 0) atomicOr(status, 2);
+    for (int c = slot; c < KP; c += 32) {
+        const double2 *ac = reinterpret_cast<const double2 *>(A + c * KP + 4 * l16);
+        const double2 a0 = ac[0], a1 = ac[1];
+        double s = a0.x * a0.x;
+        s = fma(a0.y, a0.y, s);
+        s = fma(a1.x, a1.x, s);
+        s = fma(a1.y, a1.y, s);
+        s = row16_sum_dpp(s);
+        if (l16 == 0) nrm[c] = c < k ? sqrt(s) : -1.0;
+    }
+    __syncthreads();
+    for (int c = tid; c < KP; c += blockDim.x) {
+        int rank = 0;
+#pragma unroll 4
+        for (int d = 0; d < KP; ++d)
+            if (nrm[d] > nrm[c] || (nrm[d] == nrm[c] && d < c)) ++rank;
+        order[rank] = c;
+    }
+    __syncthreads();
+    for (int t = tid; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, jj = t % KP;
+        const int c = order[jj];
+        const double sg = nrm[c] > 0.0 ? nrm[c] : 0.0;
+        U[t] = (jj < k && sg > 0.0) ? A[c * KP + i] / sg : 0.0;
+        V[t] = jj < k ? Vs[c * KP + i] : 0.0;
+    }
+    for (int jj = tid; jj < KP; jj += blockDim.x)
+        sigma[jj] = jj < k ? (nrm[order[jj]] > 0 ? nrm[order[jj]] : 0.0) : 0.0;
 }
 
 // Column statistics on the tile-CSC (values are X~): block (tile range,

@@ -610,10 +610,25 @@ struct Engine {
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
     void chol(double shift, double *Rdst) {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
-        if constexpr (KP == 64)
-            k_chol3<<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
-        else
+        if constexpr (KP == 64) {
+            k_chol_w<<<1, 64, 0, s>>>(G, k, shift, Rdst, status);
+            check();
+            k_trinv_w<<<1, 64, 0, s>>>(Rdst, k, W64, W32);
+        } else {
             k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+        }
+        check();
+    }
+
+    // out[:, :out_cols] = Y W32 for a tall f32 panel Y (n x KP).
+    void apply(const float *Y, float *out, int64_t n, int out_cols, int64_t out_ld) {
+        if constexpr (KP == 64) {
+            const unsigned grid = std::min<unsigned>(blocks_for((n + 15) / 16, 4), 2048u);
+            k_apply4<KP><<<std::max(grid, 1u), 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
+        } else {
+            const unsigned grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
+            k_apply2<KP><<<std::max(grid, 1u), 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
+        }
         check();
     }
 
@@ -626,23 +641,28 @@ struct Engine {
     float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true) {
         if (n < 0) n = m->n_obs;
         const int passes = !final ? 1 : (qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2);
-        const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
         const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, gram_blocks)));
         int64_t rpg = (n + gblocks - 1) / gblocks;
         rpg = (rpg + 15) / 16 * 16;
         for (int pass = 0; pass < passes; ++pass) {
             {
                 StageScope prof(m, s, SCXPCA_STAGE_GRAM);
-                k_gram2<KP><<<gblocks, 256, 0, s>>>(src, n, rpg, m->gpart.as<double>());
+                if constexpr (KP == 64)
+                    k_gram4<KP><<<gblocks, 256, 0, s>>>(src, n, rpg, m->gpart.as<double>());
+                else
+                    k_gram2<KP><<<gblocks, 256, 0, s>>>(src, n, rpg, m->gpart.as<double>());
+                check();
+                if constexpr (KP == 64)
+                    k_sum_parts16<KP><<<(KP * KP + 15) / 16, 256, 0, s>>>(m->gpart.as<double>(), gblocks, G);
+                else
+                    k_gram_reduce<KP><<<(KP * KP + 63) / 64, 256, 0, s>>>(m->gpart.as<double>(), gblocks, G);
             }
-            check();
-            k_gram_reduce<KP><<<(KP * KP + 63) / 64, 256, 0, s>>>(m->gpart.as<double>(), gblocks, G);
             check();
             const double shift = !final ? 1e-7 : ((qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0);
             chol(shift, R1);
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
-                k_apply2<KP><<<apply_grid, 256, 0, s>>>(src, W32, tmp, n, KP, KP);
+                apply(src, tmp, n, KP, KP);
             }
             check();
             std::swap(src, tmp);
@@ -711,7 +731,10 @@ struct Engine {
         cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
         {
             StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
-            k_jacobi_lds<KP><<<1, 256, 0, s>>>(Rt, k, Vcol, U, V, sig, status);
+            if constexpr (KP == 64)
+                k_jacobi_w<KP><<<1, 512, 0, s>>>(Rt, k, U, V, sig, status);
+            else
+                k_jacobi_lds<KP><<<1, 256, 0, s>>>(Rt, k, Vcol, U, V, sig, status);
         }
         check();
         // Uhat = Q_B U_R (n_vars x KP, f64) -> components
@@ -735,8 +758,7 @@ struct Engine {
         } else {
             emb_dev = Qother;  // n_obs x KP scratch, embeddings written with ld = nc
         }
-        const unsigned apply_grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
-        k_apply2<KP><<<apply_grid, 256, 0, s>>>(Q, W32, emb_dev, n, nc, nc);
+        apply(Q, emb_dev, n, nc, nc);
         check();
         // results back
         std::vector<double> hsig(KP), hsum(nv), hsq(nv);
