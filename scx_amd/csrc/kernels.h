@@ -88,6 +88,55 @@ __device__ __forceinline__ T wave_sum(T v) {
 }
 
 // ---------------------------------------------------------------------------
+// One-wave f64 helpers: lane broadcast and a 64-lane sum by DPP (quad
+// permutations and row mirrors inside each 16-lane row, then the four row
+// sums read by readlane), the result identical in every lane.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), l));
+    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), l));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const uint32_t lo = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), CTRL, 0xf, 0xf, false));
+    const uint32_t hi = static_cast<uint32_t>(
+        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), CTRL, 0xf, 0xf, false));
+    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
+}
+
+// 1/sqrt(d) for finite d > 0: the hardware estimate (relative error about
+// 2^-23) refined by two Newton-Raphson steps y += y (1/2 - d y^2 / 2), each
+// doubling the correct bits. Branch-free, unlike the library routine.
+__device__ __forceinline__ double rsq_nr(double d) {
+    double y = __builtin_amdgcn_rsq(d);
+    const double h = 0.5 * d;
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    y = fma(y, fma(-h * y, y, 0.5), y);
+    return y;
+}
+
+// Sum over each 16-lane DPP row, the same bits in all 16 lanes (every step
+// adds the same two operands in every lane, in either order).
+__device__ __forceinline__ double row16_sum_dpp(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v = row16_sum_dpp(v);
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
+
+
+// ---------------------------------------------------------------------------
 // Per-row preparation
 // ---------------------------------------------------------------------------
 
@@ -107,8 +156,10 @@ __global__ void __launch_bounds__(256) k_check_sorted(const int64_t *__restrict_
     for (int64_t r = tid; r < n_obs; r += nthreads) {
         const int64_t a = indptr[r], b = indptr[r + 1];
         if (b < a) bad |= 4;
-        for (int64_t p = a + 1; p < b; ++p)
+        for (int64_t p = a + 1; p < b; ++p) {
             if (idx[p] < idx[p - 1]) { bad |= 1; break; }
+            if (idx[p] == idx[p - 1]) bad |= 8;  // repeated column: the tile-CSC holds one entry per (row, gene)
+        }
     }
     if (bad) atomicOr(flag, bad);
 }
@@ -174,7 +225,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     if (a + u * kWave + lane < b) sum += static_cast<double>(v[u]);
-                sum = wave_sum(sum);
+                sum = wave_sum_dpp(sum);
                 f = sum > 0.0 ? target / sum : 1.0;
                 if (lane == 0) rowf[r] = f;
             }
@@ -192,7 +243,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             if (flags & kXfNorm) {
                 double sum = 0.0;
                 for (int64_t p = a + lane; p < b; p += kWave) sum += static_cast<double>(val[p]);
-                sum = wave_sum(sum);
+                sum = wave_sum_dpp(sum);
                 f = sum > 0.0 ? target / sum : 1.0;
                 if (lane == 0) rowf[r] = f;
             }
@@ -1017,176 +1068,195 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
     if (threadIdx.x == 0) tp[n_vars] = carry;
 }
 
-// Tile-CSC build from the split offsets (n_vars <= 8192): one block per
-// (tile, 128-gene chunk). Row r's entries of the chunk are CSR positions
-// indptr[r] + [spl[r][s], spl[r][s+1]); the chunk's entries precede all later
-// chunks' in the tile, so its output starts at sum_r spl[r][s] (a block
-// reduction). The block flattens its rows' ranges (row found by binary search
-// over the scanned lengths), builds per-gene row masks with LDS atomic ORs,
-// scans the 128 gene counts, and places each entry at start[c] +
-// popcount(mask bits of gene c in rows < r), so runs are in row order whatever
-// the schedule (deterministic). The chunk's output is assembled in LDS and
-// written contiguously; a chunk with more than kBuild3Stage entries in one
-// tile (only near-dense genes) writes its entries directly instead. When psum
-// is given, the block also writes the tile's per-gene sum and sum of squares
-// of X~ (the column statistics' partials).
-constexpr int kBuild3Stage = 4096;
-__global__ void __launch_bounds__(256) k_build_tcsc3(const int64_t *__restrict__ indptr,
+// Tile-CSC build from the split offsets (n_vars <= 8192), one block of 8
+// waves per (tile, 256-gene chunk), XCD-aware (workgroups are dealt to the 8
+// XCDs round-robin; block b runs on XCD b % 8 and takes tile
+// 8 (b / 8 / nck) + b % 8, chunk (b / 8) % nck, so all chunks of a tile, whose
+// row segments share cache lines, go through one L2; blocks past the last
+// tile exit). Wave w owns the tile's rows 32w .. 32w+31 and flattens
+// their segments of the chunk (CSR positions indptr[r] + [spl[r][2c],
+// spl[r][2c+2]) for 128-gene split offsets); entry f = 64b + lane of batch b
+// finds its row by a wave-uniform walk over the few rows the batch touches
+// (no search), and the wave keeps up to kB4Batches batches (gene, X~, row) in
+// registers, so each entry is read from memory once. Pass 1 sets bit rl of
+// mask[w][gene] (rows within a wave are distinct bits; a gene occurs at most
+// once per row, which the matrix check enforces); the block then scans the
+// per-gene counts into run starts and per-wave offsets, and pass 2 places each
+// entry at woff[w][gene] + popcount(mask bits of earlier rows of the wave):
+// runs in row order whatever the schedule (deterministic). The chunk is
+// assembled in LDS and written contiguously when it has at most kB4Stage
+// entries, directly otherwise. Longer waves process their remaining batches
+// from memory in both passes. With psum, the block also writes the tile's
+// per-gene sum and sum of squares of X~ (f64, run order).
+constexpr int kB4Genes = 256;
+constexpr int kB4Stage = 8192;
+constexpr int kB4Batches = 12;
+__global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ idx,
                                                      const float *__restrict__ xt,
-                                                     const uint32_t *__restrict__ spl, int nsplit, int64_t n_obs,
-                                                     int64_t n_vars, uint32_t *__restrict__ tptr,
+                                                     const uint32_t *__restrict__ spl, int nsplit, int nck,
+                                                     int64_t n_obs, int64_t n_vars, uint32_t *__restrict__ tptr,
                                                      uint8_t *__restrict__ trow, float *__restrict__ tval,
                                                      double *__restrict__ psum, double *__restrict__ psq) {
-    constexpr int NT = 256, CBG = 128;
-    __shared__ uint32_t mask[8][CBG];
-    __shared__ uint32_t start[CBG + 1];
-    __shared__ uint8_t pre[CBG][8];
-    __shared__ uint32_t flat[kTileRows + 1];  // scanned range lengths per row
-    __shared__ uint32_t lo_s[kTileRows];
-    __shared__ float sval[kBuild3Stage];
-    __shared__ uint8_t srow[kBuild3Stage];
-    __shared__ uint32_t wred[2][4];
-    const int t = blockIdx.x / nsplit, sc = blockIdx.x % nsplit;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NG = kB4Genes, NW = 8, U = kB4Batches;
+    constexpr int SPC = NG / 128;  // 128-gene split offsets per chunk
+    __shared__ uint32_t mask[NW][NG];
+    __shared__ uint32_t woff[NW][NG];
+    __shared__ uint32_t start[NG + 1];
+    __shared__ float sval[kB4Stage];
+    __shared__ uint8_t srow[kB4Stage];
+    __shared__ uint32_t wlo[NW], gsum[4];
+    const int slot = static_cast<int>(blockIdx.x >> 3), xcd = static_cast<int>(blockIdx.x & 7);
+    const int t = (slot / nck) * 8 + xcd, ck = slot % nck;
     const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+    if (r0 >= n_obs) return;  // padding blocks of the last XCD round
     const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t g0 = static_cast<uint32_t>(ck) * NG;
+    const int ng = (n_vars - g0) < NG ? static_cast<int>(n_vars - g0) : NG;
+    const int cs = ck * SPC, ce = (cs + SPC) < nsplit ? cs + SPC : nsplit;
     const int64_t base = indptr[r0];
-    // per row: range of this chunk, and the entries of earlier chunks
-    uint32_t len = 0u, before = 0u;
-    if (tid < nr) {
-        const uint32_t *sr = spl + (r0 + tid) * (nsplit + 1);
-        const uint32_t l0 = sr[sc], l1 = sr[sc + 1];
-        before = l0;
-        len = l1 - l0;
-        lo_s[tid] = static_cast<uint32_t>(indptr[r0 + tid] - base) + l0;  // tile-relative CSR offset
+    const int64_t p_last = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
+    // lanes 0..31: this chunk's segment of row 32w + lane
+    uint32_t len = 0u, slo = 0u;
+    int64_t gp = 0;
+    if (lane < 32 && 32 * w + lane < nr) {
+        const uint32_t *sr = spl + (r0 + 32 * w + lane) * (nsplit + 1);
+        slo = sr[cs];
+        len = sr[ce] - slo;
+        gp = indptr[r0 + 32 * w + lane] + slo;
     }
-    for (int i = tid; i < 8 * CBG; i += NT) (&mask[0][0])[i] = 0u;
-    // block scans: exclusive prefix of len (flat) and the sum of before
-    uint32_t incl = len, bsum = before;
+    uint32_t incl = len;
 #pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
+    for (int o = 1; o < 32; o <<= 1) {
         const uint32_t v = __shfl_up(incl, o);
         if (lane >= o) incl += v;
     }
-    bsum = static_cast<uint32_t>(wave_sum(static_cast<int>(bsum)));
-    if (lane == kWave - 1) wred[0][wave] = incl;
-    if (lane == 0) wred[1][wave] = bsum;
-    __syncthreads();
-    uint32_t wpre = 0u;
-    for (int w = 0; w < wave; ++w) wpre += wred[0][w];
-    const uint32_t total = wred[0][0] + wred[0][1] + wred[0][2] + wred[0][3];
-    const uint32_t cbase = wred[1][0] + wred[1][1] + wred[1][2] + wred[1][3];
-    flat[tid] = wpre + incl - len;
-    if (tid == 0) flat[kTileRows] = total;
-    __syncthreads();
-    // row of flattened entry f: largest row with flat[row] <= f (rows with
-    // len 0 share their successor's start; upper_bound skips them)
-    auto row_of = [&](uint32_t f) {
-        int lo = 0, hi = nr;  // answer in [0, nr)
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (flat[mid] <= f) lo = mid; else hi = mid;
+    const uint32_t fst = incl - len;  // flattened start of the row's segment
+    const uint32_t wtotal = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(incl), 31));
+    const uint32_t wslo = static_cast<uint32_t>(wave_sum(static_cast<int>(slo)));
+    for (int i = tid; i < NW * NG; i += blockDim.x) (&mask[0][0])[i] = 0u;
+    if (lane == 0) wlo[w] = wslo;
+    const int nb = static_cast<int>((wtotal + 63) / 64);
+    // batch b: the lane's row (-1: none) and CSR position; j is the first row
+    // whose segment reaches into the batch (wave-uniform, advanced in order)
+    int j = 0;
+    auto locate = [&](int b, int &row, int64_t &p) {
+        const uint32_t f0 = 64u * static_cast<uint32_t>(b), f = f0 + lane;
+        row = -1;
+        p = base;
+        while (j < 32) {
+            const uint32_t fs = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fst), j));
+            const uint32_t ln = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(len), j));
+            if (fs >= f0 + 64u) break;
+            if (f >= fs && f < fs + ln) {
+                const uint32_t plo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(gp)), j));
+                const uint32_t phi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(static_cast<uint64_t>(gp) >> 32)), j));
+                row = j;
+                p = static_cast<int64_t>((static_cast<uint64_t>(phi) << 32) | plo) + (f - fs);
+            }
+            if (fs + ln > f0 + 64u) break;  // the row continues into the next batch
+            ++j;
         }
-        return lo;
+        p = p < p_last ? p : p_last;
     };
-    const uint32_t g0 = static_cast<uint32_t>(sc) * CBG;
-    const int64_t p_last = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
-    // pass 1: masks. U entries per thread per batch: the row searches (LDS)
-    // first, then all U loads in flight, then the atomics.
-    constexpr int U = 8;
-    for (uint32_t f0 = 0; f0 < total; f0 += NT * U) {
-        int rr[U];
-        uint32_t c[U];
+    int rowv[U];
+    uint32_t gv[U];
+    float xv[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t f = f0 + tid + u * NT;
-            rr[u] = row_of(f < total ? f : total - 1);
+    for (int b = 0; b < U; ++b) {
+        rowv[b] = -1;
+        gv[b] = g0;
+        xv[b] = 0.f;
+        if (b < nb) {
+            int64_t p;
+            locate(b, rowv[b], p);
+            gv[b] = idx[p];
+            xv[b] = xt[p];
         }
+    }
+    const int j_ovf = j;  // walk state at the first batch held in memory only
+    __syncthreads();      // masks cleared
+    // pass 1: row masks per gene
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t f = f0 + tid + u * NT;
-            const uint32_t fc = f < total ? f : total - 1;
-            int64_t p = base + lo_s[rr[u]] + (fc - flat[rr[u]]);
-            p = p < p_last ? p : p_last;
-            c[u] = idx[p];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-            if (f0 + tid + u * NT < total) atomicOr(&mask[rr[u] >> 5][c[u] - g0], 1u << (rr[u] & 31));
+    for (int b = 0; b < U; ++b)
+        if (rowv[b] >= 0) atomicOr(&mask[w][gv[b] - g0], 1u << rowv[b]);
+    for (int b = U; b < nb; ++b) {
+        int row;
+        int64_t p;
+        locate(b, row, p);
+        const uint32_t g = idx[p];
+        if (row >= 0) atomicOr(&mask[w][g - g0], 1u << row);
     }
     __syncthreads();
-    // counts and per-word prefix of 128 genes; thread j < 128 owns gene j
-    uint32_t cnt = 0u;
-    if (tid < CBG) {
+    // per-gene counts: run starts (block scan over genes) and per-wave offsets
+    uint32_t tot = 0u;
+    if (tid < NG) {
 #pragma unroll
-        for (int w = 0; w < 8; ++w) {
-            pre[tid][w] = static_cast<uint8_t>(cnt);
-            cnt += __popc(mask[w][tid]);
+        for (int ww = 0; ww < NW; ++ww) {
+            woff[ww][tid] = tot;
+            tot += __popc(mask[ww][tid]);
         }
     }
-    uint32_t ci = cnt;
+    uint32_t ci = tot;
 #pragma unroll
     for (int o = 1; o < kWave; o <<= 1) {
         const uint32_t v = __shfl_up(ci, o);
         if (lane >= o) ci += v;
     }
-    if (lane == kWave - 1 && wave < 2) wred[0][wave] = ci;
+    if (lane == kWave - 1 && w < 4) gsum[w] = ci;
     __syncthreads();
-    if (tid == 0) start[CBG] = total;
-    if (tid < CBG) {
-        const uint32_t st = (wave == 1 ? wred[0][0] : 0u) + ci - cnt;
+    uint32_t cbase = 0u;
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) cbase += wlo[ww];
+    const uint32_t ctotal = gsum[0] + gsum[1] + gsum[2] + gsum[3];
+    if (tid < NG) {
+        uint32_t st = ci - tot;
+        for (int ww = 0; ww < w; ++ww) st += gsum[ww];
         start[tid] = st;
-        if (g0 + tid < n_vars) tptr[static_cast<int64_t>(t) * (n_vars + 1) + g0 + tid] = cbase + st;
+#pragma unroll
+        for (int ww = 0; ww < NW; ++ww) woff[ww][tid] += st;
+        if (tid < ng) tptr[static_cast<int64_t>(t) * (n_vars + 1) + g0 + tid] = cbase + st;
     }
-    if (tid == 0 && sc == nsplit - 1)
-        tptr[static_cast<int64_t>(t) * (n_vars + 1) + n_vars] = static_cast<uint32_t>(indptr[r0 + nr] - base);
+    if (tid == 0) {
+        start[NG] = ctotal;
+        if (ck == nck - 1) tptr[static_cast<int64_t>(t) * (n_vars + 1) + n_vars] = static_cast<uint32_t>(indptr[r0 + nr] - base);
+    }
     __syncthreads();
-    // pass 2: placement (LDS staging when the chunk fits)
-    const bool staged = total <= static_cast<uint32_t>(kBuild3Stage);
-    for (uint32_t f0 = 0; f0 < total; f0 += NT * U) {
-        int rr[U];
-        uint32_t c[U];
-        float x[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t f = f0 + tid + u * NT;
-            rr[u] = row_of(f < total ? f : total - 1);
+    // pass 2: placement
+    const bool staged = ctotal <= static_cast<uint32_t>(kB4Stage);
+    auto place = [&](int row, uint32_t g, float x) {
+        const uint32_t gl = g - g0;
+        const uint32_t below = row ? (mask[w][gl] & ((1u << row) - 1u)) : 0u;
+        const uint32_t pos = woff[w][gl] + __popc(below);
+        const uint8_t rt = static_cast<uint8_t>(32 * w + row);
+        if (staged) {
+            srow[pos] = rt;
+            sval[pos] = x;
+        } else {
+            trow[base + cbase + pos] = rt;
+            tval[base + cbase + pos] = x;
         }
+    };
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint32_t f = f0 + tid + u * NT;
-            const uint32_t fc = f < total ? f : total - 1;
-            int64_t p = base + lo_s[rr[u]] + (fc - flat[rr[u]]);
-            p = p < p_last ? p : p_last;
-            c[u] = idx[p];
-            x[u] = xt[p];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (f0 + tid + u * NT >= total) continue;
-            const uint32_t cc = c[u] - g0;
-            const uint32_t w = static_cast<uint32_t>(rr[u]) >> 5, bit = static_cast<uint32_t>(rr[u]) & 31u;
-            const uint32_t below = bit ? (mask[w][cc] & ((1u << bit) - 1u)) : 0u;
-            const uint32_t pos = start[cc] + pre[cc][w] + __popc(below);
-            if (staged) {
-                srow[pos] = static_cast<uint8_t>(rr[u]);
-                sval[pos] = x[u];
-            } else {
-                trow[base + cbase + pos] = static_cast<uint8_t>(rr[u]);
-                tval[base + cbase + pos] = x[u];
-            }
-        }
+    for (int b = 0; b < U; ++b)
+        if (rowv[b] >= 0) place(rowv[b], gv[b], xv[b]);
+    j = j_ovf;
+    for (int b = U; b < nb; ++b) {
+        int row;
+        int64_t p;
+        locate(b, row, p);
+        const uint32_t g = idx[p];
+        const float x = xt[p];
+        if (row >= 0) place(row, g, x);
     }
     __syncthreads();
     if (staged)
-        for (uint32_t q = tid; q < total; q += NT) {
+        for (uint32_t q = tid; q < ctotal; q += blockDim.x) {
             trow[base + cbase + q] = srow[q];
             tval[base + cbase + q] = sval[q];
         }
-    // column statistics of this tile: sum and sum of squares of X~ per gene,
-    // in run (row) order, f64
-    if (psum && tid < CBG && g0 + tid < n_vars) {
+    if (psum && tid < ng) {
         const uint32_t a = start[tid], e = start[tid + 1];
         double sm = 0.0, sq = 0.0;
         for (uint32_t q = a; q < e; ++q) {
@@ -1300,12 +1370,17 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
         const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
         const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
         const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
+        if (nr == kTileRows) {  // every tile but the last: no row checks
 #pragma unroll
-        for (int j = 0; j < QPT; ++j) {
-            const int i = tid + j * NT;
-            const bool ok = i / (KP / 4) < nr;
-            const float4 v = Q4[ok ? i : 0];
-            qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int j = 0; j < QPT; ++j) qpre[j] = Q4[tid + j * NT];
+        } else {
+#pragma unroll
+            for (int j = 0; j < QPT; ++j) {
+                const int i = tid + j * NT;
+                const bool ok = i / (KP / 4) < nr;
+                const float4 v = Q4[ok ? i : 0];
+                qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
         }
     };
     // run layout of one tile: start (tile-relative) and list offset per gene
@@ -1324,13 +1399,16 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
     // list entry f = 64 b + lane of the tile starting at tb: gene gi with
     // pos[gi] <= f, at tile offset lo[gi] + f - pos[gi]
     auto prefetch = [&](int64_t tb_, const uint32_t (&lo_)[GW], const uint32_t (&pos_)[GW + 1]) {
+        uint32_t dl[GW];  // tile offset - list offset of each gene's run (wave-uniform)
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) dl[gi] = lo_[gi] - pos_[gi];
 #pragma unroll
         for (int b = 0; b < NB; ++b) {
             const uint32_t f = static_cast<uint32_t>(b * kWave + lane);
-            uint32_t off = lo_[0] + f;
+            uint32_t d = dl[0];
 #pragma unroll
-            for (int gi = 1; gi < GW; ++gi) off = (f >= pos_[gi]) ? lo_[gi] + (f - pos_[gi]) : off;
-            int64_t p = tb_ + off;
+            for (int gi = 1; gi < GW; ++gi) d = (f >= pos_[gi]) ? dl[gi] : d;
+            int64_t p = tb_ + (f + d);
             p = p < p_max ? p : p_max;
             prow[b] = trow[p];
             pval[b] = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
@@ -1616,8 +1694,9 @@ __global__ void __launch_bounds__(256) k_sum_parts16(const double *__restrict__ 
 // loads 4 float4 of row r0 + l%16 (columns 16m + 4(l/16) .. +3), so component
 // e of load m is A[i][k] for inner index 16m + 4k + e; the matching B values
 // W[16m + 4(l/16) + e][16jt + l%16] stay in 64 registers per lane. 64 MFMAs
-// per tile; lane l then holds out[r0 + 4(l/16) + q][16jt + l%16].
-template <int KP>
+// per tile, 40 when W is upper triangular (UPPER: the blocks m > jt are zero
+// and skipped); lane l then holds out[r0 + 4(l/16) + q][16jt + l%16].
+template <int KP, bool UPPER>
 __global__ void __launch_bounds__(256) k_apply4(const float *__restrict__ Y, const float *__restrict__ W,
                                                 float *__restrict__ out, int64_t nrows, int out_cols,
                                                 int64_t out_ld) {
@@ -1653,7 +1732,7 @@ __global__ void __launch_bounds__(256) k_apply4(const float *__restrict__ Y, con
 #pragma unroll
             for (int e = 0; e < 4; ++e)
 #pragma unroll
-                for (int jt = 0; jt < 4; ++jt)
+                for (int jt = UPPER ? m : 0; jt < 4; ++jt)
                     acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[e], wr[m][e][jt], acc[jt], 0, 0, 0);
         }
 #pragma unroll
@@ -1907,54 +1986,6 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
         W32[t] = static_cast<float>(v);
     }
     if (bad && tid == 0) atomicOr(status, 1);
-}
-
-// ---------------------------------------------------------------------------
-// One-wave f64 helpers: lane broadcast and a 64-lane sum by DPP (quad
-// permutations and row mirrors inside each 16-lane row, then the four row
-// sums read by readlane), the result identical in every lane.
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u)), l));
-    const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(u >> 32)), l));
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-
-template <int CTRL>
-__device__ __forceinline__ double dpp_f64(double v) {
-    const uint64_t u = __builtin_bit_cast(uint64_t, v);
-    const uint32_t lo = static_cast<uint32_t>(
-        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u)), CTRL, 0xf, 0xf, false));
-    const uint32_t hi = static_cast<uint32_t>(
-        __builtin_amdgcn_update_dpp(0, static_cast<int>(static_cast<uint32_t>(u >> 32)), CTRL, 0xf, 0xf, false));
-    return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
-}
-
-// 1/sqrt(d) for finite d > 0: the hardware estimate (relative error about
-// 2^-23) refined by two Newton-Raphson steps y += y (1/2 - d y^2 / 2), each
-// doubling the correct bits. Branch-free, unlike the library routine.
-__device__ __forceinline__ double rsq_nr(double d) {
-    double y = __builtin_amdgcn_rsq(d);
-    const double h = 0.5 * d;
-    y = fma(y, fma(-h * y, y, 0.5), y);
-    y = fma(y, fma(-h * y, y, 0.5), y);
-    return y;
-}
-
-// Sum over each 16-lane DPP row, the same bits in all 16 lanes (every step
-// adds the same two operands in every lane, in either order).
-__device__ __forceinline__ double row16_sum_dpp(double v) {
-    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
-    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
-    v += dpp_f64<0x141>(v);  // row_half_mirror
-    v += dpp_f64<0x140>(v);  // row_mirror
-    return v;
-}
-
-__device__ __forceinline__ double wave_sum_dpp(double v) {
-    v = row16_sum_dpp(v);
-    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
 }
 
 // Cholesky factor for KP = 64 in ONE wave, no block barriers: lane c holds

@@ -237,10 +237,13 @@ void prepare(scxpca_matrix *m) {
                                                       m->use_split ? m->spl.as<uint32_t>() : nullptr);
     HIPC(hipGetLastError());
     if (m->use_split)
-        k_build_tcsc3<<<static_cast<unsigned>(m->n_tiles) * m->nsplit, 256, 0, m->stream>>>(
-            m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, m->n_obs, m->n_vars,
+    {
+        const int nck = static_cast<int>((m->n_vars + kB4Genes - 1) / kB4Genes);
+        k_build_tcsc4<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
+            m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
             m->tsq.as<double>());
+    }
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                          static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
@@ -338,6 +341,12 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
             hidx = sidx.data();
             hval = sval.data();
         }
+        // the tile-CSC holds one entry per (row, gene)
+        for (int64_t r = 0; r < x->n_obs; ++r)
+            for (int64_t p = x->indptr[r] + 1; p < x->indptr[r + 1]; ++p)
+                if (hidx[p] == hidx[p - 1])
+                    throw Error(SCXPCA_ERR_INVALID,
+                                "a CSR row repeats a column index (sum duplicate entries before calling pca)");
         m->own_indptr.ensure(sizeof(int64_t) * (x->n_obs + 1));
         m->own_idx.ensure(sizeof(uint32_t) * std::max<int64_t>(m->nnz, 1));
         m->own_val.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
@@ -365,6 +374,9 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
         if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
         if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
         if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
+        if (h & 8)
+            throw Error(SCXPCA_ERR_INVALID,
+                        "a CSR row repeats a column index (sum duplicate entries before calling pca)");
     }
     m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
     m->use_split = m->n_vars <= kMaxSplitVars;
@@ -620,11 +632,15 @@ struct Engine {
         check();
     }
 
-    // out[:, :out_cols] = Y W32 for a tall f32 panel Y (n x KP).
-    void apply(const float *Y, float *out, int64_t n, int out_cols, int64_t out_ld) {
+    // out[:, :out_cols] = Y W32 for a tall f32 panel Y (n x KP); upper: W32 is
+    // upper triangular (an inverse Cholesky factor).
+    void apply(const float *Y, float *out, int64_t n, int out_cols, int64_t out_ld, bool upper) {
         if constexpr (KP == 64) {
-            const unsigned grid = std::min<unsigned>(blocks_for((n + 15) / 16, 4), 2048u);
-            k_apply4<KP><<<std::max(grid, 1u), 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
+            const unsigned grid = std::max(std::min<unsigned>(blocks_for((n + 15) / 16, 4), 2048u), 1u);
+            if (upper)
+                k_apply4<KP, true><<<grid, 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
+            else
+                k_apply4<KP, false><<<grid, 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for((n + 31) / 32, 4), 8192u);
             k_apply2<KP><<<std::max(grid, 1u), 256, 0, s>>>(Y, W32, out, n, out_cols, out_ld);
@@ -662,7 +678,7 @@ struct Engine {
             chol(shift, R1);
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
-                apply(src, tmp, n, KP, KP);
+                apply(src, tmp, n, KP, KP, true);
             }
             check();
             std::swap(src, tmp);
@@ -758,7 +774,7 @@ struct Engine {
         } else {
             emb_dev = Qother;  // n_obs x KP scratch, embeddings written with ld = nc
         }
-        apply(Q, emb_dev, n, nc, nc);
+        apply(Q, emb_dev, n, nc, nc, false);
         check();
         // results back
         std::vector<double> hsig(KP), hsum(nv), hsq(nv);

@@ -116,3 +116,25 @@ def test_pca_parity_small(small, small_mat, n_iter, qr_method):
     np.testing.assert_allclose(got["mean"], ref["mean"], rtol=1e-6, atol=1e-12)
     np.testing.assert_allclose(got["variance_ratio"], ref["variance_ratio"], rtol=2e-4)
     assert abs(got["total_variance"] - (ref["variance"] / ref["variance_ratio"])[0]) <= 1e-6 * got["total_variance"]
+
+
+@pytest.mark.parametrize("on_device", [False, True])
+def test_repeated_column_in_a_row_is_rejected(gpu, on_device):
+    # the transposed SpMM's tile layout holds one entry per (row, gene): a row
+    # that repeats a column index is refused with a clear error, on both paths
+    indptr = np.array([0, 2, 5, 6], dtype=np.int64)
+    indices = np.array([0, 3, 1, 1, 4, 2], dtype=np.uint32)
+    data = np.ones(6, dtype=np.float32)
+    say(f"case: repeated column, on_device={on_device}")
+    if on_device:
+        import torch
+
+        dev = torch.device("cuda:0")
+        t = (torch.from_numpy(indptr).to(dev), torch.from_numpy(indices.view(np.int32)).to(dev),
+             torch.from_numpy(data).to(dev))
+        torch.cuda.synchronize()
+        with pytest.raises(ValueError, match="repeats a column index"):
+            gpu.Matrix(*t, (3, 5), target_sum=1e4, log1p=True, device=0)
+    else:
+        with pytest.raises(ValueError, match="repeats a column index"):
+            gpu.Matrix(indptr, indices, data, (3, 5), target_sum=1e4, log1p=True)
