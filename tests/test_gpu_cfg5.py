@@ -65,3 +65,64 @@ def test_cfg5_subsample_pca(sub, precision):
         sc = score_rel_err(ref["embeddings"][:, :10], got["embeddings"][:, :10].astype(np.float64))
         say(f"score rel err top10 {sc.max():.2e}")
         assert sc.max() <= 1e-3
+
+
+@pytest.mark.skipif(os.environ.get("SCX_FULL_CFG5") != "1",
+                    reason="full 2,000,000 x 33,538 matrix (4.0e9 nonzeros, 32 GB of host CSR): "
+                           "set SCX_FULL_CFG5=1 to run")
+def test_cfg5_full_size_identities(gpu):
+    """Size-independent checks at the full size, through device pointers:
+    adjoint <X~^T Q, M> = <Q, X~ M> with positive Q and M (both whole-matrix
+    passes, f64 host sums, 2e-5 relative); the gene sum of the transposed pass, sum_c (X~^T Q)[c, :] =
+    (X~ 1)^T Q with X~ 1 from the oracle's transform (1e-5 relative); and the
+    FP32 and BF16-sketch rPCA (k=100, p=20, q=4) agreeing on the singular values
+    (1e-4 relative) and the top-10 subspace of the leading separated block."""
+    import torch
+
+    w = synth.CFG5
+    say("case: generate the full 2M x 33,538 workload")
+    x = synth.generate(w)
+    say(f"nnz={x.nnz:,}")
+    dev = torch.device("cuda:0")
+    t = (torch.from_numpy(x.indptr).to(dev), torch.from_numpy(x.indices.view(np.int32)).to(dev),
+         torch.from_numpy(x.data).to(dev))
+    torch.cuda.synchronize()
+    m = gpu.Matrix(*t, x.shape, target_sum=w.target_sum, log1p=True, device=0)
+    try:
+        n, nv = x.shape
+        rng = np.random.default_rng(3)
+        k = 16
+        # Positive panels for the adjoint: with ~1.2e5 rows per gene, f32
+        # rounding of a sign-mixed sum leaves ~1e-5 of the (cancelling) total,
+        # while positive sums keep it near 5e-6 relative, so a lost tile
+        # (1.3e-4 of the rows) still shows at 2e-5.
+        q = rng.uniform(0.0, 1.0, (n, k)) / n
+        panel = rng.uniform(0.0, 1.0, (nv, k))
+        say("case: full transpose + forward, uncentered")
+        z = m.spmm(q, transpose=True)
+        y = m.spmm(panel, transpose=False)
+        a, b = float(np.sum(z * panel)), float(np.sum(q * y))
+        say(f"adjoint: {a!r} vs {b!r}")
+        assert abs(a - b) <= 2e-5 * max(abs(a), abs(b))
+        qs = rng.standard_normal((n, k)) / np.sqrt(n)
+        zs = m.spmm(qs, transpose=True)
+        view = oracle.CsrView(x.indptr, x.indices, x.data, x.shape)
+        xrow = oracle.row_sums(view.with_data(oracle.transform(view, w.target_sum, True)))
+        ref = xrow @ qs
+        assert np.abs(zs.sum(axis=0) - ref).max() <= 1e-5 * np.abs(ref).max()
+        del z, y, q, zs, qs
+        say("case: full rPCA FP32 and BF16 sketch")
+        kw = dict(n_comps=w.n_comps, n_oversamples=w.n_oversamples, n_power_iterations=w.n_power_iterations,
+                  seed=0, qr_method=1)
+        f32 = m.pca(precision=0, **kw)
+        b16 = m.pca(precision=1, **kw)
+        se = sigma_rel_err(f32["sigma"][: w.n_comps], b16["sigma"][: w.n_comps])
+        j = separated_rank(f32["sigma"], 10, 0.02)
+        ang = max_principal_angle(f32["components"].T[:, :j].astype(np.float64),
+                                  b16["components"].T[:, :j].astype(np.float64)) if j else 0.0
+        say(f"FP32 vs BF16: sigma rel max {se.max():.2e}, top{j} angle {ang:.2e}, "
+            f"device s {f32['seconds_device']:.3f} / {b16['seconds_device']:.3f}")
+        assert se.max() <= 1e-4
+        assert ang <= 1e-3
+    finally:
+        m.close()
