@@ -187,78 +187,109 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
     constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-         r < n_obs; r += wstride) {
-        const int64_t a = indptr[r], b = indptr[r + 1];
-        uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
-        int prev_last = -1;  // chunk of the last entry of the previous batch
-        // split offsets: an entry whose 128-gene chunk differs from its
-        // predecessor's opens chunks (prev, ch]; their offsets are its position
-        auto splits = [&](int64_t p0, bool ok, uint32_t c, int64_t p) {
-            const int ch = ok ? static_cast<int>(c >> 7) : 0x7fffffff;
-            int prev = __shfl_up(ch, 1);
-            if (lane == 0) prev = prev_last;
-            if (ok && ch > prev)
-                for (int g = prev + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p - a);
-            const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;
-            prev_last = __shfl(ch, cnt - 1);
-        };
-        auto emit = [&](int64_t p, uint32_t c, float v, double f) {
-            xt[p] = xtilde(v, f, flags);
-            lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
-        };
-        if (b - a <= U * kWave) {
-            // short row: every load issued at once (clamped, unconditional)
-            uint32_t c[U];
-            float v[U];
-            const int64_t last = b > a ? b - 1 : a;
+    // split offsets of one row: an entry whose 128-gene chunk differs from its
+    // predecessor's opens chunks (prev, ch]; their offsets are its position
+    auto splits = [&](uint32_t *sr, int64_t a, int64_t b, int64_t p0, bool ok, uint32_t c, int &prev_last) {
+        const int ch = ok ? static_cast<int>(c >> 7) : 0x7fffffff;
+        int prev = __shfl_up(ch, 1);
+        if (lane == 0) prev = prev_last;
+        if (ok && ch > prev)
+            for (int g = prev + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p0 + lane - a);
+        const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;
+        prev_last = __shfl(ch, cnt - 1);
+    };
+    auto splits_tail = [&](uint32_t *sr, int64_t a, int64_t b, int prev_last) {
+        // chunks after the row's last entry start at the row's end
+        for (int g = prev_last + 1 + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
+        if (lane == 0) sr[nsplit] = static_cast<uint32_t>(b - a);
+    };
+    auto factor = [&](int64_t r, double sum) {
+        double f = 1.0;
+        if (flags & kXfNorm) {
+            sum = wave_sum_dpp(sum);
+            f = sum > 0.0 ? target / sum : 1.0;
+            if (lane == 0) rowf[r] = f;
+        }
+        return f;
+    };
+    // short rows (<= U * 64 entries): every load issued at once, clamped
+    // into the row (no loads for an empty row)
+    auto load_short = [&](int64_t a, int64_t b, uint32_t (&c)[U], float (&v)[U]) {
+        if (b > a) {
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 int64_t p = a + u * kWave + lane;
-                p = p < last ? p : last;
+                p = p < b - 1 ? p : b - 1;
                 c[u] = idx[p];
                 v[u] = val[p];
             }
-            double f = 1.0;
-            if (flags & kXfNorm) {
-                double sum = 0.0;
-#pragma unroll
-                for (int u = 0; u < U; ++u)
-                    if (a + u * kWave + lane < b) sum += static_cast<double>(v[u]);
-                sum = wave_sum_dpp(sum);
-                f = sum > 0.0 ? target / sum : 1.0;
-                if (lane == 0) rowf[r] = f;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t p0 = a + u * kWave;
-                if (p0 >= b) break;
-                const int64_t p = p0 + lane;
-                const bool ok = p < b;
-                if (ok) emit(p, c[u], v[u], f);
-                if (nsplit > 0) splits(p0, ok, c[u], p);
-            }
-        } else {
-            double f = 1.0;
-            if (flags & kXfNorm) {
-                double sum = 0.0;
-                for (int64_t p = a + lane; p < b; p += kWave) sum += static_cast<double>(val[p]);
-                sum = wave_sum_dpp(sum);
-                f = sum > 0.0 ? target / sum : 1.0;
-                if (lane == 0) rowf[r] = f;
-            }
-            for (int64_t p0 = a; p0 < b; p0 += kWave) {
-                const int64_t p = p0 + lane;
-                const bool ok = p < b;
-                const uint32_t c = ok ? idx[p] : 0u;
-                if (ok) emit(p, c, val[p], f);
-                if (nsplit > 0) splits(p0, ok, c, p);
-            }
         }
-        if (nsplit > 0) {
-            // chunks after the row's last entry start at the row's end
-            for (int g = prev_last + 1 + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
-            if (lane == 0) sr[nsplit] = static_cast<uint32_t>(b - a);
+    };
+    auto finish_short = [&](int64_t r, int64_t a, int64_t b, const uint32_t (&c)[U], const float (&v)[U]) {
+        double sum = 0.0;
+        if (flags & kXfNorm) {
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (a + u * kWave + lane < b) sum += static_cast<double>(v[u]);
+        }
+        const double f = factor(r, sum);
+        uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
+        int prev_last = -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t p0 = a + u * kWave;
+            if (p0 >= b) break;
+            const int64_t p = p0 + lane;
+            const bool ok = p < b;
+            if (ok) {
+                xt[p] = xtilde(v[u], f, flags);
+                lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
+            }
+            if (nsplit > 0) splits(sr, a, b, p0, ok, c[u], prev_last);
+        }
+        if (nsplit > 0) splits_tail(sr, a, b, prev_last);
+    };
+    auto long_row = [&](int64_t r, int64_t a, int64_t b) {
+        double sum = 0.0;
+        if (flags & kXfNorm)
+            for (int64_t p = a + lane; p < b; p += kWave) sum += static_cast<double>(val[p]);
+        const double f = factor(r, sum);
+        uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
+        int prev_last = -1;
+        for (int64_t p0 = a; p0 < b; p0 += kWave) {
+            const int64_t p = p0 + lane;
+            const bool ok = p < b;
+            const uint32_t c = ok ? idx[p] : 0u;
+            if (ok) {
+                xt[p] = xtilde(val[p], f, flags);
+                lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
+            }
+            if (nsplit > 0) splits(sr, a, b, p0, ok, c, prev_last);
+        }
+        if (nsplit > 0) splits_tail(sr, a, b, prev_last);
+    };
+    // two rows per wave at a time (r and r + wstride): both rows' loads are in
+    // flight together, so each wave keeps twice the bytes outstanding
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; r < n_obs;
+         r += 2 * wstride) {
+        const int64_t r2 = r + wstride;
+        const bool has2 = r2 < n_obs;
+        const int64_t a = indptr[r], b = indptr[r + 1];
+        const int64_t a2 = has2 ? indptr[r2] : 0, b2 = has2 ? indptr[r2 + 1] : 0;
+        const bool s1 = b - a <= U * kWave, s2 = has2 && (b2 - a2 <= U * kWave);
+        uint32_t c1[U] = {}, c2[U] = {};
+        float v1[U] = {}, v2[U] = {};
+        if (s1) load_short(a, b, c1, v1);
+        if (s2) load_short(a2, b2, c2, v2);
+        if (s1)
+            finish_short(r, a, b, c1, v1);
+        else
+            long_row(r, a, b);
+        if (has2) {
+            if (s2)
+                finish_short(r2, a2, b2, c2, v2);
+            else
+                long_row(r2, a2, b2);
         }
     }
 }
@@ -618,41 +649,64 @@ __global__ void __launch_bounds__(256) k_gram64(const double *__restrict__ A, in
     for (int e = 0; e < PER; ++e) part[static_cast<int64_t>(blockIdx.x) * KP * KP + threadIdx.x + e * 256] = acc[e];
 }
 
-template <int KP>
-__global__ void __launch_bounds__(256) k_sum_parts(const double *__restrict__ part, int nparts,
-                                                   double *__restrict__ G) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= KP * KP) return;
-    double s = 0.0;
-    for (int p = 0; p < nparts; ++p) s += part[static_cast<int64_t>(p) * KP * KP + t];
-    G[t] = s;
-}
 
-// out = A W for a small f64 A (rows x KP), W f64 KP x KP; thread per element.
+// out = A W for a small f64 A (rows x KP), W f64 KP x KP: W and the block's
+// 256 / KP rows of A staged in LDS, one output element per thread (four
+// partial sums over the inner index, added in a fixed order).
 template <int KP>
 __global__ void __launch_bounds__(256) k_apply64(const double *__restrict__ A, const double *__restrict__ W,
                                                  double *__restrict__ out, float *__restrict__ out32,
                                                  int64_t nrows) {
-    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (t >= nrows * KP) return;
-    const int64_t r = t / KP;
-    const int j = static_cast<int>(t % KP);
-    double s = 0.0;
-    for (int i = 0; i <= j; ++i) s += A[r * KP + i] * W[i * KP + j];  // W upper triangular or dense below
-    for (int i = j + 1; i < KP; ++i) s += A[r * KP + i] * W[i * KP + j];
-    if (out) out[t] = s;
-    if (out32) out32[t] = static_cast<float>(s);
+    constexpr int RB = 256 / KP;  // rows per block
+    __shared__ double ws[KP * KP];
+    __shared__ double as[RB * KP];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * RB;
+    for (int i = threadIdx.x; i < KP * KP; i += blockDim.x) ws[i] = W[i];
+    for (int i = threadIdx.x; i < RB * KP; i += blockDim.x)
+        as[i] = r0 + i / KP < nrows ? A[r0 * KP + i] : 0.0;
+    __syncthreads();
+    const int rl = threadIdx.x / KP, j = threadIdx.x % KP;
+    if (r0 + rl >= nrows) return;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+#pragma unroll 4
+    for (int i = 0; i < KP; i += 4) {
+        s0 += as[rl * KP + i] * ws[i * KP + j];
+        s1 += as[rl * KP + i + 1] * ws[(i + 1) * KP + j];
+        s2 += as[rl * KP + i + 2] * ws[(i + 2) * KP + j];
+        s3 += as[rl * KP + i + 3] * ws[(i + 3) * KP + j];
+    }
+    const double sm = (s0 + s1) + (s2 + s3);
+    const int64_t t = (r0 + rl) * KP + j;
+    if (out) out[t] = sm;
+    if (out32) out32[t] = static_cast<float>(sm);
 }
 
-// C = A B for KP x KP f64 matrices (one workgroup).
+// C = A B for KP x KP f64 matrices: A staged in LDS by row tiles, B whole
+// (KP = 64: 32 KB each); one workgroup, KP * KP / 256 outputs per thread.
 template <int KP>
 __global__ void __launch_bounds__(256) k_small_gemm64(const double *__restrict__ A, const double *__restrict__ B,
                                                       double *__restrict__ C) {
-    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, j = t % KP;
-        double s = 0.0;
-        for (int q = 0; q < KP; ++q) s += A[i * KP + q] * B[q * KP + j];
-        C[t] = s;
+    static_assert(KP == 64 || KP == 128, "panel widths");
+    constexpr int RT = 64 * 64 / KP;  // rows of A per LDS tile (KP * RT doubles = 32 KB)
+    __shared__ double bs[KP == 64 ? KP * KP : 1];
+    __shared__ double at[RT * KP];
+    if constexpr (KP == 64)
+        for (int i = threadIdx.x; i < KP * KP; i += blockDim.x) bs[i] = B[i];
+    for (int i0 = 0; i0 < KP; i0 += RT) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < RT * KP; i += blockDim.x) at[i] = A[i0 * KP + i];
+        __syncthreads();
+        for (int t = threadIdx.x; t < RT * KP; t += blockDim.x) {
+            const int i = t / KP, j = t % KP;
+            double s0 = 0.0, s1 = 0.0;
+            for (int q = 0; q < KP; q += 2) {
+                const double b0 = KP == 64 ? bs[q * KP + j] : B[q * KP + j];
+                const double b1 = KP == 64 ? bs[(q + 1) * KP + j] : B[(q + 1) * KP + j];
+                s0 += at[i * KP + q] * b0;
+                s1 += at[i * KP + q + 1] * b1;
+            }
+            C[(i0 + i) * KP + j] = s0 + s1;
+        }
     }
 }
 
@@ -1520,24 +1574,6 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
     }
 }
 
-// 1^T Q from the transposed SpMM's per-block partials: 256 threads, 256 / KP
-// strided subsets per column, combined in subset order (fixed order).
-template <int KP>
-__global__ void __launch_bounds__(256) k_sum_cols(const double *__restrict__ part, int nrb,
-                                                  double *__restrict__ out) {
-    constexpr int NS = 256 / KP;
-    __shared__ double red[256];
-    const int c = threadIdx.x % KP, sub = threadIdx.x / KP;
-    double t = 0.0;
-    for (int r = sub; r < nrb; r += NS) t += part[static_cast<int64_t>(r) * KP + c];
-    red[threadIdx.x] = t;
-    __syncthreads();
-    if (sub == 0) {
-        double v = 0.0;
-        for (int q = 0; q < NS; ++q) v += red[q * KP + c];
-        out[c] = v;
-    }
-}
 
 // Gram with in-block reduction: each wave accumulates its upper tiles over its
 // rows (FP32 MFMA), the four waves of the block add into an LDS tile in a fixed

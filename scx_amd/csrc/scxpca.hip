@@ -539,7 +539,7 @@ struct Engine {
     }
 
     int gram64_blocks() const {
-        return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((m->n_vars + 255) / 256, 256)));
+        return static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((m->n_vars + 31) / 32, 256)));
     }
 
     void check() { HIPC(hipGetLastError()); }
@@ -547,7 +547,7 @@ struct Engine {
     // mc = mu^T panel (f32 for the forward epilogue)
     void mean_correction(const float *panel) {
         const int64_t nv = m->n_vars;
-        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nv + 255) / 256, kMaxColBlocks)));
+        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nv + 15) / 16, kMaxColBlocks)));
         const int64_t rpb = (nv + nb - 1) / nb;
         k_colsum<KP, float><<<nb, 256, 0, s>>>(panel, mu, nv, rpb, m->cpart.as<double>());
         check();
@@ -610,7 +610,7 @@ struct Engine {
             if (qsp) HIPC(hipMemsetAsync(qsp, 0, sizeof(double) * nrb_t * nch_t * KP, s));
         }
         if (centered) {
-            k_sum_cols<KP><<<1, 256, 0, s>>>(qsp, nrb_t * nch_t, sumq);
+            k_colsum_reduce<KP><<<KP, 256, 0, s>>>(qsp, nrb_t * nch_t, sumq, nullptr);
             check();
         }
         StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
@@ -699,7 +699,7 @@ struct Engine {
         for (int pass = 0; pass < 3; ++pass) {
             k_gram64<KP><<<nb, 256, 0, s>>>(src, nv, rpb, m->gpart.as<double>());
             check();
-            k_sum_parts<KP><<<blocks_for(KP * KP), 256, 0, s>>>(m->gpart.as<double>(), nb, G);
+            k_sum_parts16<KP><<<(KP * KP + 15) / 16, 256, 0, s>>>(m->gpart.as<double>(), nb, G);
             check();
             chol(pass == 0 ? 1e-13 : 0.0, Rs[pass]);
             k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(src, W64, bufs[pass], nullptr, nv);
