@@ -130,6 +130,9 @@ def main():
     ap.add_argument("--qr", choices=["cholesky", "householder"], default="cholesky")
     ap.add_argument("--cpu-rows", type=int, default=-1,
                     help="rows for the CPU baseline: -1 all rows, 0 skip, N the first N rows")
+    ap.add_argument("--host-input", type=int, default=0, metavar="RUNS",
+                    help="also time RUNS calls from host CSR buffers (H2D copy, validation, prepare, rPCA, "
+                         "results back to host); reported beside the device-resident value, never as it")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="before the timed region, copy MIB MiB device-to-device once (a plain copy "
                          "of known size whose FETCH_SIZE/WRITE_SIZE calibrate the traffic counters)")
@@ -235,6 +238,21 @@ def main():
                          + f"; one run of the C/OpenMP restatement of randomized_pca_inmemory (f64), "
                            f"{cores} threads"}
 
+    host_input = None
+    if args.host_input > 0:
+        kw_h = dict(kw)
+        kw_h.pop("out")
+        times = []
+        for _ in range(args.host_input):
+            torch.cuda.synchronize()
+            th = time.perf_counter()
+            mh = _lib.Matrix(x.indptr, x.indices, x.data, x.shape, target_sum=w.target_sum, log1p=True)
+            mh.pca(**kw_h)
+            mh.close()
+            times.append(time.perf_counter() - th)
+        host_input = {"value": round(float(np.median(times)), 4), "unit": "s", "runs": args.host_input,
+                      "includes": "host CSR -> HBM copy, row checks, prepare, rPCA, embeddings/components to host"}
+
     stage_ms = {s: round(v[0] / args.steps, 4) for s, v in prof.items() if v[1]}
     out = {
         "metric": METRIC,
@@ -270,6 +288,7 @@ def main():
         "stage_ms_per_step": stage_ms,
         "cpu_baseline": cpu,
         "seconds_device_last": round(res["seconds_device"], 6),
+        "host_input": host_input,
     }
     print(json.dumps(out), flush=True)
 

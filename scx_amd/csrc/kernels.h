@@ -141,27 +141,14 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
 // ---------------------------------------------------------------------------
 
 
-// Flags rows whose indices decrease (the chunked kernels need sorted rows).
-__global__ void __launch_bounds__(256) k_check_sorted(const int64_t *__restrict__ indptr,
-                                                      const uint32_t *__restrict__ idx, int64_t n_obs,
-                                                      int64_t n_vars, int *__restrict__ flag) {
-    const int64_t tid = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const int64_t nthreads = static_cast<int64_t>(gridDim.x) * blockDim.x;
-    const int64_t nnz = indptr[n_obs];
-    // one thread per nonzero: compare with the previous entry of the same row
-    int bad = 0;
-    for (int64_t p = tid; p < nnz; p += nthreads) {
-        if (idx[p] >= static_cast<uint64_t>(n_vars)) bad |= 2;
-    }
-    for (int64_t r = tid; r < n_obs; r += nthreads) {
-        const int64_t a = indptr[r], b = indptr[r + 1];
-        if (b < a) bad |= 4;
-        for (int64_t p = a + 1; p < b; ++p) {
-            if (idx[p] < idx[p - 1]) { bad |= 1; break; }
-            if (idx[p] == idx[p - 1]) bad |= 8;  // repeated column: the tile-CSC holds one entry per (row, gene)
-        }
-    }
-    if (bad) atomicOr(flag, bad);
+// indptr check before anything dereferences entries through it: flag = 1
+// when indptr[0] != 0 or any indptr[r + 1] < indptr[r].
+__global__ void __launch_bounds__(256) k_check_indptr(const int64_t *__restrict__ indptr, int64_t n_obs,
+                                                      int *__restrict__ flag) {
+    const int64_t r = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (r >= n_obs) return;
+    const bool bad = indptr[r + 1] < indptr[r] || (r == 0 && indptr[0] != 0);
+    if (bad) atomicOr(flag, 1);
 }
 
 
@@ -178,29 +165,45 @@ constexpr uint32_t kChunkMod = 512;
 // the split offsets spl[r][g] = number of entries with gene < g * 128 (rows
 // are sorted: written by the entries that open a chunk, and the row length for
 // chunks after the last entry; spl[r][nsplit] = row length).
+// The same pass validates the matrix into *flag (bits: 1 a row's indices
+// decrease, 2 an index >= n_vars, 4 indptr decreases, 8 a row repeats an
+// index); the tile-CSC builds skip all work when the flag is set, and the
+// host rejects the matrix before anything else reads these outputs.
 __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict__ indptr,
                                                       const uint32_t *__restrict__ idx,
                                                       const float *__restrict__ val, int64_t n_obs,
                                                       double target, int flags, int nsplit,
                                                       double *__restrict__ rowf, float *__restrict__ xt,
-                                                      uint16_t *__restrict__ lg, uint32_t *__restrict__ spl) {
+                                                      uint16_t *__restrict__ lg, uint32_t *__restrict__ spl,
+                                                      int64_t n_vars, int *__restrict__ flag) {
     constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
-    // split offsets of one row: an entry whose 128-gene chunk differs from its
-    // predecessor's opens chunks (prev, ch]; their offsets are its position
-    auto splits = [&](uint32_t *sr, int64_t a, int64_t b, int64_t p0, bool ok, uint32_t c, int &prev_last) {
-        const int ch = ok ? static_cast<int>(c >> 7) : 0x7fffffff;
-        int prev = __shfl_up(ch, 1);
+    int bad = 0;
+    // One 64-entry batch of a row: the index checks against the previous entry
+    // (lane 0: the previous batch's last, carried in prev_last; -1 before the
+    // first), and the split offsets: an entry whose 128-gene chunk differs from
+    // its predecessor's opens chunks (prev, ch]; their offsets are its position.
+    auto scan = [&](uint32_t *sr, int64_t a, int64_t b, int64_t p0, bool ok, uint32_t c, int64_t &prev_last) {
+        int64_t prev = static_cast<int64_t>(static_cast<uint32_t>(__shfl_up(static_cast<int>(c), 1)));
         if (lane == 0) prev = prev_last;
-        if (ok && ch > prev)
-            for (int g = prev + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p0 + lane - a);
+        if (ok) {
+            bad |= (static_cast<int64_t>(c) >= n_vars) ? 2 : 0;
+            bad |= (prev >= 0 && static_cast<int64_t>(c) < prev) ? 1 : 0;
+            bad |= (static_cast<int64_t>(c) == prev) ? 8 : 0;
+        }
+        if (nsplit > 0 && ok) {
+            const int ch = static_cast<int>(c >> 7), chp = prev >= 0 ? static_cast<int>(prev >> 7) : -1;
+            if (ch > chp)
+                for (int g = chp + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p0 + lane - a);
+        }
         const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;
-        prev_last = __shfl(ch, cnt - 1);
+        prev_last = static_cast<int64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c), cnt - 1)));
     };
-    auto splits_tail = [&](uint32_t *sr, int64_t a, int64_t b, int prev_last) {
+    auto splits_tail = [&](uint32_t *sr, int64_t a, int64_t b, int64_t prev_last) {
         // chunks after the row's last entry start at the row's end
-        for (int g = prev_last + 1 + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
+        const int chl = prev_last >= 0 ? static_cast<int>(prev_last >> 7) : -1;
+        for (int g = (chl + 1 > 0 ? chl + 1 : 0) + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
         if (lane == 0) sr[nsplit] = static_cast<uint32_t>(b - a);
     };
     auto factor = [&](int64_t r, double sum) {
@@ -234,7 +237,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         }
         const double f = factor(r, sum);
         uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
-        int prev_last = -1;
+        int64_t prev_last = -1;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t p0 = a + u * kWave;
@@ -245,7 +248,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 xt[p] = xtilde(v[u], f, flags);
                 lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
             }
-            if (nsplit > 0) splits(sr, a, b, p0, ok, c[u], prev_last);
+            scan(sr, a, b, p0, ok, c[u], prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
     };
@@ -255,7 +258,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             for (int64_t p = a + lane; p < b; p += kWave) sum += static_cast<double>(val[p]);
         const double f = factor(r, sum);
         uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
-        int prev_last = -1;
+        int64_t prev_last = -1;
         for (int64_t p0 = a; p0 < b; p0 += kWave) {
             const int64_t p = p0 + lane;
             const bool ok = p < b;
@@ -264,7 +267,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 xt[p] = xtilde(val[p], f, flags);
                 lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
             }
-            if (nsplit > 0) splits(sr, a, b, p0, ok, c, prev_last);
+            scan(sr, a, b, p0, ok, c, prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
     };
@@ -276,6 +279,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         const bool has2 = r2 < n_obs;
         const int64_t a = indptr[r], b = indptr[r + 1];
         const int64_t a2 = has2 ? indptr[r2] : 0, b2 = has2 ? indptr[r2 + 1] : 0;
+        bad |= (b < a || b2 < a2) ? 4 : 0;
         const bool s1 = b - a <= U * kWave, s2 = has2 && (b2 - a2 <= U * kWave);
         uint32_t c1[U] = {}, c2[U] = {};
         float v1[U] = {}, v2[U] = {};
@@ -292,6 +296,11 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 long_row(r2, a2, b2);
         }
     }
+    // one atomic per wave that found a problem
+    int wbad = bad;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) wbad |= __shfl_xor(wbad, o);
+    if (wbad && lane == 0) atomicOr(flag, wbad);
 }
 
 // ---------------------------------------------------------------------------
@@ -1022,8 +1031,10 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
                                                     const uint32_t *__restrict__ idx,
                                                     const float *__restrict__ val, int64_t n_obs,
                                                     int n_vars, uint32_t *__restrict__ tptr,
-                                                    uint8_t *__restrict__ trow, float *__restrict__ tval) {
+                                                    uint8_t *__restrict__ trow, float *__restrict__ tval,
+                                                    const int *__restrict__ flag) {
     constexpr int GP = (kTcscGenes < 2048) ? kTcscGenes : 2048;  // genes per pass with staging room
+    if (*flag) return;  // the matrix failed validation: nothing here may trust its indices
     __shared__ uint32_t mask[GP * 8];
     __shared__ uint32_t start[GP];
     __shared__ float sval[kTcscStage];
@@ -1151,8 +1162,10 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
                                                      const uint32_t *__restrict__ spl, int nsplit, int nck,
                                                      int64_t n_obs, int64_t n_vars, uint32_t *__restrict__ tptr,
                                                      uint8_t *__restrict__ trow, float *__restrict__ tval,
-                                                     double *__restrict__ psum, double *__restrict__ psq) {
+                                                     double *__restrict__ psum, double *__restrict__ psq,
+                                                     const int *__restrict__ flag) {
     constexpr int NG = kB4Genes, NW = 8, U = kB4Batches;
+    if (*flag) return;  // the matrix failed validation: nothing here may trust its indices
     constexpr int SPC = NG / 128;  // 128-gene split offsets per chunk
     __shared__ uint32_t mask[NW][NG];
     __shared__ uint32_t woff[NW][NG];

@@ -182,6 +182,7 @@ struct scxpca_matrix {
     DevBuf sum, sq, cnt, psum, psq, pcnt;
     // PCA workspace
     DevBuf y, q, panel, panel2, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
+    DevBuf vflag;  // validation bits of the last prepare (k_prepare_rows)
     OmegaScratch om;  // device Omega scratch (device and host buffers)
     // stage profiler (events on the matrix stream; off unless enabled)
     bool profiling = false;
@@ -223,18 +224,25 @@ struct StageScope {
 // tile-CSC copy. Runs at creation and on refresh; invalidates cached stats.
 const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr; }
 
-void prepare(scxpca_matrix *m) {
+// Row factors, X~, chunk offsets and the tile-CSC of the current CSR, with
+// the matrix validated in the same pass. Returns the validation bits of
+// k_prepare_rows (0: valid); with any bit set the builds did nothing and no
+// later stage may run on this matrix.
+int prepare(scxpca_matrix *m) {
     m->stats_ready = false;
     m->xbf_ready = false;
-    if (m->n_obs == 0) return;
+    if (m->n_obs == 0) return 0;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
+    m->vflag.ensure(sizeof(int));
+    HIPC(hipMemsetAsync(m->vflag.p, 0, sizeof(int), m->stream));
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
     static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
     k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
                                                       m->use_split ? m->nsplit : 0,
                                                       (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
                                                       m->xt.as<float>(), m->lg.as<uint16_t>(),
-                                                      m->use_split ? m->spl.as<uint32_t>() : nullptr);
+                                                      m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
+                                                      m->vflag.as<int>());
     HIPC(hipGetLastError());
     if (m->use_split)
     {
@@ -242,12 +250,13 @@ void prepare(scxpca_matrix *m) {
         k_build_tcsc4<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
-            m->tsq.as<double>());
+            m->tsq.as<double>(), m->vflag.as<int>());
     }
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                          static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
-                                                         m->trow.as<uint8_t>(), m->tval.as<float>());
+                                                         m->trow.as<uint8_t>(), m->tval.as<float>(),
+                                                         m->vflag.as<int>());
     HIPC(hipGetLastError());
     // entries per gene -> host (the transposed SpMM's gene order is built from it)
     m->gcnt.ensure(sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1));
@@ -263,8 +272,20 @@ void prepare(scxpca_matrix *m) {
     if (m->n_vars > 0)
         HIPC(hipMemcpyAsync(m->host_gcnt.data(), m->gcnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost,
                             m->stream));
+    int h = 0;
+    HIPC(hipMemcpyAsync(&h, m->vflag.p, sizeof(int), hipMemcpyDeviceToHost, m->stream));
     HIPC(hipStreamSynchronize(m->stream));
     for (auto &g : m->gorder) g.valid = false;
+    return h;
+}
+
+// The rejection for the validation bits of prepare().
+void throw_invalid(int h) {
+    if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
+    if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
+    if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
+    if (h & 8)
+        throw Error(SCXPCA_ERR_INVALID, "a CSR row repeats a column index (sum duplicate entries before calling pca)");
 }
 
 // Gene order for the transposed SpMM: genes sorted by entry count (descending,
@@ -309,75 +330,38 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
         m->idx = x->indices;
         m->val = x->data;
         HIPC(hipMemcpy(&m->nnz, x->indptr + x->n_obs, sizeof(int64_t), hipMemcpyDeviceToHost));
-    } else {
-        m->nnz = x->indptr[x->n_obs];
-        // canonicalize host rows: the chunked kernels need non-decreasing indices
-        std::vector<uint32_t> sidx;
-        std::vector<float> sval;
-        const uint32_t *hidx = x->indices;
-        const float *hval = x->data;
-        bool sorted = true;
-        for (int64_t r = 0; r < x->n_obs && sorted; ++r)
-            for (int64_t p = x->indptr[r] + 1; p < x->indptr[r + 1]; ++p)
-                if (x->indices[p] < x->indices[p - 1]) { sorted = false; break; }
-        for (int64_t p = 0; p < m->nnz; ++p)
-            if (x->indices[p] >= static_cast<uint64_t>(x->n_vars))
-                throw Error(SCXPCA_ERR_INVALID, "column index out of range");
-        if (!sorted) {
-            sidx.assign(x->indices, x->indices + m->nnz);
-            sval.assign(x->data, x->data + m->nnz);
-            std::vector<int64_t> perm;
-            for (int64_t r = 0; r < x->n_obs; ++r) {
-                const int64_t a = x->indptr[r], b = x->indptr[r + 1];
-                perm.resize(b - a);
-                for (int64_t i = 0; i < b - a; ++i) perm[i] = a + i;
-                std::stable_sort(perm.begin(), perm.end(),
-                                 [&](int64_t u, int64_t v) { return x->indices[u] < x->indices[v]; });
-                for (int64_t i = 0; i < b - a; ++i) {
-                    sidx[a + i] = x->indices[perm[i]];
-                    sval[a + i] = x->data[perm[i]];
-                }
-            }
-            hidx = sidx.data();
-            hval = sval.data();
+        if (m->n_obs > 0) {
+            // indptr first: the row pass dereferences entries through it
+            DevBuf flag;
+            flag.ensure(sizeof(int));
+            HIPC(hipMemsetAsync(flag.p, 0, sizeof(int), m->stream));
+            k_check_indptr<<<blocks_for(m->n_obs, 256), 256, 0, m->stream>>>(m->indptr, m->n_obs, flag.as<int>());
+            HIPC(hipGetLastError());
+            int h = 0;
+            HIPC(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+            HIPC(hipStreamSynchronize(m->stream));
+            if (h) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
         }
-        // the tile-CSC holds one entry per (row, gene)
+    } else {
+        if (x->indptr[0] != 0) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
         for (int64_t r = 0; r < x->n_obs; ++r)
-            for (int64_t p = x->indptr[r] + 1; p < x->indptr[r + 1]; ++p)
-                if (hidx[p] == hidx[p - 1])
-                    throw Error(SCXPCA_ERR_INVALID,
-                                "a CSR row repeats a column index (sum duplicate entries before calling pca)");
+            if (x->indptr[r + 1] < x->indptr[r]) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
+        m->nnz = x->indptr[x->n_obs];
+        // the entries are uploaded as given and checked on the device by the
+        // row pass; rows found unsorted are sorted here and uploaded again
         m->own_indptr.ensure(sizeof(int64_t) * (x->n_obs + 1));
         m->own_idx.ensure(sizeof(uint32_t) * std::max<int64_t>(m->nnz, 1));
         m->own_val.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
         HIPC(hipMemcpyAsync(m->own_indptr.p, x->indptr, sizeof(int64_t) * (x->n_obs + 1), hipMemcpyHostToDevice, m->stream));
         if (m->nnz) {
-            HIPC(hipMemcpyAsync(m->own_idx.p, hidx, sizeof(uint32_t) * m->nnz, hipMemcpyHostToDevice, m->stream));
-            HIPC(hipMemcpyAsync(m->own_val.p, hval, sizeof(float) * m->nnz, hipMemcpyHostToDevice, m->stream));
+            HIPC(hipMemcpyAsync(m->own_idx.p, x->indices, sizeof(uint32_t) * m->nnz, hipMemcpyHostToDevice, m->stream));
+            HIPC(hipMemcpyAsync(m->own_val.p, x->data, sizeof(float) * m->nnz, hipMemcpyHostToDevice, m->stream));
         }
-        HIPC(hipStreamSynchronize(m->stream));  // host staging vectors go out of scope
         m->indptr = m->own_indptr.as<int64_t>();
         m->idx = m->own_idx.as<uint32_t>();
         m->val = m->own_val.as<float>();
     }
     if (m->n_obs == 0) return;
-    if (x->on_device) {
-        DevBuf flag;
-        flag.ensure(sizeof(int));
-        HIPC(hipMemsetAsync(flag.p, 0, sizeof(int), m->stream));
-        k_check_sorted<<<blocks_for(std::max<int64_t>(m->n_obs, m->nnz), 256), 256, 0, m->stream>>>(
-            m->indptr, m->idx, m->n_obs, m->n_vars, flag.as<int>());
-        HIPC(hipGetLastError());
-        int h = 0;
-        HIPC(hipMemcpyAsync(&h, flag.p, sizeof(int), hipMemcpyDeviceToHost, m->stream));
-        HIPC(hipStreamSynchronize(m->stream));
-        if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
-        if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
-        if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
-        if (h & 8)
-            throw Error(SCXPCA_ERR_INVALID,
-                        "a CSR row repeats a column index (sum duplicate entries before calling pca)");
-    }
     m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
     m->use_split = m->n_vars <= kMaxSplitVars;
     if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
@@ -392,8 +376,29 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
         m->tsum.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
         m->tsq.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
     }
-    prepare(m);
-    HIPC(hipStreamSynchronize(m->stream));
+    int h = prepare(m);
+    if ((h & 1) && !x->on_device) {
+        // canonicalize host rows (stable sort by index) and upload again
+        std::vector<uint32_t> sidx(x->indices, x->indices + m->nnz);
+        std::vector<float> sval(x->data, x->data + m->nnz);
+        std::vector<int64_t> perm;
+        for (int64_t r = 0; r < x->n_obs; ++r) {
+            const int64_t a = x->indptr[r], b = x->indptr[r + 1];
+            perm.resize(b - a);
+            for (int64_t i = 0; i < b - a; ++i) perm[i] = a + i;
+            std::stable_sort(perm.begin(), perm.end(),
+                             [&](int64_t u, int64_t v) { return x->indices[u] < x->indices[v]; });
+            for (int64_t i = 0; i < b - a; ++i) {
+                sidx[a + i] = x->indices[perm[i]];
+                sval[a + i] = x->data[perm[i]];
+            }
+        }
+        HIPC(hipMemcpyAsync(m->own_idx.p, sidx.data(), sizeof(uint32_t) * m->nnz, hipMemcpyHostToDevice, m->stream));
+        HIPC(hipMemcpyAsync(m->own_val.p, sval.data(), sizeof(float) * m->nnz, hipMemcpyHostToDevice, m->stream));
+        HIPC(hipStreamSynchronize(m->stream));  // staging vectors go out of scope
+        h = prepare(m);
+    }
+    throw_invalid(h);
 }
 
 void ensure_xbf(scxpca_matrix *m) {
@@ -1085,7 +1090,7 @@ void project(scxpca_matrix *m, bool backward, const float *in, int64_t k, const 
     if (!user) HIPC(hipStreamSynchronize(s));
 }
 
-void refresh(scxpca_matrix *m) { prepare(m); }
+void refresh(scxpca_matrix *m) { throw_invalid(prepare(m)); }
 }  // namespace
 
 // ---------------------------------------------------------------------------
