@@ -1375,18 +1375,25 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Transposed SpMM over the tile-CSC with per-wave entry lists: the gene order
-// and block layout of k_spmm_t4, but each wave prefetches its whole next-tile
-// entry list, the runs of its GW genes concatenated (up to CAP entries), into
-// registers while it computes the current tile, and writes it to its own LDS
-// list after that tile; the gathers then read (Q-row offset, x~) from LDS. The
-// list is private to the wave (LDS operations of one wave complete in order),
-// so only the Q tile needs block barriers. Entries past CAP (rare) are read
-// directly from HBM. Entry loads are unconditional at clamped indices with
-// validity applied at use, so the waits on them stay partial.
+// Transposed SpMM over the tile-CSC with per-wave entry lists. Block (tile
+// range tr, gene chunk gc) has NWT waves; wave w owns GW genes (its gperm
+// slots: genes dealt to waves by entry count) and keeps their Z rows in
+// registers (E columns per lane, one partial per 16-lane group) across the
+// range. Per tile the Q tile (256 rows x KP) is staged in LDS between two block
+// barriers; each wave prefetches its next-tile entry list, the runs of its GW
+// genes concatenated (up to CAP entries), into registers while it computes the
+// current tile, and writes it to its own LDS list after that tile; the gathers
+// then read (Q-row offset, x~) from LDS. The list is private to the wave (LDS
+// operations of one wave complete in order), so only the Q tile needs block
+// barriers. Entries past CAP (rare) are read directly from HBM. Entry loads
+// are unconditional at clamped indices with validity applied at use, so the
+// waits on them stay partial. Blocks are XCD-aware (blockIdx = slot * 8 + xcd,
+// the gene chunks of one tile range on one XCD, sharing its Q tiles in L2).
+// Measured at 1M x 2k: 16 waves per block (one block per CU) beats two 8-wave
+// blocks per CU, which double the Q-tile staging.
 // ---------------------------------------------------------------------------
-template <int KP, int GW, int CAP, bool BF16>
-__global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ indptr,
+template <int KP, int GW, int CAP, bool BF16, int NWT>
+__global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ tptr,
                                                   const uint8_t *__restrict__ trow,
                                                   const float *__restrict__ tval,
@@ -1396,7 +1403,7 @@ __global__ void __launch_bounds__(1024) k_spmm_t5(const int64_t *__restrict__ in
                                                   int64_t n_vars, int n_tiles, float *__restrict__ part,
                                                   double *__restrict__ qsum_part) {
     constexpr int E = KP / 16;
-    constexpr int NW = 16;
+    constexpr int NW = NWT;  // waves per block
     constexpr int NT = NW * kWave;
     constexpr int NB = CAP / kWave;  // prefetch slots per lane
     constexpr int QC = KP / kWave;   // Q columns per lane in the column sums

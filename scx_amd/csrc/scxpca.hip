@@ -170,7 +170,7 @@ struct scxpca_matrix {
     DevBuf gcnt;
     std::vector<uint32_t> host_gcnt;
     struct GeneOrder {
-        int gw = 0, nch = 0;
+        int gw = 0, nch = 0, tw = 0;
         bool valid = false;
         std::vector<uint32_t> host;  // kept alive for the asynchronous upload
         DevBuf dev;
@@ -289,14 +289,14 @@ void throw_invalid(int h) {
 }
 
 // Gene order for the transposed SpMM: genes sorted by entry count (descending,
-// ties by index) are dealt to the nch*16 waves in snake order (0..W-1, W-1..0,
+// ties by index) are dealt to the nch*tw waves in snake order (0..W-1, W-1..0,
 // ...), round r filling slot r of each wave, so per-wave entry totals differ by
 // at most about one gene's count. Unused slots hold n_vars.
-const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch) {
+const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
     auto &go = m->gorder[gw == 8 ? 0 : 1];
-    if (go.valid && go.gw == gw && go.nch == nch) return go.dev.as<uint32_t>();
+    if (go.valid && go.gw == gw && go.nch == nch && go.tw == tw) return go.dev.as<uint32_t>();
     const int64_t nv = m->n_vars;
-    const int64_t W = static_cast<int64_t>(nch) * 16, S = W * gw;
+    const int64_t W = static_cast<int64_t>(nch) * tw, S = W * gw;
     std::vector<uint32_t> order(nv);
     for (int64_t c = 0; c < nv; ++c) order[c] = static_cast<uint32_t>(c);
     std::stable_sort(order.begin(), order.end(),
@@ -313,6 +313,7 @@ const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch) {
                         m->stream));
     go.gw = gw;
     go.nch = nch;
+    go.tw = tw;
     go.valid = true;
     return go.dev.as<uint32_t>();
 }
@@ -485,7 +486,8 @@ template <int KP>
 struct Engine {
     static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t5 (Z in registers)
     static constexpr int TCAP = (KP == 64) ? 256 : 128;  // per-wave entry list in k_spmm_t5
-    static constexpr int CG = 16 * GW;              // genes per k_spmm_t5 block
+    static constexpr int TW = 16;                   // waves per k_spmm_t5 block
+    static constexpr int CG = TW * GW;              // genes per k_spmm_t5 block
     static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
     static constexpr int FPF = 8;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
@@ -502,12 +504,12 @@ struct Engine {
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         nch_t = static_cast<int>((nv + CG - 1) / CG);
-        // tile ranges: about one block per CU in total (256 CUs), multiple of 8
-        int ntr = std::max(1, 256 / nch_t);
+        // tile ranges: about 1024 / TW blocks in total (256 CUs), multiple of 8
+        int ntr = std::max(1, (256 * 16 / TW) / nch_t);
         ntr = std::min(ntr, std::max(1, m->n_tiles));
         nrb_t = (ntr + 7) / 8 * 8;
         tiles_per_tr = (m->n_tiles + nrb_t - 1) / nrb_t;
-        gperm = ensure_gene_order(m, GW, nch_t);
+        gperm = ensure_gene_order(m, GW, nch_t, TW);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
@@ -599,13 +601,13 @@ struct Engine {
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
             const uint32_t *gp = gperm;
             if (bf16)
-                k_spmm_t5<KP, GW, TCAP, true><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                k_spmm_t5<KP, GW, TCAP, true, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(),
                                                                    m->txbf.as<uint16_t>(), Q, gp, nch_t, nrb_t,
                                                                    tiles_per_tr, m->n_obs, nv, m->n_tiles,
                                                                    m->tpart.as<float>(), qsp);
             else
-                k_spmm_t5<KP, GW, TCAP, false><<<grid, 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                k_spmm_t5<KP, GW, TCAP, false, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                     m->trow.as<uint8_t>(), m->tval.as<float>(), nullptr,
                                                                     Q, gp, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
                                                                     m->n_tiles, m->tpart.as<float>(), qsp);
