@@ -1375,25 +1375,29 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
 }
 
 // ---------------------------------------------------------------------------
-// Transposed SpMM over the tile-CSC with per-wave entry lists. Block (tile
-// range tr, gene chunk gc) has NWT waves; wave w owns GW genes (its gperm
-// slots: genes dealt to waves by entry count) and keeps their Z rows in
-// registers (E columns per lane, one partial per 16-lane group) across the
-// range. Per tile the Q tile (256 rows x KP) is staged in LDS between two block
-// barriers; each wave prefetches its next-tile entry list, the runs of its GW
-// genes concatenated (up to CAP entries), into registers while it computes the
-// current tile, and writes it to its own LDS list after that tile; the gathers
-// then read (Q-row offset, x~) from LDS. The list is private to the wave (LDS
-// operations of one wave complete in order), so only the Q tile needs block
-// barriers. Entries past CAP (rare) are read directly from HBM. Entry loads
-// are unconditional at clamped indices with validity applied at use, so the
-// waits on them stay partial. Blocks are XCD-aware (blockIdx = slot * 8 + xcd,
-// the gene chunks of one tile range on one XCD, sharing its Q tiles in L2).
-// Measured at 1M x 2k: 16 waves per block (one block per CU) beats two 8-wave
-// blocks per CU, which double the Q-tile staging.
+// Transposed SpMM over the tile-CSC, entries in registers. Block (tile range
+// tr, gene chunk gc) has NWT waves; wave w owns GW genes (its gperm slots:
+// genes dealt to waves by entry count, so slot 0 holds the densest genes) and
+// keeps their Z rows in registers (E columns per lane, one partial per 16-lane
+// group) across the range. Per tile the Q tile (256 rows x KP) is staged in
+// LDS between two block barriers. Each gene's run in the next tile is loaded
+// one tile ahead straight into the gather layout (lane 4l+g holds entry 4l+g
+// of a 64-entry batch): NB0 batches for slot 0, one for the other slots. The
+// address is a wave-uniform run start plus the lane's fixed slot, so a batch
+// costs two loads and no per-lane index arithmetic, and no entry list passes
+// through LDS. Entries beyond the prefetched batches (rare outside slot 0)
+// are read directly. Loads may run up to NB0 * 64 - 1 entries past the last
+// run: the tile-CSC arrays carry kEntryPad entries of padding, and lanes past
+// a run's end are masked at use. The next tile's base is this tile's base
+// plus its entry count (tptr[t][n_vars], read by the spare lane GW with the
+// run bounds), so the tile loop issues no scalar loads. Blocks are XCD-aware
+// (blockIdx = slot * 8 + xcd, the gene chunks of one tile range on one XCD,
+// sharing its Q tiles in L2).
 // ---------------------------------------------------------------------------
-template <int KP, int GW, int CAP, bool BF16, int NWT>
-__global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict__ indptr,
+constexpr int64_t kEntryPad = 256;
+
+template <int KP, int GW, int NB0, bool BF16, int NWT>
+__global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ tptr,
                                                   const uint8_t *__restrict__ trow,
                                                   const float *__restrict__ tval,
@@ -1405,22 +1409,23 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
     constexpr int E = KP / 16;
     constexpr int NW = NWT;  // waves per block
     constexpr int NT = NW * kWave;
-    constexpr int NB = CAP / kWave;  // prefetch slots per lane
-    constexpr int QC = KP / kWave;   // Q columns per lane in the column sums
-    static_assert(GW <= 16 && CAP % kWave == 0, "one gene per lane of a group");
+    constexpr int QC = KP / kWave;  // Q columns per lane in the column sums
+    static_assert(GW < 16, "one gene per lane of a group, plus the spare lane GW");
+    static_assert(NB0 >= 1 && NB0 * kWave <= kEntryPad, "slot-0 look-ahead must stay inside the padding");
     __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
-    __shared__ uint2 ents[NW][CAP];           // per-wave entry lists (Q-row byte offset, x~)
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
-    const int wave = tid / kWave, lane = tid & 63, g = lane >> 4, l = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int lane = tid & 63, g = lane >> 4, l = lane & 15;
     const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
     const int esel = 4 * l + g;
-    uint2 *ent = ents[wave];
     const int64_t gslot = (static_cast<int64_t>(gc) * NW + wave) * GW + l;
     const uint32_t gene = (l < GW) ? gperm[gslot] : static_cast<uint32_t>(n_vars);
     const bool gvalid = gene < static_cast<uint64_t>(n_vars);
-    const uint32_t gene_c = gvalid ? gene : 0u;
+    const bool spare = l == GW;  // reads the tile's entry count
+    const uint32_t ia = gvalid ? gene : 0u;
+    const uint32_t ib = gvalid ? gene + 1u : static_cast<uint32_t>(n_vars);
     float z[GW][E];
 #pragma unroll
     for (int gi = 0; gi < GW; ++gi)
@@ -1429,14 +1434,13 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
     const int t_begin = tr * tiles_per_tr;
     const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
     const int t_end = (tr < ntr) ? t_end0 : t_begin;
-    const int64_t p_max = indptr[n_obs] > 0 ? indptr[n_obs] - 1 : 0;
 
     auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
     auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
         const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
-        const uint32_t a0 = tp[gene_c], b0 = tp[gene_c + 1];
+        const uint32_t a0 = tp[ia], b0 = tp[ib];
         a = gvalid ? a0 : 0u;
-        b = gvalid ? b0 : 0u;
+        b = (gvalid || spare) ? b0 : 0u;
     };
     constexpr int QPT = kTileRows * KP / 4 / NT;
     float4 qpre[QPT];
@@ -1457,43 +1461,39 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
             }
         }
     };
-    // run layout of one tile: start (tile-relative) and list offset per gene
-    uint32_t lo[GW], pos[GW + 1];
-    uint32_t nlo[GW], npos[GW + 1];
-    auto layout = [&](uint32_t wl, uint32_t wh, uint32_t (&lo_)[GW], uint32_t (&pos_)[GW + 1]) {
-        pos_[0] = 0u;
+    // run start (tile-relative) and length of each gene in a tile, and the
+    // tile's entry count: scalars
+    uint32_t lo[GW], ln[GW], nlo[GW], nln[GW], tot = 0u, ntot = 0u;
+    auto layout = [&](uint32_t wl, uint32_t wh, uint32_t (&lo_)[GW], uint32_t (&ln_)[GW], uint32_t &tot_) {
 #pragma unroll
         for (int gi = 0; gi < GW; ++gi) {
             lo_[gi] = rl32(wl, gi);
-            pos_[gi + 1] = pos_[gi] + (rl32(wh, gi) - lo_[gi]);
+            ln_[gi] = rl32(wh, gi) - lo_[gi];
+        }
+        tot_ = rl32(wh, GW);
+    };
+    // prefetched batches in the gather layout: NB0 for slot 0, one per other slot
+    uint32_t prow0[NB0], prow[GW];
+    float pval0[NB0], pval[GW];
+    auto issue = [&](int64_t p0, uint32_t &pr, float &pv) {  // p0 wave-uniform
+        pr = (trow + p0)[esel];
+        pv = BF16 ? bf16_to_f32((txbf + p0)[esel]) : (tval + p0)[esel];
+    };
+    auto issue_tile = [&](int64_t tb_, const uint32_t (&lo_)[GW], int from, int to) {
+        for (int gi = from; gi < to; ++gi) {
+            if (gi == 0) {
+#pragma unroll
+                for (int b = 0; b < NB0; ++b) issue(tb_ + lo_[0] + b * kWave, prow0[b], pval0[b]);
+            } else {
+                issue(tb_ + lo_[gi], prow[gi], pval[gi]);
+            }
         }
     };
-    uint32_t prow[NB];
-    float pval[NB];
-    // list entry f = 64 b + lane of the tile starting at tb: gene gi with
-    // pos[gi] <= f, at tile offset lo[gi] + f - pos[gi]
-    auto prefetch = [&](int64_t tb_, const uint32_t (&lo_)[GW], const uint32_t (&pos_)[GW + 1]) {
-        uint32_t dl[GW];  // tile offset - list offset of each gene's run (wave-uniform)
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) dl[gi] = lo_[gi] - pos_[gi];
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const uint32_t f = static_cast<uint32_t>(b * kWave + lane);
-            uint32_t d = dl[0];
-#pragma unroll
-            for (int gi = 1; gi < GW; ++gi) d = (f >= pos_[gi]) ? dl[gi] : d;
-            int64_t p = tb_ + (f + d);
-            p = p < p_max ? p : p_max;
-            prow[b] = trow[p];
-            pval[b] = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
-        }
-    };
-    auto store_list = [&](uint32_t total) {
-#pragma unroll
-        for (int b = 0; b < NB; ++b) {
-            const uint32_t f = static_cast<uint32_t>(b * kWave + lane);
-            if (f < total) ent[f] = make_uint2(prow[b] * (KP * 4), __float_as_uint(pval[b]));
-        }
+    auto gather = [&](float (&acc)[E], uint32_t rr, float v, uint32_t left) {
+        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
+        const bool ok = esel < cnt;
+        gather_batch<E>(acc, ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt, reinterpret_cast<const char *>(qs),
+                        lane_ofs);
     };
 
     // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
@@ -1508,9 +1508,9 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
     if (t_begin < t_end) {
         bounds(t_begin, wlo, whi);
         tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
-        layout(wlo, whi, lo, pos);
-        prefetch(tb, lo, pos);
-        store_list(pos[GW]);
+        layout(wlo, whi, lo, ln, tot);
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) issue_tile(tb, lo, gi, gi + 1);
         load_q(t_begin);
         const int tn = t_begin + 1 < t_end ? t_begin + 1 : t_begin;
         bounds(tn, wlo, whi);
@@ -1528,41 +1528,55 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
 #pragma unroll
                 for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
         }
-        // next tile: Q tile, run layout (bounds loaded one tile ago), entries
+        // next tile: Q tile, base, run layout (bounds loaded one tile ago)
         load_q(tn);
-        const int64_t ntb = indptr[static_cast<int64_t>(tn) * kTileRows];
-        layout(wlo, whi, nlo, npos);
-        prefetch(ntb, nlo, npos);
+        const int64_t ntb = tn != t ? tb + tot : tb;
+        layout(wlo, whi, nlo, nln, ntot);
         bounds(tnn, wlo, whi);
-        // this tile from the wave's LDS list
+        uint32_t longest = 0u;
 #pragma unroll
         for (int gi = 0; gi < GW; ++gi) {
-            const uint32_t a = pos[gi], b = pos[gi + 1];
-            const uint32_t bl = b < static_cast<uint32_t>(CAP) ? b : static_cast<uint32_t>(CAP);
-            for (uint32_t o = a; o < bl; o += kWave) {
-                const int cnt = (bl - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(bl - o) : kWave;
-                const bool ok = esel < cnt;
-                const uint2 en = ent[ok ? o + esel : 0];
-                const uint32_t ofs = ok ? en.x : 0u;
-                const float x = ok ? __uint_as_float(en.y) : 0.f;
-                gather_batch<E>(z[gi], ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
-            }
-            // list overflow (rare): read the rest of the run directly
-            for (uint32_t o = (a > static_cast<uint32_t>(CAP) ? a : static_cast<uint32_t>(CAP)); o < b; o += kWave) {
-                const int cnt = (b - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(b - o) : kWave;
-                const bool ok = esel < cnt;
-                int64_t p = tb + lo[gi] + (o - a) + esel;
-                p = p < p_max ? p : p_max;
-                const uint32_t rr = trow[p];
-                const float v = BF16 ? bf16_to_f32(txbf[p]) : tval[p];
-                gather_batch<E>(z[gi], ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt,
-                                reinterpret_cast<const char *>(qs), lane_ofs);
+            const uint32_t len = ln[gi];
+            if (gi == 0) {
+                uint32_t cr[NB0];
+                float cv[NB0];
+#pragma unroll
+                for (int b = 0; b < NB0; ++b) {
+                    cr[b] = prow0[b];
+                    cv[b] = pval0[b];
+                }
+                issue_tile(ntb, nlo, 0, 1);
+                longest = len > static_cast<uint32_t>(NB0 * kWave) ? len : 0u;
+#pragma unroll
+                for (int b = 0; b < NB0; ++b)
+                    if (len > static_cast<uint32_t>(b * kWave)) gather(z[0], cr[b], cv[b], len - b * kWave);
+            } else {
+                const uint32_t cr = prow[gi];
+                const float cv = pval[gi];
+                issue_tile(ntb, nlo, gi, gi + 1);
+                longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
+                if (len > 0u) gather(z[gi], cr, cv, len);
             }
         }
-        store_list(npos[GW]);  // this wave's gathers of tile t are done with its list
+        // entries beyond the prefetched batches: read directly
+        if (longest) {
 #pragma unroll
-        for (int gi = 0; gi < GW; ++gi) { lo[gi] = nlo[gi]; pos[gi] = npos[gi]; }
-        pos[GW] = npos[GW];
+            for (int gi = 0; gi < GW; ++gi) {
+                const uint32_t len = ln[gi];
+                for (uint32_t o = (gi == 0 ? NB0 : 1) * kWave; o < len; o += kWave) {
+                    uint32_t rr;
+                    float v;
+                    issue(tb + lo[gi] + o, rr, v);
+                    gather(z[gi], rr, v, len - o);
+                }
+            }
+        }
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            lo[gi] = nlo[gi];
+            ln[gi] = nln[gi];
+        }
+        tot = ntot;
         tb = ntb;
     }
 #pragma unroll
@@ -1580,7 +1594,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
                 dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
         }
     }
-    if (do_qsum) {  // the 16 waves' column sums, added in wave order
+    if (do_qsum) {  // the waves' column sums, added in wave order
         __syncthreads();
         double *red = reinterpret_cast<double *>(qs);  // NW x KP f64 fits in the Q tile buffer
 #pragma unroll
@@ -1593,7 +1607,6 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t5(const int64_t *__restrict_
         }
     }
 }
-
 
 // Gram with in-block reduction: each wave accumulates its upper tiles over its
 // rows (FP32 MFMA), the four waves of the block add into an LDS tile in a fixed

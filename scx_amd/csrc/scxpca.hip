@@ -28,6 +28,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
@@ -111,7 +112,22 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
 }
 
 constexpr int kCB = 128;       // gene width of the row split offsets
-constexpr int64_t kMaxSplitVars = 8192;  // split offsets (and the chunked forward) up to this many genes
+constexpr int64_t kMaxChunkedFwdVars = 8192;  // the chunked forward SpMM up to this many genes
+// split offsets, the per-(tile, chunk) tile-CSC build and its per-tile column
+// partials while their metadata stays under this size; the general build beyond
+// (SCXPCA_SPLIT_MAX_BYTES, read when a matrix is created, overrides it)
+constexpr double kMaxSplitBytes = 32.0 * (1ull << 30);
+
+double split_max_bytes() {
+    const char *e = std::getenv("SCXPCA_SPLIT_MAX_BYTES");
+    if (e && *e) {
+        char *end = nullptr;
+        const double v = std::strtod(e, &end);
+        if (end != e && *end == '\0' && v >= 0.0) return v;
+        throw Error(SCXPCA_ERR_INVALID, "SCXPCA_SPLIT_MAX_BYTES must be a non-negative number");
+    }
+    return kMaxSplitBytes;
+}
 constexpr int kStatsCG = 2048; // genes per column-stats block
 constexpr int kMaxColBlocks = 1024;  // partial rows of the column-sum kernels
 
@@ -175,7 +191,8 @@ struct scxpca_matrix {
         std::vector<uint32_t> host;  // kept alive for the asynchronous upload
         DevBuf dev;
     } gorder[2];  // one per wave-slot layout (KP = 64 and KP = 128)
-    bool use_split = false;
+    bool use_split = false;    // split offsets + k_build_tcsc4 (+ per-tile column partials)
+    bool chunked_fwd = false;  // forward SpMM by k_spmm_fwd3 (needs the split offsets)
     // cached column statistics of X~
     bool stats_ready = false;
     DevBuf tsum, tsq;  // per-tile column-statistics partials (written by the tile-CSC build)
@@ -364,13 +381,16 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     }
     if (m->n_obs == 0) return;
     m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
-    m->use_split = m->n_vars <= kMaxSplitVars;
+    m->n_tiles = static_cast<int>((m->n_obs + kTileRows - 1) / kTileRows);
+    const double split_bytes = 4.0 * static_cast<double>(m->n_obs) * (m->nsplit + 1) +
+                               16.0 * static_cast<double>(m->n_tiles) * static_cast<double>(m->n_vars);
+    m->use_split = split_bytes <= split_max_bytes();
+    m->chunked_fwd = m->use_split && m->n_vars <= kMaxChunkedFwdVars;
     if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
     if (m->use_split) m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
-    m->n_tiles = static_cast<int>((m->n_obs + kTileRows - 1) / kTileRows);
     m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
-    m->trow.ensure(std::max<int64_t>(m->nnz, 1));
-    m->tval.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+    m->trow.ensure(m->nnz + kEntryPad);
+    m->tval.ensure(sizeof(float) * (m->nnz + kEntryPad));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     if (m->use_split) {
@@ -405,7 +425,7 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
 void ensure_xbf(scxpca_matrix *m) {
     if (m->xbf_ready) return;
     m->xbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
-    m->txbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    m->txbf.ensure(sizeof(uint16_t) * (m->nnz + kEntryPad));
     if (m->n_obs > 0) {
         const unsigned nb = std::min(blocks_for(m->nnz, 256), 65536u);
         k_to_bf16<<<nb, 256, 0, m->stream>>>(m->xt.as<float>(), m->nnz, m->xbf.as<uint16_t>());
@@ -484,10 +504,10 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
 
 template <int KP>
 struct Engine {
-    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t5 (Z in registers)
-    static constexpr int TCAP = (KP == 64) ? 256 : 128;  // per-wave entry list in k_spmm_t5
-    static constexpr int TW = 16;                   // waves per k_spmm_t5 block
-    static constexpr int CG = TW * GW;              // genes per k_spmm_t5 block
+    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t6 (Z in registers)
+    static constexpr int NB0 = (KP == 64) ? 4 : 1;  // slot-0 prefetch batches in k_spmm_t6
+    static constexpr int TW = 16;                   // waves per k_spmm_t6 block
+    static constexpr int CG = TW * GW;              // genes per k_spmm_t6 block
     static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
     static constexpr int FPF = 8;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
@@ -567,7 +587,7 @@ struct Engine {
         if (m->n_obs == 0) return;
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
         const float *mcp = centered ? mc32 : nullptr;
-        if (m->use_split) {
+        if (m->chunked_fwd) {
             constexpr int NWF = 16;
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
@@ -601,13 +621,13 @@ struct Engine {
             StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
             const uint32_t *gp = gperm;
             if (bf16)
-                k_spmm_t5<KP, GW, TCAP, true, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                k_spmm_t6<KP, GW, NB0, true, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(),
                                                                    m->txbf.as<uint16_t>(), Q, gp, nch_t, nrb_t,
                                                                    tiles_per_tr, m->n_obs, nv, m->n_tiles,
                                                                    m->tpart.as<float>(), qsp);
             else
-                k_spmm_t5<KP, GW, TCAP, false, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
                                                                     m->trow.as<uint8_t>(), m->tval.as<float>(), nullptr,
                                                                     Q, gp, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
                                                                     m->n_tiles, m->tpart.as<float>(), qsp);

@@ -7,8 +7,13 @@ same input:
     log1p-only branch), genes never expressed, a row holding every gene (a
     forward window longer than one 64-entry batch) and a gene expressed in
     every row (a transposed-SpMM wave list longer than its prefetch capacity);
-  * wide3k: 3,000 genes (the general tile-CSC build, many gene chunks);
-  * wide9k: 9,000 genes (the row-parallel forward SpMM without split offsets);
+  * dense60: 600 x 200 with 60 genes expressed in every row (transposed-SpMM
+    runs past the prefetched batches in every gene slot, forward windows
+    longer than one batch);
+  * wide3k: 3,000 genes (many gene chunks);
+  * wide9k: 9,000 genes (the row-parallel forward SpMM, split offsets);
+  * wide9k_general: the same matrix with SCXPCA_SPLIT_MAX_BYTES=0 (no split
+    offsets: the general tile-CSC build and the tile-CSC column statistics);
   * tiny: 40 x 30.
 Tolerances are those of tests/test_gpu_parity.py: SpMM panels to 2e-5 of the
 f64 magnitude, column statistics to 5e-7 relative, PCA singular values to 1e-4
@@ -17,6 +22,7 @@ the same Omega.
 """
 from __future__ import annotations
 
+import os
 import sys
 
 import numpy as np
@@ -86,12 +92,23 @@ def ragged_matrix():
     return c
 
 
+def dense60_matrix():
+    d = structured(600, 200, seed=15, density=0.05)
+    d[:, 20:80] = 1.0 + (np.arange(600)[:, None] + np.arange(60)[None, :]) % 9
+    return to_csr(d)
+
+
 MATRICES = {
     "ragged": ragged_matrix,
+    "dense60": dense60_matrix,
     "wide3k": lambda: to_csr(structured(1500, 3000, seed=12, density=0.05)),
     "wide9k": lambda: to_csr(structured(1200, 9000, seed=13, density=0.03)),
+    "wide9k_general": lambda: to_csr(structured(1200, 9000, seed=13, density=0.03)),
     "tiny": lambda: to_csr(structured(40, 30, seed=14, density=0.4)),
 }
+
+
+ENV = {"wide9k_general": {"SCXPCA_SPLIT_MAX_BYTES": "0"}}
 
 
 @pytest.fixture(scope="module", params=list(MATRICES))
@@ -99,7 +116,17 @@ def case(request, gpu):
     name = request.param
     csr = MATRICES[name]()
     say(f"case: Matrix({name}) {csr.shape} nnz={len(csr.data)}")
-    mat = gpu.Matrix(csr.indptr, csr.indices, csr.data, csr.shape, target_sum=1e4, log1p=True)
+    env = ENV.get(name, {})
+    saved = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        mat = gpu.Matrix(csr.indptr, csr.indices, csr.data, csr.shape, target_sum=1e4, log1p=True)
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
     view = csr.view()
     xt = view.with_data(oracle.transform(view, 1e4, True))
     mu = oracle.col_sums_and_sum_sq(xt)[0] / csr.shape[0]
