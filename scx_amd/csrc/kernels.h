@@ -1461,6 +1461,17 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
             }
         }
     };
+    // one of the QPT loads of a Q tile (issued spread over the gene loop, so
+    // the tile's 64-128 KB do not arrive as one burst ahead of the entry loads)
+    auto load_q_j = [&](int tt, int j) {
+        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
+        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
+        const int i = tid + j * NT;
+        const bool ok = i / (KP / 4) < nr;
+        const float4 v = Q4[ok ? i : 0];
+        qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+    };
     // run start (tile-relative) and length of each gene in a tile, and the
     // tile's entry count: scalars
     uint32_t lo[GW], ln[GW], nlo[GW], nln[GW], tot = 0u, ntot = 0u;
@@ -1528,8 +1539,8 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
 #pragma unroll
                 for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
         }
-        // next tile: Q tile, base, run layout (bounds loaded one tile ago)
-        load_q(tn);
+        // next tile: base, run layout (bounds loaded one tile ago); its Q tile
+        // is loaded during the gene loop
         const int64_t ntb = tn != t ? tb + tot : tb;
         layout(wlo, whi, nlo, nln, ntot);
         bounds(tnn, wlo, whi);
@@ -1557,6 +1568,9 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
                 longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
                 if (len > 0u) gather(z[gi], cr, cv, len);
             }
+#pragma unroll
+            for (int j = 0; j < QPT; ++j)
+                if ((j * GW) / QPT == gi) load_q_j(tn, j);
         }
         // entries beyond the prefetched batches: read directly
         if (longest) {

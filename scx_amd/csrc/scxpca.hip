@@ -112,9 +112,9 @@ inline unsigned blocks_for(int64_t n, int per = 256) {
 }
 
 constexpr int kCB = 128;       // gene width of the row split offsets
-constexpr int64_t kMaxChunkedFwdVars = 8192;  // the chunked forward SpMM up to this many genes
-// split offsets, the per-(tile, chunk) tile-CSC build and its per-tile column
-// partials while their metadata stays under this size; the general build beyond
+// split offsets, the per-(tile, chunk) tile-CSC build with its per-tile column
+// partials, and the chunked forward SpMM while their metadata stays under this
+// size; the general build and the row-parallel forward beyond
 // (SCXPCA_SPLIT_MAX_BYTES, read when a matrix is created, overrides it)
 constexpr double kMaxSplitBytes = 32.0 * (1ull << 30);
 
@@ -191,8 +191,7 @@ struct scxpca_matrix {
         std::vector<uint32_t> host;  // kept alive for the asynchronous upload
         DevBuf dev;
     } gorder[2];  // one per wave-slot layout (KP = 64 and KP = 128)
-    bool use_split = false;    // split offsets + k_build_tcsc4 (+ per-tile column partials)
-    bool chunked_fwd = false;  // forward SpMM by k_spmm_fwd3 (needs the split offsets)
+    bool use_split = false;  // split offsets, k_build_tcsc4 (+ per-tile column partials), k_spmm_fwd3
     // cached column statistics of X~
     bool stats_ready = false;
     DevBuf tsum, tsq;  // per-tile column-statistics partials (written by the tile-CSC build)
@@ -385,7 +384,6 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
     const double split_bytes = 4.0 * static_cast<double>(m->n_obs) * (m->nsplit + 1) +
                                16.0 * static_cast<double>(m->n_tiles) * static_cast<double>(m->n_vars);
     m->use_split = split_bytes <= split_max_bytes();
-    m->chunked_fwd = m->use_split && m->n_vars <= kMaxChunkedFwdVars;
     if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
     if (m->use_split) m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
     m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
@@ -587,7 +585,7 @@ struct Engine {
         if (m->n_obs == 0) return;
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
         const float *mcp = centered ? mc32 : nullptr;
-        if (m->chunked_fwd) {
+        if (m->use_split) {
             constexpr int NWF = 16;
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);

@@ -11,9 +11,10 @@ same input:
     runs past the prefetched batches in every gene slot, forward windows
     longer than one batch);
   * wide3k: 3,000 genes (many gene chunks);
-  * wide9k: 9,000 genes (the row-parallel forward SpMM, split offsets);
+  * wide9k: 9,000 genes (35 forward gene chunks);
   * wide9k_general: the same matrix with SCXPCA_SPLIT_MAX_BYTES=0 (no split
-    offsets: the general tile-CSC build and the tile-CSC column statistics);
+    offsets: the general tile-CSC build, the tile-CSC column statistics and
+    the row-parallel forward SpMM);
   * tiny: 40 x 30.
 Tolerances are those of tests/test_gpu_parity.py: SpMM panels to 2e-5 of the
 f64 magnitude, column statistics to 5e-7 relative, PCA singular values to 1e-4
