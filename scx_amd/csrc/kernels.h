@@ -1443,6 +1443,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         b = (gvalid || spare) ? b0 : 0u;
     };
     constexpr int QPT = kTileRows * KP / 4 / NT;
+    constexpr bool kSpreadQ = KP == 64;
     float4 qpre[QPT];
     auto load_q = [&](int tt) {
         const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
@@ -1540,7 +1541,9 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
                 for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
         }
         // next tile: base, run layout (bounds loaded one tile ago); its Q tile
-        // is loaded during the gene loop
+        // is loaded during the gene loop (KP = 64) or here in one go (KP = 128:
+        // measured faster at 2M x 33,538)
+        if (!kSpreadQ) load_q(tn);
         const int64_t ntb = tn != t ? tb + tot : tb;
         layout(wlo, whi, nlo, nln, ntot);
         bounds(tnn, wlo, whi);
@@ -1570,7 +1573,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
             }
 #pragma unroll
             for (int j = 0; j < QPT; ++j)
-                if ((j * GW) / QPT == gi) load_q_j(tn, j);
+                if (kSpreadQ && (j * GW) / QPT == gi) load_q_j(tn, j);
         }
         // entries beyond the prefetched batches: read directly
         if (longest) {
