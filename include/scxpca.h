@@ -139,6 +139,12 @@ typedef enum {
 } scxpca_stage;
 scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms_total,
                                     int64_t *launches, int32_t n_stages);
+/* Restrict the stage profiler to the stages whose bit (1 << scxpca_stage) is
+ * set in stage_mask; 0 selects every stage (the default). Each timed stage
+ * costs two event records on the stream (a few microseconds of device time
+ * each), so a caller timing one stage inside a measured loop selects only it.
+ * Extends the reference's profiler, which has no per-stage switch. */
+scxpca_status scxpca_matrix_profile_stages(scxpca_matrix *m, uint32_t stage_mask);
 
 /* Stage probes on a resident matrix (host buffers in and out), used by the
  * parity tests. Each mirrors one reference function:

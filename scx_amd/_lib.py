@@ -33,6 +33,7 @@ EXPORTS = (
     "scxpca_matrix_refresh",
     "scxpca_matrix_pca",
     "scxpca_matrix_profile",
+    "scxpca_matrix_profile_stages",
     "scxpca_matrix_stats",
     "scxpca_matrix_spmm",
     "scxpca_qr",
@@ -106,6 +107,7 @@ def lib() -> ctypes.CDLL:
     l.scxpca_matrix_pca.argtypes = [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]
     l.scxpca_matrix_stats.argtypes = [_vp, _vp, _vp, _vp, _vp]
     l.scxpca_matrix_profile.argtypes = [_vp, _i32, _vp, _vp, _i32]
+    l.scxpca_matrix_profile_stages.argtypes = [_vp, ctypes.c_uint32]
     l.scxpca_matrix_spmm.argtypes = [_vp, _i32, _i64, _vp, _vp, _i32, _vp]
     l.scxpca_qr.argtypes = [_vp, _i64, _i64, _i32, ctypes.c_int, _vp]
     l.scxpca_project_forward.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
@@ -206,9 +208,13 @@ class Matrix:
     def refresh(self):
         check(lib().scxpca_matrix_refresh(self.handle))
 
-    def profile(self, enable: bool = True) -> dict:
+    def profile(self, enable: bool = True, stages=None) -> dict:
         """Per-stage device time (ms) and launch counts since the last call;
-        then sets profiling on or off."""
+        then sets profiling on or off, for the named stages only (None: all)."""
+        mask = 0
+        for name in stages or ():
+            mask |= 1 << STAGES.index(name)
+        check(lib().scxpca_matrix_profile_stages(self.handle, mask))
         n = len(STAGES)
         ms = np.zeros(n)
         cnt = np.zeros(n, dtype=np.int64)

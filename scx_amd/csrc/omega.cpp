@@ -193,8 +193,20 @@ double omega_draw_at(uint64_t seed, uint64_t w, uint32_t *consumed) {
 // leaves the n_words computed on the device.
 bool omega_walk(const uint8_t *c, uint64_t n_words, uint64_t n_samples, uint64_t seed, uint32_t *start,
                 std::vector<std::pair<uint32_t, double>> *overrides) {
+    constexpr uint64_t kOnes = 0x0101010101010101ull;  // eight one-word draws in a row
     uint64_t w = 0;
     for (uint64_t j = 0; j < n_samples; ++j) {
+        // fast path: most draws take one word, so eight starts at once when the
+        // next eight outcome bytes all say so (the chain is the same either way)
+        while (j + 8 <= n_samples && w + 8 <= n_words) {
+            uint64_t x;
+            std::memcpy(&x, c + w, 8);
+            if (x != kOnes) break;
+            for (uint32_t t = 0; t < 8; ++t) start[j + t] = static_cast<uint32_t>(w + t);
+            j += 8;
+            w += 8;
+        }
+        if (j >= n_samples) break;
         if (w >= n_words) return false;
         start[j] = static_cast<uint32_t>(w);
         const uint8_t cw = c[w];

@@ -154,10 +154,36 @@ struct DeviceGuard {
 
 // Buffers of the device Omega generation. The host vectors stay alive with
 // their owner, so the uploads from them need no synchronization.
+// Pinned host buffer (hipHostMalloc): copies to and from it are DMA transfers
+// that need no staging through a runtime bounce buffer. Reused across calls;
+// callers synchronise the stream before rewriting or freeing it.
+struct PinnedBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf &) = delete;
+    PinnedBuf &operator=(const PinnedBuf &) = delete;
+    ~PinnedBuf() { release(); }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void ensure(size_t n) {
+        if (n <= bytes && p) return;
+        release();
+        if (n == 0) n = 16;
+        HIPC(hipHostMalloc(&p, n, hipHostMallocDefault));
+        bytes = n;
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(p); }
+};
+
 struct OmegaScratch {
     DevBuf w, c, v, start, tab, ov;
-    std::vector<uint8_t> hc;
-    std::vector<uint32_t> hstart, hpos;
+    PinnedBuf hc, hstart;  // draw outcomes in, draw starts out
+    std::vector<uint32_t> hpos;
     std::vector<double> htab, hval;
 };
 
@@ -188,7 +214,7 @@ struct scxpca_matrix {
     struct GeneOrder {
         int gw = 0, nch = 0, tw = 0;
         bool valid = false;
-        std::vector<uint32_t> host;  // kept alive for the asynchronous upload
+        PinnedBuf host;  // kept alive for the asynchronous upload
         DevBuf dev;
     } gorder[2];  // one per wave-slot layout (KP = 64 and KP = 128)
     bool use_split = false;  // split offsets, k_build_tcsc4 (+ per-tile column partials), k_spmm_fwd3
@@ -202,6 +228,7 @@ struct scxpca_matrix {
     OmegaScratch om;  // device Omega scratch (device and host buffers)
     // stage profiler (events on the matrix stream; off unless enabled)
     bool profiling = false;
+    uint32_t prof_mask = ~0u;  // stages the profiler times (bit s = scxpca_stage s)
     struct Mark { int stage; hipEvent_t a, b; };
     std::vector<Mark> marks;
 
@@ -223,7 +250,7 @@ struct StageScope {
     hipStream_t s;
     int idx = -1;
     StageScope(scxpca_matrix *mm, hipStream_t ss, int stage) : m(mm), s(ss) {
-        if (!m->profiling) return;
+        if (!m->profiling || !((m->prof_mask >> stage) & 1u)) return;
         scxpca_matrix::Mark mk{stage, nullptr, nullptr};
         HIPC(hipEventCreate(&mk.a));
         HIPC(hipEventCreate(&mk.b));
@@ -313,20 +340,26 @@ const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
     if (go.valid && go.gw == gw && go.nch == nch && go.tw == tw) return go.dev.as<uint32_t>();
     const int64_t nv = m->n_vars;
     const int64_t W = static_cast<int64_t>(nch) * tw, S = W * gw;
-    std::vector<uint32_t> order(nv);
-    for (int64_t c = 0; c < nv; ++c) order[c] = static_cast<uint32_t>(c);
-    std::stable_sort(order.begin(), order.end(),
-                     [&](uint32_t a, uint32_t b) { return m->host_gcnt[a] > m->host_gcnt[b]; });
+    // genes by entry count, descending, ties by gene index: one 64-bit key
+    // each (a matrix shell that was never prepared has no counts: all zero)
+    std::vector<uint64_t> key(nv);
+    const int64_t nc = static_cast<int64_t>(m->host_gcnt.size());
+    for (int64_t c = 0; c < nv; ++c) {
+        const uint32_t cnt = c < nc ? m->host_gcnt[c] : 0u;
+        key[c] = (static_cast<uint64_t>(~cnt) << 32) | static_cast<uint64_t>(c);
+    }
+    std::sort(key.begin(), key.end());
     // the previous upload from go.host (if any) was drained by prepare's sync
-    go.host.assign(std::max<int64_t>(S, 1), static_cast<uint32_t>(nv));
+    go.host.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
+    uint32_t *h = go.host.as<uint32_t>();
+    for (int64_t i = 0; i < std::max<int64_t>(S, 1); ++i) h[i] = static_cast<uint32_t>(nv);
     for (int64_t i = 0; i < nv; ++i) {
         const int64_t r = i / W, pos = i % W;
         const int64_t w = (r % 2 == 0) ? pos : W - 1 - pos;
-        go.host[w * gw + r] = order[i];
+        h[w * gw + r] = static_cast<uint32_t>(key[i]);
     }
-    go.dev.ensure(sizeof(uint32_t) * go.host.size());
-    HIPC(hipMemcpyAsync(go.dev.p, go.host.data(), sizeof(uint32_t) * go.host.size(), hipMemcpyHostToDevice,
-                        m->stream));
+    go.dev.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
+    HIPC(hipMemcpyAsync(go.dev.p, h, sizeof(uint32_t) * std::max<int64_t>(S, 1), hipMemcpyHostToDevice, m->stream));
     go.gw = gw;
     go.nch = nch;
     go.tw = tw;
@@ -909,26 +942,28 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
         sc.w.ensure(sizeof(uint64_t) * n_words);
         sc.c.ensure(n_words);
         sc.v.ensure(sizeof(double) * n_words);
-        sc.tab.ensure(sizeof(double) * 2 * 257);
+        if (!sc.tab.p) {  // the ziggurat tables are constants: built and uploaded once
+            sc.tab.ensure(sizeof(double) * 2 * 257);
+            sc.htab.resize(2 * 257);
+            omega_tables(sc.htab.data(), sc.htab.data() + 257);
+            HIPC(hipMemcpyAsync(sc.tab.p, sc.htab.data(), sizeof(double) * sc.htab.size(), hipMemcpyHostToDevice, s));
+        }
         ChaChaKey key;
         omega_key(seed, key.k);
-        sc.htab.resize(2 * 257);
-        omega_tables(sc.htab.data(), sc.htab.data() + 257);
-        HIPC(hipMemcpyAsync(sc.tab.p, sc.htab.data(), sizeof(double) * sc.htab.size(), hipMemcpyHostToDevice, s));
         k_chacha8_u64<<<blocks_for((n_words + 7) / 8), 256, 0, s>>>(key, n_words, sc.w.as<uint64_t>());
         HIPC(hipGetLastError());
         k_zig_outcome<<<std::min(blocks_for(n_words), 65536u), 256, 0, s>>>(
             sc.w.as<uint64_t>(), n_words, sc.tab.as<double>(), sc.tab.as<double>() + 257, sc.c.as<uint8_t>(),
             sc.v.as<double>());
         HIPC(hipGetLastError());
-        sc.hc.resize(n_words);
-        HIPC(hipMemcpyAsync(sc.hc.data(), sc.c.p, n_words, hipMemcpyDeviceToHost, s));
+        sc.hc.ensure(n_words);
+        HIPC(hipMemcpyAsync(sc.hc.p, sc.c.p, n_words, hipMemcpyDeviceToHost, s));
         HIPC(hipStreamSynchronize(s));
-        sc.hstart.resize(n);
+        sc.hstart.ensure(sizeof(uint32_t) * n);
         std::vector<std::pair<uint32_t, double>> ov;
-        if (!omega_walk(sc.hc.data(), n_words, n, seed, sc.hstart.data(), &ov)) continue;  // more words
+        if (!omega_walk(sc.hc.as<uint8_t>(), n_words, n, seed, sc.hstart.as<uint32_t>(), &ov)) continue;  // more words
         sc.start.ensure(sizeof(uint32_t) * n);
-        HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+        HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
         if (!ov.empty()) {
             sc.hpos.resize(ov.size());
             sc.hval.resize(ov.size());
@@ -957,9 +992,9 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
     sc.v.ensure(sizeof(double) * n);
     HIPC(hipMemcpyAsync(sc.v.p, sc.hval.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
     sc.start.ensure(sizeof(uint32_t) * n);
-    sc.hstart.resize(n);
-    for (uint64_t i = 0; i < n; ++i) sc.hstart[i] = static_cast<uint32_t>(i);
-    HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.data(), sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
+    sc.hstart.ensure(sizeof(uint32_t) * n);
+    for (uint64_t i = 0; i < n; ++i) sc.hstart.as<uint32_t>()[i] = static_cast<uint32_t>(i);
+    HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
     k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
                                                              static_cast<int64_t>(rows), k, panel, out64);
     HIPC(hipGetLastError());
@@ -1283,6 +1318,13 @@ scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms
         }
         m->clear_marks();
         m->profiling = enable != 0;
+    });
+}
+
+scxpca_status scxpca_matrix_profile_stages(scxpca_matrix *m, uint32_t stage_mask) {
+    return guarded([&] {
+        if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
+        m->prof_mask = stage_mask ? stage_mask : ~0u;
     });
 }
 

@@ -138,3 +138,13 @@ def test_repeated_column_in_a_row_is_rejected(gpu, on_device):
     else:
         with pytest.raises(ValueError, match="repeats a column index"):
             gpu.Matrix(indptr, indices, data, (3, 5), target_sum=1e4, log1p=True)
+
+
+@pytest.mark.parametrize("value", ["lots", "-1", "1e9x"])
+def test_invalid_split_cap_is_rejected(gpu, small, monkeypatch, value):
+    # SCXPCA_SPLIT_MAX_BYTES is read when a matrix is created; a value that is
+    # not a non-negative number is refused, not silently ignored
+    say(f"case: SCXPCA_SPLIT_MAX_BYTES={value!r}")
+    monkeypatch.setenv("SCXPCA_SPLIT_MAX_BYTES", value)
+    with pytest.raises(ValueError, match="SCXPCA_SPLIT_MAX_BYTES"):
+        gpu.Matrix(small.indptr, small.indices, small.data, small.shape, target_sum=1e4, log1p=True)
