@@ -770,6 +770,26 @@ __global__ void __launch_bounds__(256) k_unpack_panel32(const float *__restrict_
     dst[t] = src[r * KP + j];
 }
 
+// sig (kp), the column sums and sums of squares (nv each) and the status word
+// into one f64 buffer, so the end of a PCA call copies back once
+__global__ void __launch_bounds__(256) k_pack_results(const double *__restrict__ sig, int kp,
+                                                      const double *__restrict__ sum, const double *__restrict__ sq,
+                                                      int64_t nv, const int *__restrict__ status,
+                                                      double *__restrict__ out) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= kp + 2 * nv + 1) return;
+    double v;
+    if (i < kp)
+        v = sig[i];
+    else if (i < kp + nv)
+        v = sum[i - kp];
+    else if (i < kp + 2 * nv)
+        v = sq[i - kp - nv];
+    else
+        v = static_cast<double>(*status);
+    out[i] = v;
+}
+
 __global__ void k_f32_to_f64(const float *__restrict__ s, double *__restrict__ d, int64_t n) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t < n) d[t] = static_cast<double>(s[t]);

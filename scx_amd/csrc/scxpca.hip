@@ -224,7 +224,9 @@ struct scxpca_matrix {
     DevBuf sum, sq, cnt, psum, psq, pcnt;
     // PCA workspace
     DevBuf y, q, panel, panel2, panel64, b64, t64a, t64b, tpart, gpart, cpart, small, status;
-    DevBuf vflag;  // validation bits of the last prepare (k_prepare_rows)
+    PinnedBuf hgcnt;  // host side of gcnt: per-gene counts + the validation word
+    DevBuf dres;      // packed end-of-call results (k_pack_results)
+    PinnedBuf hres;   // their host copy
     OmegaScratch om;  // device Omega scratch (device and host buffers)
     // stage profiler (events on the matrix stream; off unless enabled)
     bool profiling = false;
@@ -276,8 +278,12 @@ int prepare(scxpca_matrix *m) {
     m->xbf_ready = false;
     if (m->n_obs == 0) return 0;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
-    m->vflag.ensure(sizeof(int));
-    HIPC(hipMemsetAsync(m->vflag.p, 0, sizeof(int), m->stream));
+    // gcnt holds the per-gene entry counts and, at [n_vars], the validation
+    // bits of k_prepare_rows: one copy brings both back
+    const int64_t nvc = std::max<int64_t>(m->n_vars, 1);
+    m->gcnt.ensure(sizeof(uint32_t) * (nvc + 1));
+    int *vflag = reinterpret_cast<int *>(m->gcnt.as<uint32_t>() + m->n_vars);
+    HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * (nvc + 1), m->stream));
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
     static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
     k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
@@ -285,7 +291,7 @@ int prepare(scxpca_matrix *m) {
                                                       (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
                                                       m->xt.as<float>(), m->lg.as<uint16_t>(),
                                                       m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
-                                                      m->vflag.as<int>());
+                                                      vflag);
     HIPC(hipGetLastError());
     if (m->use_split)
     {
@@ -293,17 +299,16 @@ int prepare(scxpca_matrix *m) {
         k_build_tcsc4<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
-            m->tsq.as<double>(), m->vflag.as<int>());
+            m->tsq.as<double>(), vflag);
     }
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
                                                          static_cast<int>(m->n_vars), m->tptr.as<uint32_t>(),
                                                          m->trow.as<uint8_t>(), m->tval.as<float>(),
-                                                         m->vflag.as<int>());
+                                                         vflag);
     HIPC(hipGetLastError());
-    // entries per gene -> host (the transposed SpMM's gene order is built from it)
-    m->gcnt.ensure(sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1));
-    HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * std::max<int64_t>(m->n_vars, 1), m->stream));
+    // entries per gene (counted into the zeroed gcnt) -> host: the transposed
+    // SpMM's gene order is built from them
     if (m->n_vars > 0) {
         const int tpb = 64;
         const dim3 grid(static_cast<unsigned>((m->n_vars + 255) / 256), static_cast<unsigned>((m->n_tiles + tpb - 1) / tpb));
@@ -311,15 +316,14 @@ int prepare(scxpca_matrix *m) {
                                                          m->gcnt.as<uint32_t>());
         HIPC(hipGetLastError());
     }
-    m->host_gcnt.assign(m->n_vars, 0u);
-    if (m->n_vars > 0)
-        HIPC(hipMemcpyAsync(m->host_gcnt.data(), m->gcnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost,
-                            m->stream));
-    int h = 0;
-    HIPC(hipMemcpyAsync(&h, m->vflag.p, sizeof(int), hipMemcpyDeviceToHost, m->stream));
+    m->hgcnt.ensure(sizeof(uint32_t) * (nvc + 1));
+    HIPC(hipMemcpyAsync(m->hgcnt.p, m->gcnt.p, sizeof(uint32_t) * (m->n_vars + 1), hipMemcpyDeviceToHost,
+                        m->stream));
     HIPC(hipStreamSynchronize(m->stream));
+    const uint32_t *hg = m->hgcnt.as<uint32_t>();
+    m->host_gcnt.assign(hg, hg + m->n_vars);
     for (auto &g : m->gorder) g.valid = false;
-    return h;
+    return static_cast<int>(hg[m->n_vars]);
 }
 
 // The rejection for the validation bits of prepare().
@@ -835,17 +839,23 @@ struct Engine {
         apply(Q, emb_dev, n, nc, nc, false);
         check();
         // results back
-        std::vector<double> hsig(KP), hsum(nv), hsq(nv);
-        int hstatus = 0;
-        HIPC(hipMemcpyAsync(hsig.data(), sig, sizeof(double) * KP, hipMemcpyDeviceToHost, s));
-        HIPC(hipMemcpyAsync(hsum.data(), m->sum.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-        HIPC(hipMemcpyAsync(hsq.data(), m->sq.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
-        HIPC(hipMemcpyAsync(&hstatus, status, sizeof(int), hipMemcpyDeviceToHost, s));
+        // sig, column sums, sums of squares and status: one packed copy into pinned memory
+        const int64_t nres = KP + 2 * nv + 1;
+        m->dres.ensure(sizeof(double) * nres);
+        m->hres.ensure(sizeof(double) * nres);  // the previous call's copy was drained by its sync
+        k_pack_results<<<blocks_for(nres), 256, 0, s>>>(sig, KP, m->sum.as<double>(), m->sq.as<double>(), nv, status,
+                                                       m->dres.as<double>());
+        check();
+        HIPC(hipMemcpyAsync(m->hres.p, m->dres.p, sizeof(double) * nres, hipMemcpyDeviceToHost, s));
         if (!out->on_device) {
             HIPC(hipMemcpyAsync(out->embeddings, emb_dev, sizeof(float) * n * nc, hipMemcpyDeviceToHost, s));
             HIPC(hipMemcpyAsync(out->components, comps_dev, sizeof(float) * nv * nc, hipMemcpyDeviceToHost, s));
         }
         HIPC(hipStreamSynchronize(s));
+        const double *hres = m->hres.as<double>();
+        const std::vector<double> hsig(hres, hres + KP), hsum(hres + KP, hres + KP + nv),
+            hsq(hres + KP + nv, hres + KP + 2 * nv);
+        const int hstatus = static_cast<int>(hres[KP + 2 * nv]);
         if (hstatus & 1) {
             if (p->qr_method == SCXPCA_QR_CHOLESKY)
                 throw Error(SCXPCA_ERR_LINALG,
