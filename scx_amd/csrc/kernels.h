@@ -1161,8 +1161,8 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
 // tile exit). Wave w owns the tile's rows 32w .. 32w+31 and flattens
 // their segments of the chunk (CSR positions indptr[r] + [spl[r][2c],
 // spl[r][2c+2]) for 128-gene split offsets); entry f = 64b + lane of batch b
-// finds its row by a wave-uniform walk over the few rows the batch touches
-// (no search), and the wave keeps up to kB4Batches batches (gene, X~, row) in
+// finds its row by a lane-parallel binary search over the 32 segment starts,
+// and the wave keeps up to kB4Batches batches (gene, X~, row) in
 // registers, so each entry is read from memory once. Pass 1 sets bit rl of
 // mask[w][gene] (rows within a wave are distinct bits; a gene occurs at most
 // once per row, which the matrix check enforces); the block then scans the
@@ -1225,26 +1225,32 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     for (int i = tid; i < NW * NG; i += blockDim.x) (&mask[0][0])[i] = 0u;
     if (lane == 0) wlo[w] = wslo;
     const int nb = static_cast<int>((wtotal + 63) / 64);
-    // batch b: the lane's row (-1: none) and CSR position; j is the first row
-    // whose segment reaches into the batch (wave-uniform, advanced in order)
-    int j = 0;
+    // batch b: the lane's row (-1: none) and CSR position. Entry f = 64b + lane
+    // belongs to the largest row j with fst[j] <= f (a 5-step binary search
+    // over the 32 segment starts, read across lanes by ds_bpermute; an empty
+    // row is never the largest such j unless it is the last, which the bound
+    // check then rejects), when f < fst[j] + len[j].
+    auto lane_u32 = [](int src_lane, uint32_t v) {
+        return static_cast<uint32_t>(__builtin_amdgcn_ds_bpermute(src_lane * 4, static_cast<int>(v)));
+    };
+    const uint32_t gp_lo = static_cast<uint32_t>(static_cast<uint64_t>(gp));
+    const uint32_t gp_hi = static_cast<uint32_t>(static_cast<uint64_t>(gp) >> 32);
     auto locate = [&](int b, int &row, int64_t &p) {
-        const uint32_t f0 = 64u * static_cast<uint32_t>(b), f = f0 + lane;
-        row = -1;
-        p = base;
-        while (j < 32) {
-            const uint32_t fs = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(fst), j));
-            const uint32_t ln = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(len), j));
-            if (fs >= f0 + 64u) break;
-            if (f >= fs && f < fs + ln) {
-                const uint32_t plo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(gp)), j));
-                const uint32_t phi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(static_cast<uint32_t>(static_cast<uint64_t>(gp) >> 32)), j));
-                row = j;
-                p = static_cast<int64_t>((static_cast<uint64_t>(phi) << 32) | plo) + (f - fs);
-            }
-            if (fs + ln > f0 + 64u) break;  // the row continues into the next batch
-            ++j;
+        const uint32_t f = 64u * static_cast<uint32_t>(b) + static_cast<uint32_t>(lane);
+        int lo = 0;
+        uint32_t fs = 0u;  // fst[0] = 0 <= f
+#pragma unroll
+        for (int step = 16; step > 0; step >>= 1) {
+            const uint32_t v = lane_u32(lo + step, fst);
+            const bool le = v <= f;
+            lo = le ? lo + step : lo;
+            fs = le ? v : fs;
         }
+        const uint32_t ln = lane_u32(lo, len);
+        const uint64_t g = (static_cast<uint64_t>(lane_u32(lo, gp_hi)) << 32) | lane_u32(lo, gp_lo);
+        const bool in = f < fs + ln;
+        row = in ? lo : -1;
+        p = in ? static_cast<int64_t>(g) + (f - fs) : base;
         p = p < p_last ? p : p_last;
     };
     int rowv[U];
@@ -1262,8 +1268,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
             xv[b] = xt[p];
         }
     }
-    const int j_ovf = j;  // walk state at the first batch held in memory only
-    __syncthreads();      // masks cleared
+    __syncthreads();  // masks cleared
     // pass 1: row masks per gene
 #pragma unroll
     for (int b = 0; b < U; ++b)
@@ -1328,7 +1333,6 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
 #pragma unroll
     for (int b = 0; b < U; ++b)
         if (rowv[b] >= 0) place(rowv[b], gv[b], xv[b]);
-    j = j_ovf;
     for (int b = U; b < nb; ++b) {
         int row;
         int64_t p;
@@ -1346,7 +1350,18 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     if (psum && tid < ng) {
         const uint32_t a = start[tid], e = start[tid + 1];
         double sm = 0.0, sq = 0.0;
-        for (uint32_t q = a; q < e; ++q) {
+        uint32_t q = a;
+        for (; q + 4 <= e; q += 4) {  // four loads in flight, the sums in run order
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = static_cast<double>(staged ? sval[q + u] : tval[base + cbase + q + u]);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                sm += v[u];
+                sq += v[u] * v[u];
+            }
+        }
+        for (; q < e; ++q) {
             const double v = static_cast<double>(staged ? sval[q] : tval[base + cbase + q]);
             sm += v;
             sq += v * v;
