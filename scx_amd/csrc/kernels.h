@@ -2131,9 +2131,10 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
         const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
         sh = c < k ? shift * tr : 0.0;
     }
-    // masked, shifted Gram staged row-major in LDS by a rolled loop, then one
-    // unconditional read per register
-#pragma unroll 1
+    // masked, shifted Gram staged row-major in LDS (eight rows' loads in
+    // flight: the registers of a[] are not live yet), then one unconditional
+    // read per register
+#pragma unroll 8
     for (int i = 0; i < KP; ++i) {
         const double g = (i < k && c < k) ? G[i * KP + c] : (i == c ? 1.0 : 0.0);
         Rs[i * KP + c] = (i == c) ? g + sh : g;
@@ -2176,8 +2177,8 @@ __global__ void __launch_bounds__(64) k_trinv_w(const double *__restrict__ R64, 
     __shared__ double Rs[KP * KP];
     __shared__ double rinv[KP];
     const int c = threadIdx.x;
-#pragma unroll 1
-    for (int i = 0; i < KP; ++i) Rs[i * KP + c] = R64[i * KP + c];
+#pragma unroll 8
+    for (int i = 0; i < KP; ++i) Rs[i * KP + c] = R64[i * KP + c];  // eight loads in flight
     rinv[c] = c < k ? 1.0 / R64[c * KP + c] : 1.0;
     wave_lds_sync();
     double x[KP];
