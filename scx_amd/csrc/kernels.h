@@ -936,9 +936,9 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
     };
     // First 64-entry batch of each (row, chunk) window, PF rows ahead; the
     // ring runs on across chunk boundaries (row r of chunk ch is slot r % PF).
-    uint32_t pidx[PF];
+    uint16_t pidx[PF];  // the loaded type: no extension on the loop edge
     float pval[PF];
-    auto issue = [&](int r, uint32_t wl, uint32_t wh, uint32_t &pi, float &pv) {
+    auto issue = [&](int r, uint32_t wl, uint32_t wh, uint16_t &pi, float &pv) {
         const int64_t base_r = rl64(rbase, r);
         const uint32_t lo_r = rl32(wl, r);
         int64_t p = base_r + lo_r + esel;
@@ -955,9 +955,23 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
         const int64_t g0 = static_cast<int64_t>(ch) * CG;
         bounds(ch + 1 < nch ? ch + 1 : ch, nlo, nhi);  // unconditional: see the note on loads
         __syncthreads();
-        for (int i = threadIdx.x; i < CG * KP / 4; i += blockDim.x) {
-            const int64_t gene = g0 + i / (KP / 4);
-            ms[i] = gene < n_vars ? M4[gene * (KP / 4) + i % (KP / 4)] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // the chunk's panel rows: unconditional loads at clamped rows (genes past
+        // n_vars zeroed by a select), MG in flight per wait
+        constexpr int MPT = CG * KP / 4 / (NW * kWave), MG = MPT < 2 ? MPT : 2;
+        static_assert(MPT * NW * kWave == CG * KP / 4 && MPT % MG == 0, "whole float4s per thread");
+#pragma unroll
+        for (int h = 0; h < MPT; h += MG) {
+            float4 mv[MG];
+#pragma unroll
+            for (int j = 0; j < MG; ++j) {
+                const int i = threadIdx.x + (h + j) * NW * kWave;
+                const int64_t gene = g0 + i / (KP / 4);
+                const bool ok = gene < n_vars;
+                const float4 v = M4[(ok ? gene : 0) * (KP / 4) + i % (KP / 4)];
+                mv[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int j = 0; j < MG; ++j) ms[threadIdx.x + (h + j) * NW * kWave] = mv[j];
         }
         __syncthreads();
         // Main flow: the first batch of every row's window, straight-line
@@ -967,18 +981,20 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
-            const uint32_t ci = pidx[r % PF];
-            const float cv = pval[r % PF];
-            if (r + PF < R)
-                issue(r + PF, wlo, whi, pidx[r % PF], pval[r % PF]);
-            else  // next chunk's rows (after the last chunk: harmless reads)
-                issue(r + PF - R, nlo, nhi, pidx[r % PF], pval[r % PF]);
             const uint32_t len = hi_r - lo_r;
             any_long |= len > static_cast<uint32_t>(kWave);
             const int cnt = len < static_cast<uint32_t>(kWave) ? static_cast<int>(len) : kWave;
             const bool ok = esel < cnt;
-            const uint32_t ofs = ok ? (ci & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
-            const float x = ok ? cv : 0.f;
+            // the slot is read before its next load is issued (a scheduling
+            // barrier keeps the order), so the old and new values share a
+            // register and the loop carries it without a copy
+            const uint32_t ofs = ok ? (static_cast<uint32_t>(pidx[r % PF]) & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+            const float x = ok ? pval[r % PF] : 0.f;
+            __builtin_amdgcn_sched_barrier(0);
+            if (r + PF < R)
+                issue(r + PF, wlo, whi, pidx[r % PF], pval[r % PF]);
+            else  // next chunk's rows (after the last chunk: harmless reads)
+                issue(r + PF - R, nlo, nhi, pidx[r % PF], pval[r % PF]);
             gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
         }
         // Windows longer than one batch (rare): the remaining batches, read
@@ -1520,9 +1536,9 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         tot_ = rl32(wh, GW);
     };
     // prefetched batches in the gather layout: NB0 for slot 0, one per other slot
-    uint32_t prow0[NB0], prow[GW];
+    uint8_t prow0[NB0], prow[GW];  // the loaded type: no extension on the loop edge
     float pval0[NB0], pval[GW];
-    auto issue = [&](int64_t p0, uint32_t &pr, float &pv) {  // p0 wave-uniform
+    auto issue = [&](int64_t p0, uint8_t &pr, float &pv) {  // p0 wave-uniform
         pr = (trow + p0)[esel];
         pv = BF16 ? bf16_to_f32((txbf + p0)[esel]) : (tval + p0)[esel];
     };
@@ -1541,6 +1557,21 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         const bool ok = esel < cnt;
         gather_batch<E>(acc, ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt, reinterpret_cast<const char *>(qs),
                         lane_ofs);
+    };
+    // A prefetched batch is turned into (LDS offset, value) before the next
+    // tile's load into the same registers is issued (a scheduling barrier
+    // keeps that order): the old and new values never overlap, so both use
+    // one register and the loop carries it without a copy (a copy of a
+    // register with a load in flight waits for every outstanding load).
+    auto prep = [&](uint32_t rr, float v, uint32_t left, uint32_t &ofs, float &x) {
+        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
+        const bool ok = esel < cnt;
+        ofs = ok ? rr * (KP * 4) : 0u;
+        x = ok ? v : 0.f;
+    };
+    auto gather_prepped = [&](float (&acc)[E], uint32_t ofs, float x, uint32_t left) {
+        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
+        gather_batch<E>(acc, ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
     };
 
     // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
@@ -1587,24 +1618,26 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         for (int gi = 0; gi < GW; ++gi) {
             const uint32_t len = ln[gi];
             if (gi == 0) {
-                uint32_t cr[NB0];
-                float cv[NB0];
+                uint32_t ofs[NB0];
+                float x[NB0];
 #pragma unroll
-                for (int b = 0; b < NB0; ++b) {
-                    cr[b] = prow0[b];
-                    cv[b] = pval0[b];
-                }
+                for (int b = 0; b < NB0; ++b)
+                    prep(prow0[b], pval0[b], len > static_cast<uint32_t>(b * kWave) ? len - b * kWave : 0u, ofs[b],
+                         x[b]);
+                __builtin_amdgcn_sched_barrier(0);
                 issue_tile(ntb, nlo, 0, 1);
                 longest = len > static_cast<uint32_t>(NB0 * kWave) ? len : 0u;
 #pragma unroll
                 for (int b = 0; b < NB0; ++b)
-                    if (len > static_cast<uint32_t>(b * kWave)) gather(z[0], cr[b], cv[b], len - b * kWave);
+                    if (len > static_cast<uint32_t>(b * kWave)) gather_prepped(z[0], ofs[b], x[b], len - b * kWave);
             } else {
-                const uint32_t cr = prow[gi];
-                const float cv = pval[gi];
+                uint32_t ofs;
+                float x;
+                prep(prow[gi], pval[gi], len, ofs, x);
+                __builtin_amdgcn_sched_barrier(0);
                 issue_tile(ntb, nlo, gi, gi + 1);
                 longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
-                if (len > 0u) gather(z[gi], cr, cv, len);
+                if (len > 0u) gather_prepped(z[gi], ofs, x, len);
             }
 #pragma unroll
             for (int j = 0; j < QPT; ++j)
@@ -1616,7 +1649,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
             for (int gi = 0; gi < GW; ++gi) {
                 const uint32_t len = ln[gi];
                 for (uint32_t o = (gi == 0 ? NB0 : 1) * kWave; o < len; o += kWave) {
-                    uint32_t rr;
+                    uint8_t rr;
                     float v;
                     issue(tb + lo[gi] + o, rr, v);
                     gather(z[gi], rr, v, len - o);
