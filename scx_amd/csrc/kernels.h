@@ -1495,6 +1495,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     };
     constexpr int QPT = kTileRows * KP / 4 / NT;
     constexpr bool kSpreadQ = KP == 64;
+    constexpr bool kIssueLate = KP == 128;
     float4 qpre[QPT];
     auto load_q = [&](int tt) {
         const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
@@ -1560,9 +1561,12 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     };
     // A prefetched batch is turned into (LDS offset, value) before the next
     // tile's load into the same registers is issued (a scheduling barrier
-    // keeps that order): the old and new values never overlap, so both use
-    // one register and the loop carries it without a copy (a copy of a
-    // register with a load in flight waits for every outstanding load).
+    // keeps that order), so the old and new values need not overlap. With
+    // KP = 128 the load goes out after the gene's gathers: only then does
+    // the register allocator keep each gene in one register, so the loop
+    // edge has no copy (a copy of a register with a load in flight waits for
+    // every outstanding load). With KP = 64 the edge keeps its copies but the
+    // earlier issue (a longer look-ahead) measured faster.
     auto prep = [&](uint32_t rr, float v, uint32_t left, uint32_t &ofs, float &x) {
         const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
         const bool ok = esel < cnt;
@@ -1625,19 +1629,21 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
                     prep(prow0[b], pval0[b], len > static_cast<uint32_t>(b * kWave) ? len - b * kWave : 0u, ofs[b],
                          x[b]);
                 __builtin_amdgcn_sched_barrier(0);
-                issue_tile(ntb, nlo, 0, 1);
+                if (!kIssueLate) issue_tile(ntb, nlo, 0, 1);
                 longest = len > static_cast<uint32_t>(NB0 * kWave) ? len : 0u;
 #pragma unroll
                 for (int b = 0; b < NB0; ++b)
                     if (len > static_cast<uint32_t>(b * kWave)) gather_prepped(z[0], ofs[b], x[b], len - b * kWave);
+                if (kIssueLate) issue_tile(ntb, nlo, 0, 1);
             } else {
                 uint32_t ofs;
                 float x;
                 prep(prow[gi], pval[gi], len, ofs, x);
                 __builtin_amdgcn_sched_barrier(0);
-                issue_tile(ntb, nlo, gi, gi + 1);
+                if (!kIssueLate) issue_tile(ntb, nlo, gi, gi + 1);
                 longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
                 if (len > 0u) gather_prepped(z[gi], ofs, x, len);
+                if (kIssueLate) issue_tile(ntb, nlo, gi, gi + 1);
             }
 #pragma unroll
             for (int j = 0; j < QPT; ++j)
