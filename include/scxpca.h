@@ -117,6 +117,13 @@ scxpca_status scxpca_matrix_refresh(scxpca_matrix *m);
  * matrix's own are used). Replaces run_resident_power_loop + host SVD
  * (gpu_pca_resident.rs:295-347, gpu_pca.rs:592-697). */
 scxpca_status scxpca_matrix_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out);
+/* Covariance route on a resident matrix: exact PCA from the n_vars x n_vars
+ * sample covariance of X~ (f64 eigendecomposition), k = n_comps; the
+ * oversampling, power-iteration, seed, qr_method and precision fields are
+ * ignored. Replaces covariance_pca (scx-accel/src/pca/cpu.rs:1663-1854), the
+ * route CPU method="auto" takes at <= 5,000 genes (:1592,:1643-1660);
+ * n_vars <= 16384. sigma (if given) receives sqrt((n_obs - 1) lambda). */
+scxpca_status scxpca_matrix_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out);
 
 /* Stage profiler (the reference's SCX_GPU_PROFILE stage timers,
  * scx-gpu/src/profile.rs:77-208): while enabled, every stage launch on the

@@ -17,6 +17,10 @@ Parity status:
     distribution tests); no reference fixture pins the sample bits.
   * X~ transform, column stats, SpMMs, QR, SVD, assembly, total variance —
     pinned against closed forms, brute-force dense f64 and numpy/LAPACK.
+  * ``covariance_pca`` (numpy over the same X~, SpMM and total variance) —
+    pinned against the dense SVD of the centred X~ (eigenvalues =
+    sigma^2 / (n - 1), eigenvectors = right singular vectors up to sign,
+    embeddings = U Sigma).
 """
 from __future__ import annotations
 
@@ -249,4 +253,50 @@ def randomized_pca(x, n_comps=50, n_oversamples=10, n_power_iterations=2, zero_c
         "variance_ratio": ratio,
         "mean": mean if zero_center else None,
         "sigma_all": sig,
+    }
+
+
+def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False):
+    """``covariance_pca`` (scx-accel/src/pca/cpu.rs:1663-1854) on the raw CSR
+    ``x`` with the lazy transform applied once: the f64 cross-product X~^T X~
+    and column sums (:1713-1725; products of the f32 X~ values are exact in
+    f64, the sum order is BLAS's), ``finalize_covariance`` (:303-322), the
+    symmetric eigendecomposition (LAPACK syevd through numpy, as faer's
+    self-adjoint EVD at :1748-1751), the top ``n_comps`` eigenpairs largest
+    first (:1776-1785, eigenvalues clamped at 0), the guarded total variance
+    (:1762-1771, :1029-1092) and the embeddings (X~ - mu) V by the forward
+    SpMM (:1815-1846). Eigenvector signs are the solver's (the reference's
+    are faer's): compare up to sign."""
+    import scipy.sparse as sp
+
+    v = _as_view(x)
+    n, nv = v.shape
+    if n_comps <= 0:
+        raise ValueError(_PCA_ERRORS[1])
+    if n == 0 or nv == 0:
+        raise ValueError(_PCA_ERRORS[2])
+    if n_comps > min(n, nv):
+        raise ValueError(_PCA_ERRORS[3].format(nc=n_comps, mn=min(n, nv)))
+    xt = v.with_data(transform(v, target_sum, log1p))
+    a = sp.csr_matrix((xt.data.astype(np.float64), xt.indices, xt.indptr), shape=(n, nv)).toarray()
+    g = a.T @ a
+    col_sums = a.sum(axis=0)
+    means = col_sums / n if zero_center else None
+    col_sum_sq = np.diag(g).copy()
+    denom = max(n - 1.0, 1.0)
+    cov = (g - n * np.outer(means, means)) / denom if zero_center else g / denom
+    w, z = np.linalg.eigh(cov)
+    tv, _ = total_variance(xt, col_sum_sq, means)
+    idx = np.arange(nv - 1, nv - 1 - n_comps, -1)
+    var = np.maximum(w[idx], 0.0)
+    vmat = z[:, idx]
+    ratio = var / tv if tv > 0 else np.zeros(n_comps)
+    emb = spmm_forward(xt, np.ascontiguousarray(vmat), means)
+    return {
+        "embeddings": emb,
+        "components": np.ascontiguousarray(vmat.T),
+        "variance": var,
+        "variance_ratio": ratio,
+        "mean": means,
+        "total_variance": tv,
     }

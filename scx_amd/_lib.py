@@ -32,6 +32,7 @@ EXPORTS = (
     "scxpca_matrix_destroy",
     "scxpca_matrix_refresh",
     "scxpca_matrix_pca",
+    "scxpca_matrix_pca_covariance",
     "scxpca_matrix_profile",
     "scxpca_matrix_profile_stages",
     "scxpca_matrix_stats",
@@ -105,6 +106,7 @@ def lib() -> ctypes.CDLL:
     l.scxpca_matrix_destroy.argtypes = [_vp]
     l.scxpca_matrix_refresh.argtypes = [_vp]
     l.scxpca_matrix_pca.argtypes = [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]
+    l.scxpca_matrix_pca_covariance.argtypes = [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]
     l.scxpca_matrix_stats.argtypes = [_vp, _vp, _vp, _vp, _vp]
     l.scxpca_matrix_profile.argtypes = [_vp, _i32, _vp, _vp, _i32]
     l.scxpca_matrix_profile_stages.argtypes = [_vp, ctypes.c_uint32]
@@ -240,12 +242,20 @@ class Matrix:
 
     def pca(self, n_comps=50, n_oversamples=10, n_power_iterations=2, zero_center=True, seed=0,
             qr_method=QR_HOUSEHOLDER, precision=PRECISION_FP32, out=None):
-        """Run the PCA; returns a dict of host arrays (or fills device tensors
-        given in ``out`` = dict(embeddings=..., components=...))."""
+        """Run the randomized PCA; returns a dict of host arrays (or fills
+        device tensors given in ``out`` = dict(embeddings=..., components=...))."""
+        return self._run(lib().scxpca_matrix_pca, n_comps, n_oversamples, n_power_iterations, zero_center, seed,
+                         qr_method, precision, out, max(1, min(n_comps + n_oversamples, *self.shape)))
+
+    def pca_covariance(self, n_comps=50, zero_center=True, out=None):
+        """Exact PCA by the covariance route (k = n_comps); same result dict."""
+        return self._run(lib().scxpca_matrix_pca_covariance, n_comps, 0, 0, zero_center, 0, QR_HOUSEHOLDER,
+                         PRECISION_FP32, out, max(1, n_comps))
+
+    def _run(self, fn, n_comps, n_oversamples, n_power_iterations, zero_center, seed, qr_method, precision, out, k):
         n, nv = self.shape
         prm = Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), qr_method, seed, 0.0, 0,
                      precision)
-        k = max(1, min(n_comps + n_oversamples, n, nv))
         var = np.zeros(max(n_comps, 1))
         ratio = np.zeros(max(n_comps, 1))
         mean = np.zeros(max(nv, 1))
@@ -258,7 +268,7 @@ class Matrix:
             emb, comps = out["embeddings"], out["components"]
             on_dev = 1
         res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), on_dev, 0, 0.0, 0, 0.0)
-        check(lib().scxpca_matrix_pca(self.handle, ctypes.byref(prm), ctypes.byref(res)))
+        check(fn(self.handle, ctypes.byref(prm), ctypes.byref(res)))
         return {
             "embeddings": emb,
             "components": comps,

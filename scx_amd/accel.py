@@ -26,7 +26,8 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["pca", "normalize_total", "log1p", "LazyCsr", "resolve_device", "gpu_available", "gpu_info"]
+__all__ = ["pca", "normalize_total", "log1p", "LazyCsr", "resolve_device", "resolve_method", "gpu_available",
+           "gpu_info"]
 
 BACKEND = "scx-gpu-hip"
 
@@ -200,6 +201,19 @@ def _resolve_mask_var(adata, mask_var, n_vars):
     return cols, name
 
 
+COVARIANCE_PCA_THRESHOLD = 5_000  # scx-accel/src/pca/cpu.rs:1592
+
+
+def resolve_method(method: str, n_vars: int) -> str:
+    """The solver a ``method`` request runs (resolve_cpu_pca_method,
+    scx-accel/src/pca/cpu.rs:1643-1660)."""
+    if method not in ("auto", "covariance", "randomized"):
+        raise ValueError(f"Invalid method={method!r}; expected 'auto', 'covariance', or 'randomized'")
+    if method == "auto":
+        return "covariance" if n_vars <= COVARIANCE_PCA_THRESHOLD else "randomized"
+    return method
+
+
 def _route_stamp(device_id, spmm_policy, allow_tf32, resident):
     """AccelExecutionInfo fields (scx-accel/src/route.rs:325-372)."""
     return {
@@ -228,7 +242,15 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
         n_power_iterations: int = 2, device: str = "auto", method: str = "auto", qr_method: str = "householder",
         prefer_format: str = "csr", allow_tf32: bool = False, spmm_policy: str = "default", memory_budget=None,
         mask_var=None, *, precision: str = "fp32") -> None:
-    """Randomized PCA of ``adata.X`` on the GPU; results written into ``adata``.
+    """PCA of ``adata.X`` on the GPU; results written into ``adata``.
+
+    ``method`` is resolved as the reference's CPU router does
+    (resolve_cpu_pca_method, scx-accel/src/pca/cpu.rs:1643-1660): "covariance"
+    and "randomized" as named, "auto" the covariance route at <= 5,000
+    (effective) genes, else randomized. The covariance route is exact (f64
+    eigendecomposition of the sample covariance, cpu.rs:1663-1854) and ignores
+    ``n_oversamples``, ``n_power_iterations``, ``random_state``, ``qr_method``
+    and ``precision``.
 
     ``precision`` (keyword only, not in the reference): ``"fp32"`` (default)
     or ``"bf16"`` — the BF16-sketch form streams X~ as bf16 in the sketch
@@ -272,10 +294,13 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
                     target_sum=lazy.target_sum if lazy.target_sum is not None else None, log1p=lazy.log1p,
                     device=device_id)
     try:
-        res = m.pca(n_comps=n_comps, n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
-                    zero_center=zero_center, seed=int(random_state) & ((1 << 64) - 1),
-                    qr_method=_lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER,
-                    precision=_lib.PRECISION_BF16_SKETCH if precision == "bf16" else _lib.PRECISION_FP32)
+        if resolve_method(method, n_vars) == "covariance":
+            res = m.pca_covariance(n_comps=n_comps, zero_center=zero_center)
+        else:
+            res = m.pca(n_comps=n_comps, n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
+                        zero_center=zero_center, seed=int(random_state) & ((1 << 64) - 1),
+                        qr_method=_lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER,
+                        precision=_lib.PRECISION_BF16_SKETCH if precision == "bf16" else _lib.PRECISION_FP32)
     except ValueError as e:
         # AccelError::InvalidInput surfaces as RuntimeError (pca.rs:845)
         raise RuntimeError(str(e)) from None

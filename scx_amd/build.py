@@ -46,7 +46,8 @@ def build(verbose: bool = False) -> str:
         ["g++", "-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-c",
          os.path.join(CSRC, "omega.cpp"), "-o", omega_o],
         [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, "scxpca.hip"), "-o", scx_o],
-        [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp", scx_o, omega_o],
+        [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", LIB + ".tmp", scx_o, omega_o,
+         "-L/opt/rocm/lib", "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"],
     ]
     for cmd in cmds:
         if verbose:

@@ -1699,6 +1699,147 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Covariance route (cpu.rs:1663-1854): G = X~^T X~ from the tile-CSC.
+// Block (pair p = (I, J), I <= J, of 64-gene blocks; tile range tr) has 8
+// waves. Per 256-row tile the slices X~[tile, I] and X~[tile, J] are
+// densified in LDS (row-major 256 x 64 f32, zero-filled, then each entry
+// scattered: the slice of 64 consecutive genes is one contiguous range of the
+// gene-major tile-CSC, and an entry finds its gene by a binary search over the
+// 65 run starts staged in LDS). Wave w computes the 32 x 32 quadrant
+// (w & 3) of G_IJ over rows [128 (w >> 2), 128 (w >> 2) + 128) of the tile
+// with v_mfma_f32_32x32x2f32 (lane l reads A[k0 + l/32][m0 + l%32] and
+// B[k0 + l/32][n0 + l%32]: a row-major panel with 64-float rows is
+// conflict-free), and adds the tile's f32 result into f64 accumulators. The
+// two row halves are added at the end (wave w then w + 4) and the block
+// writes one f64 partial per pair and tile range. f32 products and f32 sums
+// over one tile's rows, f64 across tiles (the reference accumulates in f64
+// throughout; the difference is ~1e-7 relative per entry).
+// ---------------------------------------------------------------------------
+constexpr int kCovGB = 64;  // genes per covariance block
+
+__global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ indptr,
+                                                  const uint32_t *__restrict__ tptr,
+                                                  const uint8_t *__restrict__ trow,
+                                                  const float *__restrict__ tval, const int2 *__restrict__ pairs,
+                                                  int npairs, int tiles_per_tr, int n_tiles, int64_t n_obs,
+                                                  int64_t n_vars, double *__restrict__ part) {
+    __shared__ float pan[2][kTileRows * kCovGB];  // 2 x 64 KB
+    __shared__ uint32_t rs[2][kCovGB + 1];         // run starts of the two slices (tile-relative)
+    const int p = blockIdx.x, tr = blockIdx.y;
+    const int2 IJ = pairs[p];
+    const bool diag = IJ.x == IJ.y;
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int64_t I0 = static_cast<int64_t>(IJ.x) * kCovGB, J0 = static_cast<int64_t>(IJ.y) * kCovGB;
+    const int t0 = tr * tiles_per_tr, t1 = (t0 + tiles_per_tr) < n_tiles ? (t0 + tiles_per_tr) : n_tiles;
+    const int q = wave & 3, half = wave >> 2;
+    const int m0 = 32 * (q >> 1), n0 = 32 * (q & 1);
+    const float *pa = pan[0];
+    const float *pb = diag ? pan[0] : pan[1];
+    v16f acc;
+    double acc64[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) acc64[i] = 0.0;
+    for (int t = t0; t < t1; ++t) {
+        const uint32_t *tp = tptr + static_cast<int64_t>(t) * (n_vars + 1);
+        const int64_t base = indptr[static_cast<int64_t>(t) * kTileRows];
+        // zero the panels, stage the run starts (genes past n_vars: empty runs)
+        float4 *pz = reinterpret_cast<float4 *>(&pan[0][0]);
+        const int nz = (diag ? 1 : 2) * kTileRows * kCovGB / 4;
+        for (int i = tid; i < nz; i += 512) pz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (tid < 2 * (kCovGB + 1)) {
+            const int s = tid / (kCovGB + 1), c = tid % (kCovGB + 1);
+            const int64_t g = (s == 0 ? I0 : J0) + c;
+            rs[s][c] = tp[g < n_vars ? g : n_vars];
+        }
+        __syncthreads();
+        // scatter the slices' entries (gene of entry e: largest c with rs[c] <= e)
+        for (int s = 0; s < (diag ? 1 : 2); ++s) {
+            const uint32_t a = rs[s][0], b = rs[s][kCovGB];
+            float *pw = pan[s];
+            for (uint32_t e = a + static_cast<uint32_t>(tid); e < b; e += 512u) {
+                int lo = 0;
+#pragma unroll
+                for (int step = 32; step > 0; step >>= 1) lo = (rs[s][lo + step] <= e) ? lo + step : lo;
+                pw[static_cast<int>(trow[base + e]) * kCovGB + lo] = tval[base + e];
+            }
+        }
+        __syncthreads();
+        // this wave's quadrant over its 128 rows of the tile
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[i] = 0.f;
+        const int rbase = 128 * half;
+#pragma unroll 8
+        for (int k0 = 0; k0 < 128; k0 += 2) {
+            const int row = rbase + k0 + (lane >> 5);
+            const float av = pa[row * kCovGB + m0 + (lane & 31)];
+            const float bv = pb[row * kCovGB + n0 + (lane & 31)];
+            acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc64[i] += static_cast<double>(acc[i]);
+        __syncthreads();  // the panels are rewritten for the next tile
+    }
+    // add the two row halves: waves 4..7 hand their f64 sums to waves 0..3
+    double *red = reinterpret_cast<double *>(&pan[0][0]);  // 4 x 64 x 16 f64 = 32 KB
+    if (half == 1) {
+#pragma unroll
+        for (int i = 0; i < 16; ++i) red[(q * 64 + lane) * 16 + i] = acc64[i];
+    }
+    __syncthreads();
+    if (half == 0) {
+        double *dst = part + (static_cast<int64_t>(tr) * npairs + p) * (kCovGB * kCovGB);
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            // D[m][n] of v_mfma_f32_32x32x2f32: n = lane % 32, m = 8 (i / 4) + 4 (lane / 32) + i % 4
+            const int m = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), n = n0 + (lane & 31);
+            dst[m * kCovGB + n] = acc64[i] + red[(q * 64 + lane) * 16 + i];
+        }
+    }
+}
+
+// cov (n_vars x n_vars f64, both triangles) from the Gram partials: the sum
+// over tile ranges in range order, then (G - n mu_i mu_j) / (n - 1)
+// (finalize_covariance, cpu.rs:303-322; mu = NULL: G / (n - 1)).
+__global__ void __launch_bounds__(256) k_cov_finalize(const double *__restrict__ part, int npairs, int ntr,
+                                                      const int *__restrict__ pair_of_blocks, int nblk,
+                                                      const double *__restrict__ mu, int64_t n_obs,
+                                                      int64_t n_vars, double *__restrict__ cov) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n_vars * n_vars) return;
+    const int64_t i = e / n_vars, j = e % n_vars;
+    const int bi = static_cast<int>(i / kCovGB), bj = static_cast<int>(j / kCovGB);
+    const bool sw = bi > bj;  // stored as (min, max) block pair, transposed inside
+    const int p = pair_of_blocks[(sw ? bj : bi) * nblk + (sw ? bi : bj)];
+    const int li = static_cast<int>((sw ? j : i) % kCovGB), lj = static_cast<int>((sw ? i : j) % kCovGB);
+    double g = 0.0;
+    for (int r = 0; r < ntr; ++r) g += part[(static_cast<int64_t>(r) * npairs + p) * (kCovGB * kCovGB) + li * kCovGB + lj];
+    const double n = static_cast<double>(n_obs);
+    const double denom = n - 1.0 > 1.0 ? n - 1.0 : 1.0;
+    cov[e] = mu ? (g - n * mu[i] * mu[j]) / denom : g / denom;
+}
+
+// Top nc eigenpairs of the ascending EVD (eigenvectors in the columns of Z,
+// column-major n x n): variance[pc] = max(w[n-1-pc], 0), components row pc =
+// Z[:, n-1-pc] (f32), and the f32 panel M (n x KP, zero past nc) for the
+// embeddings (cpu.rs:1776-1800).
+template <int KP>
+__global__ void __launch_bounds__(256) k_cov_topk(const double *__restrict__ Z, const double *__restrict__ w,
+                                                  int64_t n, int nc, float *__restrict__ comps,
+                                                  float *__restrict__ M, double *__restrict__ var) {
+    const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < nc) {
+        const double ev = w[n - 1 - e];
+        var[e] = ev > 0.0 ? ev : 0.0;
+    }
+    if (e >= n * KP) return;
+    const int64_t v = e / KP;
+    const int pc = static_cast<int>(e % KP);
+    const float z = pc < nc ? static_cast<float>(Z[(n - 1 - pc) * n + v]) : 0.f;
+    M[e] = z;
+    if (pc < nc && comps) comps[static_cast<int64_t>(pc) * n + v] = z;
+}
+
 // Gram with in-block reduction: each wave accumulates its upper tiles over its
 // rows (FP32 MFMA), the four waves of the block add into an LDS tile in a fixed
 // order, and the block writes one f64 partial. Two row pairs per iteration

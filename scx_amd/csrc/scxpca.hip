@@ -23,6 +23,8 @@
 // span the same subspace; the orthonormal panel keeps the FP32 Gram of the
 // next Y well conditioned.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <chrono>
@@ -238,8 +240,13 @@ struct scxpca_matrix {
         for (auto &mk : marks) { (void)hipEventDestroy(mk.a); (void)hipEventDestroy(mk.b); }
         marks.clear();
     }
+    // covariance route: Gram partials, pair tables, covariance / eigenvectors,
+    // eigenvalues and their workspace, the solver handle (created on first use)
+    DevBuf covpart, covtab, cov, evw, eve, evinfo, covvar;
+    rocblas_handle rb = nullptr;
     ~scxpca_matrix() {
         clear_marks();
+        if (rb) (void)rocblas_destroy_handle(rb);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -775,6 +782,53 @@ struct Engine {
         check();
     }
 
+    // Total variance with the cancellation guard (guarded_total_variance_*,
+    // cpu.rs:1029-1092) from the column sums and sums of squares; hmu gets
+    // the column means.
+    double total_variance(bool centered, const std::vector<double> &hsum, const std::vector<double> &hsq,
+                          std::vector<double> &hmu, int &recomputed) {
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        hmu.assign(nv, 0.0);
+        for (int64_t v = 0; v < nv; ++v) hmu[v] = hsum[v] / static_cast<double>(n);
+        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
+        double tv = 0.0;
+        recomputed = 0;
+        if (!centered) {
+            for (int64_t v = 0; v < nv; ++v) tv += hsq[v];
+            tv /= denom;
+        } else {
+            double tot = 0.0, sumsq = 0.0, msq = 0.0;
+            for (int64_t v = 0; v < nv; ++v) tot += hsq[v] - static_cast<double>(n) * hmu[v] * hmu[v];
+            tv = tot / denom;
+            for (int64_t v = 0; v < nv; ++v) sumsq += hsq[v];
+            for (int64_t v = 0; v < nv; ++v) msq += hmu[v] * hmu[v];
+            msq *= static_cast<double>(n);
+            if (sumsq > 0.0 && (sumsq - msq) <= 1e-7 * sumsq) {
+                // cancelled: stable centered recompute on the device
+                recomputed = 1;
+                const int nb = 1024;
+                DevBuf part;
+                part.ensure(sizeof(double) * nb);
+                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), mu, n,
+                                                 part.as<double>());
+                check();
+                std::vector<double> hp(nb);
+                std::vector<uint32_t> hcnt(nv);
+                HIPC(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+                HIPC(hipMemcpyAsync(hcnt.data(), m->cnt.p, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, s));
+                HIPC(hipStreamSynchronize(s));
+                double t = 0.0;
+                for (int i = 0; i < nb; ++i) t += hp[i];
+                for (int64_t v = 0; v < nv; ++v) {
+                    const double zeros = static_cast<double>(std::max<int64_t>(n - static_cast<int64_t>(hcnt[v]), 0));
+                    t += zeros * hmu[v] * hmu[v];
+                }
+                tv = t / denom;
+            }
+        }
+        return tv;
+    }
+
     void run(const scxpca_params *p, scxpca_result *out) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         const bool centered = p->zero_center != 0;
@@ -864,44 +918,10 @@ struct Engine {
         }
         if (hstatus & 2) throw Error(SCXPCA_ERR_LINALG, "SVD failed: Jacobi sweeps did not converge");
         // total variance with the cancellation guard (cpu.rs:1029-1092)
-        std::vector<double> hmu(nv);
-        for (int64_t v = 0; v < nv; ++v) hmu[v] = hsum[v] / static_cast<double>(n);
-        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
-        double tv = 0.0;
+        std::vector<double> hmu;
         int recomputed = 0;
-        if (!centered) {
-            for (int64_t v = 0; v < nv; ++v) tv += hsq[v];
-            tv /= denom;
-        } else {
-            double tot = 0.0, sumsq = 0.0, msq = 0.0;
-            for (int64_t v = 0; v < nv; ++v) tot += hsq[v] - static_cast<double>(n) * hmu[v] * hmu[v];
-            tv = tot / denom;
-            for (int64_t v = 0; v < nv; ++v) sumsq += hsq[v];
-            for (int64_t v = 0; v < nv; ++v) msq += hmu[v] * hmu[v];
-            msq *= static_cast<double>(n);
-            if (sumsq > 0.0 && (sumsq - msq) <= 1e-7 * sumsq) {
-                // cancelled: stable centered recompute on the device
-                recomputed = 1;
-                const int nb = 1024;
-                DevBuf part;
-                part.ensure(sizeof(double) * nb);
-                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), mu, n,
-                                                 part.as<double>());
-                check();
-                std::vector<double> hp(nb);
-                std::vector<uint32_t> hcnt(nv);
-                HIPC(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
-                HIPC(hipMemcpyAsync(hcnt.data(), m->cnt.p, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, s));
-                HIPC(hipStreamSynchronize(s));
-                double t = 0.0;
-                for (int i = 0; i < nb; ++i) t += hp[i];
-                for (int64_t v = 0; v < nv; ++v) {
-                    const double zeros = static_cast<double>(std::max<int64_t>(n - static_cast<int64_t>(hcnt[v]), 0));
-                    t += zeros * hmu[v] * hmu[v];
-                }
-                tv = t / denom;
-            }
-        }
+        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed);
+        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
         for (int j = 0; j < nc; ++j) {
             const double var = hsig[j] * hsig[j] / denom;
             if (out->variance) out->variance[j] = var;
@@ -911,6 +931,124 @@ struct Engine {
             for (int j = 0; j < k; ++j) out->sigma[j] = hsig[j];
         if (out->mean && centered) std::memcpy(out->mean, hmu.data(), sizeof(double) * nv);
         out->k = k;
+        out->total_variance = tv;
+        out->tv_recomputed = recomputed;
+    }
+
+    // Covariance route (covariance_pca, cpu.rs:1663-1854): G = X~^T X~ by
+    // k_cov_gram, cov = (G - n mu mu^T) / (n - 1), the symmetric EVD
+    // (rocSOLVER dsyevd, f64), the top n_comps eigenpairs, and the embeddings
+    // (X~ - mu) V by the forward SpMM. k = n_comps (no oversampling).
+    void run_covariance(const scxpca_params *p, scxpca_result *out) {
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        const bool centered = p->zero_center != 0;
+        const int nc = static_cast<int>(p->n_comps);
+        HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
+        ensure_stats(m);
+        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, 1.0 / static_cast<double>(n), mu);
+        check();
+        // block pairs (I <= J) of 64-gene blocks and the pair index of each
+        const int nblk = static_cast<int>((nv + kCovGB - 1) / kCovGB);
+        const int npairs = nblk * (nblk + 1) / 2;
+        std::vector<int> tab(2 * static_cast<size_t>(npairs) + static_cast<size_t>(nblk) * nblk, -1);
+        {
+            int q = 0;
+            for (int I = 0; I < nblk; ++I)
+                for (int J = I; J < nblk; ++J, ++q) {
+                    tab[2 * q] = I;
+                    tab[2 * q + 1] = J;
+                    tab[2 * static_cast<size_t>(npairs) + static_cast<size_t>(I) * nblk + J] = q;
+                }
+        }
+        m->covtab.ensure(sizeof(int) * tab.size());
+        HIPC(hipMemcpyAsync(m->covtab.p, tab.data(), sizeof(int) * tab.size(), hipMemcpyHostToDevice, s));
+        const int2 *pairs = m->covtab.as<int2>();
+        const int *pob = m->covtab.as<int>() + 2 * npairs;
+        // tile ranges: enough blocks to fill the GPU several times over
+        int ntr = std::max(1, std::min(m->n_tiles, (2048 + npairs - 1) / npairs));
+        const int tpr = (m->n_tiles + ntr - 1) / ntr;
+        ntr = (m->n_tiles + tpr - 1) / tpr;
+        m->covpart.ensure(sizeof(double) * static_cast<size_t>(ntr) * npairs * kCovGB * kCovGB);
+        m->cov.ensure(sizeof(double) * static_cast<size_t>(nv) * nv);
+        {
+            StageScope prof(m, s, SCXPCA_STAGE_GRAM);
+            k_cov_gram<<<dim3(npairs, ntr), 512, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+                                                         m->tval.as<float>(), pairs, npairs, tpr, m->n_tiles, n, nv,
+                                                         m->covpart.as<double>());
+            check();
+            k_cov_finalize<<<blocks_for(nv * nv), 256, 0, s>>>(m->covpart.as<double>(), npairs, ntr, pob, nblk,
+                                                                centered ? mu : nullptr, n, nv,
+                                                                m->cov.as<double>());
+            check();
+        }
+        // eigenpairs, ascending; eigenvectors overwrite cov (column-major)
+        m->evw.ensure(sizeof(double) * nv);
+        m->eve.ensure(sizeof(double) * nv);
+        m->evinfo.ensure(sizeof(rocblas_int));
+        if (!m->rb) {
+            if (rocblas_create_handle(&m->rb) != rocblas_status_success)
+                throw Error(SCXPCA_ERR_DEVICE, "rocblas_create_handle failed");
+        }
+        if (rocblas_set_stream(m->rb, s) != rocblas_status_success) throw Error(SCXPCA_ERR_DEVICE, "rocblas_set_stream failed");
+        {
+            StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
+            const rocblas_status st = rocsolver_dsyevd(m->rb, rocblas_evect_original, rocblas_fill_lower,
+                                                       static_cast<rocblas_int>(nv), m->cov.as<double>(),
+                                                       static_cast<rocblas_int>(nv), m->evw.as<double>(),
+                                                       m->eve.as<double>(), m->evinfo.as<rocblas_int>());
+            if (st != rocblas_status_success)
+                throw Error(SCXPCA_ERR_DEVICE, "rocsolver_dsyevd failed (status " + std::to_string(static_cast<int>(st)) + ")");
+        }
+        // top nc: variance, components, the f32 panel V (nv x KP)
+        float *comps_dev = nullptr;
+        DevBuf comps_tmp;
+        if (out->on_device) {
+            comps_dev = out->components;
+        } else {
+            comps_tmp.ensure(sizeof(float) * std::max<int64_t>(nv * nc, 1));
+            comps_dev = comps_tmp.as<float>();
+        }
+        m->covvar.ensure(sizeof(double) * std::max(nc, 1));
+        k_cov_topk<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->cov.as<double>(), m->evw.as<double>(), nv, nc, comps_dev,
+                                                           m->panel.as<float>(), m->covvar.as<double>());
+        check();
+        // embeddings (X~ - mu) V
+        float *Y = m->y.as<float>();
+        if (centered) mean_correction(m->panel.as<float>());
+        spmm_fwd(m->panel.as<float>(), centered, false, Y);
+        float *emb_dev = nullptr;
+        if (out->on_device) {
+            emb_dev = out->embeddings;
+        } else {
+            emb_dev = m->q.as<float>();  // n x KP scratch holds the packed n x nc embeddings
+        }
+        k_unpack_panel32<KP><<<blocks_for(n * nc), 256, 0, s>>>(Y, n, nc, emb_dev);
+        check();
+        // results back
+        std::vector<double> hvar(nc), hsum(nv), hsq(nv);
+        rocblas_int hinfo = 0;
+        HIPC(hipMemcpyAsync(hvar.data(), m->covvar.p, sizeof(double) * nc, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsum.data(), m->sum.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsq.data(), m->sq.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(&hinfo, m->evinfo.p, sizeof(rocblas_int), hipMemcpyDeviceToHost, s));
+        if (!out->on_device) {
+            HIPC(hipMemcpyAsync(out->embeddings, emb_dev, sizeof(float) * n * nc, hipMemcpyDeviceToHost, s));
+            HIPC(hipMemcpyAsync(out->components, comps_dev, sizeof(float) * nv * nc, hipMemcpyDeviceToHost, s));
+        }
+        HIPC(hipStreamSynchronize(s));
+        if (hinfo != 0) throw Error(SCXPCA_ERR_LINALG, "Eigendecomposition failed: the symmetric eigensolver did not converge");
+        std::vector<double> hmu;
+        int recomputed = 0;
+        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed);
+        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
+        for (int j = 0; j < nc; ++j) {
+            if (out->variance) out->variance[j] = hvar[j];
+            if (out->variance_ratio) out->variance_ratio[j] = tv > 0.0 ? hvar[j] / tv : 0.0;
+        }
+        if (out->sigma)  // singular values of the centred X~: sqrt((n - 1) lambda)
+            for (int j = 0; j < nc; ++j) out->sigma[j] = std::sqrt(hvar[j] * denom);
+        if (out->mean && centered) std::memcpy(out->mean, hmu.data(), sizeof(double) * nv);
+        out->k = nc;
         out->total_variance = tv;
         out->tv_recomputed = recomputed;
     }
@@ -1029,6 +1167,38 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     } else {
         Engine<128> eng(m, static_cast<int>(k));
         eng.run(p, out);
+    }
+    HIPC(hipEventRecord(e1, m->stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out->seconds_device = ms * 1e-3;
+}
+
+// The covariance genes limit: the n_vars x n_vars f64 covariance and the
+// Gram partials stay modest (1.7 GiB at the limit).
+constexpr int64_t kMaxCovVars = 16384;
+
+void run_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
+    validate(m, p);
+    if (p->n_comps > 128)
+        throw Error(SCXPCA_ERR_INVALID, "n_comps = " + std::to_string(p->n_comps) + " exceeds the GPU panel limit of 128");
+    if (m->n_vars > kMaxCovVars)
+        throw Error(SCXPCA_ERR_INVALID, "the covariance route supports up to " + std::to_string(kMaxCovVars) +
+                                            " genes (n_vars = " + std::to_string(m->n_vars) + "); use method='randomized'");
+    DeviceGuard dg(m->device);
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, m->stream));
+    if (p->n_comps <= 64) {
+        Engine<64> eng(m, static_cast<int>(p->n_comps));
+        eng.run_covariance(p, out);
+    } else {
+        Engine<128> eng(m, static_cast<int>(p->n_comps));
+        eng.run_covariance(p, out);
     }
     HIPC(hipEventRecord(e1, m->stream));
     HIPC(hipEventSynchronize(e1));
@@ -1328,6 +1498,13 @@ scxpca_status scxpca_matrix_profile(scxpca_matrix *m, int32_t enable, double *ms
         }
         m->clear_marks();
         m->profiling = enable != 0;
+    });
+}
+
+scxpca_status scxpca_matrix_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
+    return guarded([&] {
+        if (!m || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        run_pca_covariance(m, p, out);
     });
 }
 

@@ -328,3 +328,42 @@ def test_pca_threads_do_not_change_bits_of_spmm_stages():
     b = oracle.randomized_pca(_view(c), n_comps=5, seed=3, target_sum=1e4, log1p=True, nthreads=4)
     # SpMMs are bit-identical; the QR is column-parallel, also order-fixed
     assert np.array_equal(a["sigma_all"], b["sigma_all"])
+
+
+def test_covariance_oracle_matches_dense_svd():
+    """The covariance route's oracle against the SVD of the explicitly centred
+    X~: eigenvalues = sigma^2 / (n - 1), embeddings = U Sigma (up to sign),
+    components = right singular vectors (up to sign); and uncentred: the
+    eigenvalues of X~^T X~ / (n - 1)."""
+    import scipy.sparse as sp
+
+    from scx_amd import synth
+    x = synth.generate(synth.CFG1, n_obs=600, workers=1)
+    m = x.to_scipy()[:, :90].tocsr()
+    m.sort_indices()
+    v = oracle.CsrView(m.indptr.astype(np.int64), m.indices.astype(np.uint32), m.data.astype(np.float32), m.shape)
+    xt = v.with_data(oracle.transform(v, 1e4, True))
+    d = sp.csr_matrix((xt.data.astype(np.float64), xt.indices, xt.indptr), shape=m.shape).toarray()
+    n = m.shape[0]
+    r = oracle.covariance_pca(v, n_comps=8, zero_center=True, target_sum=1e4, log1p=True)
+    u, s, vt = np.linalg.svd(d - d.mean(axis=0), full_matrices=False)
+    np.testing.assert_allclose(r["variance"], s[:8] ** 2 / (n - 1), rtol=1e-12)
+    for j in range(8):
+        sgn = np.sign(np.dot(r["components"][j], vt[j]))
+        np.testing.assert_allclose(r["components"][j], sgn * vt[j], atol=1e-9)
+        np.testing.assert_allclose(r["embeddings"][:, j], sgn * u[:, j] * s[j], atol=1e-9 * s[0])
+    np.testing.assert_allclose(r["mean"], d.mean(axis=0), rtol=1e-13, atol=1e-15)
+    assert r["variance_ratio"].sum() <= 1.0 + 1e-12
+    ru = oracle.covariance_pca(v, n_comps=5, zero_center=False, target_sum=1e4, log1p=True)
+    w = np.linalg.eigvalsh(d.T @ d / (n - 1))[::-1]
+    np.testing.assert_allclose(ru["variance"], w[:5], rtol=1e-12)
+
+
+def test_method_routing():
+    from scx_amd import accel
+    assert accel.resolve_method("auto", 5000) == "covariance"
+    assert accel.resolve_method("auto", 5001) == "randomized"
+    assert accel.resolve_method("covariance", 30000) == "covariance"
+    assert accel.resolve_method("randomized", 10) == "randomized"
+    with pytest.raises(ValueError, match="Invalid method='pca'"):
+        accel.resolve_method("pca", 10)
