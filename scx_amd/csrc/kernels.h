@@ -1701,24 +1701,24 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
 
 // ---------------------------------------------------------------------------
 // Covariance route (cpu.rs:1663-1854): G = X~^T X~ from the tile-CSC.
-// Block (pair p = (I, J), I <= J, of 64-gene blocks; tile range tr) has 8
+// Block (pair p = (I, J), I <= J, of 64-gene blocks; tile range tr) has 16
 // waves. Per 256-row tile the slices X~[tile, I] and X~[tile, J] are
 // densified in LDS (row-major 256 x 64 f32, zero-filled, then each entry
 // scattered: the slice of 64 consecutive genes is one contiguous range of the
 // gene-major tile-CSC, and an entry finds its gene by a binary search over the
 // 65 run starts staged in LDS). Wave w computes the 32 x 32 quadrant
-// (w & 3) of G_IJ over rows [128 (w >> 2), 128 (w >> 2) + 128) of the tile
+// (w & 3) of G_IJ over rows [64 (w >> 2), 64 (w >> 2) + 64) of the tile
 // with v_mfma_f32_32x32x2f32 (lane l reads A[k0 + l/32][m0 + l%32] and
 // B[k0 + l/32][n0 + l%32]: a row-major panel with 64-float rows is
 // conflict-free), and adds the tile's f32 result into f64 accumulators. The
-// two row halves are added at the end (wave w then w + 4) and the block
+// four row quarters are added at the end (in quarter order) and the block
 // writes one f64 partial per pair and tile range. f32 products and f32 sums
 // over one tile's rows, f64 across tiles (the reference accumulates in f64
 // throughout; the difference is ~1e-7 relative per entry).
 // ---------------------------------------------------------------------------
 constexpr int kCovGB = 64;  // genes per covariance block
 
-__global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ indptr,
+__global__ void __launch_bounds__(1024) k_cov_gram(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ tptr,
                                                   const uint8_t *__restrict__ trow,
                                                   const float *__restrict__ tval, const int2 *__restrict__ pairs,
@@ -1732,7 +1732,7 @@ __global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ in
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int64_t I0 = static_cast<int64_t>(IJ.x) * kCovGB, J0 = static_cast<int64_t>(IJ.y) * kCovGB;
     const int t0 = tr * tiles_per_tr, t1 = (t0 + tiles_per_tr) < n_tiles ? (t0 + tiles_per_tr) : n_tiles;
-    const int q = wave & 3, half = wave >> 2;
+    const int q = wave & 3, quarter = wave >> 2;  // output quadrant, row quarter of the tile
     const int m0 = 32 * (q >> 1), n0 = 32 * (q & 1);
     const float *pa = pan[0];
     const float *pb = diag ? pan[0] : pan[1];
@@ -1746,7 +1746,7 @@ __global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ in
         // zero the panels, stage the run starts (genes past n_vars: empty runs)
         float4 *pz = reinterpret_cast<float4 *>(&pan[0][0]);
         const int nz = (diag ? 1 : 2) * kTileRows * kCovGB / 4;
-        for (int i = tid; i < nz; i += 512) pz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int i = tid; i < nz; i += 1024) pz[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         if (tid < 2 * (kCovGB + 1)) {
             const int s = tid / (kCovGB + 1), c = tid % (kCovGB + 1);
             const int64_t g = (s == 0 ? I0 : J0) + c;
@@ -1757,7 +1757,7 @@ __global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ in
         for (int s = 0; s < (diag ? 1 : 2); ++s) {
             const uint32_t a = rs[s][0], b = rs[s][kCovGB];
             float *pw = pan[s];
-            for (uint32_t e = a + static_cast<uint32_t>(tid); e < b; e += 512u) {
+            for (uint32_t e = a + static_cast<uint32_t>(tid); e < b; e += 1024u) {
                 int lo = 0;
 #pragma unroll
                 for (int step = 32; step > 0; step >>= 1) lo = (rs[s][lo + step] <= e) ? lo + step : lo;
@@ -1768,9 +1768,9 @@ __global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ in
         // this wave's quadrant over its 128 rows of the tile
 #pragma unroll
         for (int i = 0; i < 16; ++i) acc[i] = 0.f;
-        const int rbase = 128 * half;
+        const int rbase = 64 * quarter;
 #pragma unroll 8
-        for (int k0 = 0; k0 < 128; k0 += 2) {
+        for (int k0 = 0; k0 < 64; k0 += 2) {
             const int row = rbase + k0 + (lane >> 5);
             const float av = pa[row * kCovGB + m0 + (lane & 31)];
             const float bv = pb[row * kCovGB + n0 + (lane & 31)];
@@ -1780,20 +1780,22 @@ __global__ void __launch_bounds__(512) k_cov_gram(const int64_t *__restrict__ in
         for (int i = 0; i < 16; ++i) acc64[i] += static_cast<double>(acc[i]);
         __syncthreads();  // the panels are rewritten for the next tile
     }
-    // add the two row halves: waves 4..7 hand their f64 sums to waves 0..3
-    double *red = reinterpret_cast<double *>(&pan[0][0]);  // 4 x 64 x 16 f64 = 32 KB
-    if (half == 1) {
+    // add the row quarters in order: waves 4..15 hand their f64 sums to 0..3
+    double *red = reinterpret_cast<double *>(&pan[0][0]);  // 3 x 4 x 64 x 16 f64 = 96 KB
+    if (quarter > 0) {
 #pragma unroll
-        for (int i = 0; i < 16; ++i) red[(q * 64 + lane) * 16 + i] = acc64[i];
+        for (int i = 0; i < 16; ++i) red[(((quarter - 1) * 4 + q) * 64 + lane) * 16 + i] = acc64[i];
     }
     __syncthreads();
-    if (half == 0) {
+    if (quarter == 0) {
         double *dst = part + (static_cast<int64_t>(tr) * npairs + p) * (kCovGB * kCovGB);
 #pragma unroll
         for (int i = 0; i < 16; ++i) {
             // D[m][n] of v_mfma_f32_32x32x2f32: n = lane % 32, m = 8 (i / 4) + 4 (lane / 32) + i % 4
             const int m = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3), n = n0 + (lane & 31);
-            dst[m * kCovGB + n] = acc64[i] + red[(q * 64 + lane) * 16 + i];
+            dst[m * kCovGB + n] = ((acc64[i] + red[((0 * 4 + q) * 64 + lane) * 16 + i]) +
+                                   red[((1 * 4 + q) * 64 + lane) * 16 + i]) +
+                                  red[((2 * 4 + q) * 64 + lane) * 16 + i];
         }
     }
 }
@@ -1819,23 +1821,24 @@ __global__ void __launch_bounds__(256) k_cov_finalize(const double *__restrict__
     cov[e] = mu ? (g - n * mu[i] * mu[j]) / denom : g / denom;
 }
 
-// Top nc eigenpairs of the ascending EVD (eigenvectors in the columns of Z,
-// column-major n x n): variance[pc] = max(w[n-1-pc], 0), components row pc =
-// Z[:, n-1-pc] (f32), and the f32 panel M (n x KP, zero past nc) for the
-// embeddings (cpu.rs:1776-1800).
+// Top nc eigenpairs of an ascending EVD of nsel selected pairs (eigenvectors
+// in the columns of Z, column-major n x nsel, the largest last):
+// variance[pc] = max(w[nsel-1-pc], 0), components row pc = Z[:, nsel-1-pc]
+// (f32), and the f32 panel M (n x KP, zero past nc) for the embeddings
+// (cpu.rs:1776-1800).
 template <int KP>
 __global__ void __launch_bounds__(256) k_cov_topk(const double *__restrict__ Z, const double *__restrict__ w,
-                                                  int64_t n, int nc, float *__restrict__ comps,
+                                                  int64_t n, int64_t nsel, int nc, float *__restrict__ comps,
                                                   float *__restrict__ M, double *__restrict__ var) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (e < nc) {
-        const double ev = w[n - 1 - e];
+        const double ev = w[nsel - 1 - e];
         var[e] = ev > 0.0 ? ev : 0.0;
     }
     if (e >= n * KP) return;
     const int64_t v = e / KP;
     const int pc = static_cast<int>(e % KP);
-    const float z = pc < nc ? static_cast<float>(Z[(n - 1 - pc) * n + v]) : 0.f;
+    const float z = pc < nc ? static_cast<float>(Z[(nsel - 1 - pc) * n + v]) : 0.f;
     M[e] = z;
     if (pc < nc && comps) comps[static_cast<int64_t>(pc) * n + v] = z;
 }

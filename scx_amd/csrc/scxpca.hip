@@ -972,7 +972,7 @@ struct Engine {
         m->cov.ensure(sizeof(double) * static_cast<size_t>(nv) * nv);
         {
             StageScope prof(m, s, SCXPCA_STAGE_GRAM);
-            k_cov_gram<<<dim3(npairs, ntr), 512, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
+            k_cov_gram<<<dim3(npairs, ntr), 1024, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(),
                                                          m->tval.as<float>(), pairs, npairs, tpr, m->n_tiles, n, nv,
                                                          m->covpart.as<double>());
             check();
@@ -981,7 +981,7 @@ struct Engine {
                                                                 m->cov.as<double>());
             check();
         }
-        // eigenpairs, ascending; eigenvectors overwrite cov (column-major)
+        // all eigenpairs, ascending; eigenvectors overwrite cov (column-major)
         m->evw.ensure(sizeof(double) * nv);
         m->eve.ensure(sizeof(double) * nv);
         m->evinfo.ensure(sizeof(rocblas_int));
@@ -992,9 +992,9 @@ struct Engine {
         if (rocblas_set_stream(m->rb, s) != rocblas_status_success) throw Error(SCXPCA_ERR_DEVICE, "rocblas_set_stream failed");
         {
             StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
-            const rocblas_status st = rocsolver_dsyevd(m->rb, rocblas_evect_original, rocblas_fill_lower,
-                                                       static_cast<rocblas_int>(nv), m->cov.as<double>(),
-                                                       static_cast<rocblas_int>(nv), m->evw.as<double>(),
+            const rocblas_int rn = static_cast<rocblas_int>(nv);
+            const rocblas_status st = rocsolver_dsyevd(m->rb, rocblas_evect_original, rocblas_fill_lower, rn,
+                                                       m->cov.as<double>(), rn, m->evw.as<double>(),
                                                        m->eve.as<double>(), m->evinfo.as<rocblas_int>());
             if (st != rocblas_status_success)
                 throw Error(SCXPCA_ERR_DEVICE, "rocsolver_dsyevd failed (status " + std::to_string(static_cast<int>(st)) + ")");
@@ -1009,8 +1009,8 @@ struct Engine {
             comps_dev = comps_tmp.as<float>();
         }
         m->covvar.ensure(sizeof(double) * std::max(nc, 1));
-        k_cov_topk<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->cov.as<double>(), m->evw.as<double>(), nv, nc, comps_dev,
-                                                           m->panel.as<float>(), m->covvar.as<double>());
+        k_cov_topk<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->cov.as<double>(), m->evw.as<double>(), nv, nv, nc,
+                                                           comps_dev, m->panel.as<float>(), m->covvar.as<double>());
         check();
         // embeddings (X~ - mu) V
         float *Y = m->y.as<float>();
