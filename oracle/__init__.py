@@ -256,11 +256,12 @@ def randomized_pca(x, n_comps=50, n_oversamples=10, n_power_iterations=2, zero_c
     }
 
 
-def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False):
+def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False, nthreads=0):
     """``covariance_pca`` (scx-accel/src/pca/cpu.rs:1663-1854) on the raw CSR
     ``x`` with the lazy transform applied once: the f64 cross-product X~^T X~
-    and column sums (:1713-1725; products of the f32 X~ values are exact in
-    f64, the sum order is BLAS's), ``finalize_covariance`` (:303-322), the
+    and column sums by ``orc_covariance_gram`` (the C restatement of
+    accumulate_covariance_block, :198-291, rows in ascending order per entry),
+    ``finalize_covariance`` (:303-322), the
     symmetric eigendecomposition (LAPACK syevd through numpy, as faer's
     self-adjoint EVD at :1748-1751), the top ``n_comps`` eigenpairs largest
     first (:1776-1785, eigenvalues clamped at 0), the guarded total variance
@@ -278,9 +279,12 @@ def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False
     if n_comps > min(n, nv):
         raise ValueError(_PCA_ERRORS[3].format(nc=n_comps, mn=min(n, nv)))
     xt = v.with_data(transform(v, target_sum, log1p))
-    a = sp.csr_matrix((xt.data.astype(np.float64), xt.indices, xt.indptr), shape=(n, nv)).toarray()
-    g = a.T @ a
-    col_sums = a.sum(axis=0)
+    g = np.empty((nv, nv))
+    col_sums = np.empty(nv)
+    if lib().orc_covariance_gram(ctypes.byref(xt.c), _ptr(g), _ptr(col_sums), int(nthreads)) != 0:
+        raise ValueError("covariance oracle needs strictly increasing row indices")
+    g = np.tril(g.T)  # column-major lower triangle -> row-major lower
+    g = g + np.tril(g, -1).T
     means = col_sums / n if zero_center else None
     col_sum_sq = np.diag(g).copy()
     denom = max(n - 1.0, 1.0)
@@ -291,7 +295,7 @@ def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False
     var = np.maximum(w[idx], 0.0)
     vmat = z[:, idx]
     ratio = var / tv if tv > 0 else np.zeros(n_comps)
-    emb = spmm_forward(xt, np.ascontiguousarray(vmat), means)
+    emb = spmm_forward(xt, np.ascontiguousarray(vmat), means, nthreads=nthreads)
     return {
         "embeddings": emb,
         "components": np.ascontiguousarray(vmat.T),

@@ -676,3 +676,40 @@ int orc_randomized_pca(const orc_csr *x0, const orc_pca_params *p, double *embed
     free(b); free(qb); free(uh); free(sg); free(vt);
     return rc == 0 ? 0 : 4;
 }
+
+
+/* accumulate_covariance_block / accumulate_covariance_into (cpu.rs:198-291)
+ * for a strictly increasing CSR: the lower triangle of X^T X (entry (hi, lo)
+ * at cov[lo * n_vars + hi], column-major) and the column sums, f64. The
+ * output is split into column blocks, one per OpenMP task; every entry adds
+ * its rows in ascending order, so the result does not depend on the split or
+ * the thread count (as the reference's). Returns 1 (nothing written) when a
+ * row is not strictly increasing (the reference then takes its serial
+ * duplicate-coalescing path, which this build's GPU path rejects). */
+int orc_covariance_gram(const orc_csr *x, double *cov, double *col_sums, int nthreads) {
+    const int64_t nv = x->n_vars;
+    for (int64_t r = 0; r < x->n_obs; ++r)
+        for (int64_t p = x->indptr[r] + 1; p < x->indptr[r + 1]; ++p)
+            if (x->indices[p] <= x->indices[p - 1]) return 1;
+    memset(cov, 0, sizeof(double) * (size_t)nv * (size_t)nv);
+    memset(col_sums, 0, sizeof(double) * (size_t)nv);
+    set_threads(nthreads);
+    const int64_t nb = nv < 64 ? nv : 64;  /* column blocks */
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int64_t b = 0; b < nb; ++b) {
+        const uint32_t a0 = (uint32_t)(b * nv / nb), a1 = (uint32_t)((b + 1) * nv / nb);
+        for (int64_t r = 0; r < x->n_obs; ++r) {
+            const int64_t s0 = x->indptr[r], s1 = x->indptr[r + 1];
+            for (int64_t i = s0; i < s1; ++i) {
+                const uint32_t c1 = x->indices[i];
+                if (c1 < a0) continue;
+                if (c1 >= a1) break;
+                const double v1 = (double)x->data[i];
+                col_sums[c1] += v1;
+                double *col = cov + (size_t)c1 * (size_t)nv;
+                for (int64_t j = i; j < s1; ++j) col[x->indices[j]] += v1 * (double)x->data[j];
+            }
+        }
+    }
+    return 0;
+}

@@ -107,6 +107,10 @@ typedef struct {
     int32_t nthreads;     /* <= 0: OpenMP default */
 } orc_pca_params;
 
+/* ---- covariance route (cpu.rs:198-291): lower triangle of X^T X (column-
+ * major) and column sums, f64; 1 if a row is not strictly increasing. */
+int orc_covariance_gram(const orc_csr *x, double *cov, double *col_sums, int nthreads);
+
 /* Full randomized_pca_inmemory (cpu.rs:887-952) on the raw CSR with the lazy
  * transform applied once. Outputs (caller-allocated):
  *   embeddings n_obs x n_comps (row-major f64), components n_comps x n_vars,
