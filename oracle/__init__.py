@@ -17,6 +17,14 @@ Parity status:
     distribution tests); no reference fixture pins the sample bits.
   * X~ transform, column stats, SpMMs, QR, SVD, assembly, total variance —
     pinned against closed forms, brute-force dense f64 and numpy/LAPACK.
+  * PFlog (``pflog_delta``, ``pflog_baseline``, ``estimate_alpha``,
+    ``pflog_total_variance``, ``pflog_pca``) — pinned by the reference's own
+    known answers (the hand-computed alpha cases, the dense v4 transform and
+    its zero row sums, full-rank reconstructions of the centred / uncentred
+    transform) and the three simulated-count fixtures with a known generating
+    alpha; ``pflog_pca`` also against the dense SVD of the materialised
+    ``Z = delta + b 1^T``, and the closed-form total variance against the
+    dense ``Z``'s.
   * ``covariance_pca`` (numpy over the same X~, SpMM and total variance) —
     pinned against the dense SVD of the centred X~ (eigenvalues =
     sigma^2 / (n - 1), eigenvectors = right singular vectors up to sign,
@@ -74,6 +82,8 @@ def lib():
         _lib.orc_total_variance.restype = ctypes.c_double
         _lib.orc_randomized_pca.restype = ctypes.c_int
         _lib.orc_thin_svd.restype = ctypes.c_int
+        _lib.orc_pflog_total_variance.restype = ctypes.c_double
+        _lib.orc_pflog_pca.restype = ctypes.c_int
     return _lib
 
 
@@ -303,4 +313,121 @@ def covariance_pca(x, n_comps=50, zero_center=True, target_sum=None, log1p=False
         "variance_ratio": ratio,
         "mean": means,
         "total_variance": tv,
+    }
+
+
+# ---------------------------------------------------------------- PFlog (v4)
+ALPHA_MU_MIN = 1e-3           # AlphaOptions::default (scx-accel/src/pflog.rs:255-262)
+ALPHA_FALLBACK = 0.25
+
+
+def pflog_delta(x, four_alpha: float, form: int = 0) -> np.ndarray:
+    """delta values on x's pattern: form 0 the lazy Scale{4a} -> Log1p chain,
+    form 1 the in-memory builder (``orc_pflog_delta``)."""
+    v = _as_view(x)
+    out = np.empty_like(v.data)
+    lib().orc_pflog_delta(ctypes.byref(v.c), ctypes.c_double(four_alpha), ctypes.c_int(int(form)), _ptr(out))
+    return out
+
+
+def pflog_baseline(delta) -> np.ndarray:
+    v = _as_view(delta)
+    out = np.empty(v.shape[0])
+    lib().orc_pflog_baseline(ctypes.byref(v.c), _ptr(out))
+    return out
+
+
+def pflog_total_variance(col_sum_sq, colmean_delta, baseline, n_obs: int, n_vars: int) -> float:
+    css = np.ascontiguousarray(col_sum_sq, dtype=np.float64)
+    b = np.ascontiguousarray(baseline, dtype=np.float64)
+    cd = None if colmean_delta is None else np.ascontiguousarray(colmean_delta, dtype=np.float64)
+    return float(lib().orc_pflog_total_variance(_ptr(css), None if cd is None else _ptr(cd), _ptr(b),
+                                                _i64(n_obs), _i64(n_vars)))
+
+
+def column_moments(x):
+    """streaming_mean_var (scx-accel/src/hvg/cpu.rs:107-130) on raw values:
+    f64 column sums and sums of squares in nnz order, then
+    finalize_column_moments (scx-sparse/src/moments.rs:193-233): mean = s / n,
+    var = (ss - n * mean * mean) / max(n - 1, 1), negative clamped to 0."""
+    v = _as_view(x)
+    n, nv = v.shape
+    if n == 0:
+        return np.zeros(nv), np.zeros(nv)
+    if not np.all(np.isfinite(v.data)):
+        raise ValueError("non-finite value in the input")
+    s, ss = col_sums_and_sum_sq(v)
+    nf = float(n)
+    means = s / nf
+    var = (ss - nf * means * means) / max(nf - 1.0, 1.0)
+    var = np.where(var < 0.0, 0.0, var)
+    return means, var
+
+
+def estimate_alpha(x, mu_min: float = ALPHA_MU_MIN, fallback_alpha: float = ALPHA_FALLBACK) -> dict:
+    """estimate_alpha (scx-accel/src/pflog.rs:340-398): per-gene method-of-
+    moments alpha_g = (var - mean) / mean^2 over every gene with mean > mu_min
+    (finite ones), pooled by the median (the middle pair averaged for an even
+    count, :403-415); the fallback alpha when the pool is empty or its median
+    is not positive."""
+    v = _as_view(x)
+    if v.shape[1] == 0:
+        raise ValueError("estimate_alpha requires n_vars > 0")
+    means, var = column_moments(v)
+    keep = means > mu_min
+    m = means[keep]
+    cand = (var[keep] - m) / (m * m)
+    cand = np.sort(cand[np.isfinite(cand)])
+    n_used = int(cand.size)
+    fb = {"alpha": float(fallback_alpha), "pseudocount": 1.0 / (4.0 * fallback_alpha), "n_genes_used": n_used,
+          "fell_back": True}
+    if n_used == 0:
+        return fb
+    h = n_used // 2
+    alpha = float(cand[h]) if n_used % 2 else float(0.5 * (cand[h - 1] + cand[h]))
+    if not np.isfinite(alpha) or alpha <= 0.0:
+        return fb
+    return {"alpha": alpha, "pseudocount": 1.0 / (4.0 * alpha), "n_genes_used": n_used, "fell_back": False}
+
+
+def pflog_pca(x, n_comps=50, n_oversamples=10, n_power_iterations=2, zero_center=True, seed=0, alpha=None,
+              form=0, nthreads=0):
+    """pflog_pca (scx-accel/src/pca/cpu.rs:815-866) on the raw CSR ``x``:
+    alpha estimated (``alpha=None``) or pinned, delta = log1p(4 alpha x) in
+    the given form, the baseline from delta (pflog.rs:186-222), then the
+    C restatement. Returns a dict of f64 arrays plus the alpha fit, the
+    baseline and the total variance."""
+    v = _as_view(x)
+    n, nv = v.shape
+    if alpha is None:
+        fit = estimate_alpha(v)
+        fit["alpha_source"] = "estimated"
+    else:
+        fit = {"alpha": float(alpha), "pseudocount": 1.0 / (4.0 * alpha), "alpha_source": "pinned"}
+    delta = v.with_data(pflog_delta(v, 4.0 * fit["alpha"], form))
+    base = pflog_baseline(delta)
+    k = max(1, min(n_comps + n_oversamples, nv, n))
+    emb = np.zeros((n, max(n_comps, 1)))
+    comps = np.zeros((max(n_comps, 1), nv))
+    var = np.zeros(max(n_comps, 1))
+    ratio = np.zeros(max(n_comps, 1))
+    mean = np.zeros(max(nv, 1))
+    sig = np.zeros(k)
+    tv = ctypes.c_double(0.0)
+    prm = _Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), seed, 0.0, 0, int(nthreads))
+    rc = lib().orc_pflog_pca(ctypes.byref(delta.c), _ptr(base), ctypes.byref(prm), _ptr(emb), _ptr(comps),
+                             _ptr(var), _ptr(ratio), _ptr(mean), _ptr(sig), ctypes.byref(tv))
+    if rc != 0:
+        raise ValueError(_PCA_ERRORS[rc].format(nc=n_comps, mn=min(n, nv)))
+    return {
+        "embeddings": emb,
+        "components": comps,
+        "variance": var,
+        "variance_ratio": ratio,
+        "mean": mean if zero_center else None,
+        "sigma_all": sig,
+        "baseline": base,
+        "delta": delta.data,
+        "total_variance": tv.value,
+        **fit,
     }

@@ -598,33 +598,43 @@ double orc_total_variance(const orc_csr *x, const double *col_sum_sq, const doub
 /* randomized_pca_inmemory (cpu.rs:887-952) + build_pca_result (:1530-1585)  */
 /* ------------------------------------------------------------------------ */
 
-int orc_randomized_pca(const orc_csr *x0, const orc_pca_params *p, double *embeddings,
-                       double *components, double *variance, double *variance_ratio,
-                       double *mean_out, double *sigma_all) {
-    const int64_t n = x0->n_obs, nv = x0->n_vars, nc = p->n_comps;
-    if (nc == 0) return 1;
-    if (n == 0 || nv == 0) return 2;
-    if (nc > (n < nv ? n : nv)) return 3;
-    set_threads(p->nthreads);
-    int64_t k = nc + p->n_oversamples;
-    if (k > nv) k = nv;
-    if (k > n) k = n;
-    const int64_t nnz = x0->indptr[n];
+/* Forward and transpose SpMM of the PFlog operator Z = delta + b 1^T
+ * (streaming_spmm_forward_offset / _transpose_offset, cpu.rs:617-687): the
+ * plain (centered) product, then the row-baseline rank-1 term. */
+static void spmm_forward_offset(const orc_csr *x, const double *m, int64_t k, const double *means,
+                                const double *baseline, double *y, int nthreads) {
+    orc_spmm_forward(x, m, k, means, y, nthreads);
+    if (!baseline) return;
+    double *cs = (double *)calloc((size_t)k, sizeof(double));
+    for (int64_t v = 0; v < x->n_vars; ++v)
+        for (int64_t j = 0; j < k; ++j) cs[j] += m[v * k + j];
+    for (int64_t i = 0; i < x->n_obs; ++i)
+        for (int64_t j = 0; j < k; ++j) y[i * k + j] += baseline[i] * cs[j];
+    free(cs);
+}
 
-    float *xt = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
-    orc_transform(x0, p->target_sum, p->log1p, xt);
-    orc_csr x = *x0;
-    x.data = xt;
-
-    double *sums = (double *)malloc(sizeof(double) * (size_t)nv);
-    double *sumsq = (double *)malloc(sizeof(double) * (size_t)nv);
-    orc_col_sums_and_sum_sq(&x, sums, sumsq);
-    double *means = NULL;
-    if (p->zero_center) {
-        means = (double *)malloc(sizeof(double) * (size_t)nv);
-        for (int64_t j = 0; j < nv; ++j) means[j] = sums[j] / (double)n;
+static void spmm_transpose_offset(const orc_csr *x, const double *q, int64_t k, const double *means,
+                                  const double *baseline, double *z, int nthreads) {
+    orc_spmm_transpose(x, q, k, means, z, nthreads);
+    if (!baseline) return;
+    double *bq = (double *)calloc((size_t)k, sizeof(double));
+    for (int64_t i = 0; i < x->n_obs; ++i) {
+        if (baseline[i] == 0.0) continue;
+        for (int64_t j = 0; j < k; ++j) bq[j] += baseline[i] * q[i * k + j];
     }
+    for (int64_t v = 0; v < x->n_vars; ++v)
+        for (int64_t j = 0; j < k; ++j) z[v * k + j] += bq[j];
+    free(bq);
+}
 
+/* The power loop (cpu.rs:569-588, :835-857) and build_pca_result
+ * (:1530-1585), shared by randomized_pca_inmemory and pflog_pca: x holds the
+ * streamed values (X~, or delta for PFlog), means the centering vector or
+ * NULL, baseline the PFlog row offset or NULL, tv the total variance. */
+static int pca_core(const orc_csr *x, const double *means, const double *baseline, double tv,
+                    const orc_pca_params *p, int64_t k, double *embeddings, double *components,
+                    double *variance, double *variance_ratio, double *mean_out, double *sigma_all) {
+    const int64_t n = x->n_obs, nv = x->n_vars, nc = p->n_comps;
     double *omega = (double *)malloc(sizeof(double) * (size_t)(nv * k));
     orc_random_gaussian((uint64_t)nv, (uint64_t)k, p->seed, omega);
     double *y = (double *)malloc(sizeof(double) * (size_t)(n * k));
@@ -632,20 +642,19 @@ int orc_randomized_pca(const orc_csr *x0, const orc_pca_params *p, double *embed
     double *b = (double *)malloc(sizeof(double) * (size_t)(nv * k));
     double *qb = (double *)malloc(sizeof(double) * (size_t)(nv * k));
 
-    orc_spmm_forward(&x, omega, k, means, y, p->nthreads);
+    spmm_forward_offset(x, omega, k, means, baseline, y, p->nthreads);
     orc_qr_thin_q(y, n, k, q, p->nthreads);
     for (int64_t it = 0; it < p->n_power_iterations; ++it) {
-        orc_spmm_transpose(&x, q, k, means, b, p->nthreads);
+        spmm_transpose_offset(x, q, k, means, baseline, b, p->nthreads);
         if (p->n_power_iterations > 2) {
             orc_qr_thin_q(b, nv, k, qb, p->nthreads);
-            orc_spmm_forward(&x, qb, k, means, y, p->nthreads);
+            spmm_forward_offset(x, qb, k, means, baseline, y, p->nthreads);
         } else {
-            orc_spmm_forward(&x, b, k, means, y, p->nthreads);
+            spmm_forward_offset(x, b, k, means, baseline, y, p->nthreads);
         }
         orc_qr_thin_q(y, n, k, q, p->nthreads);
     }
-    orc_spmm_transpose(&x, q, k, means, b, p->nthreads);
-    double tv = orc_total_variance(&x, sumsq, means, NULL);
+    spmm_transpose_offset(x, q, k, means, baseline, b, p->nthreads);
 
     double *uh = (double *)malloc(sizeof(double) * (size_t)(nv * k));
     double *sg = (double *)malloc(sizeof(double) * (size_t)k);
@@ -672,9 +681,127 @@ int orc_randomized_pca(const orc_csr *x0, const orc_pca_params *p, double *embed
         if (sigma_all) memcpy(sigma_all, sg, sizeof(double) * (size_t)k);
         if (means && mean_out) memcpy(mean_out, means, sizeof(double) * (size_t)nv);
     }
-    free(xt); free(sums); free(sumsq); free(means); free(omega); free(y); free(q);
-    free(b); free(qb); free(uh); free(sg); free(vt);
+    free(omega); free(y); free(q); free(b); free(qb); free(uh); free(sg); free(vt);
     return rc == 0 ? 0 : 4;
+}
+
+static int validate_inputs(int64_t n, int64_t nv, int64_t nc) {
+    if (nc == 0) return 1;
+    if (n == 0 || nv == 0) return 2;
+    if (nc > (n < nv ? n : nv)) return 3;
+    return 0;
+}
+
+static int64_t pca_k(const orc_pca_params *p, int64_t n, int64_t nv) {
+    int64_t k = p->n_comps + p->n_oversamples;
+    if (k > nv) k = nv;
+    if (k > n) k = n;
+    return k;
+}
+
+int orc_randomized_pca(const orc_csr *x0, const orc_pca_params *p, double *embeddings,
+                       double *components, double *variance, double *variance_ratio,
+                       double *mean_out, double *sigma_all) {
+    const int64_t n = x0->n_obs, nv = x0->n_vars;
+    const int v = validate_inputs(n, nv, p->n_comps);
+    if (v) return v;
+    set_threads(p->nthreads);
+    const int64_t k = pca_k(p, n, nv);
+    const int64_t nnz = x0->indptr[n];
+
+    float *xt = (float *)malloc(sizeof(float) * (size_t)(nnz > 0 ? nnz : 1));
+    orc_transform(x0, p->target_sum, p->log1p, xt);
+    orc_csr x = *x0;
+    x.data = xt;
+
+    double *sums = (double *)malloc(sizeof(double) * (size_t)nv);
+    double *sumsq = (double *)malloc(sizeof(double) * (size_t)nv);
+    orc_col_sums_and_sum_sq(&x, sums, sumsq);
+    double *means = NULL;
+    if (p->zero_center) {
+        means = (double *)malloc(sizeof(double) * (size_t)nv);
+        for (int64_t j = 0; j < nv; ++j) means[j] = sums[j] / (double)n;
+    }
+    const double tv = orc_total_variance(&x, sumsq, means, NULL);
+    const int rc = pca_core(&x, means, NULL, tv, p, k, embeddings, components, variance, variance_ratio,
+                            mean_out, sigma_all);
+    free(xt); free(sums); free(sumsq); free(means);
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* PFlog v4 (scx-accel/src/pflog.rs; pflog_pca, cpu.rs:611-866)               */
+/* ------------------------------------------------------------------------ */
+
+void orc_pflog_delta(const orc_csr *x, double four_alpha, int form, float *out) {
+    const int64_t nnz = x->indptr[x->n_obs];
+    for (int64_t p = 0; p < nnz; ++p) {
+        const float v = x->data[p];
+        out[p] = form ? (float)log1p(four_alpha * (double)v) : log1pf((float)((double)v * four_alpha));
+    }
+}
+
+void orc_pflog_baseline(const orc_csr *delta, double *out) {
+    const double inv_d = 1.0 / (double)delta->n_vars;
+    for (int64_t r = 0; r < delta->n_obs; ++r) {
+        double s = 0.0;
+        for (int64_t p = delta->indptr[r]; p < delta->indptr[r + 1]; ++p) s += (double)delta->data[p];
+        out[r] = -s * inv_d;
+    }
+}
+
+double orc_pflog_total_variance(const double *col_sum_sq, const double *colmean_delta, const double *baseline,
+                                int64_t n, int64_t nv) {
+    double s_dd = 0.0, b2 = 0.0;
+    for (int64_t j = 0; j < nv; ++j) s_dd += col_sum_sq[j];
+    for (int64_t i = 0; i < n; ++i) b2 += baseline[i] * baseline[i];
+    const double denom = ((double)n - 1.0) > 1.0 ? ((double)n - 1.0) : 1.0;
+    const double d = (double)nv;
+    double tss;
+    if (!colmean_delta) {
+        tss = s_dd - d * b2;
+    } else {
+        double bsum = 0.0;
+        for (int64_t i = 0; i < n; ++i) bsum += baseline[i];
+        const double bmean = bsum / ((double)n > 1.0 ? (double)n : 1.0);
+        double m2 = 0.0;
+        for (int64_t j = 0; j < nv; ++j) {
+            const double mu = colmean_delta[j] + bmean;
+            m2 += mu * mu;
+        }
+        tss = s_dd - d * b2 - (double)n * m2;
+    }
+    const double tv = tss / denom;
+    return tv > 0.0 ? tv : 0.0;
+}
+
+int orc_pflog_pca(const orc_csr *delta, const double *baseline, const orc_pca_params *p, double *embeddings,
+                  double *components, double *variance, double *variance_ratio, double *mean_out,
+                  double *sigma_all, double *total_variance) {
+    const int64_t n = delta->n_obs, nv = delta->n_vars;
+    const int v = validate_inputs(n, nv, p->n_comps);
+    if (v) return v;
+    set_threads(p->nthreads);
+    const int64_t k = pca_k(p, n, nv);
+    double *sums = (double *)malloc(sizeof(double) * (size_t)nv);
+    double *sumsq = (double *)malloc(sizeof(double) * (size_t)nv);
+    orc_col_sums_and_sum_sq(delta, sums, sumsq);
+    double *cd = NULL, *means = NULL;
+    if (p->zero_center) {
+        cd = (double *)malloc(sizeof(double) * (size_t)nv);
+        means = (double *)malloc(sizeof(double) * (size_t)nv);
+        for (int64_t j = 0; j < nv; ++j) cd[j] = sums[j] / (double)n;
+        double bsum = 0.0;
+        for (int64_t i = 0; i < n; ++i) bsum += baseline[i];
+        const double bmean = bsum / ((double)n > 1.0 ? (double)n : 1.0);
+        for (int64_t j = 0; j < nv; ++j) means[j] = cd[j] + bmean;
+    }
+    const double tv = orc_pflog_total_variance(sumsq, cd, baseline, n, nv);
+    if (total_variance) *total_variance = tv;
+    const int rc = pca_core(delta, means, baseline, tv, p, k, embeddings, components, variance, variance_ratio,
+                            mean_out, sigma_all);
+    free(sums); free(sumsq); free(cd); free(means);
+    return rc;
 }
 
 

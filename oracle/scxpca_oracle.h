@@ -122,6 +122,24 @@ int orc_randomized_pca(const orc_csr *x, const orc_pca_params *p, double *embedd
                        double *components, double *variance, double *variance_ratio,
                        double *mean, double *sigma_all);
 
+/* ---- PFlog v4 (scx-accel/src/pflog.rs, pca/cpu.rs:611-866).
+ * delta = log1p(4 alpha x) on x's pattern. form 0: the lazy Scale{4a} -> Log1p
+ * chain, ((x as f64 * 4a) as f32).ln_1p() (transforms.rs:109-128); form 1: the
+ * in-memory builder, (4a * x as f64).ln_1p() as f32 (pyscx/src/accel/pflog.rs:375-397). */
+void orc_pflog_delta(const orc_csr *x, double four_alpha, int form, float *out);
+/* b_i = -(sum_j delta_ij) * (1 / n_vars), f64 left fold per row
+ * (pflog_baseline_from_delta, pflog.rs:186-222; csr.rs:543-552). */
+void orc_pflog_baseline(const orc_csr *delta, double *out);
+/* pflog_total_variance (cpu.rs:702-769); colmean_delta NULL: uncentered. */
+double orc_pflog_total_variance(const double *col_sum_sq, const double *colmean_delta, const double *baseline,
+                                int64_t n, int64_t nv);
+/* pflog_pca_with_depth (cpu.rs:815-866) on the delta CSR (values already
+ * transformed) and the baseline; outputs as orc_randomized_pca, plus the
+ * total variance. target_sum / log1p of p are ignored. */
+int orc_pflog_pca(const orc_csr *delta, const double *baseline, const orc_pca_params *p, double *embeddings,
+                  double *components, double *variance, double *variance_ratio, double *mean,
+                  double *sigma_all, double *total_variance);
+
 #ifdef __cplusplus
 }
 #endif
