@@ -125,6 +125,43 @@ scxpca_status scxpca_matrix_pca(scxpca_matrix *m, const scxpca_params *p, scxpca
  * n_vars <= 16384. sigma (if given) receives sqrt((n_obs - 1) lambda). */
 scxpca_status scxpca_matrix_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out);
 
+/* PFlog (v4) normalization (Booeshaghi et al.; scx-accel/src/pflog.rs): the
+ * resident operator Z = delta + b 1^T with delta = log1p(4 alpha x) on x's
+ * pattern and the per-cell baseline b_i = -(1/n_vars) sum_j delta_ij.
+ * delta forms: LAZY = the Scale{4a} -> Log1p chain, ((x as f64 * 4a) as
+ * f32).ln_1p() (pyscx/src/lazy_transform/transforms.rs:109-128, the backed
+ * and lazy X of pyscx/src/accel/pflog.rs:176-276); F64 = the in-memory
+ * builder (4a * x as f64).ln_1p() as f32 (pflog.rs:375-397). */
+typedef enum { SCXPCA_PFLOG_DELTA_LAZY = 0, SCXPCA_PFLOG_DELTA_F64 = 1 } scxpca_pflog_delta;
+typedef struct {
+    double alpha;           /* > 0 (finite): pinned; 0: estimate_alpha on the raw counts */
+    double mu_min;          /* AlphaOptions::mu_min (the reference default 1e-3) */
+    double fallback_alpha;  /* AlphaOptions::fallback_alpha (default 0.25) */
+    int32_t delta_form;     /* scxpca_pflog_delta */
+} scxpca_pflog_params;
+/* AlphaEstimate (pflog.rs:268-288) / the binding's AlphaMeta (pyscx/src/accel/pflog.rs:315-327). */
+typedef struct {
+    double alpha;
+    double pseudocount;     /* 1 / (4 alpha) */
+    int64_t n_genes_used;   /* -1 when pinned */
+    int32_t estimated;      /* 1: estimate_alpha, 0: pinned */
+    int32_t fell_back;      /* 1: the fallback alpha was used */
+} scxpca_pflog_fit;
+/* Resident PFlog matrix: alpha resolved as resolve_alpha
+ * (pyscx/src/accel/pflog.rs:329-350; estimate_alpha, pflog.rs:340-398, on
+ * the GPU column moments of the raw counts), delta and the baseline
+ * (pflog_baseline_from_delta, pflog.rs:186-222) built on the device, counts
+ * checked as the reference checks them (non-finite -> error; negative ->
+ * error for the F64 form). scxpca_matrix_pca on it runs pflog_pca
+ * (scx-accel/src/pca/cpu.rs:815-866): the randomized PCA of Z with the two
+ * rank-1 baseline terms in every SpMM and the closed-form total variance
+ * (:702-769); scxpca_matrix_spmm and the projection apply Z; the covariance
+ * route rejects it. */
+scxpca_status scxpca_matrix_create_pflog(const scxpca_csr *x, const scxpca_pflog_params *pp, int device,
+                                         scxpca_matrix **out);
+/* The alpha fit and (baseline != NULL) the n_obs baseline values, host f64. */
+scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, double *baseline);
+
 /* Stage profiler (the reference's SCX_GPU_PROFILE stage timers,
  * scx-gpu/src/profile.rs:77-208): while enabled, every stage launch on the
  * matrix stream is bracketed by HIP events. scxpca_matrix_profile returns the

@@ -22,6 +22,8 @@ _vp = ctypes.c_void_p
 OK, ERR_INVALID, ERR_LINALG, ERR_OOM, ERR_DEVICE, ERR_NO_DEVICE = range(6)
 QR_HOUSEHOLDER, QR_CHOLESKY = 0, 1
 PRECISION_FP32, PRECISION_BF16_SKETCH = 0, 1
+PFLOG_DELTA_LAZY, PFLOG_DELTA_F64 = 0, 1
+ALPHA_MU_MIN, ALPHA_FALLBACK = 1e-3, 0.25  # AlphaOptions::default (scx-accel/src/pflog.rs:255-262)
 STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm_fwd_bf16", "spmm_t_bf16",
           "prepare", "solve", "small")
 
@@ -29,6 +31,8 @@ STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm
 EXPORTS = (
     "scxpca_randomized_pca",
     "scxpca_matrix_create",
+    "scxpca_matrix_create_pflog",
+    "scxpca_matrix_pflog_fit",
     "scxpca_matrix_destroy",
     "scxpca_matrix_refresh",
     "scxpca_matrix_pca",
@@ -63,6 +67,15 @@ class Result(ctypes.Structure):
     _fields_ = [("embeddings", _vp), ("components", _vp), ("variance", _vp), ("variance_ratio", _vp),
                 ("mean", _vp), ("sigma", _vp), ("on_device", _i32), ("k", _i64), ("total_variance", _dbl),
                 ("tv_recomputed", _i32), ("seconds_device", _dbl)]
+
+
+class PflogParams(ctypes.Structure):
+    _fields_ = [("alpha", _dbl), ("mu_min", _dbl), ("fallback_alpha", _dbl), ("delta_form", _i32)]
+
+
+class PflogFit(ctypes.Structure):
+    _fields_ = [("alpha", _dbl), ("pseudocount", _dbl), ("n_genes_used", _i64), ("estimated", _i32),
+                ("fell_back", _i32)]
 
 
 class ScxPcaError(RuntimeError):
@@ -103,6 +116,9 @@ def lib() -> ctypes.CDLL:
     l.scxpca_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     l.scxpca_randomized_pca.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(Params), ctypes.POINTER(Result), ctypes.c_int]
     l.scxpca_matrix_create.argtypes = [ctypes.POINTER(Csr), _dbl, _i32, ctypes.c_int, ctypes.POINTER(_vp)]
+    l.scxpca_matrix_create_pflog.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(PflogParams), ctypes.c_int,
+                                             ctypes.POINTER(_vp)]
+    l.scxpca_matrix_pflog_fit.argtypes = [_vp, ctypes.POINTER(PflogFit), _vp]
     l.scxpca_matrix_destroy.argtypes = [_vp]
     l.scxpca_matrix_refresh.argtypes = [_vp]
     l.scxpca_matrix_pca.argtypes = [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]
@@ -178,9 +194,11 @@ def gaussian_device(rows: int, cols: int, seed: int, device: int = 0) -> np.ndar
 
 class Matrix:
     """A CSR matrix resident in HBM (owned copy of host arrays, or borrowed
-    device tensors), with its normalize/log1p transform fixed."""
+    device tensors), with its normalize/log1p transform fixed. With ``pflog``
+    (a dict: alpha (None: estimated), delta_form, mu_min, fallback_alpha) it
+    is the PFlog operator delta + b 1^T instead."""
 
-    def __init__(self, indptr, indices, data, shape, target_sum=None, log1p=False, device=0):
+    def __init__(self, indptr, indices, data, shape, target_sum=None, log1p=False, device=0, pflog=None):
         self._keep = (indptr, indices, data)
         on_dev = not isinstance(indptr, np.ndarray)
         if not on_dev:
@@ -192,9 +210,30 @@ class Matrix:
         self.device = device
         csr = Csr(self.shape[0], self.shape[1], ptr(indptr), ptr(indices), ptr(data), int(on_dev))
         h = _vp()
-        check(lib().scxpca_matrix_create(ctypes.byref(csr), _dbl(float(target_sum or 0.0)), int(bool(log1p)),
-                                         device, ctypes.byref(h)))
+        if pflog is None:
+            check(lib().scxpca_matrix_create(ctypes.byref(csr), _dbl(float(target_sum or 0.0)), int(bool(log1p)),
+                                             device, ctypes.byref(h)))
+        else:
+            a = pflog.get("alpha")
+            pp = PflogParams(0.0 if a is None else float(a), float(pflog.get("mu_min", ALPHA_MU_MIN)),
+                             float(pflog.get("fallback_alpha", ALPHA_FALLBACK)),
+                             int(pflog.get("delta_form", PFLOG_DELTA_LAZY)))
+            check(lib().scxpca_matrix_create_pflog(ctypes.byref(csr), ctypes.byref(pp), device, ctypes.byref(h)))
         self.handle = h
+
+    def pflog_fit(self, baseline: bool = True) -> dict:
+        """The alpha fit of a PFlog matrix and (``baseline``) its per-cell baseline."""
+        f = PflogFit()
+        b = np.empty(self.shape[0]) if baseline else None
+        check(lib().scxpca_matrix_pflog_fit(self.handle, ctypes.byref(f), ptr(b)))
+        out = {"alpha": float(f.alpha), "pseudocount": float(f.pseudocount),
+               "alpha_source": "estimated" if f.estimated else "pinned"}
+        if f.estimated:
+            out["n_genes_used"] = int(f.n_genes_used)
+            out["fell_back"] = bool(f.fell_back)
+        if baseline:
+            out["baseline"] = b
+        return out
 
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:

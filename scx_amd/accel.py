@@ -26,8 +26,8 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["pca", "normalize_total", "log1p", "LazyCsr", "resolve_device", "resolve_method", "gpu_available",
-           "gpu_info"]
+__all__ = ["pca", "pflog", "normalize_total", "log1p", "LazyCsr", "resolve_device", "resolve_method",
+           "gpu_available", "gpu_info"]
 
 BACKEND = "scx-gpu-hip"
 
@@ -329,4 +329,113 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
     }
     container = dict(adata.uns.get("scx_accel", {}) or {})
     container["pca"] = _route_stamp(device_id, spmm_policy, allow_tf32, True)
+    adata.uns["scx_accel"] = container
+
+
+# ---------------------------------------------------------------------------
+# pflog
+# ---------------------------------------------------------------------------
+
+DENSE_MAX_ELEMS = 200_000_000  # pyscx/src/accel/pflog.rs:60
+
+
+def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: int = 50, n_oversamples: int = 10,
+          n_power_iterations: int = 2, zero_center: bool = True, random_state: int = 0,
+          obsm_key: str = "X_pflog_pca", baseline_key: str = "pflog_baseline", layer_out=None, out=None,
+          store_repr: str = "delta_baseline", shard_size=None, dense_max_elems: int = DENSE_MAX_ELEMS,
+          device: str = "auto") -> None:
+    """PFlog (v4) normalization of raw counts on the GPU: the signature,
+    defaults, validation and outputs of ``pyscx.accel.pflog``
+    (pyscx/src/accel/pflog.rs:71-311).
+
+    ``alpha=None`` estimates the matrix-wide overdispersion from the counts
+    (estimate_alpha, scx-accel/src/pflog.rs:340-398); the per-cell baseline
+    goes to ``adata.obs[baseline_key]``, the fit to ``adata.uns["pflog"]``,
+    and ``store`` in ("pca", "all") runs the baseline-aware randomized PCA
+    (pflog_pca, scx-accel/src/pca/cpu.rs:815-866) into
+    ``adata.obsm[obsm_key]`` with the singular values
+    sqrt(variance (n - 1)) in ``adata.uns[f"{obsm_key}_singular_values"]``.
+    An in-memory ``X`` builds delta as ``(4a * x as f64).ln_1p() as f32``
+    (pflog.rs:375-397); a ``LazyCsr`` without transforms as the lazy
+    ``Scale{4a} -> Log1p`` chain. The dense materialisation (``store`` in
+    ("dense", "all")) and its SCX-file output are not built here: they raise
+    NotImplementedError after the checks the reference makes first."""
+    if alpha is not None and (not np.isfinite(alpha) or alpha <= 0):
+        raise ValueError(f"pflog: alpha must be positive and finite, got {alpha}")
+    kind, device_id = resolve_device(device)
+    stores = {"pca": (True, False), "baseline": (False, False), "dense": (False, True), "all": (True, True)}
+    if store not in stores:
+        raise ValueError(f'pflog: unknown store {store!r}; expected "pca", "baseline", "dense", or "all"')
+    want_pca, want_dense = stores[store]
+    if store_repr not in ("delta_baseline", "dense"):
+        raise ValueError(f'pflog: unknown store_repr {store_repr!r}; expected "delta_baseline" or "dense"')
+    if out is not None and not want_dense:
+        raise ValueError('pflog: out= streams the materialized transform to disk; pass store="dense" or '
+                         'store="all" (got store that does not materialize)')
+    if layer is not None:
+        try:
+            x = adata.layers[layer]
+        except KeyError:
+            raise ValueError(f"layer '{layer}' not found in adata.layers") from None
+    else:
+        x = adata.X
+    if want_dense:
+        raise NotImplementedError("scx_amd builds the PFlog PCA and baseline (store='pca' or 'baseline'); the "
+                                  "dense materialization of Z is not part of the GPU path")
+    if kind != "gpu":
+        raise RuntimeError("scx_amd implements the GPU route of pflog only; device='cpu' (or 'auto' without a "
+                           "visible ROCm GPU) is served by the reference CPU implementation")
+
+    import scipy.sparse as sp
+
+    if isinstance(x, LazyCsr):
+        if x.target_sum is not None or x.log1p:
+            raise ValueError("pflog requires raw counts, but X is a lazy-transformed dataset that already carries "
+                             "transforms (e.g. normalize_total / log1p). Run pflog on the raw-count X instead.")
+        raw, form = x.raw, _lib.PFLOG_DELTA_LAZY
+    else:
+        raw, form = x, _lib.PFLOG_DELTA_F64
+    if not sp.issparse(raw):
+        raw = sp.csr_matrix(np.asarray(raw))
+    raw = sp.csr_matrix(raw)
+    if not raw.has_sorted_indices:
+        raw = raw.sorted_indices()
+
+    def runtime(e):
+        # the binding maps estimate_alpha / baseline / PCA failures to RuntimeError and
+        # delta_from_raw_csr's count checks to ValueError (pflog.rs:329-350, 375-397, 460-470)
+        return ValueError(str(e)) if str(e).startswith("pflog:") else RuntimeError(str(e))
+
+    try:
+        m = _lib.Matrix(raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
+                        raw.data.astype(np.float32, copy=False), raw.shape, device=device_id,
+                        pflog={"alpha": alpha, "delta_form": form})
+    except ValueError as e:
+        raise runtime(e) from None
+    try:
+        fit = m.pflog_fit(baseline=True)
+        res = None
+        if want_pca:
+            try:
+                res = m.pca(n_comps=n_components, n_oversamples=n_oversamples,
+                            n_power_iterations=n_power_iterations, zero_center=zero_center,
+                            seed=int(random_state) & ((1 << 64) - 1), qr_method=_lib.QR_HOUSEHOLDER)
+            except (ValueError, _lib.ScxPcaError) as e:
+                raise RuntimeError(str(e)) from None
+    finally:
+        m.close()
+
+    uns = {"version": "v4", "alpha": fit["alpha"], "pseudocount": fit["pseudocount"],
+           "alpha_source": fit["alpha_source"]}
+    if fit["alpha_source"] == "estimated":
+        uns["n_genes_used"] = fit["n_genes_used"]
+        uns["fell_back"] = fit["fell_back"]
+    adata.uns["pflog"] = uns
+    adata.obs[baseline_key] = fit["baseline"]
+    if res is not None:
+        adata.obsm[obsm_key] = res["embeddings"]
+        scale = max(raw.shape[0] - 1.0, 1.0)
+        adata.uns[f"{obsm_key}_singular_values"] = np.sqrt(np.maximum(res["variance"] * scale, 0.0))
+    container = dict(adata.uns.get("scx_accel", {}) or {})
+    container["pflog"] = _route_stamp(device_id, "default", False, True)
     adata.uns["scx_accel"] = container

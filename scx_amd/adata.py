@@ -1,7 +1,8 @@
 """A minimal AnnData stand-in (anndata is not a dependency of this package).
 
-``pca`` duck-types its argument: it needs ``.X``, ``.var`` (for ``mask_var`` /
-``highly_variable``), ``.obsm``, ``.varm`` and ``.uns``. A real
+``pca`` and ``pflog`` duck-type their argument: they need ``.X``, ``.var``
+(for ``mask_var`` / ``highly_variable``), ``.obs``, ``.obsm``, ``.varm``,
+``.layers`` and ``.uns``. A real
 ``anndata.AnnData`` works unchanged; this class provides the same slots.
 """
 from __future__ import annotations
@@ -17,6 +18,7 @@ class AnnDataLite:
         self.var = var if var is not None else pd.DataFrame(index=[f"g{i}" for i in range(n_vars)])
         self.obsm = {}
         self.varm = {}
+        self.layers = {}
         self.uns = {}
 
     @property

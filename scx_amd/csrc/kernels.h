@@ -21,6 +21,12 @@ namespace scxpca {
 constexpr int kWave = 64;
 constexpr int kXfNorm = 1;
 constexpr int kXfLog1p = 2;
+// PFlog (v4): every row's factor is the matrix-wide 4 alpha (no row totals);
+// kXfLogF64 takes the log in f64 and rounds once (the in-memory delta builder);
+// kXfCheck rejects non-finite values (and, with kXfLogF64, negative counts).
+constexpr int kXfScale = 4;
+constexpr int kXfLogF64 = 8;
+constexpr int kXfCheck = 16;
 
 typedef float v16f __attribute__((ext_vector_type(16)));
 
@@ -65,8 +71,12 @@ __host__ __device__ __forceinline__ float log1p_fast(float x) {
 
 // transforms.rs:21-39: ((x as f64 * factor) as f32).ln_1p(); the factor is 1
 // for rows whose total is not positive, which reproduces the :40-72 branch.
+// PFlog: the Scale{4a} -> Log1p chain (transforms.rs:109-128) is the same
+// expression with f = 4a; the in-memory builder (pyscx/src/accel/pflog.rs:
+// 375-397) is (4a * x as f64).ln_1p() as f32.
 __device__ __forceinline__ float xtilde(float v, double f, int flags) {
-    const float s = (flags & kXfNorm) ? static_cast<float>(static_cast<double>(v) * f) : v;
+    if (flags & kXfLogF64) return static_cast<float>(log1p(f * static_cast<double>(v)));
+    const float s = (flags & (kXfNorm | kXfScale)) ? static_cast<float>(static_cast<double>(v) * f) : v;
     return (flags & kXfLog1p) ? log1p_fast(s) : s;
 }
 
@@ -167,16 +177,24 @@ constexpr uint32_t kChunkMod = 512;
 // chunks after the last entry; spl[r][nsplit] = row length).
 // The same pass validates the matrix into *flag (bits: 1 a row's indices
 // decrease, 2 an index >= n_vars, 4 indptr decreases, 8 a row repeats an
-// index); the tile-CSC builds skip all work when the flag is set, and the
+// index; with kXfCheck 16 a non-finite value or transformed value, 32 a
+// negative count under kXfLogF64, the first such entry's position in
+// ~*errpos); the tile-CSC builds skip all work when the flag is set, and the
 // host rejects the matrix before anything else reads these outputs.
+// PFlog rows (rowb != null) also get the baseline
+// b_i = -(sum_j x~_ij) * (1 / n_vars) (pflog.rs:186-222), f64 lane partials
+// and a wave reduction.
 __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict__ indptr,
                                                       const uint32_t *__restrict__ idx,
                                                       const float *__restrict__ val, int64_t n_obs,
                                                       double target, int flags, int nsplit,
                                                       double *__restrict__ rowf, float *__restrict__ xt,
                                                       uint16_t *__restrict__ lg, uint32_t *__restrict__ spl,
-                                                      int64_t n_vars, int *__restrict__ flag) {
+                                                      int64_t n_vars, int *__restrict__ flag,
+                                                      double *__restrict__ rowb,
+                                                      unsigned long long *__restrict__ errpos) {
     constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
+    const double inv_d = 1.0 / static_cast<double>(n_vars > 0 ? n_vars : 1);
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     int bad = 0;
@@ -206,7 +224,22 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         for (int g = (chl + 1 > 0 ? chl + 1 : 0) + lane; g < nsplit; g += kWave) sr[g] = static_cast<uint32_t>(b - a);
         if (lane == 0) sr[nsplit] = static_cast<uint32_t>(b - a);
     };
+    // kXfCheck: non-finite input or transformed value; a negative count for
+    // the in-memory PFlog builder; the smallest offending position wins
+    auto check_value = [&](float v, float d, int64_t p) {
+        const bool nf = !__builtin_isfinite(v) || !__builtin_isfinite(d);
+        const bool neg = (flags & kXfLogF64) && v < 0.f;
+        if (nf || neg) {
+            bad |= (nf ? 16 : 0) | (neg ? 32 : 0);
+            atomicMax(errpos, ~static_cast<unsigned long long>(p));
+        }
+    };
+    auto put_baseline = [&](int64_t r, double dsum) {
+        dsum = wave_sum_dpp(dsum);
+        if (lane == 0) rowb[r] = -dsum * inv_d;
+    };
     auto factor = [&](int64_t r, double sum) {
+        if (flags & kXfScale) return target;
         double f = 1.0;
         if (flags & kXfNorm) {
             sum = wave_sum_dpp(sum);
@@ -238,6 +271,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         const double f = factor(r, sum);
         uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
         int64_t prev_last = -1;
+        double dsum = 0.0;
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int64_t p0 = a + u * kWave;
@@ -245,12 +279,16 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             const int64_t p = p0 + lane;
             const bool ok = p < b;
             if (ok) {
-                xt[p] = xtilde(v[u], f, flags);
+                const float d = xtilde(v[u], f, flags);
+                xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
+                dsum += static_cast<double>(d);
+                if (flags & kXfCheck) check_value(v[u], d, p);
             }
             scan(sr, a, b, p0, ok, c[u], prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
+        if (rowb) put_baseline(r, dsum);
     };
     auto long_row = [&](int64_t r, int64_t a, int64_t b) {
         double sum = 0.0;
@@ -259,17 +297,23 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         const double f = factor(r, sum);
         uint32_t *sr = nsplit > 0 ? spl + r * (nsplit + 1) : nullptr;
         int64_t prev_last = -1;
+        double dsum = 0.0;
         for (int64_t p0 = a; p0 < b; p0 += kWave) {
             const int64_t p = p0 + lane;
             const bool ok = p < b;
             const uint32_t c = ok ? idx[p] : 0u;
             if (ok) {
-                xt[p] = xtilde(val[p], f, flags);
+                const float v = val[p];
+                const float d = xtilde(v, f, flags);
+                xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
+                dsum += static_cast<double>(d);
+                if (flags & kXfCheck) check_value(v, d, p);
             }
             scan(sr, a, b, p0, ok, c, prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
+        if (rowb) put_baseline(r, dsum);
     };
     // two rows per wave at a time (r and r + wstride): both rows' loads are in
     // flight together, so each wave keeps twice the bytes outstanding
@@ -492,6 +536,33 @@ __global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ A, const d
     }
 }
 
+// out = (sum x, sum x^2) over x[0..n), f64, one 1024-thread block: thread
+// partials over a fixed stride, then a fixed-order tree (deterministic).
+__global__ void __launch_bounds__(1024) k_sum_sq_f64(const double *__restrict__ x, int64_t n,
+                                                     double *__restrict__ out) {
+    __shared__ double rs[1024], rq[1024];
+    double a = 0.0, q = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 1024) {
+        const double v = x[i];
+        a += v;
+        q += v * v;
+    }
+    rs[threadIdx.x] = a;
+    rq[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1) {
+        if (static_cast<int>(threadIdx.x) < w) {
+            rs[threadIdx.x] += rs[threadIdx.x + w];
+            rq[threadIdx.x] += rq[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = rs[0];
+        out[1] = rq[0];
+    }
+}
+
 // Column j of the partials summed by block j: thread t takes partials t,
 // t+256, ...; then a fixed-order tree over the 256 thread sums.
 template <int KP>
@@ -527,7 +598,8 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
                                                   const float *__restrict__ xt,
                                                   const uint16_t *__restrict__ xbf,
                                                   const float *__restrict__ M, const float *__restrict__ mc,
-                                                  float *__restrict__ Y, int64_t n_obs) {
+                                                  float *__restrict__ Y, int64_t n_obs,
+                                                  const double *__restrict__ rowb, const float *__restrict__ cs) {
     constexpr int E = KP / 16;
     __shared__ uint2 stage[4][kWave];
     const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
@@ -570,6 +642,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
         }
         if (g == 0) {
             float4 *yr = reinterpret_cast<float4 *>(Y + r * KP + l * E);
+            const float br = rowb ? static_cast<float>(rowb[r]) : 0.f;
 #pragma unroll
             for (int e = 0; e < E / 4; ++e) {
                 float4 o;
@@ -577,18 +650,26 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
                 o.y = acc[4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
                 o.z = acc[4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
                 o.w = acc[4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
+                if (rowb) {
+                    o.x += br * cs[l * E + 4 * e + 0];
+                    o.y += br * cs[l * E + 4 * e + 1];
+                    o.z += br * cs[l * E + 4 * e + 2];
+                    o.w += br * cs[l * E + 4 * e + 3];
+                }
                 yr[e] = o;
             }
         }
     }
 }
 
-// B[v][j] = sum_rb part[rb][v][j] - mu[v] * sumq[j]  (f64; cpu.rs:1253-1267)
+// B[v][j] = sum_rb part[rb][v][j] - mu[v] * sumq[j]  (f64; cpu.rs:1253-1267),
+// + bq[j] = (b^T Q)_j for a PFlog matrix (cpu.rs:653-687)
 template <int KP>
 __global__ void __launch_bounds__(256) k_spmm_t_reduce(const float *__restrict__ part, int nrb,
                                                        int64_t n_vars, const double *__restrict__ mu,
                                                        const double *__restrict__ sumq,
-                                                       double *__restrict__ B, float *__restrict__ B32) {
+                                                       double *__restrict__ B, float *__restrict__ B32,
+                                                       const double *__restrict__ bq) {
     const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (t >= n_vars * KP) return;
     const int64_t v = t / KP;
@@ -596,6 +677,7 @@ __global__ void __launch_bounds__(256) k_spmm_t_reduce(const float *__restrict__
     double s = 0.0;
     for (int rb = 0; rb < nrb; ++rb) s += static_cast<double>(part[static_cast<int64_t>(rb) * n_vars * KP + t]);
     if (mu) s -= mu[v] * sumq[j];
+    if (bq) s += bq[j];
     if (B) B[t] = s;
     if (B32) B32[t] = static_cast<float>(s);
 }
@@ -896,7 +978,9 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
                                                        const float *__restrict__ M, const float *__restrict__ mc,
                                                        float *__restrict__ Y, const uint32_t *__restrict__ spl,
                                                        int nsplit, int cpc, int nch, int64_t n_obs,
-                                                       int64_t n_vars, int64_t nnz) {
+                                                       int64_t n_vars, int64_t nnz,
+                                                       const double *__restrict__ rowb,
+                                                       const float *__restrict__ cs) {
     constexpr int E = KP / 16;
     static_assert(R <= 16, "row metadata is held one row per lane of a 16-lane group");
     static_assert(R % PF == 0, "ring slots are static: slot = row % PF in every chunk");
@@ -1030,6 +1114,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
         }
         if (row < n_obs && g == 0) {
             float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
+            const float br = rowb ? static_cast<float>(rowb[row]) : 0.f;
 #pragma unroll
             for (int e = 0; e < E / 4; ++e) {
                 float4 o;
@@ -1037,6 +1122,12 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
                 o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
                 o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
                 o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
+                if (rowb) {
+                    o.x += br * cs[l * E + 4 * e + 0];
+                    o.y += br * cs[l * E + 4 * e + 1];
+                    o.z += br * cs[l * E + 4 * e + 2];
+                    o.w += br * cs[l * E + 4 * e + 3];
+                }
                 yr[e] = o;
             }
         }
@@ -2207,11 +2298,21 @@ __global__ void __launch_bounds__(256) k_apply2(const float *__restrict__ Y, con
 // by column, W[i][c] = -(sum_{l=i+1..c} R[i][l] W[l][c]) / R[i][i], LPC lanes
 // per column (partial sums combined by shuffles in a fixed order, within one
 // wave, so no barriers), stored transposed in the free lower triangle.
+// Pivot rule (both Cholesky kernels): floor_tau == 0 is strict (a pivot that
+// is not positive and finite sets status bit 1, as CholeskyQR2's non-SPD
+// error); floor_tau > 0 floors a pivot below floor_tau * the mean (shifted)
+// diagonal at that value and sets bit 4. R stays nonsingular, so Q = A R^-1
+// still spans A: an exactly rank-deficient A gets a floored direction that
+// the next pass normalises. The floor is relative to the matrix, not to the
+// column: a column of pure round-off is then scaled up by at most
+// 1/sqrt(floor_tau) per pass instead of without bound. A non-finite pivot
+// still sets bit 1.
 // ---------------------------------------------------------------------------
 template <int KP>
 __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int k, double shift,
                                                double *__restrict__ R64, double *__restrict__ W64,
-                                               float *__restrict__ W32, int *__restrict__ status) {
+                                               float *__restrict__ W32, int *__restrict__ status,
+                                               double floor_tau) {
     constexpr int LD = KP + 1;           // padded row; column KP holds W's diagonal
     constexpr int LPC = 256 / KP;        // lanes per column in the inverse (4 or 2)
     __shared__ double a[KP * LD];
@@ -2223,24 +2324,35 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
         a[i * LD + c] = (i < k && c < k) ? G[t] : 0.0;
     }
     __syncthreads();
+    if (tid < 64) {
+        double s = 0.0;
+        for (int i = tid; i < k; i += 64) s += a[i * LD + i];
+        s = wave_sum(s);
+        if (tid == 0) trs = s;
+    }
+    __syncthreads();
     if (shift > 0.0) {
-        if (tid < 64) {
-            double s = 0.0;
-            for (int i = tid; i < k; i += 64) s += a[i * LD + i];
-            s = wave_sum(s);
-            if (tid == 0) trs = s;
-        }
-        __syncthreads();
         for (int i = tid; i < k; i += 256) a[i * LD + i] += shift * trs;
         __syncthreads();
     }
-    bool bad = false;
+    const double mdiag = k > 0 ? trs * (1.0 + (shift > 0.0 ? shift * k : 0.0)) / k : 0.0;
+    const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;
+    bool bad = false, floored = false;
     double r_prev = 1.0, inv_prev = 1.0;
     for (int j = 0; j < k; ++j) {
         const double d = a[j * LD + j];
-        const bool ok = (d > 0.0) && isfinite(d);
-        bad |= !ok;
-        const double r = ok ? sqrt(d) : 1.0;
+        double ok_d;
+        if (floor_tau > 0.0) {
+            const bool fin = isfinite(d);
+            bad |= !fin;
+            floored |= fin && !(d > fl);
+            ok_d = (fin && d > fl) ? d : (fin ? fl : 1.0);
+        } else {
+            const bool ok = (d > 0.0) && isfinite(d);
+            bad |= !ok;
+            ok_d = ok ? d : 1.0;
+        }
+        const double r = sqrt(ok_d);
         const double inv = 1.0 / r;
         if (j > 0) {  // scale row j-1 (no thread reads it in this step)
             for (int c = j - 1 + tid; c < k; c += 256)
@@ -2262,6 +2374,7 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
             a[(k - 1) * LD + c] = (c == k - 1) ? r_prev : a[(k - 1) * LD + c] * inv_prev;
     __syncthreads();
     for (int i = tid; i < k; i += 256) rinv[i] = 1.0 / a[i * LD + i];
+    if (floored && tid == 0) atomicOr(status, 4);
     __syncthreads();
     for (int t = tid; t < KP * KP; t += 256) {
         const int i = t / KP, c = t % KP;
@@ -2301,19 +2414,19 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
 // i > j with R[j][i] read back as a broadcast (measured faster than 63
 // readlanes per step). Lanes c < i keep updating their lower-triangle
 // registers (never read), which keeps the code branch-free. k_trinv_w then
-// inverts R.
+// inverts R. The pivot rule is k_chol2's (floor_tau).
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int k, double shift,
-                                              double *__restrict__ R64, int *__restrict__ status) {
+                                              double *__restrict__ R64, int *__restrict__ status,
+                                              double floor_tau) {
     constexpr int KP = 64;
     __shared__ double rowb[2][KP];
     __shared__ double Rs[KP * KP];
     const int c = threadIdx.x;
-    double sh = 0.0;
-    if (shift > 0.0) {
-        const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
-        sh = c < k ? shift * tr : 0.0;
-    }
+    const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
+    const double sh = (shift > 0.0 && c < k) ? shift * tr : 0.0;
+    const double mdiag = k > 0 ? (tr + (shift > 0.0 ? shift * tr * k : 0.0)) / k : 0.0;
+    const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // pivot floor
     // masked, shifted Gram staged row-major in LDS (eight rows' loads in
     // flight: the registers of a[] are not live yet), then one unconditional
     // read per register
@@ -2327,13 +2440,21 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
 #pragma unroll
     for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c];
     wave_lds_sync();
-    bool bad = false;
+    bool bad = false, floored = false;
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
         const double d = readlane_f64(a[j], j);
-        const bool ok = (d > 0.0) && isfinite(d);
-        bad |= !ok;
-        const double dd = ok ? d : 1.0;  // select the input, not the result: no branch
+        double dd;  // select the input, not the result: no branch
+        if (floor_tau > 0.0) {
+            const bool fin = isfinite(d);
+            bad |= !fin;
+            floored |= fin && !(d > fl);
+            dd = (fin && d > fl) ? d : (fin ? fl : 1.0);
+        } else {
+            const bool ok = (d > 0.0) && isfinite(d);
+            bad |= !ok;
+            dd = ok ? d : 1.0;
+        }
         const double ir = rsq_nr(dd);
         const double r = dd * ir;
         const double t = (c == j) ? r : a[j] * ir;
@@ -2344,6 +2465,7 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
         for (int i = j + 1; i < KP; ++i) a[i] -= rowb[j & 1][i] * t;
     }
     if (bad && c == 0) atomicOr(status, 1);
+    if (floored && c == 0) atomicOr(status, 4);
 #pragma unroll
     for (int i = 0; i < KP; ++i) R64[i * KP + c] = (i <= c && c < k) ? a[i] : 0.0;
 }

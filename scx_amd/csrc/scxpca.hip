@@ -130,7 +130,6 @@ double split_max_bytes() {
     }
     return kMaxSplitBytes;
 }
-constexpr int kStatsCG = 2048; // genes per column-stats block
 constexpr int kMaxColBlocks = 1024;  // partial rows of the column-sum kernels
 
 struct DeviceGuard {
@@ -200,6 +199,14 @@ struct scxpca_matrix {
     double target = 0.0;
     int flags = 0;
     DevBuf rowf;
+    // PFlog (v4) matrix: X~ is delta = log1p(4 alpha x) and the operator is
+    // delta + b 1^T with the per-cell baseline b (device, f64)
+    bool pflog = false;
+    scxpca_pflog_params pf_params{};
+    scxpca_pflog_fit pf_fit{};
+    DevBuf pfb, pfstat;          // baseline; its sum and sum of squares
+    PinnedBuf hpf;               // host copy of pfstat
+    double pf_bsum = 0.0, pf_b2 = 0.0;
     DevBuf spl;
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
@@ -272,9 +279,6 @@ struct StageScope {
     }
 };
 
-// Derived per-matrix data: row factors, split offsets (small n_vars), the
-// tile-CSC copy. Runs at creation and on refresh; invalidates cached stats.
-const double *rowf_ptr(const scxpca_matrix *m) { return (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr; }
 
 // Row factors, X~, chunk offsets and the tile-CSC of the current CSR, with
 // the matrix validated in the same pass. Returns the validation bits of
@@ -285,12 +289,20 @@ int prepare(scxpca_matrix *m) {
     m->xbf_ready = false;
     if (m->n_obs == 0) return 0;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
-    // gcnt holds the per-gene entry counts and, at [n_vars], the validation
-    // bits of k_prepare_rows: one copy brings both back
-    const int64_t nvc = std::max<int64_t>(m->n_vars, 1);
-    m->gcnt.ensure(sizeof(uint32_t) * (nvc + 1));
+    // gcnt holds the per-gene entry counts, at [n_vars] the validation bits of
+    // k_prepare_rows and at the next even word the complemented position of
+    // the first entry that failed a value check: one copy brings all back
+    const int64_t epw = (m->n_vars + 2) / 2 * 2, gwords = epw + 2;
+    m->gcnt.ensure(sizeof(uint32_t) * gwords);
     int *vflag = reinterpret_cast<int *>(m->gcnt.as<uint32_t>() + m->n_vars);
-    HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * (nvc + 1), m->stream));
+    unsigned long long *errpos = reinterpret_cast<unsigned long long *>(m->gcnt.as<uint32_t>() + epw);
+    HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * gwords, m->stream));
+    const bool want_b = (m->flags & kXfScale) != 0;
+    if (want_b) {
+        m->pfb.ensure(sizeof(double) * m->n_obs);
+        m->pfstat.ensure(sizeof(double) * 2);
+        m->hpf.ensure(sizeof(double) * 2);
+    }
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
     static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
     k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
@@ -298,7 +310,7 @@ int prepare(scxpca_matrix *m) {
                                                       (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
                                                       m->xt.as<float>(), m->lg.as<uint16_t>(),
                                                       m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
-                                                      vflag);
+                                                      vflag, want_b ? m->pfb.as<double>() : nullptr, errpos);
     HIPC(hipGetLastError());
     if (m->use_split)
     {
@@ -323,23 +335,83 @@ int prepare(scxpca_matrix *m) {
                                                          m->gcnt.as<uint32_t>());
         HIPC(hipGetLastError());
     }
-    m->hgcnt.ensure(sizeof(uint32_t) * (nvc + 1));
-    HIPC(hipMemcpyAsync(m->hgcnt.p, m->gcnt.p, sizeof(uint32_t) * (m->n_vars + 1), hipMemcpyDeviceToHost,
-                        m->stream));
+    if (want_b) {  // sum and sum of squares of the baseline (fixed order)
+        k_sum_sq_f64<<<1, 1024, 0, m->stream>>>(m->pfb.as<double>(), m->n_obs, m->pfstat.as<double>());
+        HIPC(hipGetLastError());
+        HIPC(hipMemcpyAsync(m->hpf.p, m->pfstat.p, sizeof(double) * 2, hipMemcpyDeviceToHost, m->stream));
+    }
+    m->hgcnt.ensure(sizeof(uint32_t) * gwords);
+    HIPC(hipMemcpyAsync(m->hgcnt.p, m->gcnt.p, sizeof(uint32_t) * gwords, hipMemcpyDeviceToHost, m->stream));
     HIPC(hipStreamSynchronize(m->stream));
     const uint32_t *hg = m->hgcnt.as<uint32_t>();
     m->host_gcnt.assign(hg, hg + m->n_vars);
     for (auto &g : m->gorder) g.valid = false;
+    if (want_b) {
+        m->pf_bsum = m->hpf.as<double>()[0];
+        m->pf_b2 = m->hpf.as<double>()[1];
+    }
     return static_cast<int>(hg[m->n_vars]);
 }
 
-// The rejection for the validation bits of prepare().
-void throw_invalid(int h) {
+// Position of the first entry that failed k_prepare_rows' value check.
+int64_t error_position(const scxpca_matrix *m) {
+    const int64_t epw = (m->n_vars + 2) / 2 * 2;
+    unsigned long long w = 0;
+    std::memcpy(&w, m->hgcnt.as<uint32_t>() + epw, sizeof(w));
+    return static_cast<int64_t>(~w);
+}
+
+// Rust's Display of an f32 for the common cases (shortest round-trip digits).
+std::string fmt_f32(float v) {
+    if (std::isnan(v)) return "NaN";
+    if (std::isinf(v)) return v > 0 ? "inf" : "-inf";
+    char buf[64];
+    for (int prec = 1; prec <= 9; ++prec) {
+        std::snprintf(buf, sizeof(buf), "%.*g", prec, static_cast<double>(v));
+        if (std::strtof(buf, nullptr) == v) break;
+    }
+    return buf;
+}
+
+// The rejection for the validation bits of prepare(). The value checks
+// report the first offending entry as the reference does: estimate_alpha's
+// finite guard (finite.rs:22-32), delta_from_raw_csr (pyscx/src/accel/
+// pflog.rs:375-397) and pflog_baseline_from_delta's guard (pflog.rs:46-56).
+void throw_invalid(const scxpca_matrix *m, int h) {
     if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
     if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
     if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
     if (h & 8)
         throw Error(SCXPCA_ERR_INVALID, "a CSR row repeats a column index (sum duplicate entries before calling pca)");
+    if (h & (16 | 32)) {
+        const int64_t pos = error_position(m);
+        float v = 0.f;
+        HIPC(hipMemcpy(&v, m->val + pos, sizeof(float), hipMemcpyDefault));
+        if (!(m->flags & kXfScale))
+            throw Error(SCXPCA_ERR_INVALID, "HVG input contains a non-finite value (" + fmt_f32(v) +
+                                                ") at nonzero index " + std::to_string(pos) +
+                                                "; this accelerator requires finite input \u2014 filter/QC NaN "
+                                                "and Inf before computing variance");
+        if (m->flags & kXfLogF64) {
+            // the cell: last row whose start is <= pos
+            int64_t lo = 0, hi = m->n_obs;
+            while (hi - lo > 1) {
+                const int64_t mid = lo + (hi - lo) / 2;
+                int64_t a = 0;
+                HIPC(hipMemcpy(&a, m->indptr + mid, sizeof(int64_t), hipMemcpyDefault));
+                if (a <= pos) lo = mid; else hi = mid;
+            }
+            if (!std::isfinite(v))
+                throw Error(SCXPCA_ERR_INVALID, "pflog: non-finite count " + fmt_f32(v) + " at cell " +
+                                                    std::to_string(lo) + "; counts must be finite");
+            throw Error(SCXPCA_ERR_INVALID, "pflog: negative count " + fmt_f32(v) + " at cell " +
+                                                std::to_string(lo) + "; counts must be non-negative");
+        }
+        const float d = std::log1p(static_cast<float>(static_cast<double>(v) * m->target));
+        throw Error(SCXPCA_ERR_INVALID, "PFlog input contains a non-finite value (" + fmt_f32(d) +
+                                            ") at nonzero index " + std::to_string(pos) +
+                                            "; PFlog requires finite raw counts \u2014 filter/QC NaN and Inf first");
+    }
 }
 
 // Gene order for the transposed SpMM: genes sorted by entry count (descending,
@@ -378,13 +450,17 @@ const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
     return go.dev.as<uint32_t>();
 }
 
-void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1p) {
+int transform_flags(double target, int log1p) { return (target > 0.0 ? kXfNorm : 0) | (log1p ? kXfLog1p : 0); }
+
+// Upload (or borrow) the CSR and prepare it under the transform (target,
+// flags: the kXf* bits).
+void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flags) {
     if (x->n_obs < 0 || x->n_vars < 0) throw Error(SCXPCA_ERR_INVALID, "matrix shape must be non-negative");
     if (x->n_vars > 0xffffffffll) throw Error(SCXPCA_ERR_INVALID, "n_vars exceeds the u32 index range");
     m->n_obs = x->n_obs;
     m->n_vars = x->n_vars;
     m->target = target;
-    m->flags = (target > 0.0 ? kXfNorm : 0) | (log1p ? kXfLog1p : 0);
+    m->flags = flags;
     HIPC(hipStreamCreate(&m->stream));
     if (x->on_device) {
         m->indptr = x->indptr;
@@ -461,7 +537,7 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int log1
         HIPC(hipStreamSynchronize(m->stream));  // staging vectors go out of scope
         h = prepare(m);
     }
-    throw_invalid(h);
+    throw_invalid(m, h);
 }
 
 void ensure_xbf(scxpca_matrix *m) {
@@ -532,9 +608,10 @@ void ensure_stats(scxpca_matrix *m) {
     m->stats_ready = true;
 }
 
-__global__ void k_mean(const double *sum, int64_t n_vars, double inv_n, double *mu) {
+// mu = sum / n (+ shift: the PFlog baseline mean, cpu.rs:838-843)
+__global__ void k_mean(const double *sum, int64_t n_vars, double n, double shift, double *mu) {
     const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (v < n_vars) mu[v] = sum[v] * inv_n;
+    if (v < n_vars) mu[v] = sum[v] / n + shift;
 }
 
 // ---------------------------------------------------------------------------
@@ -546,6 +623,10 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
 
 template <int KP>
 struct Engine {
+    // pivot floors (relative to the column's diagonal) of the rank-tolerant
+    // Cholesky: the f32-Gram tall passes, the f64 Rayleigh-Ritz passes
+    static constexpr double kStableFloor = 1e-6;
+    static constexpr double kSmallFloor = 1e-13;
     static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t6 (Z in registers)
     static constexpr int NB0 = (KP == 64) ? 4 : 1;  // slot-0 prefetch batches in k_spmm_t6
     static constexpr int TW = 16;                   // waves per k_spmm_t6 block
@@ -556,8 +637,10 @@ struct Engine {
     hipStream_t s;
     int k = 0;
     // small KP x KP f64 scratch layout inside m->small
-    double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *partc;
-    float *W32, *mc32;
+    double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *partc, *cs64, *bq64;
+    float *W32, *mc32, *cs32;
+    // PFlog operator delta + b 1^T: the rank-1 baseline terms of both SpMMs
+    const double *rowb = nullptr;
     int *status;
     int nrb_t = 0, nch_t = 0, ncol_blk = 0, gram_waves = 0, gram_blocks = 0, tiles_per_tr = 0;
     const uint32_t *gperm = nullptr;  // wave-slot gene order of the transposed SpMM
@@ -596,13 +679,16 @@ struct Engine {
         m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(std::max(gram_waves, gram_blocks), nb64)) * KP * KP);
         m->cpart.ensure(sizeof(double) * static_cast<size_t>(std::max(kMaxColBlocks, nrb_t * nch_t)) * KP);
         constexpr size_t KK = static_cast<size_t>(KP) * KP;
-        m->small.ensure(sizeof(double) * (KK * 11 + 4 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + KP));
+        m->small.ensure(sizeof(double) * (KK * 11 + 6 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + 2 * KP));
         double *d = m->small.as<double>();
         G = d; R1 = G + KK; R2 = R1 + KK; R3 = R2 + KK; Rt = R3 + KK; W64 = Rt + KK; U = W64 + KK; V = U + KK;
         Acol = V + KK; Vcol = Acol + KK; partc = Vcol + KK;  // partc: KK doubles spare
-        sig = partc + KK; sumq = sig + KP; mc64 = sumq + KP; mu = mc64 + KP + KP;
+        sig = partc + KK; sumq = sig + KP; mc64 = sumq + KP; cs64 = mc64 + KP + KP; bq64 = cs64 + KP;
+        mu = bq64 + KP;
         W32 = reinterpret_cast<float *>(mu + std::max<int64_t>(nv, 1));
         mc32 = W32 + KK;
+        cs32 = mc32 + KP;
+        if (m->pflog && m->n_obs > 0) rowb = m->pfb.as<double>();
         m->status.ensure(sizeof(int));
         status = m->status.as<int>();
     }
@@ -612,6 +698,22 @@ struct Engine {
     }
 
     void check() { HIPC(hipGetLastError()); }
+
+    // mean(b) = sum(b) / max(n, 1) (cpu.rs:838), 0 for a plain matrix
+    double baseline_mean() const {
+        return rowb ? m->pf_bsum / std::max(static_cast<double>(m->n_obs), 1.0) : 0.0;
+    }
+
+    // out = w^T A over the rows of a tall f32 panel A (nrows x KP), f64;
+    // w == null: plain column sums
+    void weighted_colsum(const float *A, const double *w, int64_t nrows, double *out64, float *out32) {
+        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nrows + 15) / 16, kMaxColBlocks)));
+        const int64_t rpb = (nrows + nb - 1) / nb;
+        k_colsum<KP, float><<<nb, 256, 0, s>>>(A, w, nrows, rpb, m->cpart.as<double>());
+        check();
+        k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, out64, out32);
+        check();
+    }
 
     // mc = mu^T panel (f32 for the forward epilogue)
     void mean_correction(const float *panel) {
@@ -624,9 +726,10 @@ struct Engine {
         check();
     }
 
-    // sumq = 1^T Q (f64)
+    // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
     void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
         if (m->n_obs == 0) return;
+        if (rowb) weighted_colsum(panel, nullptr, m->n_vars, cs64, cs32);
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
         const float *mcp = centered ? mc32 : nullptr;
         if (m->use_split) {
@@ -637,24 +740,24 @@ struct Engine {
             if (bf16)
                 k_spmm_fwd3<KP, true, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
                     m->indptr, m->lg.as<uint16_t>(), nullptr, m->xbf.as<uint16_t>(), panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz);
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
             else
                 k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
                     m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz);
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
             if (bf16)
                 k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, nullptr, m->xbf.as<uint16_t>(), panel,
-                                                          mcp, Y, m->n_obs);
+                                                          mcp, Y, m->n_obs, rowb, cs32);
             else
                 k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
-                                                           mcp, Y, m->n_obs);
+                                                           mcp, Y, m->n_obs, rowb, cs32);
         }
         check();
     }
 
-    // B64 (and/or B32) = X~^T Q - mu sumq^T
+    // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix, cpu.rs:653-687)
     void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
         const int64_t nv = m->n_vars;
         double *qsp = centered ? m->cpart.as<double>() : nullptr;  // nrb_t x nch_t x KP partials of 1^T Q
@@ -682,21 +785,24 @@ struct Engine {
             k_colsum_reduce<KP><<<KP, 256, 0, s>>>(qsp, nrb_t * nch_t, sumq, nullptr);
             check();
         }
+        if (rowb) weighted_colsum(Q, rowb, m->n_obs, bq64, nullptr);
         StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
         k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb_t, nv,
-                                                                 centered ? mu : nullptr, sumq, B64, B32);
+                                                                 centered ? mu : nullptr, sumq, B64, B32,
+                                                                 rowb ? bq64 : nullptr);
         check();
     }
 
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
-    void chol(double shift, double *Rdst) {
+    // floor_tau: 0 strict (non-SPD is an error), > 0 the rank-tolerant pivot floor
+    void chol(double shift, double *Rdst, double floor_tau) {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
         if constexpr (KP == 64) {
-            k_chol_w<<<1, 64, 0, s>>>(G, k, shift, Rdst, status);
+            k_chol_w<<<1, 64, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
             check();
             k_trinv_w<<<1, 64, 0, s>>>(Rdst, k, W64, W32);
         } else {
-            k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status);
+            k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status, floor_tau);
         }
         check();
     }
@@ -722,7 +828,11 @@ struct Engine {
     // With final == false (a basis that only feeds the next product of the
     // power iteration) one shifted pass: Q = Y R^-1 spans Y exactly for any
     // nonsingular R, and the shift keeps the f32 Gram positive definite; the
-    // final basis gets the requested method.
+    // final basis gets the requested method. The stable mode floors pivots
+    // (kStableFloor) instead of failing, so an exactly rank-deficient Y (k
+    // above the rank, where the reference's Householder QR still returns an
+    // orthonormal basis) gets one: the floored directions are normalised by
+    // the following passes. CholeskyQR2 keeps the strict non-SPD error.
     float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true) {
         if (n < 0) n = m->n_obs;
         const int passes = !final ? 1 : (qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2);
@@ -744,7 +854,7 @@ struct Engine {
             }
             check();
             const double shift = !final ? 1e-7 : ((qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0);
-            chol(shift, R1);
+            chol(shift, R1, qr_method == SCXPCA_QR_HOUSEHOLDER ? kStableFloor : 0.0);
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
                 apply(src, tmp, n, KP, KP, true);
@@ -756,7 +866,9 @@ struct Engine {
     }
 
     // Small f64 CholeskyQR3 (shifted first pass) of A (n_vars x KP) in place
-    // into Qout; R = R3 R2 R1 written to Rt.
+    // into Qout; R = R3 R2 R1 written to Rt. Pivots are floored (kSmallFloor),
+    // never an error: B of rank below k (the reference's thin SVD accepts it)
+    // still gives an orthonormal Q_B with A = Q_B R.
     void cqr_small64(const double *A, double *Qout) {
         const int64_t nv = m->n_vars;
         const int nb = gram64_blocks();
@@ -770,7 +882,7 @@ struct Engine {
             check();
             k_sum_parts16<KP><<<(KP * KP + 15) / 16, 256, 0, s>>>(m->gpart.as<double>(), nb, G);
             check();
-            chol(pass == 0 ? 1e-13 : 0.0, Rs[pass]);
+            chol(pass == 0 ? 1e-13 : 0.0, Rs[pass], kSmallFloor);
             k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(src, W64, bufs[pass], nullptr, nv);
             check();
             src = bufs[pass];
@@ -793,6 +905,24 @@ struct Engine {
         const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
         double tv = 0.0;
         recomputed = 0;
+        if (rowb) {
+            // pflog_total_variance (cpu.rs:702-769): S_dd - D B2 (uncentered),
+            // S_dd - D B2 - n sum_j mu_j^2 with mu_j = colmean_j + mean(b)
+            // (centered); never recomputed, clamped at 0
+            double s_dd = 0.0;
+            for (int64_t v = 0; v < nv; ++v) s_dd += hsq[v];
+            double tss = s_dd - static_cast<double>(nv) * m->pf_b2;
+            if (centered) {
+                const double bm = baseline_mean();
+                double m2 = 0.0;
+                for (int64_t v = 0; v < nv; ++v) {
+                    hmu[v] += bm;
+                    m2 += hmu[v] * hmu[v];
+                }
+                tss -= static_cast<double>(n) * m2;
+            }
+            return std::max(tss / denom, 0.0);
+        }
         if (!centered) {
             for (int64_t v = 0; v < nv; ++v) tv += hsq[v];
             tv /= denom;
@@ -837,7 +967,7 @@ struct Engine {
         HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
         if (sketch_bf16) ensure_xbf(m);
         ensure_stats(m);
-        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, 1.0 / static_cast<double>(n), mu);
+        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), baseline_mean(), mu);
         check();
         // Omega -> f32 panel, generated on the device
         device_omega<KP>(s, static_cast<uint64_t>(nv), k, p->seed, m->panel.as<float>(), nullptr, m->om);
@@ -945,7 +1075,7 @@ struct Engine {
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
         ensure_stats(m);
-        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, 1.0 / static_cast<double>(n), mu);
+        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), 0.0, mu);
         check();
         // block pairs (I <= J) of 64-gene blocks and the pair index of each
         const int nblk = static_cast<int>((nv + kCovGB - 1) / kCovGB);
@@ -1182,6 +1312,7 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
 constexpr int64_t kMaxCovVars = 16384;
 
 void run_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
+    if (m->pflog) throw Error(SCXPCA_ERR_INVALID, "the covariance route is not defined for a PFlog matrix");
     validate(m, p);
     if (p->n_comps > 128)
         throw Error(SCXPCA_ERR_INVALID, "n_comps = " + std::to_string(p->n_comps) + " exceeds the GPU panel limit of 128");
@@ -1325,7 +1456,106 @@ void project(scxpca_matrix *m, bool backward, const float *in, int64_t k, const 
     if (!user) HIPC(hipStreamSynchronize(s));
 }
 
-void refresh(scxpca_matrix *m) { throw_invalid(prepare(m)); }
+// PFlog flags for the delta pass.
+int pflog_flags(const scxpca_pflog_params &pp) {
+    return kXfScale | kXfLog1p | kXfCheck | (pp.delta_form == SCXPCA_PFLOG_DELTA_F64 ? kXfLogF64 : 0);
+}
+
+// estimate_alpha (scx-accel/src/pflog.rs:340-398) from the raw column sums
+// and sums of squares: finalize_column_moments (scx-sparse/src/moments.rs:
+// 193-233) per gene, alpha_g = (var - mean) / mean^2 for every gene with
+// mean > mu_min (finite values only), the median of the pool (:403-415), the
+// fallback when the pool is empty or its median is not positive.
+#pragma clang fp contract(off)
+scxpca_pflog_fit estimate_alpha(const std::vector<double> &sum, const std::vector<double> &sq, int64_t n,
+                                const scxpca_pflog_params &pp) {
+    std::vector<double> cand;
+    if (n > 0) {
+        const double nf = static_cast<double>(n), denom = std::max(nf - 1.0, 1.0);
+        for (size_t j = 0; j < sum.size(); ++j) {
+            const double mean = sum[j] / nf;
+            const double residual = sq[j] - nf * mean * mean;
+            double var = residual / denom;
+            if (var < 0.0) var = 0.0;
+            if (!(mean > pp.mu_min)) continue;
+            const double a = (var - mean) / (mean * mean);
+            if (std::isfinite(a)) cand.push_back(a);
+        }
+    }
+    scxpca_pflog_fit f{};
+    f.estimated = 1;
+    f.n_genes_used = static_cast<int64_t>(cand.size());
+    double alpha = 0.0;
+    if (!cand.empty()) {
+        std::sort(cand.begin(), cand.end());
+        const size_t h = cand.size() / 2;
+        alpha = cand.size() % 2 ? cand[h] : 0.5 * (cand[h - 1] + cand[h]);
+    }
+    if (cand.empty() || !std::isfinite(alpha) || alpha <= 0.0) {
+        f.alpha = pp.fallback_alpha;
+        f.fell_back = 1;
+    } else {
+        f.alpha = alpha;
+        f.fell_back = 0;
+    }
+    f.pseudocount = 1.0 / (4.0 * f.alpha);
+    return f;
+}
+#pragma clang fp contract(on)
+
+// The PFlog preparation of an uploaded matrix: with an estimated alpha a raw
+// pass (X~ = x, finite check) gives the column moments, then the delta pass
+// (X~ = log1p(4 alpha x), baseline) replaces it.
+void prepare_pflog(scxpca_matrix *m, bool raw_ready) {
+    if (m->pf_fit.estimated) {
+        if (!raw_ready) {
+            m->flags = kXfCheck;
+            m->target = 0.0;
+            throw_invalid(m, prepare(m));
+        }
+        ensure_stats(m);
+        std::vector<double> hs(m->n_vars), hq(m->n_vars);
+        if (m->n_vars > 0) {
+            HIPC(hipMemcpyAsync(hs.data(), m->sum.p, sizeof(double) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
+            HIPC(hipMemcpyAsync(hq.data(), m->sq.p, sizeof(double) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
+        }
+        HIPC(hipStreamSynchronize(m->stream));
+        m->pf_fit = estimate_alpha(hs, hq, m->n_obs, m->pf_params);
+    }
+    m->flags = pflog_flags(m->pf_params);
+    m->target = 4.0 * m->pf_fit.alpha;
+    throw_invalid(m, prepare(m));
+}
+
+void build_pflog(scxpca_matrix *m, const scxpca_csr *x, const scxpca_pflog_params &pp) {
+    const double a = pp.alpha;
+    if (!(a == 0.0 || (a > 0.0 && std::isfinite(a))))
+        throw Error(SCXPCA_ERR_INVALID, "pflog: alpha must be positive and finite, got " + fmt_f32(static_cast<float>(a)));
+    if (pp.delta_form != SCXPCA_PFLOG_DELTA_LAZY && pp.delta_form != SCXPCA_PFLOG_DELTA_F64)
+        throw Error(SCXPCA_ERR_INVALID, "unknown PFlog delta form");
+    if (!(pp.fallback_alpha > 0.0 && std::isfinite(pp.fallback_alpha)) || !std::isfinite(pp.mu_min))
+        throw Error(SCXPCA_ERR_INVALID, "PFlog alpha options must be finite with a positive fallback");
+    if (x->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "PFlog requires n_vars > 0 for centering");
+    m->pflog = true;
+    m->pf_params = pp;
+    m->pf_fit = scxpca_pflog_fit{};
+    if (a == 0.0) {
+        m->pf_fit.estimated = 1;
+        build_matrix(m, x, 0.0, kXfCheck);  // the raw pass
+    } else {
+        m->pf_fit.alpha = a;
+        m->pf_fit.pseudocount = 1.0 / (4.0 * a);
+        m->pf_fit.n_genes_used = -1;
+        build_matrix(m, x, 4.0 * a, pflog_flags(pp));
+        return;
+    }
+    prepare_pflog(m, true);
+}
+
+void refresh(scxpca_matrix *m) {
+    if (m->pflog) prepare_pflog(m, false);
+    else throw_invalid(m, prepare(m));
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1342,8 +1572,34 @@ scxpca_status scxpca_matrix_create(const scxpca_csr *x, double target_sum, int32
         DeviceGuard dg(device);
         std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
         m->device = device;
-        build_matrix(m.get(), x, target_sum, log1p);
+        build_matrix(m.get(), x, target_sum, transform_flags(target_sum, log1p));
         *out = m.release();
+    });
+}
+
+scxpca_status scxpca_matrix_create_pflog(const scxpca_csr *x, const scxpca_pflog_params *pp, int device,
+                                         scxpca_matrix **out) {
+    return guarded([&] {
+        if (!x || !pp || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        *out = nullptr;
+        DeviceGuard dg(device);
+        std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
+        m->device = device;
+        build_pflog(m.get(), x, *pp);
+        *out = m.release();
+    });
+}
+
+scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, double *baseline) {
+    return guarded([&] {
+        if (!m) throw Error(SCXPCA_ERR_INVALID, "null matrix");
+        if (!m->pflog) throw Error(SCXPCA_ERR_INVALID, "not a PFlog matrix");
+        DeviceGuard dg(m->device);
+        if (fit) *fit = m->pf_fit;
+        if (baseline && m->n_obs > 0) {
+            HIPC(hipMemcpyAsync(baseline, m->pfb.p, sizeof(double) * m->n_obs, hipMemcpyDeviceToHost, m->stream));
+            HIPC(hipStreamSynchronize(m->stream));
+        }
     });
 }
 
@@ -1372,7 +1628,7 @@ scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p,
         DeviceGuard dg(device);
         std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
         m->device = device;
-        build_matrix(m.get(), x, p->target_sum, p->log1p);
+        build_matrix(m.get(), x, p->target_sum, transform_flags(p->target_sum, p->log1p));
         run_pca(m.get(), p, out);
     });
 }
@@ -1388,7 +1644,9 @@ scxpca_status scxpca_matrix_stats(scxpca_matrix *m, double *sums, double *sum_sq
         if (sum_sq) HIPC(hipMemcpyAsync(sum_sq, m->sq.p, sizeof(double) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
         if (col_nnz) HIPC(hipMemcpyAsync(col_nnz, m->cnt.p, sizeof(uint32_t) * m->n_vars, hipMemcpyDeviceToHost, m->stream));
         if (row_factor) {
-            if (m->flags & kXfNorm)
+            if (m->flags & kXfScale)
+                for (int64_t r = 0; r < m->n_obs; ++r) row_factor[r] = m->target;
+            else if (m->flags & kXfNorm)
                 HIPC(hipMemcpyAsync(row_factor, m->rowf.p, sizeof(double) * m->n_obs, hipMemcpyDeviceToHost, m->stream));
             else
                 for (int64_t r = 0; r < m->n_obs; ++r) row_factor[r] = 1.0;

@@ -148,3 +148,33 @@ def test_invalid_split_cap_is_rejected(gpu, small, monkeypatch, value):
     monkeypatch.setenv("SCXPCA_SPLIT_MAX_BYTES", value)
     with pytest.raises(ValueError, match="SCXPCA_SPLIT_MAX_BYTES"):
         gpu.Matrix(small.indptr, small.indices, small.data, small.shape, target_sum=1e4, log1p=True)
+
+
+def test_rank_deficient_matrix(gpu):
+    """k above the rank (10 distinct cells, each repeated 50 times): the stable
+    mode returns the oracle's (Householder) spectrum, zeros beyond the rank;
+    CholeskyQR2 reports its non-SPD error, as the reference's cuSOLVER route
+    does (scx-gpu/src/cusolver.rs:381-385)."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(5)
+    base = rng.negative_binomial(2, 0.4, size=(10, 40)).astype(np.float32)
+    x = sp.csr_matrix(np.repeat(base, 50, axis=0))
+    say("case: rank-deficient 500x40 rank<=10, k=30")
+    ref = oracle.randomized_pca(oracle.CsrView(x.indptr, x.indices, x.data, x.shape), n_comps=20, n_oversamples=10,
+                                n_power_iterations=2, zero_center=True, seed=0, target_sum=1e4, log1p=True)
+    m = gpu.Matrix(x.indptr.astype(np.int64), x.indices.astype(np.uint32), x.data, x.shape, target_sum=1e4,
+                   log1p=True)
+    try:
+        got = m.pca(n_comps=20, n_oversamples=10, n_power_iterations=2, seed=0, qr_method=0)
+        with pytest.raises(gpu.LinAlgError, match="CholeskyQR2 failed: non-SPD"):
+            m.pca(n_comps=20, n_oversamples=10, n_power_iterations=2, seed=0, qr_method=1)
+    finally:
+        m.close()
+    s_ref, s_got = ref["sigma_all"], got["sigma"]
+    say("rank-deficient sigma", s_got[:11], "ref", s_ref[:11])
+    assert np.all(sigma_rel_err(s_ref[:9], s_got[:9]) <= 1e-4)
+    assert np.all(s_got[9:] <= 1e-4 * s_got[0]) and np.all(s_ref[9:] <= 1e-9 * s_ref[0])
+    assert max_principal_angle(ref["components"].T[:, :9], got["components"].T[:, :9].astype(np.float64)) <= 1e-3
+    np.testing.assert_allclose(got["variance_ratio"][:9], ref["variance_ratio"][:9], rtol=2e-4)
+    assert np.all(np.isfinite(got["embeddings"])) and np.all(np.isfinite(got["components"]))

@@ -62,6 +62,22 @@ def test_pca_cpu_route_is_not_silently_served():
         accel.pca(_adata(), device="cpu")
 
 
+@pytest.mark.parametrize("kw,exc,msg", [
+    (dict(alpha=-1.0), ValueError, "pflog: alpha must be positive and finite, got -1.0"),
+    (dict(alpha=float("nan")), ValueError, "pflog: alpha must be positive and finite"),
+    (dict(store="x"), ValueError, 'pflog: unknown store \'x\'; expected "pca", "baseline", "dense", or "all"'),
+    (dict(store_repr="csr"), ValueError, "pflog: unknown store_repr"),
+    (dict(out="z.scx"), ValueError, "pflog: out= streams the materialized transform to disk"),
+    (dict(layer="counts"), ValueError, "layer 'counts' not found in adata.layers"),
+    (dict(store="dense"), NotImplementedError, "dense materialization of Z is not part of the GPU path"),
+    (dict(), RuntimeError, "GPU route of pflog only"),
+])
+def test_pflog_argument_validation(kw, exc, msg):
+    # pyscx/src/accel/pflog.rs:113-174: checked in this order, before any device work
+    with pytest.raises(exc, match=re.escape(msg)):
+        accel.pflog(_adata(), device="cpu", **kw)
+
+
 def test_lazy_chain_matches_oracle_transform():
     c = synth.random_csr(200, 60, 0.2, seed=2)
     ad = AnnDataLite(c.to_scipy())
