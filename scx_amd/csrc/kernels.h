@@ -74,8 +74,9 @@ __host__ __device__ __forceinline__ float log1p_fast(float x) {
 // PFlog: the Scale{4a} -> Log1p chain (transforms.rs:109-128) is the same
 // expression with f = 4a; the in-memory builder (pyscx/src/accel/pflog.rs:
 // 375-397) is (4a * x as f64).ln_1p() as f32.
+template <bool LOGF64>  // LOGF64: the in-memory PFlog builder (kXfLogF64)
 __device__ __forceinline__ float xtilde(float v, double f, int flags) {
-    if (flags & kXfLogF64) return static_cast<float>(log1p(f * static_cast<double>(v)));
+    if constexpr (LOGF64) return static_cast<float>(log1p(f * static_cast<double>(v)));
     const float s = (flags & (kXfNorm | kXfScale)) ? static_cast<float>(static_cast<double>(v) * f) : v;
     return (flags & kXfLog1p) ? log1p_fast(s) : s;
 }
@@ -184,6 +185,9 @@ constexpr uint32_t kChunkMod = 512;
 // PFlog rows (rowb != null) also get the baseline
 // b_i = -(sum_j x~_ij) * (1 / n_vars) (pflog.rs:186-222), f64 lane partials
 // and a wave reduction.
+// PFLOG: the value checks (kXfCheck) and the baseline rowb; LOGF64: kXfLogF64
+// (separate instantiations: the f64 log costs the plain pass registers)
+template <bool PFLOG, bool LOGF64>
 __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict__ indptr,
                                                       const uint32_t *__restrict__ idx,
                                                       const float *__restrict__ val, int64_t n_obs,
@@ -194,7 +198,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                                                       double *__restrict__ rowb,
                                                       unsigned long long *__restrict__ errpos) {
     constexpr int U = 4;  // rows of up to U * 64 entries are loaded in one go
-    const double inv_d = 1.0 / static_cast<double>(n_vars > 0 ? n_vars : 1);
+    const double inv_d = PFLOG ? 1.0 / static_cast<double>(n_vars > 0 ? n_vars : 1) : 0.0;
     const int lane = threadIdx.x & 63;
     const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / kWave);
     int bad = 0;
@@ -239,7 +243,8 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
         if (lane == 0) rowb[r] = -dsum * inv_d;
     };
     auto factor = [&](int64_t r, double sum) {
-        if (flags & kXfScale) return target;
+        if constexpr (PFLOG)
+            if (flags & kXfScale) return target;
         double f = 1.0;
         if (flags & kXfNorm) {
             sum = wave_sum_dpp(sum);
@@ -279,16 +284,19 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             const int64_t p = p0 + lane;
             const bool ok = p < b;
             if (ok) {
-                const float d = xtilde(v[u], f, flags);
+                const float d = xtilde<LOGF64>(v[u], f, flags);
                 xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
-                dsum += static_cast<double>(d);
-                if (flags & kXfCheck) check_value(v[u], d, p);
+                if constexpr (PFLOG) {
+                    dsum += static_cast<double>(d);
+                    if (flags & kXfCheck) check_value(v[u], d, p);
+                }
             }
             scan(sr, a, b, p0, ok, c[u], prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
-        if (rowb) put_baseline(r, dsum);
+        if constexpr (PFLOG)
+            if (rowb) put_baseline(r, dsum);
     };
     auto long_row = [&](int64_t r, int64_t a, int64_t b) {
         double sum = 0.0;
@@ -304,16 +312,19 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             const uint32_t c = ok ? idx[p] : 0u;
             if (ok) {
                 const float v = val[p];
-                const float d = xtilde(v, f, flags);
+                const float d = xtilde<LOGF64>(v, f, flags);
                 xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
-                dsum += static_cast<double>(d);
-                if (flags & kXfCheck) check_value(v, d, p);
+                if constexpr (PFLOG) {
+                    dsum += static_cast<double>(d);
+                    if (flags & kXfCheck) check_value(v, d, p);
+                }
             }
             scan(sr, a, b, p0, ok, c, prev_last);
         }
         if (nsplit > 0) splits_tail(sr, a, b, prev_last);
-        if (rowb) put_baseline(r, dsum);
+        if constexpr (PFLOG)
+            if (rowb) put_baseline(r, dsum);
     };
     // two rows per wave at a time (r and r + wstride): both rows' loads are in
     // flight together, so each wave keeps twice the bytes outstanding
@@ -536,21 +547,34 @@ __global__ void __launch_bounds__(256) k_colsum(const T *__restrict__ A, const d
     }
 }
 
-// out = (sum x, sum x^2) over x[0..n), f64, one 1024-thread block: thread
-// partials over a fixed stride, then a fixed-order tree (deterministic).
-__global__ void __launch_bounds__(1024) k_sum_sq_f64(const double *__restrict__ x, int64_t n,
-                                                     double *__restrict__ out) {
-    __shared__ double rs[1024], rq[1024];
+// (sum x, sum x^2) over x[0..n), f64, deterministic: block b of the first
+// launch (gridDim.x = nparts) sums its contiguous chunk (thread partials over
+// a fixed stride, then a fixed-order tree) into part[b]; the second launch
+// (one block, x = part, n = nparts, part = null) reduces the parts, read as
+// (sum, sum of squares) pairs, into out.
+__global__ void __launch_bounds__(256) k_sum_sq_f64(const double *__restrict__ x, int64_t n,
+                                                    double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double rs[256], rq[256];
     double a = 0.0, q = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 1024) {
-        const double v = x[i];
-        a += v;
-        q += v * v;
+    if (part) {
+        const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+        const int64_t b0 = static_cast<int64_t>(blockIdx.x) * chunk;
+        const int64_t b1 = b0 + chunk < n ? b0 + chunk : n;
+        for (int64_t i = b0 + threadIdx.x; i < b1; i += 256) {
+            const double v = x[i];
+            a += v;
+            q += v * v;
+        }
+    } else {
+        for (int64_t i = threadIdx.x; i < n; i += 256) {
+            a += x[2 * i];
+            q += x[2 * i + 1];
+        }
     }
     rs[threadIdx.x] = a;
     rq[threadIdx.x] = q;
     __syncthreads();
-    for (int w = 512; w > 0; w >>= 1) {
+    for (int w = 128; w > 0; w >>= 1) {
         if (static_cast<int>(threadIdx.x) < w) {
             rs[threadIdx.x] += rs[threadIdx.x + w];
             rq[threadIdx.x] += rq[threadIdx.x + w];
@@ -558,8 +582,56 @@ __global__ void __launch_bounds__(1024) k_sum_sq_f64(const double *__restrict__ 
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        out[0] = rs[0];
-        out[1] = rq[0];
+        double *o = part ? part + 2 * blockIdx.x : out;
+        o[0] = rs[0];
+        o[1] = rq[0];
+    }
+}
+
+// part[blk][j] = sum over the block's rows r of w[r] * A[r][j] (w == null:
+// 1), f64, for a tall f32 panel A (nrows x KP): 16-byte loads (KP / 4 lanes per
+// row), four rows per lane in flight, the row lanes reduced in a fixed order.
+template <int KP>
+__global__ void __launch_bounds__(256) k_wcolsum4(const float *__restrict__ A, const double *__restrict__ w,
+                                                  int64_t nrows, int64_t rows_per_blk, double *__restrict__ part) {
+    constexpr int CG = KP / 4, RL = 256 / CG;
+    __shared__ double red[RL][KP];
+    const int cg = threadIdx.x % CG, rl = threadIdx.x / CG;
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * rows_per_blk;
+    const int64_t r1 = r0 + rows_per_blk < nrows ? r0 + rows_per_blk : nrows;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    int64_t r = r0 + rl;
+    for (; r + 3 * RL < r1; r += 4 * RL) {
+        float4 v[4];
+        double ww[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[u] = *reinterpret_cast<const float4 *>(A + (r + u * RL) * KP + 4 * cg);
+            ww[u] = w ? w[r + u * RL] : 1.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            acc[0] += ww[u] * static_cast<double>(v[u].x);
+            acc[1] += ww[u] * static_cast<double>(v[u].y);
+            acc[2] += ww[u] * static_cast<double>(v[u].z);
+            acc[3] += ww[u] * static_cast<double>(v[u].w);
+        }
+    }
+    for (; r < r1; r += RL) {
+        const float4 v = *reinterpret_cast<const float4 *>(A + r * KP + 4 * cg);
+        const double ww = w ? w[r] : 1.0;
+        acc[0] += ww * static_cast<double>(v.x);
+        acc[1] += ww * static_cast<double>(v.y);
+        acc[2] += ww * static_cast<double>(v.z);
+        acc[3] += ww * static_cast<double>(v.w);
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[rl][4 * cg + e] = acc[e];
+    __syncthreads();
+    if (threadIdx.x < KP) {
+        double t = 0.0;
+        for (int i = 0; i < RL; ++i) t += red[i][threadIdx.x];
+        part[static_cast<int64_t>(blockIdx.x) * KP + threadIdx.x] = t;
     }
 }
 
@@ -592,7 +664,7 @@ __global__ void __launch_bounds__(256) k_colsum_reduce(const double *__restrict_
 // summed with cross-lane shuffles.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool BF16>
+template <int KP, bool BF16, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j] (PFlog)
 __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ idx,
                                                   const float *__restrict__ xt,
@@ -642,7 +714,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
         }
         if (g == 0) {
             float4 *yr = reinterpret_cast<float4 *>(Y + r * KP + l * E);
-            const float br = rowb ? static_cast<float>(rowb[r]) : 0.f;
+            const float br = OFFSET ? static_cast<float>(rowb[r]) : 0.f;
 #pragma unroll
             for (int e = 0; e < E / 4; ++e) {
                 float4 o;
@@ -650,7 +722,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
                 o.y = acc[4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
                 o.z = acc[4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
                 o.w = acc[4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
-                if (rowb) {
+                if constexpr (OFFSET) {
                     o.x += br * cs[l * E + 4 * e + 0];
                     o.y += br * cs[l * E + 4 * e + 1];
                     o.z += br * cs[l * E + 4 * e + 2];
@@ -970,7 +1042,7 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
 // the four group partials of a row are summed and the row is written once.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool BF16, int R, int NW, int CG, int PF>
+template <int KP, bool BF16, int R, int NW, int CG, int PF, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j]
 __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict__ indptr,
                                                        const uint16_t *__restrict__ lg,
                                                        const float *__restrict__ xt,
@@ -1114,7 +1186,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
         }
         if (row < n_obs && g == 0) {
             float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
-            const float br = rowb ? static_cast<float>(rowb[row]) : 0.f;
+            const float br = OFFSET ? static_cast<float>(rowb[row]) : 0.f;
 #pragma unroll
             for (int e = 0; e < E / 4; ++e) {
                 float4 o;
@@ -1122,7 +1194,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
                 o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
                 o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
                 o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
-                if (rowb) {
+                if constexpr (OFFSET) {
                     o.x += br * cs[l * E + 4 * e + 0];
                     o.y += br * cs[l * E + 4 * e + 1];
                     o.z += br * cs[l * E + 4 * e + 2];
@@ -2414,8 +2486,11 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
 // i > j with R[j][i] read back as a broadcast (measured faster than 63
 // readlanes per step). Lanes c < i keep updating their lower-triangle
 // registers (never read), which keeps the code branch-free. k_trinv_w then
-// inverts R. The pivot rule is k_chol2's (floor_tau).
+// inverts R. The pivot rule is k_chol2's; FLOOR selects the floor (per step
+// a max with the floor, non-finite values checked on the input and on the
+// final diagonal: per-step compares would cost scalar registers).
 // ---------------------------------------------------------------------------
+template <bool FLOOR>
 __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int k, double shift,
                                               double *__restrict__ R64, int *__restrict__ status,
                                               double floor_tau) {
@@ -2426,7 +2501,7 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
     const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
     const double sh = (shift > 0.0 && c < k) ? shift * tr : 0.0;
     const double mdiag = k > 0 ? (tr + (shift > 0.0 ? shift * tr * k : 0.0)) / k : 0.0;
-    const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // pivot floor
+    const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // FLOOR: the pivot floor
     // masked, shifted Gram staged row-major in LDS (eight rows' loads in
     // flight: the registers of a[] are not live yet), then one unconditional
     // read per register
@@ -2440,16 +2515,20 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
 #pragma unroll
     for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c];
     wave_lds_sync();
-    bool bad = false, floored = false;
+    bool bad = false;
+    double rdiag = 1.0;  // FLOOR: R[c][c], kept by lane c
+    if constexpr (FLOOR) {
+        // non-finite input is checked once here and on the final diagonal, so
+        // the step needs no compare (per-step masks cost scalar registers)
+#pragma unroll
+        for (int i = 0; i < KP; ++i) bad |= !isfinite(a[i]);
+    }
 #pragma unroll
     for (int j = 0; j < KP; ++j) {
         const double d = readlane_f64(a[j], j);
         double dd;  // select the input, not the result: no branch
-        if (floor_tau > 0.0) {
-            const bool fin = isfinite(d);
-            bad |= !fin;
-            floored |= fin && !(d > fl);
-            dd = (fin && d > fl) ? d : (fin ? fl : 1.0);
+        if constexpr (FLOOR) {
+            dd = fmax(d, fl);
         } else {
             const bool ok = (d > 0.0) && isfinite(d);
             bad |= !ok;
@@ -2458,14 +2537,22 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
         const double ir = rsq_nr(dd);
         const double r = dd * ir;
         const double t = (c == j) ? r : a[j] * ir;
+        if constexpr (FLOOR) rdiag = (c == j) ? r : rdiag;
         a[j] = t;
         rowb[j & 1][c] = t;
         wave_lds_sync();
 #pragma unroll
         for (int i = j + 1; i < KP; ++i) a[i] -= rowb[j & 1][i] * t;
     }
+    if constexpr (FLOOR) {
+        // R[c][c] = sqrt(max(pivot, floor)): equal to sqrt(floor) where it floored
+        const bool mine = c < k;
+        bad |= mine && !isfinite(rdiag);
+        const bool floored = mine && rdiag * rdiag <= fl * (1.0 + 1e-12);
+        if (__ballot(floored) != 0 && c == 0) atomicOr(status, 4);
+        bad = __ballot(bad) != 0;
+    }
     if (bad && c == 0) atomicOr(status, 1);
-    if (floored && c == 0) atomicOr(status, 4);
 #pragma unroll
     for (int i = 0; i < KP; ++i) R64[i * KP + c] = (i <= c && c < k) ? a[i] : 0.0;
 }

@@ -298,19 +298,22 @@ int prepare(scxpca_matrix *m) {
     unsigned long long *errpos = reinterpret_cast<unsigned long long *>(m->gcnt.as<uint32_t>() + epw);
     HIPC(hipMemsetAsync(m->gcnt.p, 0, sizeof(uint32_t) * gwords, m->stream));
     const bool want_b = (m->flags & kXfScale) != 0;
+    constexpr int kSumParts = 256;  // blocks of the baseline sum
     if (want_b) {
         m->pfb.ensure(sizeof(double) * m->n_obs);
-        m->pfstat.ensure(sizeof(double) * 2);
+        m->pfstat.ensure(sizeof(double) * (2 + 2 * kSumParts));
         m->hpf.ensure(sizeof(double) * 2);
     }
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
     static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
-    k_prepare_rows<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
-                                                      m->use_split ? m->nsplit : 0,
-                                                      (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
-                                                      m->xt.as<float>(), m->lg.as<uint16_t>(),
-                                                      m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
-                                                      vflag, want_b ? m->pfb.as<double>() : nullptr, errpos);
+    auto prep = (m->flags & kXfLogF64) ? k_prepare_rows<true, true>
+                : (m->flags & (kXfCheck | kXfScale)) ? k_prepare_rows<true, false> : k_prepare_rows<false, false>;
+    prep<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
+                                            m->use_split ? m->nsplit : 0,
+                                            (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
+                                            m->xt.as<float>(), m->lg.as<uint16_t>(),
+                                            m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
+                                            vflag, want_b ? m->pfb.as<double>() : nullptr, errpos);
     HIPC(hipGetLastError());
     if (m->use_split)
     {
@@ -336,7 +339,10 @@ int prepare(scxpca_matrix *m) {
         HIPC(hipGetLastError());
     }
     if (want_b) {  // sum and sum of squares of the baseline (fixed order)
-        k_sum_sq_f64<<<1, 1024, 0, m->stream>>>(m->pfb.as<double>(), m->n_obs, m->pfstat.as<double>());
+        double *parts = m->pfstat.as<double>() + 2;
+        k_sum_sq_f64<<<kSumParts, 256, 0, m->stream>>>(m->pfb.as<double>(), m->n_obs, parts, nullptr);
+        HIPC(hipGetLastError());
+        k_sum_sq_f64<<<1, 256, 0, m->stream>>>(parts, kSumParts, nullptr, m->pfstat.as<double>());
         HIPC(hipGetLastError());
         HIPC(hipMemcpyAsync(m->hpf.p, m->pfstat.p, sizeof(double) * 2, hipMemcpyDeviceToHost, m->stream));
     }
@@ -707,9 +713,9 @@ struct Engine {
     // out = w^T A over the rows of a tall f32 panel A (nrows x KP), f64;
     // w == null: plain column sums
     void weighted_colsum(const float *A, const double *w, int64_t nrows, double *out64, float *out32) {
-        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nrows + 15) / 16, kMaxColBlocks)));
+        const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nrows + 63) / 64, kMaxColBlocks)));
         const int64_t rpb = (nrows + nb - 1) / nb;
-        k_colsum<KP, float><<<nb, 256, 0, s>>>(A, w, nrows, rpb, m->cpart.as<double>());
+        k_wcolsum4<KP><<<nb, 256, 0, s>>>(A, w, nrows, rpb, m->cpart.as<double>());
         check();
         k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, out64, out32);
         check();
@@ -726,34 +732,40 @@ struct Engine {
         check();
     }
 
-    // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
-    void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
-        if (m->n_obs == 0) return;
-        if (rowb) weighted_colsum(panel, nullptr, m->n_vars, cs64, cs32);
-        StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
-        const float *mcp = centered ? mc32 : nullptr;
+    template <bool OFF>
+    void launch_fwd(const float *panel, const float *mcp, bool bf16, float *Y) {
         if (m->use_split) {
             constexpr int NWF = 16;
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
             if (bf16)
-                k_spmm_fwd3<KP, true, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
+                k_spmm_fwd3<KP, true, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
                     m->indptr, m->lg.as<uint16_t>(), nullptr, m->xbf.as<uint16_t>(), panel, mcp, Y,
                     m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
             else
-                k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF><<<grid, NWF * 64, 0, s>>>(
+                k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
                     m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
                     m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
             if (bf16)
-                k_spmm_fwd<KP, true><<<grid, 256, 0, s>>>(m->indptr, m->idx, nullptr, m->xbf.as<uint16_t>(), panel,
+                k_spmm_fwd<KP, true, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, nullptr, m->xbf.as<uint16_t>(), panel,
                                                           mcp, Y, m->n_obs, rowb, cs32);
             else
-                k_spmm_fwd<KP, false><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
+                k_spmm_fwd<KP, false, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
                                                            mcp, Y, m->n_obs, rowb, cs32);
         }
+    }
+
+    // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
+    void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
+        if (m->n_obs == 0) return;
+        if (rowb) weighted_colsum(panel, nullptr, m->n_vars, cs64, cs32);
+        StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
+        const float *mcp = centered ? mc32 : nullptr;
+        if (rowb) launch_fwd<true>(panel, mcp, bf16, Y);
+        else launch_fwd<false>(panel, mcp, bf16, Y);
         check();
     }
 
@@ -798,7 +810,10 @@ struct Engine {
     void chol(double shift, double *Rdst, double floor_tau) {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
         if constexpr (KP == 64) {
-            k_chol_w<<<1, 64, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
+            if (floor_tau > 0.0)
+                k_chol_w<true><<<1, 64, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
+            else
+                k_chol_w<false><<<1, 64, 0, s>>>(G, k, shift, Rdst, status, 0.0);
             check();
             k_trinv_w<<<1, 64, 0, s>>>(Rdst, k, W64, W32);
         } else {
