@@ -159,6 +159,7 @@ PCA_CASES = {
     "cfg1_head_est_f64_q4": (lambda: cfg1_head(), None, F64, 50, 10, 4, True),
     "cfg1_head_uncentered": (lambda: cfg1_head(), None, F64, 50, 10, 2, False),
     "cfg1_head_lazy_pinned": (lambda: cfg1_head(), 0.2, LAZY, 40, 20, 3, True),
+    "cfg1_head_lazy_est": (lambda: cfg1_head(), None, LAZY, 50, 10, 2, True),
     "ragged_empty": (lambda: with_empty_rows(shape_case("ragged"), [0, 7, 100]), None, F64, 20, 10, 3, True),
     "wide3k_k100": (lambda: shape_case("wide3k"), None, F64, 100, 20, 3, True),
     "tiny": (lambda: shape_case("tiny"), None, F64, 5, 3, 3, True),
@@ -192,6 +193,33 @@ def test_pflog_pca_matches_oracle(gpu, case):
         np.testing.assert_allclose(got["mean"], ref["mean"], rtol=1e-6, atol=1e-12)
     else:
         assert got["mean"] is None
+
+
+def test_pflog_bf16_sketch_after_refresh(gpu):
+    """The BF16 sketch form on a PFlog matrix (its bf16 copies come from the
+    delta values), around a refresh. The sketch rounds delta to bf16 (~4e-3
+    relative), which at 3,000 cells moves the top-10 Rayleigh-Ritz sigma by up
+    to a few 1e-4 (measured 2.4e-4): bar 1e-3; it rotates the top-3 subspace
+    by a few 1e-3 rad (measured 5.3e-3): bar 2e-2. An FP32 call after the
+    refresh must meet the FP32 bars (1e-4, 1e-3 rad), which pins the
+    rewritten values."""
+    x = cfg1_head()
+    say("case: pflog bf16 sketch")
+    ref = oracle.pflog_pca(view(x), n_comps=50, n_oversamples=10, n_power_iterations=2, seed=0, alpha=None, form=F64)
+    m = make(gpu, x)
+    try:
+        for precision, bar, abar in ((gpu.PRECISION_BF16_SKETCH, 1e-3, 2e-2), (gpu.PRECISION_BF16_SKETCH, 1e-3, 2e-2),
+                                     (gpu.PRECISION_FP32, 1e-4, 1e-3)):
+            got = m.pca(n_comps=50, n_oversamples=10, n_power_iterations=2, seed=0, qr_method=1, precision=precision)
+            se = sigma_rel_err(ref["sigma_all"][:10], got["sigma"][:10])
+            say(f"pflog precision={precision}: sigma rel err top10 {se.max():.2e}")
+            assert se.max() <= bar
+            ang = max_principal_angle(ref["components"].T[:, :3], got["components"].T[:, :3].astype(np.float64))
+            say(f"pflog precision={precision}: angle top3 {ang:.2e}")
+            assert ang <= abar
+            m.refresh()
+    finally:
+        m.close()
 
 
 @pytest.mark.parametrize("zero_center,alpha", [(True, 1.0), (False, 1.0), (True, 0.5)])
