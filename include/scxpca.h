@@ -103,6 +103,16 @@ typedef struct scxpca_matrix scxpca_matrix;
 scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p,
                                     scxpca_result *out, int device);
 
+/* Streamed: the same PCA for a host CSR larger than HBM (on_device = 0; rows
+ * sorted, no repeated column in a row). Replaces the StreamingPcaOperator
+ * route (scx-gpu/src/linear_operator.rs:61-266, pinned staging
+ * scx-gpu/src/staging.rs): every pass moves the CSR over PCIe in shards of
+ * shard_rows rows through two device slots (pinned staging, an asynchronous
+ * copy stream overlapping the next shard's transfer with the current shard's
+ * products); HBM holds two shards plus the n_obs x KP panels. FP32 only. */
+scxpca_status scxpca_randomized_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard_rows,
+                                             scxpca_result *out, int device);
+
 /* Resident matrix: keep the CSR (and its per-row transform factors) in HBM
  * across calls. Replaces try_build_resident_csr / ResidentPcaOperator
  * (scx-gpu/src/gpu_pca_resident.rs:87-283). target_sum/log1p fix X~. */

@@ -30,6 +30,7 @@ STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm
 # Every symbol include/scxpca.h declares.
 EXPORTS = (
     "scxpca_randomized_pca",
+    "scxpca_randomized_pca_streamed",
     "scxpca_matrix_create",
     "scxpca_matrix_create_pflog",
     "scxpca_matrix_pflog_fit",
@@ -115,6 +116,8 @@ def lib() -> ctypes.CDLL:
     l.scxpca_last_error.restype = ctypes.c_size_t
     l.scxpca_last_error.argtypes = [ctypes.c_char_p, ctypes.c_size_t]
     l.scxpca_randomized_pca.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(Params), ctypes.POINTER(Result), ctypes.c_int]
+    l.scxpca_randomized_pca_streamed.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(Params), _i64,
+                                                 ctypes.POINTER(Result), ctypes.c_int]
     l.scxpca_matrix_create.argtypes = [ctypes.POINTER(Csr), _dbl, _i32, ctypes.c_int, ctypes.POINTER(_vp)]
     l.scxpca_matrix_create_pflog.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(PflogParams), ctypes.c_int,
                                              ctypes.POINTER(_vp)]
@@ -326,6 +329,35 @@ class Matrix:
 
     def project_backward(self, dz, k, mean, dw, stream=None):
         check(lib().scxpca_project_backward(self.handle, ptr(dz), _i64(k), ptr(mean), ptr(dw), _vp(stream or 0)))
+
+
+def randomized_pca_streamed(indptr, indices, data, shape, shard_rows, n_comps=50, n_oversamples=10,
+                            n_power_iterations=2, zero_center=True, seed=0, qr_method=QR_HOUSEHOLDER,
+                            target_sum=None, log1p=False, device=0) -> dict:
+    """The randomized PCA of a host CSR streamed to the GPU in shards of
+    ``shard_rows`` rows on every pass (for matrices larger than HBM); the
+    result dict of ``Matrix.pca``."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    n, nv = int(shape[0]), int(shape[1])
+    csr = Csr(n, nv, ptr(indptr), ptr(indices), ptr(data), 0)
+    prm = Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), qr_method, seed,
+                 float(target_sum or 0.0), int(bool(log1p)), PRECISION_FP32)
+    k = max(1, min(n_comps + n_oversamples, n, nv))
+    var = np.zeros(max(n_comps, 1))
+    ratio = np.zeros(max(n_comps, 1))
+    mean = np.zeros(max(nv, 1))
+    sigma = np.zeros(k)
+    emb = np.empty((n, n_comps), dtype=np.float32)
+    comps = np.empty((n_comps, nv), dtype=np.float32)
+    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0)
+    check(lib().scxpca_randomized_pca_streamed(ctypes.byref(csr), ctypes.byref(prm), _i64(int(shard_rows)),
+                                               ctypes.byref(res), device))
+    return {"embeddings": emb, "components": comps, "variance": var[:n_comps], "variance_ratio": ratio[:n_comps],
+            "mean": mean[:nv] if zero_center else None, "sigma": sigma[: res.k], "k": int(res.k),
+            "total_variance": float(res.total_variance), "tv_recomputed": bool(res.tv_recomputed),
+            "seconds_device": float(res.seconds_device)}
 
 
 def qr(a: np.ndarray, qr_method: int = QR_HOUSEHOLDER, device: int = 0) -> np.ndarray:

@@ -456,6 +456,27 @@ const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
     return go.dev.as<uint32_t>();
 }
 
+// Layout choice and (grow-only) buffers of the derived data for the current
+// n_obs / n_vars / nnz of m, before a prepare.
+void size_derived(scxpca_matrix *m) {
+    m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
+    m->n_tiles = static_cast<int>((m->n_obs + kTileRows - 1) / kTileRows);
+    const double split_bytes = 4.0 * static_cast<double>(m->n_obs) * (m->nsplit + 1) +
+                               16.0 * static_cast<double>(m->n_tiles) * static_cast<double>(m->n_vars);
+    m->use_split = split_bytes <= split_max_bytes();
+    if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
+    if (m->use_split) m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
+    m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
+    m->trow.ensure(m->nnz + kEntryPad);
+    m->tval.ensure(sizeof(float) * (m->nnz + kEntryPad));
+    m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
+    m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
+    if (m->use_split) {
+        m->tsum.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
+        m->tsq.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
+    }
+}
+
 int transform_flags(double target, int log1p) { return (target > 0.0 ? kXfNorm : 0) | (log1p ? kXfLog1p : 0); }
 
 // Upload (or borrow) the CSR and prepare it under the transform (target,
@@ -505,22 +526,7 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flag
         m->val = m->own_val.as<float>();
     }
     if (m->n_obs == 0) return;
-    m->nsplit = static_cast<int>((m->n_vars + kCB - 1) / kCB);
-    m->n_tiles = static_cast<int>((m->n_obs + kTileRows - 1) / kTileRows);
-    const double split_bytes = 4.0 * static_cast<double>(m->n_obs) * (m->nsplit + 1) +
-                               16.0 * static_cast<double>(m->n_tiles) * static_cast<double>(m->n_vars);
-    m->use_split = split_bytes <= split_max_bytes();
-    if (m->flags & kXfNorm) m->rowf.ensure(sizeof(double) * m->n_obs);
-    if (m->use_split) m->spl.ensure(sizeof(uint32_t) * m->n_obs * (m->nsplit + 1));
-    m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
-    m->trow.ensure(m->nnz + kEntryPad);
-    m->tval.ensure(sizeof(float) * (m->nnz + kEntryPad));
-    m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
-    m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
-    if (m->use_split) {
-        m->tsum.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
-        m->tsq.ensure(sizeof(double) * static_cast<size_t>(m->n_tiles) * std::max<int64_t>(m->n_vars, 1));
-    }
+    size_derived(m);
     int h = prepare(m);
     if ((h & 1) && !x->on_device) {
         // canonicalize host rows (stable sort by index) and upload again
@@ -912,8 +918,11 @@ struct Engine {
     // Total variance with the cancellation guard (guarded_total_variance_*,
     // cpu.rs:1029-1092) from the column sums and sums of squares; hmu gets
     // the column means.
+    // stored_ss() returns sum over the stored entries of (x~ - mu)^2 (the
+    // stable recompute's device pass)
+    template <class StoredSS>
     double total_variance(bool centered, const std::vector<double> &hsum, const std::vector<double> &hsq,
-                          std::vector<double> &hmu, int &recomputed) {
+                          std::vector<double> &hmu, int &recomputed, StoredSS &&stored_ss) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         hmu.assign(nv, 0.0);
         for (int64_t v = 0; v < nv; ++v) hmu[v] = hsum[v] / static_cast<double>(n);
@@ -951,19 +960,10 @@ struct Engine {
             if (sumsq > 0.0 && (sumsq - msq) <= 1e-7 * sumsq) {
                 // cancelled: stable centered recompute on the device
                 recomputed = 1;
-                const int nb = 1024;
-                DevBuf part;
-                part.ensure(sizeof(double) * nb);
-                k_centered_ss<<<nb, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), mu, n,
-                                                 part.as<double>());
-                check();
-                std::vector<double> hp(nb);
+                double t = stored_ss();
                 std::vector<uint32_t> hcnt(nv);
-                HIPC(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
                 HIPC(hipMemcpyAsync(hcnt.data(), m->cnt.p, sizeof(uint32_t) * nv, hipMemcpyDeviceToHost, s));
                 HIPC(hipStreamSynchronize(s));
-                double t = 0.0;
-                for (int i = 0; i < nb; ++i) t += hp[i];
                 for (int64_t v = 0; v < nv; ++v) {
                     const double zeros = static_cast<double>(std::max<int64_t>(n - static_cast<int64_t>(hcnt[v]), 0));
                     t += zeros * hmu[v] * hmu[v];
@@ -974,36 +974,73 @@ struct Engine {
         return tv;
     }
 
+    // sum over the stored entries of (x~ - mu)^2 on a resident matrix mm (f64
+    // block partials in a fixed order)
+    double stored_centered_ss(scxpca_matrix *mm, const double *mu_dev) {
+        const int nb = 1024;
+        DevBuf part;
+        part.ensure(sizeof(double) * nb);
+        k_centered_ss<<<nb, 256, 0, s>>>(mm->indptr, mm->idx, mm->xt.as<float>(), mu_dev, mm->n_obs,
+                                         part.as<double>());
+        check();
+        std::vector<double> hp(nb);
+        HIPC(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        HIPC(hipStreamSynchronize(s));
+        double t = 0.0;
+        for (int i = 0; i < nb; ++i) t += hp[i];
+        return t;
+    }
+
+    // The operator of a resident matrix: its statistics, SpMMs and stable
+    // recompute on the matrix itself.
+    struct ResidentOps {
+        Engine *e;
+        void stats() { ensure_stats(e->m); }
+        void fwd(const float *panel, bool centered, bool bf16, float *Y) { e->spmm_fwd(panel, centered, bf16, Y); }
+        void t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
+            e->spmm_t(Q, centered, bf16, B64, B32);
+        }
+        double stored_ss() { return e->stored_centered_ss(e->m, e->mu); }
+    };
+
     void run(const scxpca_params *p, scxpca_result *out) {
+        ResidentOps ops{this};
+        run_with(ops, p, out);
+    }
+
+    // The power loop, Rayleigh-Ritz and results over an operator (ResidentOps,
+    // or the streamed shards of a StreamOps).
+    template <class Ops>
+    void run_with(Ops &ops, const scxpca_params *p, scxpca_result *out) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         const bool centered = p->zero_center != 0;
         const bool sketch_bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
         if (sketch_bf16) ensure_xbf(m);
-        ensure_stats(m);
+        ops.stats();
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), baseline_mean(), mu);
         check();
         // Omega -> f32 panel, generated on the device
         device_omega<KP>(s, static_cast<uint64_t>(nv), k, p->seed, m->panel.as<float>(), nullptr, m->om);
         float *Y = m->y.as<float>(), *T = m->q.as<float>();
         if (centered) mean_correction(m->panel.as<float>());
-        spmm_fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
+        ops.fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
         const int64_t n_it = p->n_power_iterations;
         float *Q = cqr_tall(Y, T, p->qr_method, -1, n_it == 0);
         float *Qother = (Q == Y) ? T : Y;
         for (int64_t it = 0; it < n_it; ++it) {
             // B (f32 n_vars x KP) -> basis by the tall MFMA CholeskyQR
-            spmm_t(Q, centered, sketch_bf16, nullptr, m->panel.as<float>());
+            ops.t(Q, centered, sketch_bf16, nullptr, m->panel.as<float>());
             float *qb = cqr_tall(m->panel.as<float>(), m->panel2.as<float>(), p->qr_method, nv, false);
             if (centered) mean_correction(qb);
-            spmm_fwd(qb, centered, sketch_bf16, Qother);
+            ops.fwd(qb, centered, sketch_bf16, Qother);
             float *Qn = cqr_tall(Qother, Q, p->qr_method, -1, it + 1 == n_it);
             Qother = (Qn == Q) ? Qother : Q;
             Q = Qn;
         }
         // Final B pass in FP32 (also in the BF16 sketch form).
-        spmm_t(Q, centered, false, m->b64.as<double>(), nullptr);
+        ops.t(Q, centered, false, m->b64.as<double>(), nullptr);
         // Rayleigh-Ritz: B = Q_B Rt, Rt = U_R S V^T
         cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
         {
@@ -1065,7 +1102,7 @@ struct Engine {
         // total variance with the cancellation guard (cpu.rs:1029-1092)
         std::vector<double> hmu;
         int recomputed = 0;
-        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed);
+        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed, [&] { return ops.stored_ss(); });
         const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
         for (int j = 0; j < nc; ++j) {
             const double var = hsig[j] * hsig[j] / denom;
@@ -1184,7 +1221,8 @@ struct Engine {
         if (hinfo != 0) throw Error(SCXPCA_ERR_LINALG, "Eigendecomposition failed: the symmetric eigensolver did not converge");
         std::vector<double> hmu;
         int recomputed = 0;
-        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed);
+        const double tv = total_variance(centered, hsum, hsq, hmu, recomputed,
+                                         [&] { return stored_centered_ss(m, mu); });
         const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
         for (int j = 0; j < nc; ++j) {
             if (out->variance) out->variance[j] = hvar[j];
@@ -1574,6 +1612,246 @@ void refresh(scxpca_matrix *m) {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Streaming operator for a CSR larger than HBM (StreamingPcaOperator,
+// scx-gpu/src/linear_operator.rs:61-266; PinnedCsrSlot / GpuCsrSlot,
+// scx-gpu/src/staging.rs). The CSR stays in host memory; every pass of the
+// power loop (the statistics, each forward and transposed product, the
+// stable total-variance recompute) drives it shard by shard through two
+// device slots. A shard's rows go host -> pinned staging (a host copy) ->
+// HBM (an asynchronous copy on a copy stream), so the next shard's transfer
+// runs while the current shard is prepared and multiplied on the compute
+// stream. Each shard is a matrix of its own (the resident prepare: X~, the
+// split offsets, the tile-CSC) whose products write the shard's rows of the
+// global Y, read the shard's rows of the global Q, and add their n_vars x KP
+// f64 transposed contribution into one accumulator; the centering
+// B = sum_s X~_s^T Q_s - mu (1^T Q) is applied once, after the last shard.
+// Nothing is cached between passes: every pass moves the whole CSR over PCIe,
+// as the reference's streaming operator re-uploads on every multiply.
+// ---------------------------------------------------------------------------
+
+__global__ void k_rebase(int64_t *ip, int64_t n, int64_t base) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) ip[i] -= base;
+}
+
+__global__ void k_add_f64(double *acc, const double *x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] += x[i];
+}
+
+__global__ void k_add_u32(uint32_t *acc, const uint32_t *x, int64_t n) {
+    const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) acc[i] += x[i];
+}
+
+// B[v][j] = acc[v][j] - mu[v] sumq[j] (mu null: no centering); B64 / B32 optional
+template <int KP>
+__global__ void k_center_b(const double *acc, const double *mu, const double *sumq, int64_t n_vars, double *B64,
+                           float *B32) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= n_vars * KP) return;
+    double v = acc[t];
+    if (mu) v -= mu[t / KP] * sumq[t % KP];
+    if (B64) B64[t] = v;
+    if (B32) B32[t] = static_cast<float>(v);
+}
+
+struct StreamSlot {
+    DevBuf ip, ix, dv;        // the shard's CSR in HBM (indptr rebased to 0)
+    PinnedBuf hip, hix, hdv;  // pinned staging
+    hipEvent_t copied = nullptr;
+    scxpca_matrix m;          // the shard matrix over ip / ix / dv (stream borrowed)
+    int64_t r0 = 0;
+    ~StreamSlot() {
+        if (copied) (void)hipEventDestroy(copied);
+        m.stream = nullptr;  // the run's compute stream, not the slot's
+    }
+};
+
+template <int KP>
+struct StreamOps {
+    Engine<KP> &eg;  // the engine of the shell matrix: global panels, QR, results
+    const scxpca_csr *x;
+    int64_t shard_rows, n, nv;
+    int nshards, k;
+    hipStream_t cs, copy = nullptr;
+    StreamSlot slot[2];
+    DevBuf acc;
+
+    StreamOps(Engine<KP> &e, const scxpca_csr *xx, int64_t rows_per_shard)
+        : eg(e), x(xx), shard_rows(rows_per_shard), n(xx->n_obs), nv(xx->n_vars), k(e.k), cs(e.s) {
+        nshards = static_cast<int>((n + shard_rows - 1) / shard_rows);
+        HIPC(hipStreamCreate(&copy));
+        for (auto &sl : slot) {
+            HIPC(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+            sl.m.stream = cs;
+            sl.m.device = eg.m->device;
+        }
+    }
+    ~StreamOps() {
+        if (copy) {
+            (void)hipStreamSynchronize(copy);
+            (void)hipStreamDestroy(copy);
+        }
+        (void)hipStreamSynchronize(cs);
+    }
+
+    // host rows of shard s -> pinned staging -> the slot's device CSR (copy stream)
+    void stage(int s) {
+        StreamSlot &sl = slot[s & 1];
+        const int64_t r0 = static_cast<int64_t>(s) * shard_rows, r1 = std::min(n, r0 + shard_rows);
+        const int64_t a = x->indptr[r0], nnz = x->indptr[r1] - a, rows = r1 - r0;
+        sl.hip.ensure(sizeof(int64_t) * (rows + 1));
+        sl.hix.ensure(sizeof(uint32_t) * std::max<int64_t>(nnz, 1));
+        sl.hdv.ensure(sizeof(float) * std::max<int64_t>(nnz, 1));
+        std::memcpy(sl.hip.p, x->indptr + r0, sizeof(int64_t) * (rows + 1));
+        std::memcpy(sl.hix.p, x->indices + a, sizeof(uint32_t) * nnz);
+        std::memcpy(sl.hdv.p, x->data + a, sizeof(float) * nnz);
+        sl.ip.ensure(sizeof(int64_t) * (rows + 1));
+        sl.ix.ensure(sizeof(uint32_t) * (nnz + kEntryPad));
+        sl.dv.ensure(sizeof(float) * (nnz + kEntryPad));
+        HIPC(hipMemcpyAsync(sl.ip.p, sl.hip.p, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice, copy));
+        if (nnz) {
+            HIPC(hipMemcpyAsync(sl.ix.p, sl.hix.p, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, copy));
+            HIPC(hipMemcpyAsync(sl.dv.p, sl.hdv.p, sizeof(float) * nnz, hipMemcpyHostToDevice, copy));
+        }
+        HIPC(hipEventRecord(sl.copied, copy));
+    }
+
+    // the staged shard s as a prepared matrix (the prepare waits on the compute stream)
+    StreamSlot &load(int s) {
+        StreamSlot &sl = slot[s & 1];
+        const int64_t r0 = static_cast<int64_t>(s) * shard_rows, r1 = std::min(n, r0 + shard_rows);
+        const int64_t a = x->indptr[r0];
+        HIPC(hipStreamWaitEvent(cs, sl.copied, 0));
+        k_rebase<<<blocks_for(r1 - r0 + 1), 256, 0, cs>>>(sl.ip.as<int64_t>(), r1 - r0 + 1, a);
+        HIPC(hipGetLastError());
+        scxpca_matrix *m = &sl.m;
+        m->n_obs = r1 - r0;
+        m->n_vars = nv;
+        m->nnz = x->indptr[r1] - a;
+        m->indptr = sl.ip.as<int64_t>();
+        m->idx = sl.ix.as<uint32_t>();
+        m->val = sl.dv.as<float>();
+        m->target = eg.m->target;
+        m->flags = eg.m->flags;
+        size_derived(m);
+        throw_invalid(m, prepare(m));
+        sl.r0 = r0;
+        return sl;
+    }
+
+    // f(slot) for every shard in order; shard s + 1 is staged while s computes
+    template <class F>
+    void each_shard(F &&f) {
+        stage(0);
+        for (int sh = 0; sh < nshards; ++sh) {
+            StreamSlot &sl = load(sh);
+            f(sl);
+            if (sh + 1 < nshards) stage(sh + 1);
+        }
+    }
+
+    void stats() {
+        scxpca_matrix *g = eg.m;
+        g->sum.ensure(sizeof(double) * nv);
+        g->sq.ensure(sizeof(double) * nv);
+        g->cnt.ensure(sizeof(uint32_t) * nv);
+        HIPC(hipMemsetAsync(g->sum.p, 0, sizeof(double) * nv, cs));
+        HIPC(hipMemsetAsync(g->sq.p, 0, sizeof(double) * nv, cs));
+        HIPC(hipMemsetAsync(g->cnt.p, 0, sizeof(uint32_t) * nv, cs));
+        each_shard([&](StreamSlot &sl) {
+            ensure_stats(&sl.m);
+            k_add_f64<<<blocks_for(nv), 256, 0, cs>>>(g->sum.as<double>(), sl.m.sum.as<double>(), nv);
+            k_add_f64<<<blocks_for(nv), 256, 0, cs>>>(g->sq.as<double>(), sl.m.sq.as<double>(), nv);
+            k_add_u32<<<blocks_for(nv), 256, 0, cs>>>(g->cnt.as<uint32_t>(), sl.m.cnt.as<uint32_t>(), nv);
+            HIPC(hipGetLastError());
+        });
+        g->stats_ready = true;
+    }
+
+    // Y = X~ panel - 1 mc^T, shard by shard into the shard's rows of Y
+    void fwd(const float *panel, bool centered, bool bf16, float *Y) {
+        (void)bf16;
+        each_shard([&](StreamSlot &sl) {
+            Engine<KP> es(&sl.m, k);
+            if (centered) HIPC(hipMemcpyAsync(es.mc32, eg.mc32, sizeof(float) * KP, hipMemcpyDeviceToDevice, cs));
+            es.spmm_fwd(panel, centered, false, Y + sl.r0 * KP);
+        });
+    }
+
+    // B = sum_s X~_s^T Q_s - mu (1^T Q)
+    void t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
+        (void)bf16;
+        acc.ensure(sizeof(double) * nv * KP);
+        HIPC(hipMemsetAsync(acc.p, 0, sizeof(double) * nv * KP, cs));
+        each_shard([&](StreamSlot &sl) {
+            Engine<KP> es(&sl.m, k);
+            es.spmm_t(Q + sl.r0 * KP, false, false, sl.m.b64.as<double>(), nullptr);
+            k_add_f64<<<blocks_for(nv * KP), 256, 0, cs>>>(acc.as<double>(), sl.m.b64.as<double>(), nv * KP);
+            HIPC(hipGetLastError());
+        });
+        if (centered) eg.weighted_colsum(Q, nullptr, n, eg.sumq, nullptr);
+        k_center_b<KP><<<blocks_for(nv * KP), 256, 0, cs>>>(acc.as<double>(), centered ? eg.mu : nullptr, eg.sumq, nv,
+                                                            B64, B32);
+        HIPC(hipGetLastError());
+    }
+
+    double stored_ss() {
+        double t = 0.0;
+        each_shard([&](StreamSlot &sl) { t += eg.stored_centered_ss(&sl.m, eg.mu); });
+        return t;
+    }
+};
+
+void run_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard_rows, scxpca_result *out,
+                      int device) {
+    if (x->on_device) throw Error(SCXPCA_ERR_INVALID, "the streamed PCA reads a host CSR (on_device = 0)");
+    if (shard_rows <= 0) throw Error(SCXPCA_ERR_INVALID, "shard_rows must be positive");
+    if (p->precision != SCXPCA_PRECISION_FP32)
+        throw Error(SCXPCA_ERR_INVALID, "the streamed PCA computes in FP32 (precision must be FP32)");
+    if (x->n_obs < 0 || x->n_vars < 0) throw Error(SCXPCA_ERR_INVALID, "matrix shape must be non-negative");
+    if (x->n_vars > 0xffffffffll) throw Error(SCXPCA_ERR_INVALID, "n_vars exceeds the u32 index range");
+    if (x->n_obs > 0 && x->indptr[0] != 0) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
+    for (int64_t r = 0; r < x->n_obs; ++r)
+        if (x->indptr[r + 1] < x->indptr[r]) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
+    DeviceGuard dg(device);
+    scxpca_matrix shell;  // global panels and statistics; no CSR of its own
+    shell.device = device;
+    HIPC(hipStreamCreate(&shell.stream));
+    shell.n_obs = x->n_obs;
+    shell.n_vars = x->n_vars;
+    shell.nnz = x->n_obs > 0 ? x->indptr[x->n_obs] : 0;
+    shell.target = p->target_sum;
+    shell.flags = transform_flags(p->target_sum, p->log1p);
+    validate(&shell, p);
+    const int64_t k = pca_k(&shell, p);
+    if (k > 128)
+        throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples = " + std::to_string(k) +
+                                            " exceeds the GPU panel limit of 128");
+    hipEvent_t e0, e1;
+    HIPC(hipEventCreate(&e0));
+    HIPC(hipEventCreate(&e1));
+    HIPC(hipEventRecord(e0, shell.stream));
+    if (k <= 64) {
+        Engine<64> eg(&shell, static_cast<int>(k));
+        StreamOps<64> ops(eg, x, shard_rows);
+        eg.run_with(ops, p, out);
+    } else {
+        Engine<128> eg(&shell, static_cast<int>(k));
+        StreamOps<128> ops(eg, x, shard_rows);
+        eg.run_with(ops, p, out);
+    }
+    HIPC(hipEventRecord(e1, shell.stream));
+    HIPC(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPC(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    out->seconds_device = ms * 1e-3;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
 
@@ -1615,6 +1893,14 @@ scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, d
             HIPC(hipMemcpyAsync(baseline, m->pfb.p, sizeof(double) * m->n_obs, hipMemcpyDeviceToHost, m->stream));
             HIPC(hipStreamSynchronize(m->stream));
         }
+    });
+}
+
+scxpca_status scxpca_randomized_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard_rows,
+                                             scxpca_result *out, int device) {
+    return guarded([&] {
+        if (!x || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        run_pca_streamed(x, p, shard_rows, out, device);
     });
 }
 

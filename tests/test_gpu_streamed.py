@@ -1,0 +1,134 @@
+"""GPU streamed PCA (scxpca_randomized_pca_streamed: the StreamingPcaOperator
+route, scx-gpu/src/linear_operator.rs:61-266, for a host CSR larger than HBM)
+against the resident matrix and the oracle on the same inputs.
+
+The streamed route runs the resident kernels per shard, so against the
+resident result only reduction order differs (per-shard f64 sums of the
+transposed partials, per-shard column statistics): singular values to 1e-5
+relative, the mean to 1e-12, the variance ratio to 1e-5, the top-3 principal
+angle to 1e-4 rad. Against the oracle the randomized path's bars (1e-4, 1e-3
+rad). Shard sizes cover one shard, uneven shards, shards of one 256-row tile
+and shards smaller than a tile.
+"""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle
+from scx_amd import synth
+from tests.pca_metrics import max_principal_angle, sigma_rel_err
+from tests.test_gpu_shapes import MATRICES
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def cfg1_head(n=3000):
+    x = synth.generate(synth.CFG1, n_obs=n, workers=1)
+    return sp.csr_matrix((x.data, x.indices.astype(np.int64), x.indptr), shape=x.shape)
+
+
+def shape_case(name):
+    c = MATRICES[name]()
+    return sp.csr_matrix((c.data, c.indices.astype(np.int64), c.indptr), shape=c.shape)
+
+
+def arrays(x):
+    return x.indptr.astype(np.int64), x.indices.astype(np.uint32), x.data.astype(np.float32), x.shape
+
+
+CASES = {
+    "cfg1_one_shard": (cfg1_head, 3000, 50, 10, 2, True, 1),
+    "cfg1_uneven_700": (cfg1_head, 700, 50, 10, 2, True, 1),
+    "cfg1_tile_256": (cfg1_head, 256, 50, 10, 3, True, 0),
+    "cfg1_small_97": (cfg1_head, 97, 20, 10, 2, True, 1),
+    "cfg1_uncentered_1000": (cfg1_head, 1000, 50, 10, 2, False, 1),
+    "wide3k_k100_400": (lambda: shape_case("wide3k"), 400, 100, 20, 3, True, 1),
+    "ragged_333": (lambda: shape_case("ragged"), 333, 20, 10, 3, True, 0),
+    "tiny_7": (lambda: shape_case("tiny"), 7, 5, 3, 3, True, 1),
+}
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_streamed_matches_resident_and_oracle(gpu, case):
+    make_x, shard, nc, p, q, centered, qr = CASES[case]
+    x = make_x()
+    ip, ix, dv, shape = arrays(x)
+    say(f"case: streamed {case} {shape} nnz={len(dv)} shard_rows={shard}")
+    got = gpu.randomized_pca_streamed(ip, ix, dv, shape, shard, n_comps=nc, n_oversamples=p, n_power_iterations=q,
+                                      zero_center=centered, seed=0, qr_method=qr, target_sum=1e4, log1p=True)
+    m = gpu.Matrix(ip, ix, dv, shape, target_sum=1e4, log1p=True)
+    try:
+        res = m.pca(n_comps=nc, n_oversamples=p, n_power_iterations=q, zero_center=centered, seed=0, qr_method=qr)
+    finally:
+        m.close()
+    ref = oracle.randomized_pca(oracle.CsrView(ip, ix, dv, shape), n_comps=nc, n_oversamples=p, n_power_iterations=q,
+                                zero_center=centered, seed=0, target_sum=1e4, log1p=True)
+    top = min(nc, 10)
+    s_res = sigma_rel_err(res["sigma"][:top], got["sigma"][:top]).max()
+    s_ref = sigma_rel_err(ref["sigma_all"][:top], got["sigma"][:top]).max()
+    ang = max_principal_angle(res["components"].T[:, :3].astype(np.float64), got["components"].T[:, :3].astype(np.float64))
+    say(f"{case}: sigma vs resident {s_res:.2e}, vs oracle {s_ref:.2e}, angle vs resident {ang:.2e}")
+    assert s_res <= 1e-5
+    assert s_ref <= 1e-4
+    assert ang <= 1e-4
+    assert max_principal_angle(ref["components"].T[:, :3], got["components"].T[:, :3].astype(np.float64)) <= 1e-3
+    np.testing.assert_allclose(got["variance_ratio"][:top], res["variance_ratio"][:top], rtol=1e-5)
+    assert abs(got["total_variance"] - res["total_variance"]) <= 1e-12 * res["total_variance"]
+    if centered:
+        np.testing.assert_allclose(got["mean"], res["mean"], rtol=1e-12, atol=1e-300)
+    else:
+        assert got["mean"] is None
+    assert got["embeddings"].shape == (shape[0], nc) and np.all(np.isfinite(got["embeddings"]))
+
+
+@pytest.mark.parametrize("n,shard", [(1000, 1000), (1000, 130)])
+def test_streamed_stable_total_variance(gpu, n, shard):
+    """The reference's cancellation matrix (scx-accel/src/pca/cpu.rs:2201-2214):
+    the closed form cancels, the guard recomputes over the stored entries
+    shard by shard; the centred variance is 3n / (n - 1) exactly."""
+    rows = np.arange(n)
+    dense = np.zeros((n, 3), dtype=np.float32)
+    dense[:, 0] = 1.0e6 + np.where(rows % 2 == 0, 1.0, -1.0)
+    dense[:, 1] = rows % 5
+    x = sp.csr_matrix(dense)
+    ip, ix, dv, shape = arrays(x)
+    say(f"case: streamed cancellation n={n} shard_rows={shard}")
+    got = gpu.randomized_pca_streamed(ip, ix, dv, shape, shard, n_comps=2, n_oversamples=1, n_power_iterations=2,
+                                      seed=0)
+    m = gpu.Matrix(ip, ix, dv, shape)
+    try:
+        res = m.pca(n_comps=2, n_oversamples=1, n_power_iterations=2, seed=0)
+    finally:
+        m.close()
+    exact = 3.0 * n / (n - 1)
+    assert got["tv_recomputed"] and res["tv_recomputed"]
+    assert abs(got["total_variance"] - exact) <= 1e-9 * exact
+    assert abs(res["total_variance"] - exact) <= 1e-9 * exact
+
+
+def test_streamed_rejections(gpu):
+    x = shape_case("tiny")
+    ip, ix, dv, shape = arrays(x)
+    with pytest.raises(ValueError, match="shard_rows must be positive"):
+        gpu.randomized_pca_streamed(ip, ix, dv, shape, 0, n_comps=5)
+    with pytest.raises(ValueError, match="exceeds matrix rank bound"):
+        gpu.randomized_pca_streamed(ip, ix, dv, shape, 10, n_comps=31)
+    bad = ip.copy()
+    bad[5] = bad[4] - 1
+    with pytest.raises(ValueError, match="indptr must be non-decreasing"):
+        gpu.randomized_pca_streamed(bad, ix, dv, shape, 10, n_comps=5)
+    unsorted = ix.copy()
+    a, b = ip[3], ip[4]
+    assert b - a >= 2
+    unsorted[a], unsorted[a + 1] = unsorted[a + 1], unsorted[a]
+    with pytest.raises(ValueError, match="sorted column indices"):
+        gpu.randomized_pca_streamed(ip, unsorted, dv, shape, 10, n_comps=5)
