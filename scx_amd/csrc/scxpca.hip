@@ -35,6 +35,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/scxpca.h"
@@ -1656,6 +1657,34 @@ __global__ void k_center_b(const double *acc, const double *mu, const double *su
     if (B32) B32[t] = static_cast<float>(v);
 }
 
+// dst <- src, n bytes, split over up to nt host threads (page-aligned pieces)
+void par_memcpy(void *dst, const void *src, size_t n, int nt) {
+    if (nt <= 1 || n < (size_t(8) << 20)) {
+        std::memcpy(dst, src, n);
+        return;
+    }
+    const size_t per = ((n + nt - 1) / nt + 4095) & ~size_t(4095);
+    std::vector<std::thread> th;
+    for (size_t off = 0; off < n; off += per) {
+        const size_t len = std::min(per, n - off);
+        th.emplace_back([=] { std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off, len); });
+    }
+    for (auto &t : th) t.join();
+}
+
+// true when p is page-locked host memory (hipHostMalloc'd): copies can then
+// go to the device directly; pageable memory (an error or an unregistered
+// type here) is staged through pinned buffers
+bool host_pinned(const void *p) {
+    hipPointerAttribute_t a;
+    const hipError_t e = hipPointerGetAttributes(&a, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // clear the query's error
+        return false;
+    }
+    return a.type == hipMemoryTypeHost;
+}
+
 struct StreamSlot {
     DevBuf ip, ix, dv;        // the shard's CSR in HBM (indptr rebased to 0)
     PinnedBuf hip, hix, hdv;  // pinned staging
@@ -1677,11 +1706,17 @@ struct StreamOps {
     hipStream_t cs, copy = nullptr;
     StreamSlot slot[2];
     DevBuf acc;
+    bool pinned_ix = false, pinned_dv = false;  // the caller's arrays are page-locked: no staging
+    int host_threads = 1;
+    static constexpr size_t kChunk = size_t(64) << 20;  // staging piece: its transfer overlaps the next copy
 
     StreamOps(Engine<KP> &e, const scxpca_csr *xx, int64_t rows_per_shard)
         : eg(e), x(xx), shard_rows(rows_per_shard), n(xx->n_obs), nv(xx->n_vars), k(e.k), cs(e.s) {
         nshards = static_cast<int>((n + shard_rows - 1) / shard_rows);
         HIPC(hipStreamCreate(&copy));
+        pinned_ix = x->indptr[n] > 0 && host_pinned(x->indices);
+        pinned_dv = x->indptr[n] > 0 && host_pinned(x->data);
+        host_threads = static_cast<int>(std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
         for (auto &sl : slot) {
             HIPC(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
             sl.m.stream = cs;
@@ -1696,25 +1731,35 @@ struct StreamOps {
         (void)hipStreamSynchronize(cs);
     }
 
-    // host rows of shard s -> pinned staging -> the slot's device CSR (copy stream)
+    // host bytes -> HBM on the copy stream: directly from page-locked memory,
+    // else through the pinned staging buffer in kChunk pieces (threaded host
+    // copy of piece i + 1 while piece i transfers)
+    void put(void *ddev, PinnedBuf &pin, const void *src, size_t bytes, bool src_pinned) {
+        if (bytes == 0) return;
+        if (src_pinned) {
+            HIPC(hipMemcpyAsync(ddev, src, bytes, hipMemcpyHostToDevice, copy));
+            return;
+        }
+        pin.ensure(bytes);
+        for (size_t off = 0; off < bytes; off += kChunk) {
+            const size_t len = std::min(kChunk, bytes - off);
+            par_memcpy(static_cast<char *>(pin.p) + off, static_cast<const char *>(src) + off, len, host_threads);
+            HIPC(hipMemcpyAsync(static_cast<char *>(ddev) + off, static_cast<char *>(pin.p) + off, len,
+                                hipMemcpyHostToDevice, copy));
+        }
+    }
+
+    // host rows of shard s -> the slot's device CSR (copy stream)
     void stage(int s) {
         StreamSlot &sl = slot[s & 1];
         const int64_t r0 = static_cast<int64_t>(s) * shard_rows, r1 = std::min(n, r0 + shard_rows);
         const int64_t a = x->indptr[r0], nnz = x->indptr[r1] - a, rows = r1 - r0;
-        sl.hip.ensure(sizeof(int64_t) * (rows + 1));
-        sl.hix.ensure(sizeof(uint32_t) * std::max<int64_t>(nnz, 1));
-        sl.hdv.ensure(sizeof(float) * std::max<int64_t>(nnz, 1));
-        std::memcpy(sl.hip.p, x->indptr + r0, sizeof(int64_t) * (rows + 1));
-        std::memcpy(sl.hix.p, x->indices + a, sizeof(uint32_t) * nnz);
-        std::memcpy(sl.hdv.p, x->data + a, sizeof(float) * nnz);
         sl.ip.ensure(sizeof(int64_t) * (rows + 1));
         sl.ix.ensure(sizeof(uint32_t) * (nnz + kEntryPad));
         sl.dv.ensure(sizeof(float) * (nnz + kEntryPad));
-        HIPC(hipMemcpyAsync(sl.ip.p, sl.hip.p, sizeof(int64_t) * (rows + 1), hipMemcpyHostToDevice, copy));
-        if (nnz) {
-            HIPC(hipMemcpyAsync(sl.ix.p, sl.hix.p, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice, copy));
-            HIPC(hipMemcpyAsync(sl.dv.p, sl.hdv.p, sizeof(float) * nnz, hipMemcpyHostToDevice, copy));
-        }
+        put(sl.ip.p, sl.hip, x->indptr + r0, sizeof(int64_t) * (rows + 1), false);
+        put(sl.ix.p, sl.hix, x->indices + a, sizeof(uint32_t) * nnz, pinned_ix);
+        put(sl.dv.p, sl.hdv, x->data + a, sizeof(float) * nnz, pinned_dv);
         HIPC(hipEventRecord(sl.copied, copy));
     }
 

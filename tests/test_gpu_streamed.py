@@ -132,3 +132,23 @@ def test_streamed_rejections(gpu):
     unsorted[a], unsorted[a + 1] = unsorted[a + 1], unsorted[a]
     with pytest.raises(ValueError, match="sorted column indices"):
         gpu.randomized_pca_streamed(ip, unsorted, dv, shape, 10, n_comps=5)
+
+
+def test_streamed_from_page_locked_host_memory(gpu):
+    """Page-locked host arrays are sent to the device directly (no staging
+    copy); the result is bit-identical to the pageable (staged) run."""
+    import torch
+
+    x = cfg1_head(2000)
+    ip, ix, dv, shape = arrays(x)
+    pix = torch.from_numpy(ix.view(np.int32)).pin_memory()
+    pdv = torch.from_numpy(dv).pin_memory()
+    pinned_ix, pinned_dv = pix.numpy().view(np.uint32), pdv.numpy()
+    assert pinned_ix.ctypes.data == pix.data_ptr() and pinned_dv.ctypes.data == pdv.data_ptr()
+    say("case: streamed page-locked vs pageable")
+    kw = dict(n_comps=20, n_oversamples=10, n_power_iterations=2, seed=0, qr_method=1, target_sum=1e4, log1p=True)
+    a = gpu.randomized_pca_streamed(ip, ix, dv, shape, 300, **kw)
+    b = gpu.randomized_pca_streamed(ip, pinned_ix, pinned_dv, shape, 300, **kw)
+    np.testing.assert_array_equal(a["sigma"], b["sigma"])
+    np.testing.assert_array_equal(a["embeddings"], b["embeddings"])
+    np.testing.assert_array_equal(a["components"], b["components"])
