@@ -22,6 +22,8 @@ GPU) raises: the CPU path of the reference is not part of this package.
 """
 from __future__ import annotations
 
+import os
+
 import numpy as np
 
 from . import _lib
@@ -202,6 +204,57 @@ def _resolve_mask_var(adata, mask_var, n_vars):
 
 
 COVARIANCE_PCA_THRESHOLD = 5_000  # scx-accel/src/pca/cpu.rs:1592
+RESIDENT_VRAM_HEADROOM = 1.2      # scx-gpu/src/gpu_pca_resident.rs:54
+
+
+def _resident_bytes(n: int, nv: int, nnz: int, k: int, bf16: bool = False) -> int:
+    """HBM a resident matrix and its PCA workspace take in this build (the CSR
+    copy, X~, chunk indices, tile-CSC, split offsets and per-tile partials
+    when they stay under 32 GiB, the n x KP panels and the n_vars x KP f64
+    workspace)."""
+    kp = 64 if k <= 64 else 128
+    tiles = (n + 255) // 256
+    nsplit = (nv + 127) // 128
+    b = 8 * (n + 1) + 8 * nnz                         # indptr, indices, values
+    b += 8 * n + 4 * nnz + 2 * nnz + 5 * (nnz + 256)  # row factors, X~, chunk index, tile rows + values
+    b += 4 * tiles * (nv + 1)                         # tile offsets
+    split = 4 * n * (nsplit + 1) + 16 * tiles * nv
+    if split <= 32 << 30:
+        b += split
+    b += 2 * 4 * n * kp + 2 * 4 * nv * kp + 4 * 8 * nv * kp + 4 * 256 * nv * kp + 8 * 1024 * kp * kp
+    if bf16:
+        b += 4 * nnz
+    return int(b)
+
+
+def _free_hbm(device_id: int) -> int:
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            return int(torch.cuda.mem_get_info(device_id)[0])
+    except Exception:
+        pass
+    return int(_lib.device_info(device_id)["total_memory"])
+
+
+def resident_fits(n: int, nv: int, nnz: int, k: int, device_id: int, bf16: bool = False) -> bool:
+    """The residency pre-flight of try_build_resident_csr
+    (scx-gpu/src/gpu_pca_resident.rs:87-135): resident when the matrix and
+    its workspace fit free HBM / 1.2 and SCX_GPU_PCA_RESIDENT is not "0"
+    (read per call here; the reference reads it once per process)."""
+    if os.environ.get("SCX_GPU_PCA_RESIDENT") == "0":
+        return False
+    return _resident_bytes(n, nv, nnz, k, bf16) <= _free_hbm(device_id) / RESIDENT_VRAM_HEADROOM
+
+
+def stream_shard_rows(n: int, nv: int, nnz: int, k: int, device_id: int) -> int:
+    """Rows per shard of the streamed route: two shards' structures within an
+    eighth of the free HBM, a multiple of 256 rows, at least 256."""
+    budget = _free_hbm(device_id) / RESIDENT_VRAM_HEADROOM / 8
+    per_row = 8 + 8 * k + 21 * (nnz / max(n, 1)) + 4 * ((nv + 127) // 128 + 1) + 20 * (nv + 1) / 256
+    rows = int(budget / (2 * per_row)) // 256 * 256
+    return max(256, min(rows, max(n, 1)))
 
 
 def resolve_method(method: str, n_vars: int) -> str:
@@ -289,25 +342,40 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
         raw = raw.sorted_indices()
     n_obs, n_vars = raw.shape
 
-    m = _lib.Matrix(raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
-                    raw.data.astype(np.float32, copy=False), raw.shape,
-                    target_sum=lazy.target_sum if lazy.target_sum is not None else None, log1p=lazy.log1p,
-                    device=device_id)
+    route = resolve_method(method, n_vars)
+    qr = _lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER
+    seed = int(random_state) & ((1 << 64) - 1)
+    k = max(1, min(n_comps + n_oversamples, n_obs, n_vars))
+    resident = route == "covariance" or resident_fits(n_obs, n_vars, raw.nnz, k, device_id, precision == "bf16")
+    ip, ix, dv = (raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
+                  raw.data.astype(np.float32, copy=False))
     try:
-        if resolve_method(method, n_vars) == "covariance":
-            res = m.pca_covariance(n_comps=n_comps, zero_center=zero_center)
+        if not resident:
+            # the streaming operator (gpu_pca.rs:566-590): the CSR stays on the host
+            if precision == "bf16":
+                raise ValueError("precision='bf16' needs the matrix resident in HBM; the streamed route is FP32")
+            res = _lib.randomized_pca_streamed(ip, ix, dv, raw.shape,
+                                               stream_shard_rows(n_obs, n_vars, raw.nnz, k, device_id),
+                                               n_comps=n_comps, n_oversamples=n_oversamples,
+                                               n_power_iterations=n_power_iterations, zero_center=zero_center,
+                                               seed=seed, qr_method=qr, target_sum=lazy.target_sum,
+                                               log1p=lazy.log1p, device=device_id)
         else:
-            res = m.pca(n_comps=n_comps, n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
-                        zero_center=zero_center, seed=int(random_state) & ((1 << 64) - 1),
-                        qr_method=_lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER,
-                        precision=_lib.PRECISION_BF16_SKETCH if precision == "bf16" else _lib.PRECISION_FP32)
+            m = _lib.Matrix(ip, ix, dv, raw.shape, target_sum=lazy.target_sum, log1p=lazy.log1p, device=device_id)
+            try:
+                if route == "covariance":
+                    res = m.pca_covariance(n_comps=n_comps, zero_center=zero_center)
+                else:
+                    res = m.pca(n_comps=n_comps, n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
+                                zero_center=zero_center, seed=seed, qr_method=qr,
+                                precision=_lib.PRECISION_BF16_SKETCH if precision == "bf16" else _lib.PRECISION_FP32)
+            finally:
+                m.close()
     except ValueError as e:
         # AccelError::InvalidInput surfaces as RuntimeError (pca.rs:845)
         raise RuntimeError(str(e)) from None
     except _lib.ScxPcaError as e:
         raise RuntimeError(f"GPU PCA failed: {e}") from None
-    finally:
-        m.close()
 
     pcs = np.zeros((full_n_vars, n_comps), dtype=np.float32)
     if masked is not None:
@@ -328,7 +396,7 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
         },
     }
     container = dict(adata.uns.get("scx_accel", {}) or {})
-    container["pca"] = _route_stamp(device_id, spmm_policy, allow_tf32, True)
+    container["pca"] = _route_stamp(device_id, spmm_policy, allow_tf32, resident)
     adata.uns["scx_accel"] = container
 
 

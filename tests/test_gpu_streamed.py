@@ -152,3 +152,31 @@ def test_streamed_from_page_locked_host_memory(gpu):
     np.testing.assert_array_equal(a["sigma"], b["sigma"])
     np.testing.assert_array_equal(a["embeddings"], b["embeddings"])
     np.testing.assert_array_equal(a["components"], b["components"])
+
+
+def test_accel_pca_streams_when_not_resident(gpu, monkeypatch):
+    """accel.pca takes the streamed route when the residency pre-flight
+    declines (SCX_GPU_PCA_RESIDENT=0 forces it, as in
+    scx-gpu/src/gpu_pca_resident.rs:71-74): same result as the resident route,
+    stamped resident_csr = False; the BF16 sketch needs residency."""
+    from scx_amd import accel
+    from scx_amd.adata import AnnDataLite
+
+    x = cfg1_head(2000)
+    out = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("SCX_GPU_PCA_RESIDENT", env)
+        a = AnnDataLite(x.copy())
+        accel.normalize_total(a, target_sum=1e4)
+        accel.log1p(a)
+        say(f"case: accel.pca SCX_GPU_PCA_RESIDENT={env}")
+        accel.pca(a, n_comps=20, device="gpu", method="randomized")
+        out[env] = a
+    r, s_ = out["1"], out["0"]
+    assert r.uns["scx_accel"]["pca"]["resident_csr"] is True
+    assert s_.uns["scx_accel"]["pca"]["resident_csr"] is False
+    np.testing.assert_allclose(s_.uns["pca"]["variance"], r.uns["pca"]["variance"], rtol=1e-5)
+    assert max_principal_angle(r.varm["PCs"][:, :3].astype(np.float64), s_.varm["PCs"][:, :3].astype(np.float64)) <= 1e-4
+    a = AnnDataLite(x.copy())
+    with pytest.raises(RuntimeError, match="needs the matrix resident"):
+        accel.pca(a, n_comps=20, device="gpu", method="randomized", precision="bf16")

@@ -78,6 +78,18 @@ def test_pflog_argument_validation(kw, exc, msg):
         accel.pflog(_adata(), device="cpu", **kw)
 
 
+def test_residency_preflight(monkeypatch):
+    # SCX_GPU_PCA_RESIDENT=0 declines before any device query (gpu_pca_resident.rs:71-74)
+    monkeypatch.setenv("SCX_GPU_PCA_RESIDENT", "0")
+    assert accel.resident_fits(1000, 100, 10_000, 60, 0) is False
+    # the footprint model grows with every dimension and sits near 19-30 bytes per nonzero at 1M x 2k
+    b = accel._resident_bytes(1_000_000, 2000, 162_701_712, 60)
+    assert accel._resident_bytes(2_000_000, 2000, 162_701_712, 60) > b
+    assert accel._resident_bytes(1_000_000, 4000, 162_701_712, 60) > b
+    assert accel._resident_bytes(1_000_000, 2000, 200_000_000, 60) > b
+    assert 19 * 162_701_712 < b < 30 * 162_701_712
+
+
 def test_lazy_chain_matches_oracle_transform():
     c = synth.random_csr(200, 60, 0.2, seed=2)
     ad = AnnDataLite(c.to_scipy())
