@@ -342,3 +342,26 @@ def test_accel_pflog_end_to_end(gpu):
     accel.pflog(b, alpha=ref["alpha"], store="baseline", device="gpu")
     assert "X_pflog_pca" not in b.obsm and b.uns["pflog"]["alpha_source"] == "pinned"
     np.testing.assert_allclose(b.obs["pflog_baseline"], ref["baseline"], rtol=1e-9)
+
+
+def test_pflog_general_layout(gpu, monkeypatch):
+    """The general layout (no split offsets: the row-parallel forward kernel
+    with the baseline epilogue, stats from the tile-CSC), forced by a zero
+    metadata cap, against the oracle."""
+    monkeypatch.setenv("SCXPCA_SPLIT_MAX_BYTES", "0")
+    x = shape_case("wide3k")
+    say("case: pflog general layout wide3k")
+    ref = oracle.pflog_pca(view(x), n_comps=30, n_oversamples=10, n_power_iterations=3, seed=0, alpha=None, form=F64)
+    m = make(gpu, x)
+    try:
+        fit = m.pflog_fit()
+        got = m.pca(n_comps=30, n_oversamples=10, n_power_iterations=3, seed=0, qr_method=1)
+    finally:
+        m.close()
+    assert fit["alpha"] == ref["alpha"]
+    np.testing.assert_allclose(fit["baseline"], ref["baseline"], rtol=1e-9, atol=1e-300)
+    se = sigma_rel_err(ref["sigma_all"][:10], got["sigma"][:10])
+    say(f"pflog general: sigma rel err top10 {se.max():.2e}")
+    assert se.max() <= 1e-4
+    assert max_principal_angle(ref["components"].T[:, :3], got["components"].T[:, :3].astype(np.float64)) <= 1e-3
+    assert abs(got["total_variance"] - ref["total_variance"]) <= 1e-9 * ref["total_variance"]

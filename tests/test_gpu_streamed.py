@@ -180,3 +180,24 @@ def test_accel_pca_streams_when_not_resident(gpu, monkeypatch):
     a = AnnDataLite(x.copy())
     with pytest.raises(RuntimeError, match="needs the matrix resident"):
         accel.pca(a, n_comps=20, device="gpu", method="randomized", precision="bf16")
+
+
+def test_streamed_general_layout(gpu, monkeypatch):
+    """Shards in the general layout (zero metadata cap: no split offsets, the
+    row-parallel forward, stats from the tile-CSC) against the resident
+    matrix in the same layout."""
+    monkeypatch.setenv("SCXPCA_SPLIT_MAX_BYTES", "0")
+    x = shape_case("wide3k")
+    ip, ix, dv, shape = arrays(x)
+    say("case: streamed general layout wide3k shard_rows=500")
+    kw = dict(n_comps=30, n_oversamples=10, n_power_iterations=3, seed=0, qr_method=1)
+    got = gpu.randomized_pca_streamed(ip, ix, dv, shape, 500, target_sum=1e4, log1p=True, **kw)
+    m = gpu.Matrix(ip, ix, dv, shape, target_sum=1e4, log1p=True)
+    try:
+        res = m.pca(**kw)
+    finally:
+        m.close()
+    se = sigma_rel_err(res["sigma"][:10], got["sigma"][:10]).max()
+    say(f"streamed general: sigma vs resident {se:.2e}")
+    assert se <= 1e-5
+    np.testing.assert_allclose(got["mean"], res["mean"], rtol=1e-12, atol=1e-300)
