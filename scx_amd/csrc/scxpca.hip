@@ -1716,7 +1716,7 @@ struct StreamOps {
         HIPC(hipStreamCreate(&copy));
         pinned_ix = x->indptr[n] > 0 && host_pinned(x->indices);
         pinned_dv = x->indptr[n] > 0 && host_pinned(x->data);
-        host_threads = static_cast<int>(std::max(1u, std::min(8u, std::thread::hardware_concurrency())));
+        host_threads = static_cast<int>(std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
         for (auto &sl : slot) {
             HIPC(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
             sl.m.stream = cs;
