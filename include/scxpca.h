@@ -171,6 +171,11 @@ scxpca_status scxpca_matrix_create_pflog(const scxpca_csr *x, const scxpca_pflog
                                          scxpca_matrix **out);
 /* The alpha fit and (baseline != NULL) the n_obs baseline values, host f64. */
 scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, double *baseline);
+/* The dense transform Z = delta + b 1^T of a PFlog matrix, n_obs x n_vars
+ * row-major f32 (each row f32(b_i), delta added at the stored entries), into
+ * a host (on_device = 0) or device buffer. Replaces materialize_dense
+ * (pyscx/src/accel/pflog.rs:506-533). */
+scxpca_status scxpca_matrix_pflog_dense(scxpca_matrix *m, float *z, int32_t on_device);
 
 /* Stage profiler (the reference's SCX_GPU_PROFILE stage timers,
  * scx-gpu/src/profile.rs:77-208): while enabled, every stage launch on the

@@ -34,6 +34,7 @@ EXPORTS = (
     "scxpca_matrix_create",
     "scxpca_matrix_create_pflog",
     "scxpca_matrix_pflog_fit",
+    "scxpca_matrix_pflog_dense",
     "scxpca_matrix_destroy",
     "scxpca_matrix_refresh",
     "scxpca_matrix_pca",
@@ -122,6 +123,7 @@ def lib() -> ctypes.CDLL:
     l.scxpca_matrix_create_pflog.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(PflogParams), ctypes.c_int,
                                              ctypes.POINTER(_vp)]
     l.scxpca_matrix_pflog_fit.argtypes = [_vp, ctypes.POINTER(PflogFit), _vp]
+    l.scxpca_matrix_pflog_dense.argtypes = [_vp, _vp, _i32]
     l.scxpca_matrix_destroy.argtypes = [_vp]
     l.scxpca_matrix_refresh.argtypes = [_vp]
     l.scxpca_matrix_pca.argtypes = [_vp, ctypes.POINTER(Params), ctypes.POINTER(Result)]
@@ -223,6 +225,12 @@ class Matrix:
                              int(pflog.get("delta_form", PFLOG_DELTA_LAZY)))
             check(lib().scxpca_matrix_create_pflog(ctypes.byref(csr), ctypes.byref(pp), device, ctypes.byref(h)))
         self.handle = h
+
+    def pflog_dense(self) -> np.ndarray:
+        """The dense PFlog transform Z (n_obs x n_vars, f32) of a PFlog matrix."""
+        z = np.empty(self.shape, dtype=np.float32)
+        check(lib().scxpca_matrix_pflog_dense(self.handle, ptr(z), 0))
+        return z
 
     def pflog_fit(self, baseline: bool = True) -> dict:
         """The alpha fit of a PFlog matrix and (``baseline``) its per-cell baseline."""

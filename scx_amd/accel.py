@@ -422,12 +422,14 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
     and ``store`` in ("pca", "all") runs the baseline-aware randomized PCA
     (pflog_pca, scx-accel/src/pca/cpu.rs:815-866) into
     ``adata.obsm[obsm_key]`` with the singular values
-    sqrt(variance (n - 1)) in ``adata.uns[f"{obsm_key}_singular_values"]``.
+    sqrt(variance (n - 1)) in ``adata.uns[f"{obsm_key}_singular_values"]``;
+    ``store`` in ("dense", "all") writes the dense Z (f32) to
+    ``adata.layers[layer_out or "pflog"]`` under the ``dense_max_elems`` guard.
     An in-memory ``X`` builds delta as ``(4a * x as f64).ln_1p() as f32``
     (pflog.rs:375-397); a ``LazyCsr`` without transforms as the lazy
-    ``Scale{4a} -> Log1p`` chain. The dense materialisation (``store`` in
-    ("dense", "all")) and its SCX-file output are not built here: they raise
-    NotImplementedError after the checks the reference makes first."""
+    ``Scale{4a} -> Log1p`` chain. The SCX-file output (``out=``) is not built
+    here: it raises NotImplementedError after the checks the reference makes
+    first."""
     if alpha is not None and (not np.isfinite(alpha) or alpha <= 0):
         raise ValueError(f"pflog: alpha must be positive and finite, got {alpha}")
     kind, device_id = resolve_device(device)
@@ -447,9 +449,9 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
             raise ValueError(f"layer '{layer}' not found in adata.layers") from None
     else:
         x = adata.X
-    if want_dense:
-        raise NotImplementedError("scx_amd builds the PFlog PCA and baseline (store='pca' or 'baseline'); the "
-                                  "dense materialization of Z is not part of the GPU path")
+    if want_dense and out is not None:
+        raise NotImplementedError("pflog: out= writes an SCX file (the storage layer), which scx_amd does not "
+                                  "build; without out= the dense transform lands in adata.layers")
     if kind != "gpu":
         raise RuntimeError("scx_amd implements the GPU route of pflog only; device='cpu' (or 'auto' without a "
                            "visible ROCm GPU) is served by the reference CPU implementation")
@@ -480,9 +482,11 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
                         pflog={"alpha": alpha, "delta_form": form})
     except ValueError as e:
         raise runtime(e) from None
+    n_elems = raw.shape[0] * raw.shape[1]
     try:
         fit = m.pflog_fit(baseline=True)
         res = None
+        dense = m.pflog_dense() if want_dense and n_elems <= dense_max_elems else None
         if want_pca:
             try:
                 res = m.pca(n_comps=n_components, n_oversamples=n_oversamples,
@@ -504,6 +508,15 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
         adata.obsm[obsm_key] = res["embeddings"]
         scale = max(raw.shape[0] - 1.0, 1.0)
         adata.uns[f"{obsm_key}_singular_values"] = np.sqrt(np.maximum(res["variance"] * scale, 0.0))
+    if want_dense:
+        if dense is None:  # the in-memory guard (pyscx/src/accel/pflog.rs:477-487)
+            n_obs_, n_vars_ = raw.shape
+            raise RuntimeError(
+                f"pflog: dense materialization is {n_obs_}\u00d7{n_vars_} = {n_elems} elements, over the "
+                f"dense_max_elems={dense_max_elems} guard. The exact PFlog transform is dense; pass out=<path.scx> "
+                f"to stream it to disk (any size), use store=\"pca\" for an out-of-core embedding, or raise "
+                f"dense_max_elems if you have the RAM.")
+        adata.layers[layer_out or "pflog"] = dense
     container = dict(adata.uns.get("scx_accel", {}) or {})
     container["pflog"] = _route_stamp(device_id, "default", False, True)
     adata.uns["scx_accel"] = container

@@ -621,6 +621,24 @@ void ensure_stats(scxpca_matrix *m) {
     m->stats_ready = true;
 }
 
+// The dense PFlog transform Z (pyscx/src/accel/pflog.rs:506-533): every row
+// filled with f32(b_i), then delta added at the stored entries (f32, as the
+// reference); two launches, so the fill is complete before the adds, and
+// each (row, column) is stored once, so the adds do not race.
+__global__ void k_pflog_fill(const double *b, int64_t n_obs, int64_t n_vars, float *z) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = t; i < n_obs * n_vars; i += stride) z[i] = static_cast<float>(b[i / n_vars]);
+}
+
+__global__ void k_pflog_scatter(const int64_t *indptr, const uint32_t *idx, const float *delta, int64_t n_obs,
+                                int64_t n_vars, float *z) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wstride = static_cast<int64_t>(gridDim.x) * (blockDim.x / 64);
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + threadIdx.x / 64; r < n_obs; r += wstride)
+        for (int64_t p = indptr[r] + lane; p < indptr[r + 1]; p += 64) z[r * n_vars + idx[p]] += delta[p];
+}
+
 // mu = sum / n (+ shift: the PFlog baseline mean, cpu.rs:838-843)
 __global__ void k_mean(const double *sum, int64_t n_vars, double n, double shift, double *mu) {
     const int64_t v = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1946,6 +1964,29 @@ scxpca_status scxpca_randomized_pca_streamed(const scxpca_csr *x, const scxpca_p
     return guarded([&] {
         if (!x || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
         run_pca_streamed(x, p, shard_rows, out, device);
+    });
+}
+
+scxpca_status scxpca_matrix_pflog_dense(scxpca_matrix *m, float *z, int32_t on_device) {
+    return guarded([&] {
+        if (!m || !z) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (!m->pflog) throw Error(SCXPCA_ERR_INVALID, "not a PFlog matrix");
+        DeviceGuard dg(m->device);
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        if (n == 0) return;
+        DevBuf tmp;
+        float *dz = z;
+        if (!on_device) {
+            tmp.ensure(sizeof(float) * n * nv);
+            dz = tmp.as<float>();
+        }
+        k_pflog_fill<<<std::min(blocks_for(n * nv), 65536u), 256, 0, m->stream>>>(m->pfb.as<double>(), n, nv, dz);
+        HIPC(hipGetLastError());
+        k_pflog_scatter<<<std::min(blocks_for(n, 4), 65536u), 256, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(),
+                                                                                 n, nv, dz);
+        HIPC(hipGetLastError());
+        if (!on_device) HIPC(hipMemcpyAsync(z, dz, sizeof(float) * n * nv, hipMemcpyDeviceToHost, m->stream));
+        HIPC(hipStreamSynchronize(m->stream));
     });
 }
 

@@ -365,3 +365,29 @@ def test_pflog_general_layout(gpu, monkeypatch):
     assert se.max() <= 1e-4
     assert max_principal_angle(ref["components"].T[:, :3], got["components"].T[:, :3].astype(np.float64)) <= 1e-3
     assert abs(got["total_variance"] - ref["total_variance"]) <= 1e-9 * ref["total_variance"]
+
+
+def test_accel_pflog_dense(gpu):
+    """store="all": the dense Z in adata.layers equals the reference's
+    materialisation (f32(b_i) + delta_ij in f32, pyscx/src/accel/pflog.rs:506-533)
+    bit for bit on the oracle's delta and the returned baseline; the
+    in-memory guard raises after the PCA outputs are written."""
+    from scx_amd import accel
+    from scx_amd.adata import AnnDataLite
+
+    x = cfg1_head(500)
+    say("case: accel.pflog store=all")
+    a = AnnDataLite(x.copy())
+    accel.pflog(a, store="all", n_components=10, device="gpu", layer_out="Z")
+    ref = oracle.pflog_pca(view(x), n_comps=10, n_oversamples=10, n_power_iterations=2, seed=0, alpha=None, form=F64)
+    b32 = np.asarray(a.obs["pflog_baseline"], dtype=np.float64).astype(np.float32)
+    expect = np.repeat(b32[:, None], x.shape[1], axis=1)
+    rows = np.repeat(np.arange(x.shape[0]), np.diff(x.indptr))
+    expect[rows, x.indices] += ref["delta"]
+    assert a.layers["Z"].dtype == np.float32 and a.layers["Z"].shape == x.shape
+    np.testing.assert_array_equal(a.layers["Z"], expect)
+    assert "X_pflog_pca" in a.obsm
+    g = AnnDataLite(x.copy())
+    with pytest.raises(RuntimeError, match="over the dense_max_elems=1000 guard"):
+        accel.pflog(g, store="all", n_components=10, device="gpu", dense_max_elems=1000)
+    assert "X_pflog_pca" in g.obsm and "pflog" not in g.layers

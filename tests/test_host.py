@@ -69,7 +69,7 @@ def test_pca_cpu_route_is_not_silently_served():
     (dict(store_repr="csr"), ValueError, "pflog: unknown store_repr"),
     (dict(out="z.scx"), ValueError, "pflog: out= streams the materialized transform to disk"),
     (dict(layer="counts"), ValueError, "layer 'counts' not found in adata.layers"),
-    (dict(store="dense"), NotImplementedError, "dense materialization of Z is not part of the GPU path"),
+    (dict(store="dense", out="z.scx"), NotImplementedError, "pflog: out= writes an SCX file"),
     (dict(), RuntimeError, "GPU route of pflog only"),
 ])
 def test_pflog_argument_validation(kw, exc, msg):
