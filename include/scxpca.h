@@ -177,6 +177,17 @@ scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, d
  * (pyscx/src/accel/pflog.rs:506-533). */
 scxpca_status scxpca_matrix_pflog_dense(scxpca_matrix *m, float *z, int32_t on_device);
 
+/* PFlog PCA of a host CSR larger than HBM, streamed as
+ * scxpca_randomized_pca_streamed streams (pflog_pca_with_depth,
+ * scx-accel/src/pca/cpu.rs:815-866, streams its delta source): alpha pinned or
+ * estimated by a streamed raw pass, every shard's delta and baseline built
+ * from its own rows. fit (may be NULL) receives the alpha fit, baseline (may
+ * be NULL) the n_obs baseline (host f64). The transform fields of p are
+ * ignored. FP32 only. */
+scxpca_status scxpca_pflog_pca_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, const scxpca_params *p,
+                                        int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit,
+                                        double *baseline, int device);
+
 /* Stage profiler (the reference's SCX_GPU_PROFILE stage timers,
  * scx-gpu/src/profile.rs:77-208): while enabled, every stage launch on the
  * matrix stream is bracketed by HIP events. scxpca_matrix_profile returns the

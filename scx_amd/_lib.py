@@ -31,6 +31,7 @@ STAGES = ("stats", "spmm_fwd", "spmm_t", "spmm_t_reduce", "gram", "apply", "spmm
 EXPORTS = (
     "scxpca_randomized_pca",
     "scxpca_randomized_pca_streamed",
+    "scxpca_pflog_pca_streamed",
     "scxpca_matrix_create",
     "scxpca_matrix_create_pflog",
     "scxpca_matrix_pflog_fit",
@@ -119,6 +120,8 @@ def lib() -> ctypes.CDLL:
     l.scxpca_randomized_pca.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(Params), ctypes.POINTER(Result), ctypes.c_int]
     l.scxpca_randomized_pca_streamed.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(Params), _i64,
                                                  ctypes.POINTER(Result), ctypes.c_int]
+    l.scxpca_pflog_pca_streamed.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(PflogParams), ctypes.POINTER(Params),
+                                            _i64, ctypes.POINTER(Result), ctypes.POINTER(PflogFit), _vp, ctypes.c_int]
     l.scxpca_matrix_create.argtypes = [ctypes.POINTER(Csr), _dbl, _i32, ctypes.c_int, ctypes.POINTER(_vp)]
     l.scxpca_matrix_create_pflog.argtypes = [ctypes.POINTER(Csr), ctypes.POINTER(PflogParams), ctypes.c_int,
                                              ctypes.POINTER(_vp)]
@@ -366,6 +369,45 @@ def randomized_pca_streamed(indptr, indices, data, shape, shard_rows, n_comps=50
             "mean": mean[:nv] if zero_center else None, "sigma": sigma[: res.k], "k": int(res.k),
             "total_variance": float(res.total_variance), "tv_recomputed": bool(res.tv_recomputed),
             "seconds_device": float(res.seconds_device)}
+
+
+def pflog_pca_streamed(indptr, indices, data, shape, shard_rows, alpha=None, delta_form=PFLOG_DELTA_LAZY,
+                       n_comps=50, n_oversamples=10, n_power_iterations=2, zero_center=True, seed=0,
+                       qr_method=QR_HOUSEHOLDER, mu_min=ALPHA_MU_MIN, fallback_alpha=ALPHA_FALLBACK, device=0):
+    """PFlog PCA of a host CSR streamed to the GPU in shards on every pass;
+    returns (the result dict of ``Matrix.pca``, the fit dict of
+    ``Matrix.pflog_fit`` with the baseline)."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    n, nv = int(shape[0]), int(shape[1])
+    csr = Csr(n, nv, ptr(indptr), ptr(indices), ptr(data), 0)
+    pp = PflogParams(0.0 if alpha is None else float(alpha), float(mu_min), float(fallback_alpha), int(delta_form))
+    prm = Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), qr_method, seed, 0.0, 0,
+                 PRECISION_FP32)
+    k = max(1, min(n_comps + n_oversamples, n, nv))
+    var = np.zeros(max(n_comps, 1))
+    ratio = np.zeros(max(n_comps, 1))
+    mean = np.zeros(max(nv, 1))
+    sigma = np.zeros(k)
+    emb = np.empty((n, n_comps), dtype=np.float32)
+    comps = np.empty((n_comps, nv), dtype=np.float32)
+    base = np.zeros(n)
+    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0)
+    f = PflogFit()
+    check(lib().scxpca_pflog_pca_streamed(ctypes.byref(csr), ctypes.byref(pp), ctypes.byref(prm),
+                                          _i64(int(shard_rows)), ctypes.byref(res), ctypes.byref(f), ptr(base),
+                                          device))
+    result = {"embeddings": emb, "components": comps, "variance": var[:n_comps], "variance_ratio": ratio[:n_comps],
+              "mean": mean[:nv] if zero_center else None, "sigma": sigma[: res.k], "k": int(res.k),
+              "total_variance": float(res.total_variance), "tv_recomputed": bool(res.tv_recomputed),
+              "seconds_device": float(res.seconds_device)}
+    fit = {"alpha": float(f.alpha), "pseudocount": float(f.pseudocount),
+           "alpha_source": "estimated" if f.estimated else "pinned", "baseline": base}
+    if f.estimated:
+        fit["n_genes_used"] = int(f.n_genes_used)
+        fit["fell_back"] = bool(f.fell_back)
+    return result, fit
 
 
 def qr(a: np.ndarray, qr_method: int = QR_HOUSEHOLDER, device: int = 0) -> np.ndarray:

@@ -732,7 +732,7 @@ struct Engine {
 
     // mean(b) = sum(b) / max(n, 1) (cpu.rs:838), 0 for a plain matrix
     double baseline_mean() const {
-        return rowb ? m->pf_bsum / std::max(static_cast<double>(m->n_obs), 1.0) : 0.0;
+        return m->pflog ? m->pf_bsum / std::max(static_cast<double>(m->n_obs), 1.0) : 0.0;
     }
 
     // out = w^T A over the rows of a tall f32 panel A (nrows x KP), f64;
@@ -948,7 +948,7 @@ struct Engine {
         const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
         double tv = 0.0;
         recomputed = 0;
-        if (rowb) {
+        if (m->pflog) {
             // pflog_total_variance (cpu.rs:702-769): S_dd - D B2 (uncentered),
             // S_dd - D B2 - n sum_j mu_j^2 with mu_j = colmean_j + mean(b)
             // (centered); never recomputed, clamped at 0
@@ -1725,6 +1725,10 @@ struct StreamOps {
     StreamSlot slot[2];
     DevBuf acc;
     bool pinned_ix = false, pinned_dv = false;  // the caller's arrays are page-locked: no staging
+    // shard transform: the shell's (normalize / log1p), PFlog with the shell's
+    // alpha (pflog), or the raw pass of the alpha estimate (raw)
+    bool pflog = false, raw = false;
+    double *baseline_out = nullptr;  // host n_obs: the PFlog baseline, gathered by stats()
     int host_threads = 1;
     static constexpr size_t kChunk = size_t(64) << 20;  // staging piece: its transfer overlaps the next copy
 
@@ -1796,8 +1800,20 @@ struct StreamOps {
         m->indptr = sl.ip.as<int64_t>();
         m->idx = sl.ix.as<uint32_t>();
         m->val = sl.dv.as<float>();
-        m->target = eg.m->target;
-        m->flags = eg.m->flags;
+        if (raw) {
+            m->pflog = false;
+            m->target = 0.0;
+            m->flags = kXfCheck;
+        } else if (pflog) {
+            m->pflog = true;
+            m->pf_params = eg.m->pf_params;
+            m->pf_fit = eg.m->pf_fit;
+            m->target = eg.m->target;
+            m->flags = pflog_flags(m->pf_params);
+        } else {
+            m->target = eg.m->target;
+            m->flags = eg.m->flags;
+        }
         size_derived(m);
         throw_invalid(m, prepare(m));
         sl.r0 = r0;
@@ -1823,8 +1839,16 @@ struct StreamOps {
         HIPC(hipMemsetAsync(g->sum.p, 0, sizeof(double) * nv, cs));
         HIPC(hipMemsetAsync(g->sq.p, 0, sizeof(double) * nv, cs));
         HIPC(hipMemsetAsync(g->cnt.p, 0, sizeof(uint32_t) * nv, cs));
+        if (pflog) g->pf_bsum = g->pf_b2 = 0.0;
         each_shard([&](StreamSlot &sl) {
             ensure_stats(&sl.m);
+            if (pflog) {  // the shards' baseline sums, in shard order
+                g->pf_bsum += sl.m.pf_bsum;
+                g->pf_b2 += sl.m.pf_b2;
+                if (baseline_out)
+                    HIPC(hipMemcpyAsync(baseline_out + sl.r0, sl.m.pfb.p, sizeof(double) * sl.m.n_obs,
+                                        hipMemcpyDeviceToHost, cs));
+            }
             k_add_f64<<<blocks_for(nv), 256, 0, cs>>>(g->sum.as<double>(), sl.m.sum.as<double>(), nv);
             k_add_f64<<<blocks_for(nv), 256, 0, cs>>>(g->sq.as<double>(), sl.m.sq.as<double>(), nv);
             k_add_u32<<<blocks_for(nv), 256, 0, cs>>>(g->cnt.as<uint32_t>(), sl.m.cnt.as<uint32_t>(), nv);
@@ -1867,8 +1891,23 @@ struct StreamOps {
     }
 };
 
+// pflog_pca on a streamed host CSR (pflog_pca_with_depth, cpu.rs:815-866,
+// which streams its delta source): the alpha estimate by a streamed raw pass
+// (estimate_alpha, pflog.rs:340-398), then the PCA over PFlog shards (each
+// shard's delta and baseline from its own rows; the shards' b (1^T M) terms
+// fill their rows of Y and their 1 (b_s^T Q_s) terms sum to 1 (b^T Q)).
+void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, const scxpca_params *p,
+                        int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit, double *baseline,
+                        int device);
+
 void run_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard_rows, scxpca_result *out,
                       int device) {
+    run_pflog_streamed(x, nullptr, p, shard_rows, out, nullptr, nullptr, device);
+}
+
+void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, const scxpca_params *p,
+                        int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit, double *baseline,
+                        int device) {
     if (x->on_device) throw Error(SCXPCA_ERR_INVALID, "the streamed PCA reads a host CSR (on_device = 0)");
     if (shard_rows <= 0) throw Error(SCXPCA_ERR_INVALID, "shard_rows must be positive");
     if (p->precision != SCXPCA_PRECISION_FP32)
@@ -1887,6 +1926,26 @@ void run_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard
     shell.nnz = x->n_obs > 0 ? x->indptr[x->n_obs] : 0;
     shell.target = p->target_sum;
     shell.flags = transform_flags(p->target_sum, p->log1p);
+    if (pp) {
+        const double a = pp->alpha;
+        if (!(a == 0.0 || (a > 0.0 && std::isfinite(a))))
+            throw Error(SCXPCA_ERR_INVALID, "pflog: alpha must be positive and finite, got " + fmt_f32(static_cast<float>(a)));
+        if (pp->delta_form != SCXPCA_PFLOG_DELTA_LAZY && pp->delta_form != SCXPCA_PFLOG_DELTA_F64)
+            throw Error(SCXPCA_ERR_INVALID, "unknown PFlog delta form");
+        if (!(pp->fallback_alpha > 0.0 && std::isfinite(pp->fallback_alpha)) || !std::isfinite(pp->mu_min))
+            throw Error(SCXPCA_ERR_INVALID, "PFlog alpha options must be finite with a positive fallback");
+        if (x->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "PFlog requires n_vars > 0 for centering");
+        shell.pflog = true;
+        shell.pf_params = *pp;
+        shell.pf_fit = scxpca_pflog_fit{};
+        if (a > 0.0) {
+            shell.pf_fit.alpha = a;
+            shell.pf_fit.pseudocount = 1.0 / (4.0 * a);
+            shell.pf_fit.n_genes_used = -1;
+        } else {
+            shell.pf_fit.estimated = 1;
+        }
+    }
     validate(&shell, p);
     const int64_t k = pca_k(&shell, p);
     if (k > 128)
@@ -1896,15 +1955,34 @@ void run_pca_streamed(const scxpca_csr *x, const scxpca_params *p, int64_t shard
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     HIPC(hipEventRecord(e0, shell.stream));
+    auto drive = [&](auto &eg, auto &ops) {
+        if (pp && shell.pf_fit.estimated) {  // the raw pass: column moments of the counts -> alpha
+            ops.raw = true;
+            ops.stats();
+            std::vector<double> hs(x->n_vars), hq(x->n_vars);
+            HIPC(hipMemcpyAsync(hs.data(), shell.sum.p, sizeof(double) * x->n_vars, hipMemcpyDeviceToHost, shell.stream));
+            HIPC(hipMemcpyAsync(hq.data(), shell.sq.p, sizeof(double) * x->n_vars, hipMemcpyDeviceToHost, shell.stream));
+            HIPC(hipStreamSynchronize(shell.stream));
+            shell.pf_fit = estimate_alpha(hs, hq, x->n_obs, *pp);
+            ops.raw = false;
+        }
+        if (pp) {
+            shell.target = 4.0 * shell.pf_fit.alpha;
+            ops.pflog = true;
+            ops.baseline_out = baseline;
+        }
+        eg.run_with(ops, p, out);
+    };
     if (k <= 64) {
         Engine<64> eg(&shell, static_cast<int>(k));
         StreamOps<64> ops(eg, x, shard_rows);
-        eg.run_with(ops, p, out);
+        drive(eg, ops);
     } else {
         Engine<128> eg(&shell, static_cast<int>(k));
         StreamOps<128> ops(eg, x, shard_rows);
-        eg.run_with(ops, p, out);
+        drive(eg, ops);
     }
+    if (fit) *fit = shell.pf_fit;
     HIPC(hipEventRecord(e1, shell.stream));
     HIPC(hipEventSynchronize(e1));
     float ms = 0.f;
@@ -1956,6 +2034,15 @@ scxpca_status scxpca_matrix_pflog_fit(scxpca_matrix *m, scxpca_pflog_fit *fit, d
             HIPC(hipMemcpyAsync(baseline, m->pfb.p, sizeof(double) * m->n_obs, hipMemcpyDeviceToHost, m->stream));
             HIPC(hipStreamSynchronize(m->stream));
         }
+    });
+}
+
+scxpca_status scxpca_pflog_pca_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, const scxpca_params *p,
+                                        int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit,
+                                        double *baseline, int device) {
+    return guarded([&] {
+        if (!x || !pp || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        run_pflog_streamed(x, pp, p, shard_rows, out, fit, baseline, device);
     });
 }
 

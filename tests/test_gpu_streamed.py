@@ -201,3 +201,30 @@ def test_streamed_general_layout(gpu, monkeypatch):
     say(f"streamed general: sigma vs resident {se:.2e}")
     assert se <= 1e-5
     np.testing.assert_allclose(got["mean"], res["mean"], rtol=1e-12, atol=1e-300)
+
+
+@pytest.mark.parametrize("alpha,form,shard", [(None, 1, 700), (0.2, 0, 97), (None, 0, 3000)])
+def test_streamed_pflog_matches_resident(gpu, alpha, form, shard):
+    """PFlog over streamed shards (scxpca_pflog_pca_streamed) against the
+    resident PFlog matrix: alpha and the baseline identical (the raw moments
+    are exact sums; each row's baseline is summed in the same order), the
+    spectrum to 1e-5, the closed-form total variance to 1e-12."""
+    x = cfg1_head(3000)
+    ip, ix, dv, shape = arrays(x)
+    say(f"case: streamed pflog alpha={alpha} form={form} shard_rows={shard}")
+    kw = dict(n_comps=30, n_oversamples=10, n_power_iterations=3, seed=0, qr_method=1)
+    got, fit = gpu.pflog_pca_streamed(ip, ix, dv, shape, shard, alpha=alpha, delta_form=form, **kw)
+    m = gpu.Matrix(ip, ix, dv, shape, pflog={"alpha": alpha, "delta_form": form})
+    try:
+        rfit = m.pflog_fit()
+        res = m.pca(**kw)
+    finally:
+        m.close()
+    assert fit["alpha"] == rfit["alpha"] and fit["alpha_source"] == rfit["alpha_source"]
+    np.testing.assert_array_equal(fit["baseline"], rfit["baseline"])
+    se = sigma_rel_err(res["sigma"][:10], got["sigma"][:10]).max()
+    say(f"streamed pflog: sigma vs resident {se:.2e}")
+    assert se <= 1e-5
+    assert abs(got["total_variance"] - res["total_variance"]) <= 1e-12 * res["total_variance"]
+    np.testing.assert_allclose(got["mean"], res["mean"], rtol=1e-12, atol=1e-300)
+    assert max_principal_angle(res["components"].T[:, :3].astype(np.float64), got["components"].T[:, :3].astype(np.float64)) <= 1e-4
