@@ -476,26 +476,44 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
         # delta_from_raw_csr's count checks to ValueError (pflog.rs:329-350, 375-397, 460-470)
         return ValueError(str(e)) if str(e).startswith("pflog:") else RuntimeError(str(e))
 
-    try:
-        m = _lib.Matrix(raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
-                        raw.data.astype(np.float32, copy=False), raw.shape, device=device_id,
-                        pflog={"alpha": alpha, "delta_form": form})
-    except ValueError as e:
-        raise runtime(e) from None
     n_elems = raw.shape[0] * raw.shape[1]
-    try:
-        fit = m.pflog_fit(baseline=True)
-        res = None
-        dense = m.pflog_dense() if want_dense and n_elems <= dense_max_elems else None
-        if want_pca:
-            try:
-                res = m.pca(n_comps=n_components, n_oversamples=n_oversamples,
-                            n_power_iterations=n_power_iterations, zero_center=zero_center,
-                            seed=int(random_state) & ((1 << 64) - 1), qr_method=_lib.QR_HOUSEHOLDER)
-            except (ValueError, _lib.ScxPcaError) as e:
-                raise RuntimeError(str(e)) from None
-    finally:
-        m.close()
+    ip, ix, dv = (raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
+                  raw.data.astype(np.float32, copy=False))
+    k = max(1, min(n_components + n_oversamples, *raw.shape))
+    seed = int(random_state) & ((1 << 64) - 1)
+    res = dense = None
+    if want_pca and not resident_fits(raw.shape[0], raw.shape[1], raw.nnz, k, device_id):
+        # the matrix does not stay in HBM: PFlog over streamed shards
+        if want_dense:
+            raise RuntimeError("pflog: the dense transform needs the matrix resident in HBM; use store='pca'")
+        try:
+            res, fit = _lib.pflog_pca_streamed(ip, ix, dv, raw.shape,
+                                               stream_shard_rows(raw.shape[0], raw.shape[1], raw.nnz, k, device_id),
+                                               alpha=alpha, delta_form=form, n_comps=n_components,
+                                               n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
+                                               zero_center=zero_center, seed=seed, qr_method=_lib.QR_HOUSEHOLDER,
+                                               device=device_id)
+        except ValueError as e:
+            raise runtime(e) from None
+        resident = False
+    else:
+        resident = True
+        try:
+            m = _lib.Matrix(ip, ix, dv, raw.shape, device=device_id, pflog={"alpha": alpha, "delta_form": form})
+        except ValueError as e:
+            raise runtime(e) from None
+        try:
+            fit = m.pflog_fit(baseline=True)
+            dense = m.pflog_dense() if want_dense and n_elems <= dense_max_elems else None
+            if want_pca:
+                try:
+                    res = m.pca(n_comps=n_components, n_oversamples=n_oversamples,
+                                n_power_iterations=n_power_iterations, zero_center=zero_center, seed=seed,
+                                qr_method=_lib.QR_HOUSEHOLDER)
+                except (ValueError, _lib.ScxPcaError) as e:
+                    raise RuntimeError(str(e)) from None
+        finally:
+            m.close()
 
     uns = {"version": "v4", "alpha": fit["alpha"], "pseudocount": fit["pseudocount"],
            "alpha_source": fit["alpha_source"]}
@@ -518,5 +536,5 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
                 f"dense_max_elems if you have the RAM.")
         adata.layers[layer_out or "pflog"] = dense
     container = dict(adata.uns.get("scx_accel", {}) or {})
-    container["pflog"] = _route_stamp(device_id, "default", False, True)
+    container["pflog"] = _route_stamp(device_id, "default", False, resident)
     adata.uns["scx_accel"] = container

@@ -391,3 +391,24 @@ def test_accel_pflog_dense(gpu):
     with pytest.raises(RuntimeError, match="over the dense_max_elems=1000 guard"):
         accel.pflog(g, store="all", n_components=10, device="gpu", dense_max_elems=1000)
     assert "X_pflog_pca" in g.obsm and "pflog" not in g.layers
+
+
+def test_accel_pflog_streams_when_not_resident(gpu, monkeypatch):
+    """accel.pflog takes the streamed PFlog route when the residency pre-flight
+    declines: the same alpha, baseline and spectrum, stamped resident_csr = False."""
+    from scx_amd import accel
+    from scx_amd.adata import AnnDataLite
+
+    x = cfg1_head(2000)
+    out = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("SCX_GPU_PCA_RESIDENT", env)
+        a = AnnDataLite(x.copy())
+        say(f"case: accel.pflog SCX_GPU_PCA_RESIDENT={env}")
+        accel.pflog(a, n_components=20, device="gpu")
+        out[env] = a
+    r, s_ = out["1"], out["0"]
+    assert s_.uns["scx_accel"]["pflog"]["resident_csr"] is False and r.uns["scx_accel"]["pflog"]["resident_csr"] is True
+    assert s_.uns["pflog"] == r.uns["pflog"]
+    np.testing.assert_array_equal(np.asarray(s_.obs["pflog_baseline"]), np.asarray(r.obs["pflog_baseline"]))
+    np.testing.assert_allclose(s_.uns["X_pflog_pca_singular_values"], r.uns["X_pflog_pca_singular_values"], rtol=1e-5)
