@@ -126,3 +126,35 @@ def test_cfg5_full_size_identities(gpu):
         assert ang <= 1e-3
     finally:
         m.close()
+
+
+@pytest.mark.skipif(os.environ.get("SCX_FULL_CFG5") != "1",
+                    reason="full 2,000,000 x 33,538 matrix (4.0e9 nonzeros, 32 GB of host CSR): "
+                           "set SCX_FULL_CFG5=1 to run")
+def test_cfg5_full_size_streamed_matches_resident(gpu):
+    """The streamed operator at the full size (8 shards of 250,000 rows, the
+    whole CSR over PCIe on each of the 11 passes) against the resident
+    matrix: singular values to 1e-5, the gap-separated leading block to
+    1e-4 rad (measured: sigma 2.5e-7; top-3 8.4e-5 rad with sigma 2 and 3
+    0.04 % apart), mean to 1e-12."""
+    w = synth.WORKLOADS["pca_2m_x_33538_k100"]
+    x = synth.generate(w)
+    say(f"case: full-size streamed vs resident {x.shape} nnz={x.nnz}")
+    kw = dict(n_comps=w.n_comps, n_oversamples=w.n_oversamples, n_power_iterations=w.n_power_iterations, seed=0,
+              qr_method=1)
+    m = gpu.Matrix(x.indptr, x.indices, x.data, x.shape, target_sum=w.target_sum, log1p=True)
+    try:
+        res = m.pca(**kw)
+    finally:
+        m.close()
+    got = gpu.randomized_pca_streamed(x.indptr, x.indices, x.data, x.shape, 250_000, target_sum=w.target_sum,
+                                      log1p=True, **kw)
+    se = sigma_rel_err(res["sigma"][:10], got["sigma"][:10]).max()
+    # the leading block a 1 % gap separates (sigma 2 and 3 nearly coincide here)
+    j = max(1, separated_rank(res["sigma"], 10, 0.01))
+    ang = max_principal_angle(res["components"].T[:, :j].astype(np.float64), got["components"].T[:, :j].astype(np.float64))
+    say(f"full-size streamed: sigma vs resident {se:.2e}, angle top{j} {ang:.2e}, "
+        f"device s {res['seconds_device']:.3f} / {got['seconds_device']:.3f}")
+    assert se <= 1e-5
+    assert ang <= 1e-4
+    np.testing.assert_allclose(got["mean"], res["mean"], rtol=1e-12, atol=1e-300)
