@@ -94,12 +94,22 @@ typedef struct {
     double total_variance;
     int32_t tv_recomputed;   /* 1 if the cancellation guard re-ran the stable pass */
     double seconds_device;   /* device time of the PCA (events on the library stream) */
+    int32_t resident;        /* 1: the CSR was resident in HBM; 0: streamed from host memory
+                                (GpuPcaResult::resident_csr, scx-gpu/src/gpu_pca.rs:76-98) */
 } scxpca_result;
 
 typedef struct scxpca_matrix scxpca_matrix;
 
 /* One-shot: upload (or borrow) the CSR, run the PCA, free device memory.
- * Replaces randomized_pca_gpu (scx-accel/src/pca/gpu.rs:32-78). */
+ * Replaces randomized_pca_gpu (scx-accel/src/pca/gpu.rs:32-78). A host CSR
+ * goes through the residency pre-flight of try_build_resident_csr
+ * (scx-gpu/src/gpu_pca_resident.rs:87-135: resident when this build's
+ * footprint fits free HBM / 1.2 and SCX_GPU_PCA_RESIDENT is not "0"); a
+ * runtime failure (out of memory, HIP error) while the resident matrix is
+ * built is a decline, and the call continues on the streamed operator
+ * (decline_on_runtime_failure, scx-gpu/src/gpu_pca.rs:547-550,
+ * scx-gpu/src/error.rs:141-162); input defects still fail the call.
+ * out->resident tells which route ran. A device CSR is always resident. */
 scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p,
                                     scxpca_result *out, int device);
 

@@ -69,7 +69,7 @@ class Params(ctypes.Structure):
 class Result(ctypes.Structure):
     _fields_ = [("embeddings", _vp), ("components", _vp), ("variance", _vp), ("variance_ratio", _vp),
                 ("mean", _vp), ("sigma", _vp), ("on_device", _i32), ("k", _i64), ("total_variance", _dbl),
-                ("tv_recomputed", _i32), ("seconds_device", _dbl)]
+                ("tv_recomputed", _i32), ("seconds_device", _dbl), ("resident", _i32)]
 
 
 class PflogParams(ctypes.Structure):
@@ -320,7 +320,7 @@ class Matrix:
         else:
             emb, comps = out["embeddings"], out["components"]
             on_dev = 1
-        res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), on_dev, 0, 0.0, 0, 0.0)
+        res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), on_dev, 0, 0.0, 0, 0.0, 0)
         check(fn(self.handle, ctypes.byref(prm), ctypes.byref(res)))
         return {
             "embeddings": emb,
@@ -340,6 +340,35 @@ class Matrix:
 
     def project_backward(self, dz, k, mean, dw, stream=None):
         check(lib().scxpca_project_backward(self.handle, ptr(dz), _i64(k), ptr(mean), ptr(dw), _vp(stream or 0)))
+
+
+def _result_dict(res, emb, comps, var, ratio, mean, sigma, n_comps, nv, zero_center) -> dict:
+    return {"embeddings": emb, "components": comps, "variance": var[:n_comps], "variance_ratio": ratio[:n_comps],
+            "mean": mean[:nv] if zero_center else None, "sigma": sigma[: res.k], "k": int(res.k),
+            "total_variance": float(res.total_variance), "tv_recomputed": bool(res.tv_recomputed),
+            "seconds_device": float(res.seconds_device), "resident": bool(res.resident)}
+
+
+def randomized_pca(indptr, indices, data, shape, n_comps=50, n_oversamples=10, n_power_iterations=2,
+                   zero_center=True, seed=0, qr_method=QR_HOUSEHOLDER, target_sum=None, log1p=False,
+                   precision=PRECISION_FP32, device=0) -> dict:
+    """One-shot randomized PCA of a host CSR (``scxpca_randomized_pca``):
+    resident when the pre-flight admits it and the build succeeds, else the
+    streamed operator; the result dict of ``Matrix.pca`` plus ``resident``."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(indices, dtype=np.uint32)
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    n, nv = int(shape[0]), int(shape[1])
+    csr = Csr(n, nv, ptr(indptr), ptr(indices), ptr(data), 0)
+    prm = Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), qr_method, seed,
+                 float(target_sum or 0.0), int(bool(log1p)), precision)
+    k = max(1, min(n_comps + n_oversamples, n, nv))
+    var, ratio, mean, sigma = np.zeros(max(n_comps, 1)), np.zeros(max(n_comps, 1)), np.zeros(max(nv, 1)), np.zeros(k)
+    emb = np.empty((n, n_comps), dtype=np.float32)
+    comps = np.empty((n_comps, nv), dtype=np.float32)
+    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0, 0)
+    check(lib().scxpca_randomized_pca(ctypes.byref(csr), ctypes.byref(prm), ctypes.byref(res), device))
+    return _result_dict(res, emb, comps, var, ratio, mean, sigma, n_comps, nv, zero_center)
 
 
 def randomized_pca_streamed(indptr, indices, data, shape, shard_rows, n_comps=50, n_oversamples=10,
@@ -362,7 +391,7 @@ def randomized_pca_streamed(indptr, indices, data, shape, shard_rows, n_comps=50
     sigma = np.zeros(k)
     emb = np.empty((n, n_comps), dtype=np.float32)
     comps = np.empty((n_comps, nv), dtype=np.float32)
-    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0)
+    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0, 0)
     check(lib().scxpca_randomized_pca_streamed(ctypes.byref(csr), ctypes.byref(prm), _i64(int(shard_rows)),
                                                ctypes.byref(res), device))
     return {"embeddings": emb, "components": comps, "variance": var[:n_comps], "variance_ratio": ratio[:n_comps],
@@ -393,7 +422,7 @@ def pflog_pca_streamed(indptr, indices, data, shape, shard_rows, alpha=None, del
     emb = np.empty((n, n_comps), dtype=np.float32)
     comps = np.empty((n_comps, nv), dtype=np.float32)
     base = np.zeros(n)
-    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0)
+    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0, 0)
     f = PflogFit()
     check(lib().scxpca_pflog_pca_streamed(ctypes.byref(csr), ctypes.byref(pp), ctypes.byref(prm),
                                           _i64(int(shard_rows)), ctypes.byref(res), ctypes.byref(f), ptr(base),

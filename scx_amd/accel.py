@@ -257,14 +257,28 @@ def stream_shard_rows(n: int, nv: int, nnz: int, k: int, device_id: int) -> int:
     return max(256, min(rows, max(n, 1)))
 
 
-def resolve_method(method: str, n_vars: int) -> str:
-    """The solver a ``method`` request runs (resolve_cpu_pca_method,
-    scx-accel/src/pca/cpu.rs:1643-1660)."""
+def resolve_method(method: str, n_vars: int, device_kind: str = "gpu") -> str:
+    """The solver a ``method`` request runs. On the GPU every request but an
+    explicit ``"covariance"`` is the randomized route, as the reference's GPU
+    dispatch resolves it (pyscx/src/accel/pca.rs:384-389: the native GPU core
+    is randomized only); ``"covariance"`` keeps this build's exact GPU
+    covariance route reachable on request. On the CPU router
+    (resolve_cpu_pca_method, scx-accel/src/pca/cpu.rs:1643-1660) ``"auto"``
+    is the covariance route at <= 5,000 genes."""
     if method not in ("auto", "covariance", "randomized"):
         raise ValueError(f"Invalid method={method!r}; expected 'auto', 'covariance', or 'randomized'")
     if method == "auto":
+        if device_kind == "gpu":
+            return "randomized"
         return "covariance" if n_vars <= COVARIANCE_PCA_THRESHOLD else "randomized"
     return method
+
+
+def _runtime_failure(e: BaseException) -> bool:
+    """A device-side failure (out of memory, HIP error) rather than an input
+    defect: the resident build declines on it (GpuError::is_runtime_failure,
+    scx-gpu/src/error.rs:77-102)."""
+    return isinstance(e, MemoryError) or (isinstance(e, _lib.ScxPcaError) and e.status == _lib.ERR_DEVICE)
 
 
 def _route_stamp(device_id, spmm_policy, allow_tf32, resident):
@@ -297,13 +311,20 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
         mask_var=None, *, precision: str = "fp32") -> None:
     """PCA of ``adata.X`` on the GPU; results written into ``adata``.
 
-    ``method`` is resolved as the reference's CPU router does
-    (resolve_cpu_pca_method, scx-accel/src/pca/cpu.rs:1643-1660): "covariance"
-    and "randomized" as named, "auto" the covariance route at <= 5,000
-    (effective) genes, else randomized. The covariance route is exact (f64
-    eigendecomposition of the sample covariance, cpu.rs:1663-1854) and ignores
-    ``n_oversamples``, ``n_power_iterations``, ``random_state``, ``qr_method``
-    and ``precision``.
+    ``method``: "auto" and "randomized" run the randomized route, as the
+    reference's GPU dispatch resolves every method (pyscx/src/accel/pca.rs:
+    384-389); an explicit "covariance" runs this build's exact GPU covariance
+    route (f64 eigendecomposition of the sample covariance, the algorithm of
+    covariance_pca, scx-accel/src/pca/cpu.rs:1663-1854), which ignores
+    ``n_oversamples``, ``n_power_iterations``, ``random_state``,
+    ``qr_method`` and ``precision``.
+
+    The randomized route keeps the CSR resident in HBM when the pre-flight
+    admits it (gpu_pca_resident.rs:87-135); when it does not, or when
+    building the resident matrix fails at run time (out of memory, a HIP
+    error), the CSR is streamed from host memory shard by shard instead
+    (decline_on_runtime_failure, scx-gpu/src/gpu_pca.rs:547-550); the route
+    stamp's ``resident_csr`` records which ran.
 
     ``precision`` (keyword only, not in the reference): ``"fp32"`` (default)
     or ``"bf16"`` — the BF16-sketch form streams X~ as bf16 in the sketch
@@ -342,26 +363,40 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
         raw = raw.sorted_indices()
     n_obs, n_vars = raw.shape
 
-    route = resolve_method(method, n_vars)
+    route = resolve_method(method, n_vars, kind)
     qr = _lib.QR_CHOLESKY if qr_method == "cholesky" else _lib.QR_HOUSEHOLDER
     seed = int(random_state) & ((1 << 64) - 1)
     k = max(1, min(n_comps + n_oversamples, n_obs, n_vars))
     resident = route == "covariance" or resident_fits(n_obs, n_vars, raw.nnz, k, device_id, precision == "bf16")
     ip, ix, dv = (raw.indptr.astype(np.int64, copy=False), raw.indices.astype(np.uint32, copy=False),
                   raw.data.astype(np.float32, copy=False))
+    def streamed():
+        # the streaming operator (gpu_pca.rs:566-590): the CSR stays on the host
+        if precision == "bf16":
+            raise ValueError("precision='bf16' needs the matrix resident in HBM; the streamed route is FP32")
+        return _lib.randomized_pca_streamed(ip, ix, dv, raw.shape,
+                                            stream_shard_rows(n_obs, n_vars, raw.nnz, k, device_id),
+                                            n_comps=n_comps, n_oversamples=n_oversamples,
+                                            n_power_iterations=n_power_iterations, zero_center=zero_center,
+                                            seed=seed, qr_method=qr, target_sum=lazy.target_sum,
+                                            log1p=lazy.log1p, device=device_id)
+
     try:
+        m = None
+        if resident:
+            try:
+                m = _lib.Matrix(ip, ix, dv, raw.shape, target_sum=lazy.target_sum, log1p=lazy.log1p,
+                                device=device_id)
+            except (MemoryError, _lib.ScxPcaError) as e:
+                # a runtime failure while building the resident matrix declines to
+                # the streamed operator; the randomized route only (the covariance
+                # route needs the resident matrix)
+                if not _runtime_failure(e) or route != "randomized":
+                    raise
+                resident = False
         if not resident:
-            # the streaming operator (gpu_pca.rs:566-590): the CSR stays on the host
-            if precision == "bf16":
-                raise ValueError("precision='bf16' needs the matrix resident in HBM; the streamed route is FP32")
-            res = _lib.randomized_pca_streamed(ip, ix, dv, raw.shape,
-                                               stream_shard_rows(n_obs, n_vars, raw.nnz, k, device_id),
-                                               n_comps=n_comps, n_oversamples=n_oversamples,
-                                               n_power_iterations=n_power_iterations, zero_center=zero_center,
-                                               seed=seed, qr_method=qr, target_sum=lazy.target_sum,
-                                               log1p=lazy.log1p, device=device_id)
+            res = streamed()
         else:
-            m = _lib.Matrix(ip, ix, dv, raw.shape, target_sum=lazy.target_sum, log1p=lazy.log1p, device=device_id)
             try:
                 if route == "covariance":
                     res = m.pca_covariance(n_comps=n_comps, zero_center=zero_center)

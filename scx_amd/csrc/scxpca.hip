@@ -384,7 +384,9 @@ std::string fmt_f32(float v) {
 // report the first offending entry as the reference does: estimate_alpha's
 // finite guard (finite.rs:22-32), delta_from_raw_csr (pyscx/src/accel/
 // pflog.rs:375-397) and pflog_baseline_from_delta's guard (pflog.rs:46-56).
-void throw_invalid(const scxpca_matrix *m, int h) {
+// row_base / entry_base: the matrix's first row and entry inside the caller's
+// CSR (a streamed shard reports positions in the whole matrix).
+void throw_invalid(const scxpca_matrix *m, int h, int64_t row_base = 0, int64_t entry_base = 0) {
     if (h & 4) throw Error(SCXPCA_ERR_INVALID, "indptr must be non-decreasing");
     if (h & 2) throw Error(SCXPCA_ERR_INVALID, "column index out of range");
     if (h & 1) throw Error(SCXPCA_ERR_INVALID, "device CSR rows must have sorted column indices");
@@ -396,7 +398,7 @@ void throw_invalid(const scxpca_matrix *m, int h) {
         HIPC(hipMemcpy(&v, m->val + pos, sizeof(float), hipMemcpyDefault));
         if (!(m->flags & kXfScale))
             throw Error(SCXPCA_ERR_INVALID, "HVG input contains a non-finite value (" + fmt_f32(v) +
-                                                ") at nonzero index " + std::to_string(pos) +
+                                                ") at nonzero index " + std::to_string(pos + entry_base) +
                                                 "; this accelerator requires finite input \u2014 filter/QC NaN "
                                                 "and Inf before computing variance");
         if (m->flags & kXfLogF64) {
@@ -410,13 +412,13 @@ void throw_invalid(const scxpca_matrix *m, int h) {
             }
             if (!std::isfinite(v))
                 throw Error(SCXPCA_ERR_INVALID, "pflog: non-finite count " + fmt_f32(v) + " at cell " +
-                                                    std::to_string(lo) + "; counts must be finite");
+                                                    std::to_string(lo + row_base) + "; counts must be finite");
             throw Error(SCXPCA_ERR_INVALID, "pflog: negative count " + fmt_f32(v) + " at cell " +
-                                                std::to_string(lo) + "; counts must be non-negative");
+                                                std::to_string(lo + row_base) + "; counts must be non-negative");
         }
         const float d = std::log1p(static_cast<float>(static_cast<double>(v) * m->target));
         throw Error(SCXPCA_ERR_INVALID, "PFlog input contains a non-finite value (" + fmt_f32(d) +
-                                            ") at nonzero index " + std::to_string(pos) +
+                                            ") at nonzero index " + std::to_string(pos + entry_base) +
                                             "; PFlog requires finite raw counts \u2014 filter/QC NaN and Inf first");
     }
 }
@@ -480,6 +482,14 @@ void size_derived(scxpca_matrix *m) {
 
 int transform_flags(double target, int log1p) { return (target > 0.0 ? kXfNorm : 0) | (log1p ? kXfLog1p : 0); }
 
+// SCXPCA_FAIL_RESIDENT_BUILD=1 makes every resident build of a host CSR fail
+// as if HBM ran out while its derived structures were allocated: a host-side
+// switch for exercising the decline to the streamed operator.
+bool forced_resident_failure() {
+    const char *e = std::getenv("SCXPCA_FAIL_RESIDENT_BUILD");
+    return e && e[0] == '1' && e[1] == '\0';
+}
+
 // Upload (or borrow) the CSR and prepare it under the transform (target,
 // flags: the kXf* bits).
 void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flags) {
@@ -527,6 +537,8 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flag
         m->val = m->own_val.as<float>();
     }
     if (m->n_obs == 0) return;
+    if (!x->on_device && forced_resident_failure())
+        throw Error(SCXPCA_ERR_OOM, "GPU out of memory: resident CSR build (forced by SCXPCA_FAIL_RESIDENT_BUILD)");
     size_derived(m);
     int h = prepare(m);
     if ((h & 1) && !x->on_device) {
@@ -1377,6 +1389,7 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     out->seconds_device = ms * 1e-3;
+    out->resident = 1;
 }
 
 // The covariance genes limit: the n_vars x n_vars f64 covariance and the
@@ -1410,6 +1423,7 @@ void run_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result 
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     out->seconds_device = ms * 1e-3;
+    out->resident = 1;
 }
 
 template <int KP>
@@ -1815,7 +1829,7 @@ struct StreamOps {
             m->flags = eg.m->flags;
         }
         size_derived(m);
-        throw_invalid(m, prepare(m));
+        throw_invalid(m, prepare(m), r0, a);
         sl.r0 = r0;
         return sl;
     }
@@ -1990,6 +2004,47 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     out->seconds_device = ms * 1e-3;
+    out->resident = 0;
+}
+
+// HBM a resident matrix and its PCA workspace take in this build: the CSR
+// copy, row factors, X~ and its chunk index, the tile-CSC, the split offsets
+// and per-tile partials while they stay under the metadata cap, the n x KP
+// panels and the n_vars x KP workspace (scx_amd.accel._resident_bytes is the
+// same model).
+double resident_footprint(int64_t n, int64_t nv, int64_t nnz, int64_t k) {
+    const double kp = k <= 64 ? 64.0 : 128.0;
+    const double tiles = static_cast<double>((n + kTileRows - 1) / kTileRows);
+    const double nsplit = static_cast<double>((nv + kCB - 1) / kCB);
+    const double dn = static_cast<double>(n), dv = static_cast<double>(nv), dz = static_cast<double>(nnz);
+    double b = 8.0 * (dn + 1) + 8.0 * dz;
+    b += 8.0 * dn + 4.0 * dz + 2.0 * dz + 5.0 * (dz + kEntryPad);
+    b += 4.0 * tiles * (dv + 1);
+    const double split = 4.0 * dn * (nsplit + 1) + 16.0 * tiles * dv;
+    if (split <= kMaxSplitBytes) b += split;
+    b += 2 * 4.0 * dn * kp + 2 * 4.0 * dv * kp + 4 * 8.0 * dv * kp + 4 * 256.0 * dv * kp + 8.0 * 1024 * kp * kp;
+    return b;
+}
+
+// try_build_resident_csr's pre-flight (scx-gpu/src/gpu_pca_resident.rs:
+// 54,71-74,87-135): resident when the footprint fits free HBM / 1.2 and
+// SCX_GPU_PCA_RESIDENT is not "0".
+bool resident_preflight(int64_t n, int64_t nv, int64_t nnz, int64_t k, size_t free_bytes) {
+    const char *e = std::getenv("SCX_GPU_PCA_RESIDENT");
+    if (e && std::strcmp(e, "0") == 0) return false;
+    return resident_footprint(n, nv, nnz, k) <= static_cast<double>(free_bytes) / 1.2;
+}
+
+// Rows per shard of the streamed operator: two shards' structures within an
+// eighth of the usable free HBM, a multiple of the tile height, at least one
+// tile (scx_amd.accel.stream_shard_rows is the same rule).
+int64_t streamed_shard_rows(int64_t n, int64_t nv, int64_t nnz, int64_t k, size_t free_bytes) {
+    const double budget = static_cast<double>(free_bytes) / 1.2 / 8.0;
+    const double per_row = 8.0 + 8.0 * static_cast<double>(k) +
+                           21.0 * static_cast<double>(nnz) / static_cast<double>(std::max<int64_t>(n, 1)) +
+                           4.0 * static_cast<double>((nv + kCB - 1) / kCB + 1) + 20.0 * static_cast<double>(nv + 1) / 256.0;
+    int64_t rows = static_cast<int64_t>(budget / (2.0 * per_row)) / kTileRows * kTileRows;
+    return std::max<int64_t>(kTileRows, std::min<int64_t>(rows, std::max<int64_t>(n, 1)));
 }
 
 // ---------------------------------------------------------------------------
@@ -2100,10 +2155,37 @@ scxpca_status scxpca_randomized_pca(const scxpca_csr *x, const scxpca_params *p,
         if (p->n_comps <= 0) throw Error(SCXPCA_ERR_INVALID, "n_components must be > 0");
         if (x->n_obs == 0 || x->n_vars == 0) throw Error(SCXPCA_ERR_INVALID, "matrix must be non-empty");
         DeviceGuard dg(device);
+        const bool host = !x->on_device;
+        const int64_t nnz = host ? x->indptr[x->n_obs] : 0;
+        const int64_t k = std::min<int64_t>(std::min<int64_t>(p->n_comps + p->n_oversamples, x->n_vars), x->n_obs);
+        size_t free_b = 0, total_b = 0;
+        HIPC(hipMemGetInfo(&free_b, &total_b));
+        if (host && p->precision == SCXPCA_PRECISION_FP32 &&
+            !resident_preflight(x->n_obs, x->n_vars, nnz, k, free_b)) {
+            // declined by the pre-flight: the streamed operator (gpu_pca.rs:566-590)
+            run_pca_streamed(x, p, streamed_shard_rows(x->n_obs, x->n_vars, nnz, k, free_b), out, device);
+            out->resident = 0;
+            return;
+        }
         std::unique_ptr<scxpca_matrix> m(new scxpca_matrix());
         m->device = device;
-        build_matrix(m.get(), x, p->target_sum, transform_flags(p->target_sum, p->log1p));
+        try {
+            build_matrix(m.get(), x, p->target_sum, transform_flags(p->target_sum, p->log1p));
+        } catch (const Error &e) {
+            // a runtime failure while building the resident matrix is a decline
+            // (decline_on_runtime_failure, scx-gpu/src/error.rs:141-162); input
+            // defects propagate, as does anything the streamed route cannot take
+            const bool runtime = e.status == SCXPCA_ERR_OOM || e.status == SCXPCA_ERR_DEVICE;
+            if (!runtime || !host || p->precision != SCXPCA_PRECISION_FP32) throw;
+            m.reset();
+            (void)hipGetLastError();  // the failed allocation's error must not surface at a later launch check
+            HIPC(hipMemGetInfo(&free_b, &total_b));
+            run_pca_streamed(x, p, streamed_shard_rows(x->n_obs, x->n_vars, nnz, k, free_b), out, device);
+            out->resident = 0;
+            return;
+        }
         run_pca(m.get(), p, out);
+        out->resident = 1;
     });
 }
 

@@ -93,10 +93,12 @@ def test_covariance_rejects_too_many_components(gpu):
         mat.close()
 
 
-def test_accel_auto_routes_to_covariance(gpu):
-    """pca(method="auto") at <= 5,000 genes writes the covariance route's
-    result (variance equal to the direct covariance call), method="randomized"
-    the randomized one (a different, approximate result)."""
+def test_accel_auto_routes_to_randomized(gpu):
+    """On the GPU pca(method="auto") runs the randomized route, as the
+    reference's GPU dispatch does (pyscx/src/accel/pca.rs:384-389): its result
+    equals method="randomized" bit for bit; an explicit method="covariance"
+    writes the covariance route's result (variance equal to the direct
+    covariance call), which differs from the approximate randomized one."""
     from scx_amd import accel
     from scx_amd.adata import AnnDataLite
 
@@ -117,6 +119,6 @@ def test_accel_auto_routes_to_covariance(gpu):
         say(f"case: accel.pca method={method}")
         accel.pca(a, n_comps=10, device="gpu", method=method)
         results[method] = a.uns["pca"]["variance"]
-    np.testing.assert_allclose(results["auto"], direct["variance"], rtol=1e-12)
+    np.testing.assert_array_equal(results["auto"], results["randomized"])
     np.testing.assert_allclose(results["covariance"], direct["variance"], rtol=1e-12)
     assert not np.allclose(results["randomized"], direct["variance"], rtol=1e-12, atol=0)

@@ -182,6 +182,79 @@ def test_accel_pca_streams_when_not_resident(gpu, monkeypatch):
         accel.pca(a, n_comps=20, device="gpu", method="randomized", precision="bf16")
 
 
+def test_resident_build_failure_declines_to_streaming(gpu, monkeypatch):
+    """A runtime failure while the resident matrix is built (forced from the
+    host by SCXPCA_FAIL_RESIDENT_BUILD=1, reported as out of memory) is a
+    decline: accel.pca and the one-shot C entry continue on the streamed
+    operator and return the resident route's result, stamped
+    resident_csr = False (decline_on_runtime_failure,
+    scx-gpu/src/gpu_pca.rs:547-550, scx-gpu/src/error.rs:141-162). An input
+    defect still fails the call."""
+    from scx_amd import accel
+    from scx_amd.adata import AnnDataLite
+
+    x = cfg1_head(2000)
+    ip, ix, dv, shape = arrays(x)
+    kw = dict(n_comps=20, n_oversamples=10, n_power_iterations=2, seed=0, qr_method=1, target_sum=1e4, log1p=True)
+    say("case: one-shot resident")
+    ref = gpu.randomized_pca(ip, ix, dv, shape, **kw)
+    assert ref["resident"] is True
+    monkeypatch.setenv("SCXPCA_FAIL_RESIDENT_BUILD", "1")
+    with pytest.raises(MemoryError, match="forced by SCXPCA_FAIL_RESIDENT_BUILD"):
+        gpu.Matrix(ip, ix, dv, shape, target_sum=1e4, log1p=True)
+    say("case: one-shot with the resident build failing")
+    got = gpu.randomized_pca(ip, ix, dv, shape, **kw)
+    assert got["resident"] is False
+    se = sigma_rel_err(ref["sigma"][:10], got["sigma"][:10]).max()
+    say(f"declined one-shot: sigma vs resident {se:.2e}")
+    assert se <= 1e-5
+    a = AnnDataLite(x.copy())
+    accel.normalize_total(a, target_sum=1e4)
+    accel.log1p(a)
+    say("case: accel.pca with the resident build failing")
+    accel.pca(a, n_comps=20, device="gpu")
+    assert a.uns["scx_accel"]["pca"]["resident_csr"] is False
+    np.testing.assert_allclose(a.uns["pca"]["variance"][:10], ref["variance"][:10], rtol=1e-5)
+    # an input defect still fails the call (here found by the streamed
+    # operator's own checks); the BF16 sketch cannot stream, so its build
+    # failure propagates
+    bad = ix.copy()
+    bad[5] = shape[1] + 3
+    with pytest.raises(ValueError, match="column index out of range"):
+        gpu.randomized_pca(ip, bad, dv, shape, **kw)
+    with pytest.raises(MemoryError, match="forced by SCXPCA_FAIL_RESIDENT_BUILD"):
+        gpu.randomized_pca(ip, ix, dv, shape, precision=gpu.PRECISION_BF16_SKETCH, **kw)
+    monkeypatch.delenv("SCXPCA_FAIL_RESIDENT_BUILD")
+    monkeypatch.setenv("SCX_GPU_PCA_RESIDENT", "0")
+    say("case: one-shot with residency declined by the pre-flight")
+    pre = gpu.randomized_pca(ip, ix, dv, shape, **kw)
+    assert pre["resident"] is False
+    np.testing.assert_array_equal(pre["sigma"], got["sigma"])
+
+
+def test_streamed_error_positions_are_global(gpu):
+    """A bad count in a later shard is reported at its position in the whole
+    matrix (cell and nonzero index), as the resident route reports it."""
+    x = cfg1_head(1200)
+    ip, ix, dv, shape = arrays(x)
+    p_bad = int(ip[900]) + 3
+    cell = 900
+    neg = dv.copy()
+    neg[p_bad] = -2.0
+    say("case: streamed pflog negative count in shard 4 of 300-row shards")
+    with pytest.raises(ValueError, match=f"negative count -2 at cell {cell};"):
+        gpu.pflog_pca_streamed(ip, ix, neg, shape, 300, alpha=0.2, delta_form=1, n_comps=10)
+    with pytest.raises(ValueError, match=f"negative count -2 at cell {cell};"):
+        gpu.Matrix(ip, ix, neg, shape, pflog={"alpha": 0.2, "delta_form": 1})
+    nan = dv.copy()
+    nan[p_bad] = np.inf
+    say("case: streamed pflog non-finite value (lazy form) in shard 4")
+    with pytest.raises(ValueError, match=f"at nonzero index {p_bad};"):
+        gpu.pflog_pca_streamed(ip, ix, nan, shape, 300, alpha=0.2, delta_form=0, n_comps=10)
+    with pytest.raises(ValueError, match=f"at nonzero index {p_bad};"):
+        gpu.Matrix(ip, ix, nan, shape, pflog={"alpha": 0.2, "delta_form": 0})
+
+
 def test_streamed_general_layout(gpu, monkeypatch):
     """Shards in the general layout (zero metadata cap: no split offsets, the
     row-parallel forward, stats from the tile-CSC) against the resident

@@ -360,9 +360,14 @@ def test_covariance_oracle_matches_dense_svd():
 
 
 def test_method_routing():
+    """GPU: every method but an explicit "covariance" is randomized
+    (pyscx/src/accel/pca.rs:384-389); the CPU router's "auto" is covariance at
+    <= 5,000 genes (cpu.rs:1643-1660)."""
     from scx_amd import accel
-    assert accel.resolve_method("auto", 5000) == "covariance"
+    assert accel.resolve_method("auto", 5000) == "randomized"
     assert accel.resolve_method("auto", 5001) == "randomized"
+    assert accel.resolve_method("auto", 5000, "cpu") == "covariance"
+    assert accel.resolve_method("auto", 5001, "cpu") == "randomized"
     assert accel.resolve_method("covariance", 30000) == "covariance"
     assert accel.resolve_method("randomized", 10) == "randomized"
     with pytest.raises(ValueError, match="Invalid method='pca'"):
