@@ -212,7 +212,7 @@ def _resident_bytes(n: int, nv: int, nnz: int, k: int, bf16: bool = False) -> in
     copy, X~, chunk indices, tile-CSC, split offsets and per-tile partials
     when they stay under 32 GiB, the n x KP panels and the n_vars x KP f64
     workspace)."""
-    kp = 64 if k <= 64 else 128
+    kp = 64 if k <= 64 else (k + 127) // 128 * 128
     tiles = (n + 255) // 256
     nsplit = (nv + 127) // 128
     b = 8 * (n + 1) + 8 * nnz                         # indptr, indices, values
@@ -222,6 +222,8 @@ def _resident_bytes(n: int, nv: int, nnz: int, k: int, bf16: bool = False) -> in
     if split <= 32 << 30:
         b += split
     b += 2 * 4 * n * kp + 2 * 4 * nv * kp + 4 * 8 * nv * kp + 4 * 256 * nv * kp + 8 * 1024 * kp * kp
+    if k > 128:  # column-major copies of the wide route
+        b += 4 * max(n, nv) * k + 16 * nv * k
     if bf16:
         b += 4 * nnz
     return int(b)

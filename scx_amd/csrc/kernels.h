@@ -461,6 +461,101 @@ __global__ void __launch_bounds__(256) k_omega_panel(const double *__restrict__ 
     if (panel) panel[t] = static_cast<float>(x);
 }
 
+// ---------------------------------------------------------------------------
+// Wide panels (k' > 128). A panel of kpw = nb * 128 columns is stored as nb
+// blocks of 128 columns, block-major: block b of an n x kpw panel is the
+// row-major n x 128 panel at offset b * n * 128, so every block is a panel the
+// KP = 128 SpMM kernels take as it is. The library QR and SVD work on
+// column-major n x k copies.
+// ---------------------------------------------------------------------------
+constexpr int kWideBlock = 128;
+
+__host__ __device__ __forceinline__ int64_t wide_index(int64_t rows, int64_t r, int j) {
+    return static_cast<int64_t>(j / kWideBlock) * rows * kWideBlock + r * kWideBlock + (j % kWideBlock);
+}
+
+// Omega element (r, j) = draw r * k + j into a block-major f32 panel (zero past k).
+__global__ void __launch_bounds__(256) k_omega_blocks(const double *__restrict__ v, const uint32_t *__restrict__ start,
+                                                      int64_t rows, int k, int kpw, float *__restrict__ panel) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= rows * kpw) return;
+    const int64_t r = t / kpw;
+    const int j = static_cast<int>(t % kpw);
+    panel[wide_index(rows, r, j)] = j < k ? static_cast<float>(v[start[r * k + j]]) : 0.f;
+}
+
+// dst (column-major rows x k, ld = rows) <- src (block-major), 64 x 64 tiles
+// through LDS: rows read and columns written contiguously. Grid (rows / 64,
+// k / 64), 256 threads.
+template <typename TS, typename TD>
+__global__ void __launch_bounds__(256) k_blocks_to_cols(const TS *__restrict__ src, int64_t rows, int k,
+                                                        TD *__restrict__ dst) {
+    __shared__ TD tile[64][65];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int j0 = static_cast<int>(blockIdx.y) * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t r = r0 + i;
+        const int j = j0 + tx;
+        tile[i][tx] = (r < rows && j < k) ? static_cast<TD>(src[wide_index(rows, r, j)]) : TD(0);
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int j = j0 + i;
+        const int64_t r = r0 + tx;
+        if (r < rows && j < k) dst[static_cast<int64_t>(j) * rows + r] = tile[tx][i];
+    }
+}
+
+// dst (block-major, kpw columns, zero at columns >= k) <- src (column-major
+// rows x k, ld = rows). Grid (rows / 64, kpw / 64), 256 threads.
+template <typename T>
+__global__ void __launch_bounds__(256) k_cols_to_blocks(const T *__restrict__ src, int64_t rows, int k, int kpw,
+                                                        T *__restrict__ dst) {
+    __shared__ T tile[64][65];
+    const int64_t r0 = static_cast<int64_t>(blockIdx.x) * 64;
+    const int j0 = static_cast<int>(blockIdx.y) * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int i = ty; i < 64; i += 4) {
+        const int j = j0 + i;
+        const int64_t r = r0 + tx;
+        tile[i][tx] = (r < rows && j < k) ? src[static_cast<int64_t>(j) * rows + r] : T(0);
+    }
+    __syncthreads();
+    for (int i = ty; i < 64; i += 4) {
+        const int64_t r = r0 + i;
+        const int j = j0 + tx;
+        if (r < rows && j < kpw) dst[wide_index(rows, r, j)] = tile[tx][i];
+    }
+}
+
+// W (column-major k x nc, f32) = (V S)[:, :nc] from the SVD's V^T (column-major
+// k x k) and the singular values S: the embedding weights of the wide panel.
+__global__ void __launch_bounds__(256) k_wide_emb_weights(const double *__restrict__ Vt, const double *__restrict__ S,
+                                                          int k, int nc, float *__restrict__ W) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= static_cast<int64_t>(k) * nc) return;
+    const int l = static_cast<int>(t % k), j = static_cast<int>(t / k);
+    W[t] = static_cast<float>(Vt[j + static_cast<int64_t>(l) * k] * S[j]);
+}
+
+// status |= bit when a library routine reported a nonzero info word
+__global__ void k_info_status(const int *__restrict__ info, int bit, int *__restrict__ status) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && *info != 0) atomicOr(status, bit);
+}
+
+// columns [c0, c0 + ncols) of a row-major dst (ld columns) <- the first ncols
+// columns of a KP-wide panel
+template <int KP>
+__global__ void __launch_bounds__(256) k_unpack_cols(const float *__restrict__ src, int64_t rows, int ncols,
+                                                     float *__restrict__ dst, int ld, int c0) {
+    const int64_t t = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= rows * ncols) return;
+    const int64_t r = t / ncols;
+    const int j = static_cast<int>(t % ncols);
+    dst[r * ld + c0 + j] = src[r * KP + j];
+}
+
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
                                                  uint16_t *__restrict__ out) {
@@ -1990,17 +2085,19 @@ __global__ void __launch_bounds__(256) k_cov_finalize(const double *__restrict__
 // (f32), and the f32 panel M (n x KP, zero past nc) for the embeddings
 // (cpu.rs:1776-1800).
 template <int KP>
+// c0: the first component of this KP-wide panel (components past 128 come in
+// further panels; var is written by the c0 = 0 launch).
 __global__ void __launch_bounds__(256) k_cov_topk(const double *__restrict__ Z, const double *__restrict__ w,
                                                   int64_t n, int64_t nsel, int nc, float *__restrict__ comps,
-                                                  float *__restrict__ M, double *__restrict__ var) {
+                                                  float *__restrict__ M, double *__restrict__ var, int c0) {
     const int64_t e = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (e < nc) {
+    if (c0 == 0 && e < nc) {
         const double ev = w[nsel - 1 - e];
         var[e] = ev > 0.0 ? ev : 0.0;
     }
     if (e >= n * KP) return;
     const int64_t v = e / KP;
-    const int pc = static_cast<int>(e % KP);
+    const int pc = c0 + static_cast<int>(e % KP);
     const float z = pc < nc ? static_cast<float>(Z[(nsel - 1 - pc) * n + v]) : 0.f;
     M[e] = z;
     if (pc < nc && comps) comps[static_cast<int64_t>(pc) * n + v] = z;

@@ -662,7 +662,8 @@ __global__ void k_mean(const double *sum, int64_t n_vars, double n, double shift
 // ---------------------------------------------------------------------------
 
 template <int KP>
-void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc);
+void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc,
+                  int kpw = 0);
 
 template <int KP>
 struct Engine {
@@ -1222,21 +1223,30 @@ struct Engine {
             comps_dev = comps_tmp.as<float>();
         }
         m->covvar.ensure(sizeof(double) * std::max(nc, 1));
-        k_cov_topk<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->cov.as<double>(), m->evw.as<double>(), nv, nv, nc,
-                                                           comps_dev, m->panel.as<float>(), m->covvar.as<double>());
-        check();
-        // embeddings (X~ - mu) V
+        // embeddings (X~ - mu) V, KP components at a time (more than KP
+        // components: one forward pass per KP-wide slice of V)
         float *Y = m->y.as<float>();
-        if (centered) mean_correction(m->panel.as<float>());
-        spmm_fwd(m->panel.as<float>(), centered, false, Y);
+        DevBuf emb_tmp;
         float *emb_dev = nullptr;
         if (out->on_device) {
             emb_dev = out->embeddings;
-        } else {
+        } else if (nc <= KP) {
             emb_dev = m->q.as<float>();  // n x KP scratch holds the packed n x nc embeddings
+        } else {
+            emb_tmp.ensure(sizeof(float) * n * nc);
+            emb_dev = emb_tmp.as<float>();
         }
-        k_unpack_panel32<KP><<<blocks_for(n * nc), 256, 0, s>>>(Y, n, nc, emb_dev);
-        check();
+        for (int c0 = 0; c0 < nc; c0 += KP) {
+            const int ncb = std::min(KP, nc - c0);
+            k_cov_topk<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->cov.as<double>(), m->evw.as<double>(), nv, nv, nc,
+                                                               comps_dev, m->panel.as<float>(),
+                                                               m->covvar.as<double>(), c0);
+            check();
+            if (centered) mean_correction(m->panel.as<float>());
+            spmm_fwd(m->panel.as<float>(), centered, false, Y);
+            k_unpack_cols<KP><<<blocks_for(n * ncb), 256, 0, s>>>(Y, n, ncb, emb_dev, nc, c0);
+            check();
+        }
         // results back
         std::vector<double> hvar(nc), hsum(nv), hsq(nv);
         rocblas_int hinfo = 0;
@@ -1292,8 +1302,19 @@ int64_t pca_k(const scxpca_matrix *m, const scxpca_params *p) {
 // per-position draw outcomes in parallel, the chain of draw starts walked on
 // the host, values gathered into the f32 panel (KP columns) and/or f64 out64.
 // Scratch comes from the DevBufs given (the matrix's, or the caller's).
+// kpw > 0: panel is a block-major wide panel of kpw columns (k > 128).
 template <int KP>
-void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc) {
+void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *panel, double *out64, OmegaScratch &sc,
+                  int kpw) {
+    auto gather = [&]() {
+        if (kpw > 0)
+            k_omega_blocks<<<blocks_for(static_cast<int64_t>(rows) * kpw), 256, 0, s>>>(
+                sc.v.as<double>(), sc.start.as<uint32_t>(), static_cast<int64_t>(rows), k, kpw, panel);
+        else
+            k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
+                                                                     static_cast<int64_t>(rows), k, panel, out64);
+        HIPC(hipGetLastError());
+    };
     const uint64_t n = rows * static_cast<uint64_t>(k);
     if (n == 0) return;
     uint64_t n_words = n + n / 16 + 4096;
@@ -1342,9 +1363,7 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
                                                                     static_cast<int>(ov.size()));
             HIPC(hipGetLastError());
         }
-        k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
-                                                                 static_cast<int64_t>(rows), k, panel, out64);
-        HIPC(hipGetLastError());
+        gather();
         return;
     }
     // the chain needed far more draws than expected: the host stream
@@ -1357,17 +1376,202 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
     sc.hstart.ensure(sizeof(uint32_t) * n);
     for (uint64_t i = 0; i < n; ++i) sc.hstart.as<uint32_t>()[i] = static_cast<uint32_t>(i);
     HIPC(hipMemcpyAsync(sc.start.p, sc.hstart.p, sizeof(uint32_t) * n, hipMemcpyHostToDevice, s));
-    k_omega_panel<KP><<<blocks_for(rows * KP), 256, 0, s>>>(sc.v.as<double>(), sc.start.as<uint32_t>(),
-                                                             static_cast<int64_t>(rows), k, panel, out64);
-    HIPC(hipGetLastError());
+    gather();
 }
+
+// ---------------------------------------------------------------------------
+// k' = n_comps + n_oversamples > 128: the power loop over wide panels. A panel
+// is nb blocks of 128 columns (block-major, kernels.h), and every product runs
+// block by block through the KP = 128 engine's SpMMs, on the resident or the
+// streamed operator alike (the CSR is read nb times per pass). The dense
+// algebra is the reference GPU path's (scx-gpu/src/cusolver.rs:87-409,
+// gpu_pca.rs:592-697) on rocBLAS / rocSOLVER: the tall f32 panels are
+// orthonormalized by Householder QR (geqrf + orgqr; qr_method householder) or
+// by CholeskyQR2 (Gram by sgemm, potrf, trsm, twice; a failed potrf is the
+// non-SPD error; qr_method cholesky) in column-major copies, and the
+// Rayleigh-Ritz SVD of B (n_vars x k', f64) is gesvd. Every basis gets the
+// requested QR (no shifted single pass for intermediate bases).
+// ---------------------------------------------------------------------------
+struct WideEngine {
+    scxpca_matrix *m;
+    Engine<128> &eg;
+    hipStream_t s;
+    int k = 0, nb = 0, kpw = 0;
+    DevBuf col, col64, tau, U, Vt, S, E, W, G, info;
+
+    WideEngine(scxpca_matrix *mm, Engine<128> &e, int kk) : m(mm), eg(e), s(mm->stream), k(kk) {
+        if (m->n_obs > INT32_MAX || m->n_vars > INT32_MAX)
+            throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples > 128 needs n_obs and n_vars below 2^31");
+        nb = (k + kWideBlock - 1) / kWideBlock;
+        kpw = nb * kWideBlock;
+        const int64_t n = std::max<int64_t>(m->n_obs, 1), nv = std::max<int64_t>(m->n_vars, 1);
+        m->y.ensure(sizeof(float) * n * kpw);
+        m->q.ensure(sizeof(float) * n * kpw);
+        m->panel.ensure(sizeof(float) * nv * kpw);   // M
+        m->panel2.ensure(sizeof(float) * nv * kpw);  // B (f32, sketch passes)
+        m->b64.ensure(sizeof(double) * nv * kpw);    // B (f64, final pass)
+        col.ensure(sizeof(float) * std::max(n, nv) * k);
+        col64.ensure(sizeof(double) * nv * k);
+        tau.ensure(sizeof(float) * k);
+        U.ensure(sizeof(double) * nv * k);
+        Vt.ensure(sizeof(double) * k * k);
+        S.ensure(sizeof(double) * k);
+        E.ensure(sizeof(double) * k);
+        W.ensure(sizeof(float) * k * k);
+        G.ensure(sizeof(float) * k * k);
+        info.ensure(sizeof(rocblas_int) * 2);
+        if (!m->rb && rocblas_create_handle(&m->rb) != rocblas_status_success)
+            throw Error(SCXPCA_ERR_DEVICE, "rocblas_create_handle failed");
+        if (rocblas_set_stream(m->rb, s) != rocblas_status_success)
+            throw Error(SCXPCA_ERR_DEVICE, "rocblas_set_stream failed");
+    }
+
+    void check() { HIPC(hipGetLastError()); }
+    static void rbcheck(rocblas_status st, const char *what) {
+        if (st != rocblas_status_success)
+            throw Error(SCXPCA_ERR_DEVICE, std::string(what) + " failed (status " + std::to_string(static_cast<int>(st)) + ")");
+    }
+
+    // Qout (block-major rows x kpw) <- an orthonormal basis of A (block-major);
+    // col keeps the basis column-major (rows x k)
+    void qr(const float *A, int64_t rows, float *Qout, int qr_method) {
+        const rocblas_int r = static_cast<rocblas_int>(rows), kk = k;
+        float *c = col.as<float>();
+        const unsigned gr = static_cast<unsigned>((rows + 63) / 64);
+        k_blocks_to_cols<float, float><<<dim3(gr, (k + 63) / 64), 256, 0, s>>>(A, rows, k, c);
+        check();
+        if (qr_method == SCXPCA_QR_HOUSEHOLDER) {
+            rbcheck(rocsolver_sgeqrf(m->rb, r, kk, c, r, tau.as<float>()), "rocsolver_sgeqrf");
+            rbcheck(rocsolver_sorgqr(m->rb, r, kk, kk, c, r, tau.as<float>()), "rocsolver_sorgqr");
+        } else {
+            const float one = 1.f, zero = 0.f;
+            for (int pass = 0; pass < 2; ++pass) {
+                rbcheck(rocblas_sgemm(m->rb, rocblas_operation_transpose, rocblas_operation_none, kk, kk, r, &one, c, r,
+                                      c, r, &zero, G.as<float>(), kk),
+                        "rocblas_sgemm");
+                rbcheck(rocsolver_spotrf(m->rb, rocblas_fill_upper, kk, G.as<float>(), kk, info.as<rocblas_int>()),
+                        "rocsolver_spotrf");
+                k_info_status<<<1, 64, 0, s>>>(info.as<rocblas_int>(), 1, eg.status);
+                check();
+                rbcheck(rocblas_strsm(m->rb, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                      rocblas_diagonal_non_unit, r, kk, &one, G.as<float>(), kk, c, r),
+                        "rocblas_strsm");
+            }
+        }
+        k_cols_to_blocks<float><<<dim3(gr, kpw / 64), 256, 0, s>>>(c, rows, k, kpw, Qout);
+        check();
+    }
+
+    template <class Ops>
+    void run_with(Ops &ops, const scxpca_params *p, scxpca_result *out) {
+        const int64_t n = m->n_obs, nv = m->n_vars;
+        const bool centered = p->zero_center != 0;
+        const bool bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
+        const int nc = static_cast<int>(p->n_comps);
+        HIPC(hipMemsetAsync(eg.status, 0, sizeof(int), s));
+        if (bf16) ensure_xbf(m);
+        ops.stats();
+        k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), eg.baseline_mean(), eg.mu);
+        check();
+        float *M = m->panel.as<float>(), *B32 = m->panel2.as<float>(), *Y = m->y.as<float>(), *Q = m->q.as<float>();
+        double *B64 = m->b64.as<double>();
+        const int64_t bn = n * kWideBlock, bv = nv * kWideBlock;  // block strides
+        device_omega<128>(s, static_cast<uint64_t>(nv), k, p->seed, M, nullptr, m->om, kpw);
+        auto fwd_all = [&](const float *Mp, float *Yp) {
+            for (int b = 0; b < nb; ++b) {
+                if (centered) eg.mean_correction(Mp + b * bv);
+                ops.fwd(Mp + b * bv, centered, bf16, Yp + b * bn);
+            }
+        };
+        auto t_all = [&](const float *Qp, bool lowp, double *Bd, float *Bf) {
+            for (int b = 0; b < nb; ++b)
+                ops.t(Qp + b * bn, centered, lowp, Bd ? Bd + b * bv : nullptr, Bf ? Bf + b * bv : nullptr);
+        };
+        fwd_all(M, Y);
+        qr(Y, n, Q, p->qr_method);
+        for (int64_t it = 0; it < p->n_power_iterations; ++it) {
+            t_all(Q, bf16, nullptr, B32);
+            qr(B32, nv, M, p->qr_method);
+            fwd_all(M, Y);
+            qr(Y, n, Q, p->qr_method);
+        }
+        // final B pass in FP32 (also in the BF16 sketch form); col still holds Q
+        t_all(Q, false, B64, nullptr);
+        double *b = col64.as<double>();
+        k_blocks_to_cols<double, double><<<dim3(static_cast<unsigned>((nv + 63) / 64), (k + 63) / 64), 256, 0, s>>>(
+            B64, nv, k, b);
+        check();
+        rbcheck(rocsolver_dgesvd(m->rb, rocblas_svect_singular, rocblas_svect_singular, static_cast<rocblas_int>(nv), k,
+                                 b, static_cast<rocblas_int>(nv), S.as<double>(), U.as<double>(),
+                                 static_cast<rocblas_int>(nv), Vt.as<double>(), k, E.as<double>(), rocblas_outofplace,
+                                 info.as<rocblas_int>() + 1),
+                "rocsolver_dgesvd");
+        k_info_status<<<1, 64, 0, s>>>(info.as<rocblas_int>() + 1, 2, eg.status);
+        check();
+        // components = U[:, :nc]^T; embeddings = Q (V S)[:, :nc]
+        DevBuf comps_tmp, emb_tmp;
+        float *comps_dev = out->components, *emb_dev = out->embeddings;
+        if (!out->on_device) {
+            comps_tmp.ensure(sizeof(float) * nv * nc);
+            emb_tmp.ensure(sizeof(float) * n * nc);
+            comps_dev = comps_tmp.as<float>();
+            emb_dev = emb_tmp.as<float>();
+        }
+        k_f64_to_f32<<<blocks_for(nv * nc), 256, 0, s>>>(U.as<double>(), comps_dev, nv * nc);
+        check();
+        k_wide_emb_weights<<<blocks_for(static_cast<int64_t>(k) * nc), 256, 0, s>>>(Vt.as<double>(), S.as<double>(), k,
+                                                                                   nc, W.as<float>());
+        check();
+        {
+            const float one = 1.f, zero = 0.f;
+            rbcheck(rocblas_sgemm(m->rb, rocblas_operation_transpose, rocblas_operation_transpose, nc,
+                                  static_cast<rocblas_int>(n), k, &one, W.as<float>(), k, col.as<float>(),
+                                  static_cast<rocblas_int>(n), &zero, emb_dev, nc),
+                    "rocblas_sgemm");
+        }
+        std::vector<double> hsig(k), hsum(nv), hsq(nv);
+        int hstatus = 0;
+        HIPC(hipMemcpyAsync(hsig.data(), S.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsum.data(), m->sum.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(hsq.data(), m->sq.p, sizeof(double) * nv, hipMemcpyDeviceToHost, s));
+        HIPC(hipMemcpyAsync(&hstatus, eg.status, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (!out->on_device) {
+            HIPC(hipMemcpyAsync(out->embeddings, emb_dev, sizeof(float) * n * nc, hipMemcpyDeviceToHost, s));
+            HIPC(hipMemcpyAsync(out->components, comps_dev, sizeof(float) * nv * nc, hipMemcpyDeviceToHost, s));
+        }
+        HIPC(hipStreamSynchronize(s));
+        if (hstatus & 1) {
+            if (p->qr_method == SCXPCA_QR_CHOLESKY)
+                throw Error(SCXPCA_ERR_LINALG, "CholeskyQR2 failed: non-SPD; retry with qr_method='householder'");
+            throw Error(SCXPCA_ERR_LINALG, "Householder QR failed");
+        }
+        if (hstatus & 2) throw Error(SCXPCA_ERR_LINALG, "SVD failed: the bidiagonal iteration did not converge");
+        std::vector<double> hmu;
+        int recomputed = 0;
+        const double tv = eg.total_variance(centered, hsum, hsq, hmu, recomputed, [&] { return ops.stored_ss(); });
+        const double denom = std::max(static_cast<double>(n) - 1.0, 1.0);
+        for (int j = 0; j < nc; ++j) {
+            const double var = hsig[j] * hsig[j] / denom;
+            if (out->variance) out->variance[j] = var;
+            if (out->variance_ratio) out->variance_ratio[j] = tv > 0.0 ? var / tv : 0.0;
+        }
+        if (out->sigma)
+            for (int j = 0; j < k; ++j) out->sigma[j] = hsig[j];
+        if (out->mean && centered) std::memcpy(out->mean, hmu.data(), sizeof(double) * nv);
+        out->k = k;
+        out->total_variance = tv;
+        out->tv_recomputed = recomputed;
+    }
+
+    void run(const scxpca_params *p, scxpca_result *out) {
+        typename Engine<128>::ResidentOps ops{&eg};
+        run_with(ops, p, out);
+    }
+};
 
 void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     validate(m, p);
     const int64_t k = pca_k(m, p);
-    if (k > 128)
-        throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples = " + std::to_string(k) +
-                                            " exceeds the GPU panel limit of 128");
     DeviceGuard dg(m->device);
     if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_xbf(m);
     ensure_stats(m);
@@ -1378,9 +1582,13 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     if (k <= 64) {
         Engine<64> eng(m, static_cast<int>(k));
         eng.run(p, out);
-    } else {
+    } else if (k <= 128) {
         Engine<128> eng(m, static_cast<int>(k));
         eng.run(p, out);
+    } else {
+        Engine<128> eng(m, 128);
+        WideEngine wide(m, eng, static_cast<int>(k));
+        wide.run(p, out);
     }
     HIPC(hipEventRecord(e1, m->stream));
     HIPC(hipEventSynchronize(e1));
@@ -1399,8 +1607,6 @@ constexpr int64_t kMaxCovVars = 16384;
 void run_pca_covariance(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     if (m->pflog) throw Error(SCXPCA_ERR_INVALID, "the covariance route is not defined for a PFlog matrix");
     validate(m, p);
-    if (p->n_comps > 128)
-        throw Error(SCXPCA_ERR_INVALID, "n_comps = " + std::to_string(p->n_comps) + " exceeds the GPU panel limit of 128");
     if (m->n_vars > kMaxCovVars)
         throw Error(SCXPCA_ERR_INVALID, "the covariance route supports up to " + std::to_string(kMaxCovVars) +
                                             " genes (n_vars = " + std::to_string(m->n_vars) + "); use method='randomized'");
@@ -1962,9 +2168,6 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
     }
     validate(&shell, p);
     const int64_t k = pca_k(&shell, p);
-    if (k > 128)
-        throw Error(SCXPCA_ERR_INVALID, "n_comps + n_oversamples = " + std::to_string(k) +
-                                            " exceeds the GPU panel limit of 128");
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
@@ -1991,10 +2194,15 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
         Engine<64> eg(&shell, static_cast<int>(k));
         StreamOps<64> ops(eg, x, shard_rows);
         drive(eg, ops);
-    } else {
+    } else if (k <= 128) {
         Engine<128> eg(&shell, static_cast<int>(k));
         StreamOps<128> ops(eg, x, shard_rows);
         drive(eg, ops);
+    } else {
+        Engine<128> eg(&shell, 128);
+        StreamOps<128> ops(eg, x, shard_rows);
+        WideEngine wide(&shell, eg, static_cast<int>(k));
+        drive(wide, ops);
     }
     if (fit) *fit = shell.pf_fit;
     HIPC(hipEventRecord(e1, shell.stream));
@@ -2013,7 +2221,7 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
 // panels and the n_vars x KP workspace (scx_amd.accel._resident_bytes is the
 // same model).
 double resident_footprint(int64_t n, int64_t nv, int64_t nnz, int64_t k) {
-    const double kp = k <= 64 ? 64.0 : 128.0;
+    const double kp = k <= 64 ? 64.0 : static_cast<double>((k + 127) / 128 * 128);
     const double tiles = static_cast<double>((n + kTileRows - 1) / kTileRows);
     const double nsplit = static_cast<double>((nv + kCB - 1) / kCB);
     const double dn = static_cast<double>(n), dv = static_cast<double>(nv), dz = static_cast<double>(nnz);
@@ -2023,6 +2231,7 @@ double resident_footprint(int64_t n, int64_t nv, int64_t nnz, int64_t k) {
     const double split = 4.0 * dn * (nsplit + 1) + 16.0 * tiles * dv;
     if (split <= kMaxSplitBytes) b += split;
     b += 2 * 4.0 * dn * kp + 2 * 4.0 * dv * kp + 4 * 8.0 * dv * kp + 4 * 256.0 * dv * kp + 8.0 * 1024 * kp * kp;
+    if (k > 128) b += 4.0 * std::max(dn, dv) * k + 16.0 * dv * k;  // column-major copies of the wide route
     return b;
 }
 
