@@ -255,6 +255,20 @@ scxpca_status scxpca_project_forward(scxpca_matrix *m, const float *w, int64_t k
 scxpca_status scxpca_project_backward(scxpca_matrix *m, const float *dz, int64_t k,
                                       const float *mean, float *dw, void *stream);
 
+/* scxpca_project_backward without any device allocation. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight training or serving loops). The same
+ * kernels and results as scxpca_project_backward; every buffer beyond dz, dw
+ * and mean (the transposed SpMM's tile-range partials, the 1^T dZ sums, the
+ * f64 mean, the packed dZ / dW when k is not 64 or 128, the gene order when
+ * the matrix has none yet) lives in one static per-device scratch array of
+ * 64 MiB, held by each GPU the library runs on until the process ends. A
+ * matrix and k that need more scratch are rejected (SCXPCA_ERR_INVALID, the
+ * size in the message). All calls on one GPU share that array: calls that
+ * overlap on one GPU race. */
+scxpca_status scxpca_project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k,
+                                              const float *mean, float *dw, void *stream);
+
 /* Ω: ChaCha8Rng::seed_from_u64(seed) + StandardNormal, row-major rows x cols
  * f64, draw i -> element i (random_gaussian, cpu.rs:1174-1178). Host only. */
 scxpca_status scxpca_gaussian(uint64_t rows, uint64_t cols, uint64_t seed, double *out);

@@ -47,6 +47,7 @@ EXPORTS = (
     "scxpca_qr",
     "scxpca_project_forward",
     "scxpca_project_backward",
+    "scxpca_project_backward_noalloc",
     "scxpca_gaussian",
     "scxpca_gaussian_device",
     "scxpca_device_count",
@@ -138,6 +139,7 @@ def lib() -> ctypes.CDLL:
     l.scxpca_qr.argtypes = [_vp, _i64, _i64, _i32, ctypes.c_int, _vp]
     l.scxpca_project_forward.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
     l.scxpca_project_backward.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
+    l.scxpca_project_backward_noalloc.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp]
     l.scxpca_gaussian.argtypes = [_u64, _u64, _u64, _vp]
     l.scxpca_gaussian_device.argtypes = [_u64, _u64, _u64, ctypes.c_int, _vp]
     l.scxpca_device_count.argtypes = []
@@ -340,6 +342,11 @@ class Matrix:
 
     def project_backward(self, dz, k, mean, dw, stream=None):
         check(lib().scxpca_project_backward(self.handle, ptr(dz), _i64(k), ptr(mean), ptr(dw), _vp(stream or 0)))
+
+    def project_backward_noalloc(self, dz, k, mean, dw, stream=None):
+        """project_backward with no device allocation (a fixed per-device scratch array)."""
+        check(lib().scxpca_project_backward_noalloc(self.handle, ptr(dz), _i64(k), ptr(mean), ptr(dw),
+                                                    _vp(stream or 0)))
 
 
 def _result_dict(res, emb, comps, var, ratio, mean, sigma, n_comps, nv, zero_center) -> dict:

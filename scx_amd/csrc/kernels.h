@@ -556,6 +556,13 @@ __global__ void __launch_bounds__(256) k_unpack_cols(const float *__restrict__ s
     dst[r * ld + c0 + j] = src[r * KP + j];
 }
 
+// p[0, n) = 0 (32-bit words): a fill kernel where a memset node is to be
+// avoided (work a caller may capture into a HIP graph).
+__global__ void __launch_bounds__(256) k_zero_words(uint32_t *__restrict__ p, int64_t n) {
+    const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+    for (int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = 0u;
+}
+
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
                                                  uint16_t *__restrict__ out) {

@@ -33,6 +33,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -427,9 +428,8 @@ void throw_invalid(const scxpca_matrix *m, int h, int64_t row_base = 0, int64_t 
 // ties by index) are dealt to the nch*tw waves in snake order (0..W-1, W-1..0,
 // ...), round r filling slot r of each wave, so per-wave entry totals differ by
 // at most about one gene's count. Unused slots hold n_vars.
-const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
-    auto &go = m->gorder[gw == 8 ? 0 : 1];
-    if (go.valid && go.gw == gw && go.nch == nch && go.tw == tw) return go.dev.as<uint32_t>();
+// The dealt order into h[0, max(S, 1)), S = nch * tw * gw slots.
+void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, uint32_t *h) {
     const int64_t nv = m->n_vars;
     const int64_t W = static_cast<int64_t>(nch) * tw, S = W * gw;
     // genes by entry count, descending, ties by gene index: one 64-bit key
@@ -441,15 +441,27 @@ const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
         key[c] = (static_cast<uint64_t>(~cnt) << 32) | static_cast<uint64_t>(c);
     }
     std::sort(key.begin(), key.end());
-    // the previous upload from go.host (if any) was drained by prepare's sync
-    go.host.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
-    uint32_t *h = go.host.as<uint32_t>();
     for (int64_t i = 0; i < std::max<int64_t>(S, 1); ++i) h[i] = static_cast<uint32_t>(nv);
     for (int64_t i = 0; i < nv; ++i) {
         const int64_t r = i / W, pos = i % W;
         const int64_t w = (r % 2 == 0) ? pos : W - 1 - pos;
         h[w * gw + r] = static_cast<uint32_t>(key[i]);
     }
+}
+
+bool gene_order_valid(const scxpca_matrix *m, int gw, int nch, int tw) {
+    const auto &go = m->gorder[gw == 8 ? 0 : 1];
+    return go.valid && go.gw == gw && go.nch == nch && go.tw == tw;
+}
+
+const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
+    auto &go = m->gorder[gw == 8 ? 0 : 1];
+    if (gene_order_valid(m, gw, nch, tw)) return go.dev.as<uint32_t>();
+    const int64_t S = static_cast<int64_t>(nch) * tw * gw;
+    // the previous upload from go.host (if any) was drained by prepare's sync
+    go.host.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
+    deal_gene_order(m, gw, nch, tw, go.host.as<uint32_t>());
+    uint32_t *h = go.host.as<uint32_t>();
     go.dev.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
     HIPC(hipMemcpyAsync(go.dev.p, h, sizeof(uint32_t) * std::max<int64_t>(S, 1), hipMemcpyHostToDevice, m->stream));
     go.gw = gw;
@@ -690,14 +702,20 @@ struct Engine {
     const uint32_t *gperm = nullptr;  // wave-slot gene order of the transposed SpMM
     int64_t rows_per_wave = 0, rows_per_colblk = 0, rows_per_gblk = 0;
 
+    // Grid of the transposed SpMM: nch gene chunks x nrb tile ranges of tpr
+    // tiles (about 1024 / TW blocks in total for 256 CUs, nrb a multiple of 8
+    // for the XCD-aware block order).
+    static void t_geometry(const scxpca_matrix *mm, int &nch, int &nrb, int &tpr) {
+        nch = static_cast<int>((mm->n_vars + CG - 1) / CG);
+        int ntr = std::max(1, (256 * 16 / TW) / std::max(nch, 1));
+        ntr = std::min(ntr, std::max(1, mm->n_tiles));
+        nrb = (ntr + 7) / 8 * 8;
+        tpr = (mm->n_tiles + nrb - 1) / nrb;
+    }
+
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
-        nch_t = static_cast<int>((nv + CG - 1) / CG);
-        // tile ranges: about 1024 / TW blocks in total (256 CUs), multiple of 8
-        int ntr = std::max(1, (256 * 16 / TW) / nch_t);
-        ntr = std::min(ntr, std::max(1, m->n_tiles));
-        nrb_t = (ntr + 7) / 8 * 8;
-        tiles_per_tr = (m->n_tiles + nrb_t - 1) / nrb_t;
+        t_geometry(m, nch_t, nrb_t, tiles_per_tr);
         gperm = ensure_gene_order(m, GW, nch_t, TW);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
@@ -749,14 +767,18 @@ struct Engine {
     }
 
     // out = w^T A over the rows of a tall f32 panel A (nrows x KP), f64;
-    // w == null: plain column sums
-    void weighted_colsum(const float *A, const double *w, int64_t nrows, double *out64, float *out32) {
+    // w == null: plain column sums. cpart: kMaxColBlocks x KP f64 partials.
+    static void colsum_into(hipStream_t st, const float *A, const double *w, int64_t nrows, double *cpart,
+                            double *out64, float *out32) {
         const int nb = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((nrows + 63) / 64, kMaxColBlocks)));
         const int64_t rpb = (nrows + nb - 1) / nb;
-        k_wcolsum4<KP><<<nb, 256, 0, s>>>(A, w, nrows, rpb, m->cpart.as<double>());
-        check();
-        k_colsum_reduce<KP><<<KP, 256, 0, s>>>(m->cpart.as<double>(), nb, out64, out32);
-        check();
+        k_wcolsum4<KP><<<nb, 256, 0, st>>>(A, w, nrows, rpb, cpart);
+        HIPC(hipGetLastError());
+        k_colsum_reduce<KP><<<KP, 256, 0, st>>>(cpart, nb, out64, out32);
+        HIPC(hipGetLastError());
+    }
+    void weighted_colsum(const float *A, const double *w, int64_t nrows, double *out64, float *out32) {
+        colsum_into(s, A, w, nrows, m->cpart.as<double>(), out64, out32);
     }
 
     // mc = mu^T panel (f32 for the forward epilogue)
@@ -807,40 +829,65 @@ struct Engine {
         check();
     }
 
-    // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix, cpu.rs:653-687)
-    void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
-        const int64_t nv = m->n_vars;
-        double *qsp = centered ? m->cpart.as<double>() : nullptr;  // nrb_t x nch_t x KP partials of 1^T Q
-        if (m->n_obs > 0) {
-            const unsigned grid = static_cast<unsigned>(nrb_t * nch_t);
-            StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
-            const uint32_t *gp = gperm;
+    // Scratch of one transposed SpMM (transposed() below): the tile-range
+    // partials of Z (nrb x n_vars x KP f32), of 1^T Q (nrb x nch x KP f64),
+    // the column-sum partials (kMaxColBlocks x KP f64) and 1^T Q, b^T Q (KP f64).
+    struct TScratch {
+        float *tpart;
+        double *qsp, *cpart, *sumq, *bq;
+    };
+
+    // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix,
+    // rowb its baseline; cpu.rs:653-687) over the buffers in sc. mu null: no
+    // centering. Every partial is written by the launch (the grid covers all
+    // nrb x nch slots), so sc needs no clearing between calls.
+    static void transposed(scxpca_matrix *mm, hipStream_t st, const uint32_t *gp, int nch, int nrb, int tpr,
+                           const float *Q, const double *mu, const double *rowb, bool bf16, const TScratch &sc,
+                           double *B64, float *B32) {
+        const int64_t nv = mm->n_vars;
+        double *qsp = mu ? sc.qsp : nullptr;
+        if (mm->n_obs > 0) {
+            const unsigned grid = static_cast<unsigned>(nrb * nch);
+            StageScope prof(mm, st, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
             if (bf16)
-                k_spmm_t6<KP, GW, NB0, true, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
-                                                                   m->trow.as<uint8_t>(), m->tval.as<float>(),
-                                                                   m->txbf.as<uint16_t>(), Q, gp, nch_t, nrb_t,
-                                                                   tiles_per_tr, m->n_obs, nv, m->n_tiles,
-                                                                   m->tpart.as<float>(), qsp);
+                k_spmm_t6<KP, GW, NB0, true, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
+                                                                    mm->trow.as<uint8_t>(), mm->tval.as<float>(),
+                                                                    mm->txbf.as<uint16_t>(), Q, gp, nch, nrb, tpr,
+                                                                    mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
             else
-                k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
-                                                                    m->trow.as<uint8_t>(), m->tval.as<float>(), nullptr,
-                                                                    Q, gp, nch_t, nrb_t, tiles_per_tr, m->n_obs, nv,
-                                                                    m->n_tiles, m->tpart.as<float>(), qsp);
-            check();
-        } else {
-            HIPC(hipMemsetAsync(m->tpart.p, 0, sizeof(float) * nrb_t * nv * KP, s));
-            if (qsp) HIPC(hipMemsetAsync(qsp, 0, sizeof(double) * nrb_t * nch_t * KP, s));
+                k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
+                                                                     mm->trow.as<uint8_t>(), mm->tval.as<float>(),
+                                                                     nullptr, Q, gp, nch, nrb, tpr, mm->n_obs, nv,
+                                                                     mm->n_tiles, sc.tpart, qsp);
+            HIPC(hipGetLastError());
+        } else {  // no rows: zero partials (a fill kernel, no memset node)
+            k_zero_words<<<blocks_for(static_cast<int64_t>(nrb) * nv * KP), 256, 0, st>>>(
+                reinterpret_cast<uint32_t *>(sc.tpart), static_cast<int64_t>(nrb) * nv * KP);
+            HIPC(hipGetLastError());
+            if (qsp) {
+                k_zero_words<<<blocks_for(2 * static_cast<int64_t>(nrb) * nch * KP), 256, 0, st>>>(
+                    reinterpret_cast<uint32_t *>(qsp), 2 * static_cast<int64_t>(nrb) * nch * KP);
+                HIPC(hipGetLastError());
+            }
         }
-        if (centered) {
-            k_colsum_reduce<KP><<<KP, 256, 0, s>>>(qsp, nrb_t * nch_t, sumq, nullptr);
-            check();
+        if (mu) {
+            k_colsum_reduce<KP><<<KP, 256, 0, st>>>(qsp, nrb * nch, sc.sumq, nullptr);
+            HIPC(hipGetLastError());
         }
-        if (rowb) weighted_colsum(Q, rowb, m->n_obs, bq64, nullptr);
-        StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
-        k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb_t, nv,
-                                                                 centered ? mu : nullptr, sumq, B64, B32,
-                                                                 rowb ? bq64 : nullptr);
-        check();
+        if (rowb) colsum_into(st, Q, rowb, mm->n_obs, sc.cpart, sc.bq, nullptr);
+        StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T_REDUCE);
+        k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, st>>>(sc.tpart, nrb, nv, mu, sc.sumq, B64, B32,
+                                                                  rowb ? sc.bq : nullptr);
+        HIPC(hipGetLastError());
+    }
+
+    // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix)
+    // on the matrix's own scratch
+    void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
+        // 1^T Q partials in cpart (nrb_t x nch_t x KP), then b^T Q's in the
+        // same buffer: colsum_into runs after the 1^T Q reduction
+        const TScratch sc{m->tpart.as<float>(), m->cpart.as<double>(), m->cpart.as<double>(), sumq, bq64};
+        transposed(m, s, gperm, nch_t, nrb_t, tiles_per_tr, Q, centered ? mu : nullptr, rowb, bf16, sc, B64, B32);
     }
 
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
@@ -1748,6 +1795,112 @@ void project(scxpca_matrix *m, bool backward, const float *in, int64_t k, const 
     if (!user) HIPC(hipStreamSynchronize(s));
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free projection backward (scxpca_project_backward_noalloc): the
+// same kernels and results as project<KP>(backward), with every buffer
+// beyond the caller's inputs and outputs (the transposed SpMM's tile-range
+// partials, the 1^T dZ / b^T dZ sums, the f64 mean, the packed dZ / dW of a
+// k below the panel width, the wave-slot gene order when the matrix has none
+// yet) carved from one file-scope device array. Each GPU holds its own copy
+// of the array (at its own address, found once per device by
+// hipGetSymbolAddress); calls on one GPU share it, so calls that overlap on
+// one GPU race. Every region a call reads is written earlier in that call.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The device address of `scratch` on device dev (the current device).
+void *noalloc_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxScratchDevices)
+        throw Error(SCXPCA_ERR_NO_DEVICE, "device ordinal beyond the allocation-free entry point's table");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error(SCXPCA_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = ptr;
+    }
+    return g_scratch_ptr[dev];
+}
+
+template <int KP>
+void project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, const float *mean, float *dw,
+                              hipStream_t user) {
+    using E = Engine<KP>;
+    hipStream_t s = user ? user : m->stream;
+    const int64_t n = m->n_obs, nv = m->n_vars;
+    int nch = 0, nrb = 0, tpr = 0;
+    E::t_geometry(m, nch, nrb, tpr);
+    const bool pack = k != KP;
+    const bool have_order = gene_order_valid(m, E::GW, nch, E::TW);
+    const int64_t slots = std::max<int64_t>(static_cast<int64_t>(nch) * E::TW * E::GW, 1);
+    size_t off = 0;
+    auto take = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) / 256 * 256;
+        return o;
+    };
+    const size_t o_tpart = take(sizeof(float) * nrb * nv * KP);
+    const size_t o_qsp = take(sizeof(double) * nrb * nch * KP);
+    const size_t o_cpart = take(sizeof(double) * kMaxColBlocks * KP);
+    const size_t o_sumq = take(sizeof(double) * KP), o_bq = take(sizeof(double) * KP);
+    const size_t o_mu = mean ? take(sizeof(double) * std::max<int64_t>(nv, 1)) : 0;
+    const size_t o_in = pack ? take(sizeof(float) * std::max<int64_t>(n, 1) * KP) : 0;
+    const size_t o_out = pack ? take(sizeof(float) * std::max<int64_t>(nv, 1) * KP) : 0;
+    const size_t o_perm = have_order ? 0 : take(sizeof(uint32_t) * slots);
+    if (off > kNoAllocScratchBytes)
+        throw Error(SCXPCA_ERR_INVALID, "scxpca_project_backward_noalloc: this matrix and k need " +
+                                            std::to_string(off) + " bytes of scratch, above the entry point's fixed " +
+                                            std::to_string(kNoAllocScratchBytes) +
+                                            " bytes; use scxpca_project_backward");
+    char *base = static_cast<char *>(noalloc_scratch(m->device));
+    const uint32_t *gp = nullptr;
+    if (have_order) {
+        gp = m->gorder[E::GW == 8 ? 0 : 1].dev.template as<uint32_t>();
+    } else {  // dealt on the host, staged through pageable memory (no allocation on the device)
+        std::vector<uint32_t> h(slots);
+        deal_gene_order(m, E::GW, nch, E::TW, h.data());
+        HIPC(hipMemcpyAsync(base + o_perm, h.data(), sizeof(uint32_t) * slots, hipMemcpyHostToDevice, s));
+        HIPC(hipStreamSynchronize(s));
+        gp = reinterpret_cast<const uint32_t *>(base + o_perm);
+    }
+    double *mu = nullptr;
+    if (mean) {
+        mu = reinterpret_cast<double *>(base + o_mu);
+        k_f32_to_f64<<<blocks_for(nv), 256, 0, s>>>(mean, mu, nv);
+        HIPC(hipGetLastError());
+    }
+    const float *qp = dz;
+    float *dst = dw;
+    if (pack) {
+        float *packed = reinterpret_cast<float *>(base + o_in);
+        k_pack_panel32<KP><<<blocks_for(n * KP), 256, 0, s>>>(dz, n, static_cast<int>(k), packed);
+        HIPC(hipGetLastError());
+        qp = packed;
+        dst = reinterpret_cast<float *>(base + o_out);
+    }
+    const typename E::TScratch sc{reinterpret_cast<float *>(base + o_tpart), reinterpret_cast<double *>(base + o_qsp),
+                                  reinterpret_cast<double *>(base + o_cpart), reinterpret_cast<double *>(base + o_sumq),
+                                  reinterpret_cast<double *>(base + o_bq)};
+    const double *rowb = (m->pflog && n > 0) ? m->pfb.as<double>() : nullptr;
+    E::transposed(m, s, gp, nch, nrb, tpr, qp, mu, rowb, false, sc, nullptr, dst);
+    if (pack) {
+        k_unpack_panel32<KP><<<blocks_for(nv * k), 256, 0, s>>>(dst, nv, static_cast<int>(k), dw);
+        HIPC(hipGetLastError());
+    }
+    if (!user) HIPC(hipStreamSynchronize(s));
+}
+
 // PFlog flags for the delta pass.
 int pflog_flags(const scxpca_pflog_params &pp) {
     return kXfScale | kXfLog1p | kXfCheck | (pp.delta_form == SCXPCA_PFLOG_DELTA_F64 ? kXfLogF64 : 0);
@@ -2497,6 +2650,17 @@ scxpca_status scxpca_project_backward(scxpca_matrix *m, const float *dz, int64_t
         DeviceGuard dg(m->device);
         if (k <= 64) project<64>(m, true, dz, k, mean, dw, static_cast<hipStream_t>(stream));
         else project<128>(m, true, dz, k, mean, dw, static_cast<hipStream_t>(stream));
+    });
+}
+
+scxpca_status scxpca_project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, const float *mean,
+                                              float *dw, void *stream) {
+    return guarded([&] {
+        if (!m || !dz || !dw) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (k <= 0 || k > 128) throw Error(SCXPCA_ERR_INVALID, "k must be in 1..128");
+        DeviceGuard dg(m->device);
+        if (k <= 64) project_backward_noalloc<64>(m, dz, k, mean, dw, static_cast<hipStream_t>(stream));
+        else project_backward_noalloc<128>(m, dz, k, mean, dw, static_cast<hipStream_t>(stream));
     });
 }
 
