@@ -72,3 +72,73 @@ def dense_pca(xd: np.ndarray, n_comps: int, zero_center: bool = True):
         "variance": s[:n_comps] ** 2 / max(n - 1, 1),
         "total_variance": float((xc ** 2).sum() / max(n - 1, 1)),
     }
+
+
+def per_pc_cosine(ref: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """|cos| between matching columns (sign-agnostic), the reference's GPU
+    gate metric (pyscx/tests/test_accel_pca_gpu.py:176-181)."""
+    a = np.asarray(ref, dtype=np.float64)
+    b = np.asarray(x, dtype=np.float64)
+    num = np.abs(np.sum(a * b, axis=0))
+    return num / np.maximum(np.linalg.norm(a, axis=0) * np.linalg.norm(b, axis=0), 1e-300)
+
+
+def exact_pca(x, kmax: int, zero_center: bool = True, device: str = "cpu"):
+    """Exact top-``kmax`` PCA of a dense f64 matrix (n x nv, numpy) by the
+    symmetric eigendecomposition of its Gram matrix in the smaller dimension
+    (LAPACK syevd through numpy on the CPU, or through torch in f64 on the
+    GPU for shapes whose CPU eigendecomposition would take minutes).
+    Singular values are exact to ~eps * sigma_1^2 / sigma_j relative, far
+    below the 1e-6 the comparisons need. Returns sigma (kmax), components
+    (kmax x nv, orthonormal rows) and embeddings (n x kmax) = Xc V."""
+    n, nv = x.shape
+    if device == "cpu":
+        xc = x - x.mean(axis=0) if zero_center else x
+        cell_space = n < nv
+        g = xc @ xc.T if cell_space else xc.T @ xc
+        w, v = np.linalg.eigh(g)
+        w, v = w[::-1][:kmax], v[:, ::-1][:, :kmax]
+        sig = np.sqrt(np.maximum(w, 0.0))
+        if cell_space:
+            comps = (xc.T @ v / np.maximum(sig, 1e-300)).T
+        else:
+            comps = v.T
+        emb = xc @ comps.T
+        return {"sigma": sig, "components": comps, "embeddings": emb}
+    import torch
+
+    dev = torch.device("cuda:0")
+    xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)).to(dev)
+    if zero_center:
+        xt -= xt.mean(dim=0, keepdim=True)
+    cell_space = n < nv
+    g = xt @ xt.t().contiguous() if cell_space else xt.t().contiguous() @ xt
+    w, v = torch.linalg.eigh(g)
+    del g
+    w = torch.flip(w, dims=[0])[:kmax]
+    v = torch.flip(v, dims=[1])[:, :kmax].contiguous()
+    sig = torch.sqrt(torch.clamp(w, min=0.0))
+    if cell_space:
+        comps = (xt.t().contiguous() @ v / torch.clamp(sig, min=1e-300)).t().contiguous()
+    else:
+        comps = v.t().contiguous()
+    emb = xt @ comps.t().contiguous()
+    torch.cuda.synchronize()
+    out = {"sigma": sig.cpu().numpy(), "components": comps.cpu().numpy(), "embeddings": emb.cpu().numpy()}
+    del xt, v, comps, emb
+    torch.cuda.empty_cache()
+    return out
+
+
+def dense_xt(csr_x, target_sum, log1p=True):
+    """Dense f64 X~ of a synth Csr with the oracle's transform (the
+    reference's f32 rounding, glibc log1pf)."""
+    import oracle
+
+    v = oracle.CsrView(csr_x.indptr, csr_x.indices, csr_x.data, csr_x.shape)
+    xt = oracle.transform(v, target_sum, log1p)
+    n, nv = csr_x.shape
+    d = np.zeros((n, nv))
+    rows = np.repeat(np.arange(n), np.diff(csr_x.indptr))
+    d[rows, csr_x.indices.astype(np.int64)] = xt
+    return d
