@@ -92,6 +92,12 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
     return static_cast<uint16_t>(u >> 16);
 }
 
+// The BF16 sketch form's packed tile-CSC entry: bf16(x~) (bits 0-15), row in
+// the 256-row tile (16-23), gene mod 64 (24-29).
+__device__ __forceinline__ uint32_t pack_bf16_entry(float x, uint32_t row, uint32_t gene) {
+    return static_cast<uint32_t>(f32_to_bf16_rne(x)) | ((row & 255u) << 16) | ((gene & 63u) << 24);
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -1464,7 +1470,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
                                                      int64_t n_obs, int64_t n_vars, uint32_t *__restrict__ tptr,
                                                      uint8_t *__restrict__ trow, float *__restrict__ tval,
                                                      double *__restrict__ psum, double *__restrict__ psq,
-                                                     const int *__restrict__ flag) {
+                                                     const int *__restrict__ flag, uint32_t *__restrict__ tpk) {
     constexpr int NG = kB4Genes, NW = 8, U = kB4Batches;
     if (*flag) return;  // the matrix failed validation: nothing here may trust its indices
     constexpr int SPC = NG / 128;  // 128-gene split offsets per chunk
@@ -1473,6 +1479,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     __shared__ uint32_t start[NG + 1];
     __shared__ float sval[kB4Stage];
     __shared__ uint8_t srow[kB4Stage];
+    __shared__ uint8_t sgene[kB4Stage];  // gene in the chunk, for the packed bf16 entries
     __shared__ uint32_t wlo[NW], gsum[4];
     const int slot = static_cast<int>(blockIdx.x >> 3), xcd = static_cast<int>(blockIdx.x & 7);
     const int t = (slot / nck) * 8 + xcd, ck = slot % nck;
@@ -1606,9 +1613,11 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
         if (staged) {
             srow[pos] = rt;
             sval[pos] = x;
+            sgene[pos] = static_cast<uint8_t>(gl);
         } else {
             trow[base + cbase + pos] = rt;
             tval[base + cbase + pos] = x;
+            if (tpk) tpk[base + cbase + pos] = pack_bf16_entry(x, rt, gl);
         }
     };
 #pragma unroll
@@ -1627,6 +1636,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
         for (uint32_t q = tid; q < ctotal; q += blockDim.x) {
             trow[base + cbase + q] = srow[q];
             tval[base + cbase + q] = sval[q];
+            if (tpk) tpk[base + cbase + q] = pack_bf16_entry(sval[q], srow[q], sgene[q]);
         }
     if (psum && tid < ng) {
         const uint32_t a = start[tid], e = start[tid + 1];

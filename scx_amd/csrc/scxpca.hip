@@ -41,6 +41,7 @@
 
 #include "../../include/scxpca.h"
 #include "kernels.h"
+#include "kernels_mfma.h"
 
 namespace scxpca {
 void gaussian_matrix(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
@@ -213,10 +214,11 @@ struct scxpca_matrix {
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
     DevBuf lg;    // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
-    DevBuf xbf;
-    bool xbf_ready = false;
+    bool xbf_ready = false;     // tpk holds the packed bf16 entries of the current prepare
+    bool tpk_in_build = false;  // the BF16 form ran on this matrix: the tile-CSC build writes tpk too
+    DevBuf qpk, mpk;         // the sketch products' panels, hi / lo bf16 in the MFMA B-operand layout
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
-    DevBuf tptr, trow, tval, txbf;
+    DevBuf tptr, trow, tval, tpk;
     int n_tiles = 0;
     // entries per gene (host copy) and the load-balanced gene order of the
     // transposed SpMM for one wave-slot layout (GW genes x 16 waves x nch)
@@ -323,7 +325,8 @@ int prepare(scxpca_matrix *m) {
         k_build_tcsc4<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
-            m->tsq.as<double>(), vflag);
+            m->tsq.as<double>(), vflag, m->tpk_in_build ? m->tpk.as<uint32_t>() : nullptr);
+        m->xbf_ready = m->tpk_in_build;
     }
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
@@ -484,6 +487,7 @@ void size_derived(scxpca_matrix *m) {
     m->tptr.ensure(sizeof(uint32_t) * static_cast<size_t>(m->n_tiles) * (m->n_vars + 1));
     m->trow.ensure(m->nnz + kEntryPad);
     m->tval.ensure(sizeof(float) * (m->nnz + kEntryPad));
+    if (m->tpk_in_build) m->tpk.ensure(sizeof(uint32_t) * (m->nnz + kEntryPad));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     if (m->use_split) {
@@ -577,16 +581,20 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flag
     throw_invalid(m, h);
 }
 
+// The BF16 sketch passes' X~: the tile-CSC entries packed as bf16(x~) | row
+// << 16 | (gene mod 64) << 24 (one 4-byte load per entry, the gene without
+// a search over the run starts)
 void ensure_xbf(scxpca_matrix *m) {
     if (m->xbf_ready) return;
-    m->xbf.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
-    m->txbf.ensure(sizeof(uint16_t) * (m->nnz + kEntryPad));
-    if (m->n_obs > 0) {
-        const unsigned nb = std::min(blocks_for(m->nnz, 256), 65536u);
-        k_to_bf16<<<nb, 256, 0, m->stream>>>(m->xt.as<float>(), m->nnz, m->xbf.as<uint16_t>());
-        HIPC(hipGetLastError());
-        k_to_bf16<<<nb, 256, 0, m->stream>>>(m->tval.as<float>(), m->nnz, m->txbf.as<uint16_t>());
+    m->tpk.ensure(sizeof(uint32_t) * (m->nnz + kEntryPad));
+    if (m->n_obs > 0 && m->n_vars > 0) {
+        const int64_t groups = static_cast<int64_t>(m->n_tiles) * ((m->n_vars + 63) / 64);
+        k_pack_tile_entries<<<blocks_for(groups, 4), 256, 0, m->stream>>>(
+            m->indptr, m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->n_vars, m->n_tiles,
+            m->tpk.as<uint32_t>());
     }
+    // later prepares of this matrix (the split layout) write the packed entries in the tile-CSC build
+    m->tpk_in_build = true;
     HIPC(hipGetLastError());
     m->xbf_ready = true;
 }
@@ -736,7 +744,11 @@ struct Engine {
         m->b64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64a.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64b.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
-        m->tpart.ensure(sizeof(float) * static_cast<size_t>(nrb_t) * std::max<int64_t>(nv, 1) * KP);
+        {
+            int ngg = 0, nrb_m = 0, tpr_m = 0;  // the BF16 form's transposed partials share tpart
+            t_mfma_geometry(m, ngg, nrb_m, tpr_m);
+            m->tpart.ensure(sizeof(float) * static_cast<size_t>(std::max(nrb_t, nrb_m)) * std::max<int64_t>(nv, 1) * KP);
+        }
         const int nb64 = gram64_blocks();
         m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(std::max(gram_waves, gram_blocks), nb64)) * KP * KP);
         m->cpart.ensure(sizeof(double) * static_cast<size_t>(std::max(kMaxColBlocks, nrb_t * nch_t)) * KP);
@@ -794,28 +806,48 @@ struct Engine {
 
     template <bool OFF>
     void launch_fwd(const float *panel, const float *mcp, bool bf16, float *Y) {
+        if (bf16) {  // the BF16 sketch form: densified tiles on the bf16 matrix cores
+            const int64_t kpad = static_cast<int64_t>(mfma_nch()) * kFG;
+            m->mpk.ensure(sizeof(uint16_t) * 2 * static_cast<size_t>(std::max<int64_t>(kpad, 8)) * KP);
+            uint16_t *mh = m->mpk.as<uint16_t>(), *ml = mh + static_cast<size_t>(kpad) * KP;
+            k_pack_hilo<KP><<<blocks_for(kpad / 8 * KP), 256, 0, s>>>(panel, m->n_vars, kpad, mh, ml);
+            check();
+            const unsigned grid = static_cast<unsigned>((m->n_tiles + 1) / 2 * (KP / 64));
+            k_spmm_fwd_mfma<OFF, kMfmaPrefetch><<<grid, 512, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
+                                                                     m->tpk.as<uint32_t>(), mh, ml, KP, KP / 64,
+                                                                     m->n_obs, m->n_vars, m->n_tiles, mcp, rowb,
+                                                                     cs32, Y);
+            return;
+        }
         if (m->use_split) {
             constexpr int NWF = 16;
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
-            if (bf16)
-                k_spmm_fwd3<KP, true, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->lg.as<uint16_t>(), nullptr, m->xbf.as<uint16_t>(), panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
-            else
-                k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
+            k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
+                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
+                m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
-            if (bf16)
-                k_spmm_fwd<KP, true, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, nullptr, m->xbf.as<uint16_t>(), panel,
-                                                          mcp, Y, m->n_obs, rowb, cs32);
-            else
-                k_spmm_fwd<KP, false, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
-                                                           mcp, Y, m->n_obs, rowb, cs32);
+            k_spmm_fwd<KP, false, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
+                                                       mcp, Y, m->n_obs, rowb, cs32);
         }
+    }
+
+    // 64-gene chunks of the bf16 MFMA forward product; entry batches each wave
+    // of the bf16 MFMA kernels loads one step ahead
+    static constexpr int kMfmaPrefetch = 28;
+    static constexpr int kTWaves = 4;  // waves (64 genes each) per block of the bf16 transposed product
+    int mfma_nch() const { return static_cast<int>((m->n_vars + kFG - 1) / kFG); }
+    // grid of the bf16 MFMA transposed product: tile ranges (a multiple of 8,
+    // enough blocks to fill 256 CUs about twice) x 256-gene groups x column halves
+    static void t_mfma_geometry(const scxpca_matrix *mm, int &ngg, int &nrb, int &tpr) {
+        ngg = static_cast<int>((mm->n_vars + kTWaves * kTG - 1) / (kTWaves * kTG));
+        const int per = std::max(1, ngg * (KP / 64));
+        int ntr = std::max(1, (512 + per - 1) / per);
+        ntr = std::min(ntr, std::max(1, mm->n_tiles));
+        nrb = (ntr + 7) / 8 * 8;
+        tpr = (mm->n_tiles + nrb - 1) / nrb;
     }
 
     // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
@@ -842,23 +874,17 @@ struct Engine {
     // centering. Every partial is written by the launch (the grid covers all
     // nrb x nch slots), so sc needs no clearing between calls.
     static void transposed(scxpca_matrix *mm, hipStream_t st, const uint32_t *gp, int nch, int nrb, int tpr,
-                           const float *Q, const double *mu, const double *rowb, bool bf16, const TScratch &sc,
+                           const float *Q, const double *mu, const double *rowb, const TScratch &sc,
                            double *B64, float *B32) {
         const int64_t nv = mm->n_vars;
         double *qsp = mu ? sc.qsp : nullptr;
         if (mm->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb * nch);
-            StageScope prof(mm, st, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
-            if (bf16)
-                k_spmm_t6<KP, GW, NB0, true, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
-                                                                    mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                                                                    mm->txbf.as<uint16_t>(), Q, gp, nch, nrb, tpr,
-                                                                    mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
-            else
-                k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
-                                                                     mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                                                                     nullptr, Q, gp, nch, nrb, tpr, mm->n_obs, nv,
-                                                                     mm->n_tiles, sc.tpart, qsp);
+            StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T);
+            k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
+                                                                 mm->trow.as<uint8_t>(), mm->tval.as<float>(),
+                                                                 nullptr, Q, gp, nch, nrb, tpr, mm->n_obs, nv,
+                                                                 mm->n_tiles, sc.tpart, qsp);
             HIPC(hipGetLastError());
         } else {  // no rows: zero partials (a fill kernel, no memset node)
             k_zero_words<<<blocks_for(static_cast<int64_t>(nrb) * nv * KP), 256, 0, st>>>(
@@ -884,10 +910,42 @@ struct Engine {
     // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix)
     // on the matrix's own scratch
     void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
+        if (bf16 && m->n_obs > 0) {
+            spmm_t_mfma(Q, centered, B64, B32);
+            return;
+        }
         // 1^T Q partials in cpart (nrb_t x nch_t x KP), then b^T Q's in the
         // same buffer: colsum_into runs after the 1^T Q reduction
         const TScratch sc{m->tpart.as<float>(), m->cpart.as<double>(), m->cpart.as<double>(), sumq, bq64};
-        transposed(m, s, gperm, nch_t, nrb_t, tiles_per_tr, Q, centered ? mu : nullptr, rowb, bf16, sc, B64, B32);
+        transposed(m, s, gperm, nch_t, nrb_t, tiles_per_tr, Q, centered ? mu : nullptr, rowb, sc, B64, B32);
+    }
+
+    // The BF16 sketch form's transposed product: Q split into hi / lo bf16
+    // (packed per 256-row tile), densified X~ tiles on the bf16 matrix cores
+    // into tile-range partials, 1^T Q (and b^T Q) in f64 from the f32 Q, then
+    // the same reduction and centering as the FP32 form.
+    void spmm_t_mfma(const float *Q, bool centered, double *B64, float *B32) {
+        const int64_t nv = m->n_vars;
+        int ngg = 0, nrb = 0, tpr = 0;
+        t_mfma_geometry(m, ngg, nrb, tpr);
+        const int64_t kpad = static_cast<int64_t>(m->n_tiles) * kTileRows;
+        m->qpk.ensure(sizeof(uint16_t) * 2 * static_cast<size_t>(kpad) * KP);
+        uint16_t *qh = m->qpk.as<uint16_t>(), *ql = qh + static_cast<size_t>(kpad) * KP;
+        {
+            StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_BF16);
+            k_pack_hilo<KP><<<blocks_for(kpad / 8 * KP), 256, 0, s>>>(Q, m->n_obs, kpad, qh, ql);
+            check();
+            k_spmm_t_mfma<kTWaves, kMfmaPrefetch><<<static_cast<unsigned>(nrb * ngg * (KP / 64)), kTWaves * 64, 0, s>>>(
+                m->indptr, m->tptr.as<uint32_t>(), m->tpk.as<uint32_t>(), qh, ql, KP, ngg, KP / 64, nrb, tpr, nv,
+                m->n_tiles, m->tpart.as<float>());
+            check();
+        }
+        if (centered) weighted_colsum(Q, nullptr, m->n_obs, sumq, nullptr);
+        if (rowb) weighted_colsum(Q, rowb, m->n_obs, bq64, nullptr);
+        StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
+        k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb, nv, centered ? mu : nullptr,
+                                                                 sumq, B64, B32, rowb ? bq64 : nullptr);
+        check();
     }
 
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
@@ -1893,7 +1951,7 @@ void project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, cons
                                   reinterpret_cast<double *>(base + o_cpart), reinterpret_cast<double *>(base + o_sumq),
                                   reinterpret_cast<double *>(base + o_bq)};
     const double *rowb = (m->pflog && n > 0) ? m->pfb.as<double>() : nullptr;
-    E::transposed(m, s, gp, nch, nrb, tpr, qp, mu, rowb, false, sc, nullptr, dst);
+    E::transposed(m, s, gp, nch, nrb, tpr, qp, mu, rowb, sc, nullptr, dst);
     if (pack) {
         k_unpack_panel32<KP><<<blocks_for(nv * k), 256, 0, s>>>(dst, nv, static_cast<int>(k), dw);
         HIPC(hipGetLastError());

@@ -166,6 +166,49 @@ def test_spmm_transpose(case, k):
     assert np.abs(got - ref).max() / np.abs(ref).max() < 2e-5
 
 
+def _bf16(x):
+    """Round to bf16 (round to nearest even), as f64."""
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _hilo(a):
+    """The panel as the bf16 products see it: bf16(a) + bf16(a - bf16(a))."""
+    a32 = np.asarray(a, dtype=np.float32)
+    hi = _bf16(a32)
+    return hi + _bf16((a32.astype(np.float64) - hi).astype(np.float32))
+
+
+@pytest.mark.parametrize("k", [60, 100, 5])
+def test_spmm_bf16(case, k):
+    """The BF16 sketch form's products (densified bf16 tiles on the matrix
+    cores): both directions against the f64 product of bf16-rounded X~ and the
+    panel's hi + lo bf16 split, 2e-5 of the f64 magnitude (f32 accumulation
+    and the centering's cancellation, the FP32 form's stage bar), and against
+    the FP32 form's product to the bf16 rounding of X~ (2^-9 per value; 1e-2
+    of the magnitude after the centering's cancellation)."""
+    name, csr, mat, xt, mu = case
+    xb = xt.with_data(_bf16(xt.data).astype(np.float32))
+    rng = np.random.default_rng(200 + k)
+    m = rng.standard_normal((csr.shape[1], k))
+    q = rng.standard_normal((csr.shape[0], k))
+    say(f"case: {name} bf16 forward + transpose k={k}")
+    got_f = mat.spmm(m, transpose=False, means=mu, bf16=True)
+    ref_f = oracle.spmm_forward(xb, _hilo(m), mu)
+    got_t = mat.spmm(q, transpose=True, means=mu, bf16=True)
+    ref_t = oracle.spmm_transpose(xb, _hilo(q), mu)
+    ef = np.abs(got_f - ref_f).max() / np.abs(ref_f).max()
+    et = np.abs(got_t - ref_t).max() / np.abs(ref_t).max()
+    f32_f = oracle.spmm_forward(xt, m, mu)
+    f32_t = oracle.spmm_transpose(xt, q, mu)
+    say(f"{name} bf16 k={k}: fwd {ef:.2e} t {et:.2e}; vs fp32 form "
+        f"{np.abs(got_f - f32_f).max() / np.abs(f32_f).max():.2e} / {np.abs(got_t - f32_t).max() / np.abs(f32_t).max():.2e}")
+    assert ef < 2e-5 and et < 2e-5
+    assert np.abs(got_f - f32_f).max() <= 1e-2 * np.abs(f32_f).max()
+    assert np.abs(got_t - f32_t).max() <= 1e-2 * np.abs(f32_t).max()
+
+
 def test_pca(case):
     name, csr, mat, xt, mu = case
     n, nv = csr.shape
