@@ -119,6 +119,7 @@ STREAM_METRIC = ("streamed rPCA wall-s (CSR in host memory, moved over PCIe in s
                  "1M cells×2k genes k=50, ≤1e-4 err vs FP64")
 COV_METRIC = "covariance-route PCA wall-s (exact, method='auto' at <= 5,000 genes), 1M cells×2k genes k=50"
 MFMA_F32_PEAK_TFLOPS = 157.3
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 MFMA
 
 
 def run_covariance(args, x, w, m, dev):
@@ -299,6 +300,115 @@ def run_streamed(args, x, w):
     print(json.dumps(out), flush=True)
 
 
+def _cpu_model() -> str:
+    """The host CPU model (lscpu's 'Model name', read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _median_runs(fn, runs):
+    fn()  # warm-up
+    ts = []
+    for _ in range(max(runs, 1)):
+        tc = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - tc)
+    ts.sort()
+    return ts[len(ts) // 2], ts
+
+
+def cpu_baseline_rpca(args, x, w, pflog=False):
+    """The oracle (the C/OpenMP restatement of the reference CPU path, f64)
+    on row samples of the same matrix: all host cores (at most 16) and one
+    thread, one warm-up and the median of --cpu-runs runs each, scaled
+    linearly in the row count to the full matrix."""
+    import oracle
+
+    n = x.shape[0]
+    cores = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count(), 16)
+
+    def run(rows, threads):
+        xs = x.head(rows)
+        xv = oracle.CsrView(xs.indptr, xs.indices, xs.data, xs.shape)
+        if pflog:
+            return lambda: oracle.pflog_pca(xv, n_comps=w.n_comps, n_oversamples=w.n_oversamples,
+                                            n_power_iterations=w.n_power_iterations, seed=0, alpha=None, form=1,
+                                            nthreads=threads)
+        return lambda: oracle.randomized_pca(xv, n_comps=w.n_comps, n_oversamples=w.n_oversamples,
+                                             n_power_iterations=w.n_power_iterations, seed=0,
+                                             target_sum=w.target_sum, log1p=True, nthreads=threads)
+
+    rows = min(n, 250_000, max(n // 4, 1)) if args.cpu_rows < 0 else min(args.cpu_rows, n)
+    med, ts = _median_runs(run(rows, cores), args.cpu_runs)
+    what = ("alpha estimate, delta, baseline and pflog_pca" if pflog else "randomized_pca_inmemory")
+    out = {"value": round(med * n / rows, 3), "unit": "s", "cores": cores, "kind": "port",
+           "sample": f"first {rows:,} of {n:,} rows; 1 warm-up + median of {len(ts)} runs ({med:.3f} s), scaled "
+                     f"linearly to {n:,} rows; the C/OpenMP restatement of {what} (f64), {cores} threads",
+           "runs_s": [round(t, 3) for t in ts], "cpu_model": _cpu_model()}
+    if args.cpu_rows_1t != 0:
+        rows1 = min(n, 25_000, max(n // 40, 1)) if args.cpu_rows_1t < 0 else min(args.cpu_rows_1t, n)
+        med1, ts1 = _median_runs(run(rows1, 1), args.cpu_runs)
+        out["one_thread"] = {"value": round(med1 * n / rows1, 3), "unit": "s", "cores": 1,
+                             "sample": f"first {rows1:,} rows; 1 warm-up + median of {len(ts1)} runs ({med1:.3f} s), "
+                                       f"scaled linearly to {n:,} rows",
+                             "runs_s": [round(t, 3) for t in ts1]}
+    return out
+
+
+def accuracy_block(x, w, qr, rows=50_000):
+    """Accuracy of the GPU result on the first `rows` cells (the bench's
+    randomized route, FP32 and BF16 sketch): singular values against the
+    oracle's f64 randomized PCA on the same Omega (the path's <= 1e-4
+    contract) and against the exact PCA of the explicitly centred X~ (dense
+    FP64, LAPACK eigendecomposition of the Gram matrix): the algorithm's own
+    approximation error, which the f64 randomized PCA shares."""
+    import numpy as np
+
+    import oracle
+    from scx_amd import _lib
+
+    n, nv = x.shape
+    rows = min(rows, n)
+    if nv > 5000:
+        return None
+    xs = x.head(rows)
+    view = oracle.CsrView(xs.indptr, xs.indices, xs.data, xs.shape)
+    ref = oracle.randomized_pca(view, n_comps=w.n_comps, n_oversamples=w.n_oversamples,
+                                n_power_iterations=w.n_power_iterations, seed=0, target_sum=w.target_sum,
+                                log1p=True, nthreads=min(16, os.cpu_count() or 1))
+    xt = oracle.transform(view, w.target_sum, True)
+    d = np.zeros((rows, nv))
+    d[np.repeat(np.arange(rows), np.diff(xs.indptr)), xs.indices.astype(np.int64)] = xt
+    d -= d.mean(axis=0)
+    ev = np.linalg.eigvalsh(d.T @ d)[::-1]
+    s_exact = np.sqrt(np.maximum(ev, 0.0))
+    del d
+    m = _lib.Matrix(xs.indptr, xs.indices, xs.data, xs.shape, target_sum=w.target_sum, log1p=True)
+    out = {"cells": rows, "reference": "oracle f64 rPCA (same Omega) and dense FP64 exact PCA of the subsample"}
+    try:
+        for name, prec in (("fp32", _lib.PRECISION_FP32), ("bf16", _lib.PRECISION_BF16_SKETCH)):
+            got = m.pca(n_comps=w.n_comps, n_oversamples=w.n_oversamples, n_power_iterations=w.n_power_iterations,
+                        seed=0, qr_method=qr, precision=prec)
+            nc = w.n_comps
+            e64 = np.abs(got["sigma"][:nc] - ref["sigma_all"][:nc]) / ref["sigma_all"][:nc]
+            eex = np.abs(got["sigma"][:nc] - s_exact[:nc]) / s_exact[:nc]
+            out[name] = {"sigma_rel_vs_f64_rpca_max": float(f"{e64.max():.3g}"),
+                         "sigma_rel_vs_dense_fp64_top1": float(f"{eex[0]:.3g}"),
+                         "sigma_rel_vs_dense_fp64_top10_max": float(f"{eex[:10].max():.3g}"),
+                         "sigma_rel_vs_dense_fp64_max": float(f"{eex.max():.3g}")}
+        e_alg = np.abs(ref["sigma_all"][:w.n_comps] - s_exact[:w.n_comps]) / s_exact[:w.n_comps]
+        out["f64_rpca_sigma_rel_vs_dense_fp64_max"] = float(f"{e_alg.max():.3g}")
+    finally:
+        m.close()
+    return out
+
+
 def _gen_batch(b):
     from scx_amd import synth
 
@@ -315,8 +425,16 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--qr", choices=["cholesky", "householder"], default="cholesky")
     ap.add_argument("--cpu-rows", type=int, default=-1,
-                    help="rows for the CPU baseline: -1 all rows, 0 skip, N the first N rows")
-    ap.add_argument("--host-input", type=int, default=0, metavar="RUNS",
+                    help="rows of the all-cores CPU baseline sample: -1 a quarter of the matrix (at most "
+                         "250,000 rows), 0 skip the CPU baseline, N the first N rows")
+    ap.add_argument("--cpu-rows-1t", type=int, default=-1,
+                    help="rows of the one-thread CPU baseline sample: -1 a fortieth of the matrix (at most "
+                         "25,000 rows), 0 skip")
+    ap.add_argument("--cpu-runs", type=int, default=5,
+                    help="timed CPU baseline runs after one warm-up (the median is reported)")
+    ap.add_argument("--legs", type=int, default=5, metavar="STEPS",
+                    help="steps of the extra timed legs (qr=householder, the BF16 sketch form); 0 skips them")
+    ap.add_argument("--host-input", type=int, default=3, metavar="RUNS",
                     help="also time RUNS calls from host CSR buffers (H2D copy, validation, prepare, rPCA, "
                          "results back to host); reported beside the device-resident value, never as it")
     ap.add_argument("--shard-rows", type=int, default=250_000,
@@ -425,6 +543,17 @@ def main():
     dom_ms = prof[dom][0] / max(prof[dom][1], 1)
     bytes_per = spmm_bytes(x.nnz, n, nv, k, bf16=dom.endswith("bf16"))
     achieved = bytes_per / (dom_ms * 1e-3) / 1e9
+    mfma_roof = None
+    if dom.endswith("bf16"):
+        # the BF16 form's products run densified tiles on the bf16 matrix cores:
+        # dense-equivalent flops 2 n nv_pad KP x 2 (the panel's hi and lo terms)
+        kp = 64 if k <= 64 else 128
+        nv_pad = (nv + 63) // 64 * 64
+        fl = 2.0 * n * nv_pad * kp * 2
+        mfma_roof = {"bound": "mfma", "kernel": dom, "achieved": round(fl / (dom_ms * 1e-3) / 1e12, 1),
+                     "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(fl / (dom_ms * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4),
+                     "flops_per_launch": fl, "avg_launch_ms": round(dom_ms, 4)}
 
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmm_traffic.json")
@@ -434,30 +563,46 @@ def main():
         if rec:
             traffic = rec["traffic_bytes"]
 
-    cpu = None
-    if args.cpu_rows != 0:
-        import oracle
+    # the Gram of the tall CholeskyQR (the one dense contraction) on FP32 MFMA:
+    # algorithmic flops 2 n k'^2 per launch over the stage's mean launch time
+    gram = None
+    if stage_prof.get("gram", (0.0, 0))[1]:
+        g_ms = stage_prof["gram"][0] / stage_prof["gram"][1]
+        g_flops = 2.0 * n * k * k
+        gram = {"launches_per_step": stage_prof["gram"][1] // max(args.profile_steps, 1),
+                "avg_ms": round(g_ms, 4), "flops_per_launch": g_flops,
+                "achieved_tflops": round(g_flops / (g_ms * 1e-3) / 1e12, 2), "peak_tflops": MFMA_F32_PEAK_TFLOPS,
+                "frac": round(g_flops / (g_ms * 1e-3) / 1e12 / MFMA_F32_PEAK_TFLOPS, 4),
+                "note": "stage = k_gram4 (16x16x4 f32 MFMA, f64 partials) + k_sum_parts16"}
 
-        rows = n if args.cpu_rows < 0 else min(args.cpu_rows, n)
-        xs = x.head(rows)
-        cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-        cores = min(cores, 16)
-        xv = oracle.CsrView(xs.indptr, xs.indices, xs.data, xs.shape)
-        tc = time.perf_counter()
-        if pflog:
-            oracle.pflog_pca(xv, n_comps=w.n_comps, n_oversamples=w.n_oversamples,
-                             n_power_iterations=w.n_power_iterations, seed=0, alpha=None, form=1, nthreads=cores)
-        else:
-            oracle.randomized_pca(xv, n_comps=w.n_comps, n_oversamples=w.n_oversamples,
-                                  n_power_iterations=w.n_power_iterations, seed=0, target_sum=w.target_sum,
-                                  log1p=True, nthreads=cores)
-        tcpu = time.perf_counter() - tc
-        what = ("alpha estimate, delta, baseline and pflog_pca (the C/OpenMP restatement, f64)" if pflog else
-                "the C/OpenMP restatement of randomized_pca_inmemory (f64)")
-        cpu = {"value": round(tcpu * n / rows, 3), "unit": "s", "cores": cores, "kind": "port",
-               "sample": (f"the full {n:,}-row matrix" if rows == n else
-                          f"first {rows:,} of {n:,} rows of the same matrix, {tcpu:.2f}s, scaled linearly")
-                         + f"; one run of {what}, {cores} threads"}
+    # extra legs on the same resident matrix, after the timed region
+    legs = None
+    if args.legs > 0 and not pflog:
+        legs = {}
+        for name, kw_leg in (("qr_householder", dict(kw, qr_method=_lib.QR_HOUSEHOLDER)),
+                             ("bf16_sketch", dict(kw, precision=_lib.PRECISION_BF16_SKETCH)),
+                             ("fp32", dict(kw, precision=_lib.PRECISION_FP32))):
+            if (name == "qr_householder" and qr == _lib.QR_HOUSEHOLDER) or (name == "fp32" and not bf16):
+                continue
+            m.refresh()
+            r0 = m.pca(**kw_leg)
+            torch.cuda.synchronize()
+            tl = time.perf_counter()
+            for _ in range(args.legs):
+                m.refresh()
+                r1 = m.pca(**kw_leg)
+            torch.cuda.synchronize()
+            ms_leg = (time.perf_counter() - tl) * 1e3 / args.legs
+            se = np.abs(r1["sigma"][:w.n_comps] - res["sigma"][:w.n_comps]) / np.abs(res["sigma"][:w.n_comps])
+            legs[name] = {"ms_per_step": round(ms_leg, 4), "steps": args.legs,
+                          "sigma_rel_vs_headline_max": float(f"{se.max():.3g}")}
+            del r0
+
+    cpu = cpu_baseline_rpca(args, x, w, pflog) if args.cpu_rows != 0 else None
+
+    accuracy = None
+    if not pflog and args.legs > 0:
+        accuracy = accuracy_block(x, w, qr)
 
     host_input = None
     if args.host_input > 0:
@@ -497,7 +642,7 @@ def main():
             "n_obs": n, "n_vars": nv, "nnz": x.nnz, "k": k,
             "parallelism": "single GPU",
         },
-        "roofline": {
+        "roofline": mfma_roof or {
             "bound": "hbm",
             "kernel": dom,
             "achieved": round(achieved, 1),
@@ -509,6 +654,9 @@ def main():
             "bytes_per_launch": bytes_per,
         },
         "stage_ms_per_step": stage_ms,
+        "gram_mfma": gram,
+        "legs": legs,
+        "accuracy": accuracy if not pflog else None,
         "cpu_baseline": cpu,
         "seconds_device_last": round(res["seconds_device"], 6),
         "host_input": host_input,
