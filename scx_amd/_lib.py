@@ -53,6 +53,8 @@ EXPORTS = (
     "scxpca_device_count",
     "scxpca_device_info",
     "scxpca_last_error",
+    "scxshard_read_info",
+    "scxshard_decode_csr",
 )
 
 
@@ -80,6 +82,20 @@ class PflogParams(ctypes.Structure):
 class PflogFit(ctypes.Structure):
     _fields_ = [("alpha", _dbl), ("pseudocount", _dbl), ("n_genes_used", _i64), ("estimated", _i32),
                 ("fell_back", _i32)]
+
+
+class ShardInfo(ctypes.Structure):
+    _fields_ = [("n_major", _i64), ("n_minor", _i64), ("nnz", _i64), ("global_offset", _i64), ("codec_id", _i32),
+                ("value_encoding", _i32), ("index_dtype", _i32), ("shard_format_version", _i32),
+                ("shard_type", _i32), ("n_groups", _i32)]
+
+
+class ShardStats(ctypes.Structure):
+    _fields_ = [("host_uploaded_bytes", _u64), ("device_decoded_bytes", _u64), ("fully_device_decoded", _i32),
+                ("n_shards_scx1_gpu", ctypes.c_uint32), ("n_shards_shufdelta_gpu", ctypes.c_uint32),
+                ("n_shards_host_bounced", ctypes.c_uint32), ("n_shards_other_gpu", ctypes.c_uint32),
+                ("n_groups", ctypes.c_uint32), ("host_seconds", _dbl), ("upload_seconds", _dbl),
+                ("device_seconds", _dbl), ("total_seconds", _dbl)]
 
 
 class ScxPcaError(RuntimeError):
@@ -145,6 +161,9 @@ def lib() -> ctypes.CDLL:
     l.scxpca_device_count.argtypes = []
     l.scxpca_device_info.argtypes = [ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_i64),
                                      ctypes.POINTER(_i32)]
+    l.scxshard_read_info.argtypes = [_vp, _u64, ctypes.POINTER(ShardInfo)]
+    l.scxshard_decode_csr.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _vp,
+                                      ctypes.POINTER(ShardStats)]
     _lib = l
     return l
 

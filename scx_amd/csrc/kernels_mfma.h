@@ -296,7 +296,6 @@ __global__ void __launch_bounds__(512) k_spmm_fwd_mfma(const int64_t *__restrict
                                                      int64_t n_obs, int64_t n_vars, int n_tiles,
                                                      const float *__restrict__ mc, const double *__restrict__ rowb,
                                                      const float *__restrict__ cs, float *__restrict__ Y) {
-    constexpr int NT = 512;
     constexpr int SLAB_U4 = 64 * kFSlabLd / 8;  // uint4 per wave slab
     constexpr int MU4 = 8 * 64;                 // uint4 per chunk of one term (8 k-groups x 64 columns)
     __shared__ uint4 slab[8][SLAB_U4];

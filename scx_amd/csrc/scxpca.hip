@@ -2790,3 +2790,6 @@ size_t scxpca_last_error(char *buf, size_t len) {
 }
 
 }  // extern "C"
+
+// On-GPU shard decode (include/scxshard.h).
+#include "shard_decode.h"
