@@ -216,6 +216,98 @@ def run_covariance(args, x, w, m, dev):
     print(json.dumps(out), flush=True)
 
 
+DECODE_METRIC = "SCX shard decode wall-s, host shard bytes -> device CSR (i64/i32/f32), Scx1 shards of 16,384 rows"
+
+
+def run_decode(args, x, w):
+    """--method decode: the workload's CSR written as Scx1 shards (16,384 rows,
+    framed at 256 rows: the reference writer's defaults) by the library's
+    writer, then decoded onto the GPU. A step is one decode of every shard,
+    from host bytes to the device CSR (host walks + decompression, the upload
+    and the kernels); stats split it."""
+    import numpy as np
+    import torch
+
+    from scx_amd import shards as host
+
+    vals = x.data.astype(np.uint16)
+    if not np.array_equal(vals.astype(np.float32), x.data):
+        raise SystemExit("decode bench: the synthetic counts do not fit u16")
+    t0 = time.time()
+    shards = host.encode_csr_shards(x.indptr, x.indices, vals, x.shape[1], rows_per_shard=16384, codec="scx1",
+                                    row_group_rows=256)
+    enc_bytes = sum(len(s) for s in shards)
+    print(f"# encoded {len(shards)} shards, {enc_bytes / 2**20:.1f} MiB in {time.time() - t0:.1f}s", file=sys.stderr,
+          flush=True)
+    csr = None
+    for _ in range(args.warmup):
+        csr = host.decode_csr_shards_to_device(shards)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        csr = host.decode_csr_shards_to_device(shards)
+        stats.append(csr.stats)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    ok = (np.array_equal(csr.indptr.cpu().numpy(), x.indptr) and np.array_equal(csr.indices.cpu().numpy(),
+                                                                                x.indices.view(np.int32))
+          and np.array_equal(csr.data.cpu().numpy(), x.data))
+    med = lambda k: float(np.median([s[k] for s in stats]))  # noqa: E731
+    dev_s = med("device_seconds")
+    n, nnz = x.shape[0], x.nnz
+    # algorithmic device bytes: the staged payload read once, the tables, the
+    # decoded indices and values written once
+    payload = stats[-1]["host_uploaded_bytes"] - (n + 1) * 8
+    moved = payload + nnz * 8
+    out_bytes = (n + 1) * 8 + nnz * 8
+    cpu = None
+    if args.cpu_rows != 0:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from oracle import shards as S
+
+        cores = min(len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count(), 16)
+
+        def cpu_decode():
+            with ThreadPoolExecutor(cores) as ex:
+                list(ex.map(S.decode_shard, shards))
+
+        med_c, runs = _median_runs(cpu_decode, max(1, min(args.cpu_runs, 3)))
+        cpu = {"value": round(med_c, 4), "unit": "s", "cores": cores, "kind": "port",
+               "sample": f"all {len(shards)} shards, 1 warm-up + median of {len(runs)}; the oracle's bit-serial "
+                         f"C restatement of decode_shard_scipy, one shard per thread",
+               "runs_s": [round(t, 4) for t in runs], "cpu_model": _cpu_model()}
+    out = {
+        "metric": DECODE_METRIC,
+        "value": round(ms / 1e3, 6),
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": f"{w.name}: {n:,} x {x.shape[1]:,} CSR, nnz={nnz:,} as {len(shards)} Scx1 shards "
+                               f"(16,384 rows, framed at 256 rows, u16 indices and values), "
+                               f"{enc_bytes / 2**20:.1f} MiB encoded",
+                   "n_obs": n, "n_vars": x.shape[1], "nnz": nnz, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_forbp_rows + k_rice_blocks", "achieved": round(moved / dev_s / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(moved / dev_s / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_ms": round(dev_s * 1e3, 4), "bytes_per_launch": moved},
+        "decoded_gbs_end_to_end": round(out_bytes / (ms * 1e-3) / 1e9, 2),
+        "stage_ms": {"host": round(med("host_seconds") * 1e3, 3), "upload": round(med("upload_seconds") * 1e3, 3),
+                     "device": round(dev_s * 1e3, 3), "total": round(med("total_seconds") * 1e3, 3)},
+        "host_uploaded_bytes": stats[-1]["host_uploaded_bytes"],
+        "round_trip_exact": bool(ok),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def run_streamed(args, x, w):
     """The streamed operator on the host CSR: every pass of the power loop moves
     the CSR to HBM in shards of --shard-rows rows (pinned staging, a copy stream
@@ -442,7 +534,8 @@ def main():
     ap.add_argument("--pinned-host", action="store_true",
                     help="--method streamed: hold the host CSR's indices and values in page-locked memory "
                          "(sent to the device directly instead of through the pinned staging copy)")
-    ap.add_argument("--method", choices=["randomized", "covariance", "pflog", "streamed"], default="randomized",
+    ap.add_argument("--method", choices=["randomized", "covariance", "pflog", "streamed", "decode"],
+                    default="randomized",
                     help="randomized (the headline metric), the exact covariance route, PFlog (v4) "
                          "normalization with its baseline-aware randomized PCA on the raw counts, or the "
                          "streamed operator for a CSR larger than HBM (host CSR, shards over PCIe each pass)")
@@ -480,6 +573,9 @@ def main():
 
     if args.method == "streamed":
         run_streamed(args, x, w)
+        return
+    if args.method == "decode":
+        run_decode(args, x, w)
         return
 
     dev = torch.device("cuda:0")

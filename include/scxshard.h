@@ -82,6 +82,21 @@ scxpca_status scxshard_decode_csr(const uint8_t *const *shards, const uint64_t *
                                   int64_t *indptr, int32_t *indices, float *data, int64_t n_rows, int64_t nnz,
                                   int32_t device, void *stream, scxshard_stats *stats);
 
+/* Writes rows [0, n_rows) of a host CSR as one CSR shard section -- the
+ * layout of the reference writer's encode_shard / encode_shard_framed
+ * (scx-codec/src/dispatch.rs:36-60, scx-format-io/src/encoder.rs:524-690,
+ * header and block index scx-format/src/shard.rs:98-122, :440-478) -- for
+ * codec 1 (Scx1; integer values) or 0 (None). values: n = indptr[n_rows] -
+ * indptr[0] elements of the value encoding's width (0 u8, 1 u16, 2 u32, 3 f32,
+ * 4 f16). index_dtype 0 writes u16 indices. row_group_rows > 0 frames the
+ * shard (shard format 2, one block-index entry per group). out == NULL
+ * returns the size in *written; otherwise cap must hold it. The 8 checksum
+ * bytes of the header are left zero (no reader verifies them). Host only. */
+scxpca_status scxshard_encode(const int64_t *indptr, int64_t n_rows, const uint32_t *indices, const void *values,
+                              int32_t value_encoding, int32_t codec, int32_t index_dtype, int64_t n_minor,
+                              int64_t global_offset, int32_t row_group_rows, uint8_t *out, uint64_t cap,
+                              uint64_t *written);
+
 #ifdef __cplusplus
 }
 #endif

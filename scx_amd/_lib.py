@@ -55,6 +55,7 @@ EXPORTS = (
     "scxpca_last_error",
     "scxshard_read_info",
     "scxshard_decode_csr",
+    "scxshard_encode",
 )
 
 
@@ -164,6 +165,8 @@ def lib() -> ctypes.CDLL:
     l.scxshard_read_info.argtypes = [_vp, _u64, ctypes.POINTER(ShardInfo)]
     l.scxshard_decode_csr.argtypes = [_vp, _vp, _i64, _vp, _vp, _vp, _i64, _i64, _i32, _vp,
                                       ctypes.POINTER(ShardStats)]
+    l.scxshard_encode.argtypes = [_vp, _i64, _vp, _vp, _i32, _i32, _i32, _i64, _i64, _i32, _vp, _u64,
+                                  ctypes.POINTER(_u64)]
     _lib = l
     return l
 

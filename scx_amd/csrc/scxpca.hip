@@ -2793,3 +2793,4 @@ size_t scxpca_last_error(char *buf, size_t len) {
 
 // On-GPU shard decode (include/scxshard.h).
 #include "shard_decode.h"
+#include "shard_encode.h"

@@ -64,9 +64,10 @@ struct RowJob {        // one non-empty FOR-BP row
 struct RiceJob {       // one Rice block
     uint64_t bit;      // bit offset of the first code in the staging buffer
     uint64_t out;
-    uint32_t n;
-    uint32_t k;
+    uint32_t nk;       // values | k << 16
+    uint32_t nbits;    // bits from `bit` to the end of the block's last code
 };
+constexpr int kRiceLdsWords = 4096;  // per wave: 16 KB of staged block bytes
 struct PlaneJob {      // one index or value sub-stream held as bytes
     uint64_t src;      // staging byte offset
     uint64_t out;      // first output element
@@ -91,19 +92,36 @@ __device__ __forceinline__ uint64_t window(const uint32_t *__restrict__ w, uint6
     return v >> (bit & 31);
 }
 
-// FOR-BP rows: a wavefront per row, four rows per workgroup. Lane i of a
-// 64-entry chunk extracts delta i; the wave's inclusive prefix sum plus the
-// running carry (frame_min + earlier chunks) gives the index.
+// FOR-BP rows: a wavefront per row, four rows per workgroup. Each lane takes
+// four consecutive entries of a 256-entry slice, extracts their deltas, and a
+// wave prefix sum of the lane sums plus the running carry (frame_min + earlier
+// slices) rebuilds the indices.
 // (forbp_decode_kernel / forbp_decode_bp4x_kernel semantics,
 // scx-gpu/kernels/forbp_decode.cu:13-64, forbp_decode_bp4x.cu:41-119; the
 // u32-overflow and index-bound checks of scx-codec's decoder and
 // u32_vec_to_i32_bounded, scx-codec/src/forbp.rs:478-483, dispatch.rs:250.)
+__device__ __forceinline__ uint32_t forbp_delta(const uint32_t *__restrict__ stage, const RowJob &r, uint32_t i,
+                                                uint32_t fb, uint64_t mask, uint32_t full, uint64_t tail_bit) {
+    if (i < full) {
+        // value p of a 128-chunk: lane p & 3, slot p >> 2 of that lane's
+        // little-endian u32 stream at 16-byte stride
+        const uint32_t c = i / kSimd, p = i % kSimd, l = p & 3u, slot = p >> 2;
+        const uint32_t bitpos = slot * fb, w0 = bitpos >> 5, bin = bitpos & 31u;
+        const uint64_t byte = r.src + (uint64_t)c * fb * 16u + w0 * 16u + l * 4u;
+        uint64_t v = (uint32_t)window(stage, byte * 8u) >> bin;
+        if (bin + fb > 32u) v |= (uint64_t)(uint32_t)window(stage, (byte + 16u) * 8u) << (32u - bin);
+        return (uint32_t)(v & mask);
+    }
+    return (uint32_t)(window(stage, tail_bit + (uint64_t)(i - full) * fb) & mask);
+}
+
 __global__ __launch_bounds__(256) void k_forbp_rows(const uint32_t *__restrict__ stage,
                                                     const RowJob *__restrict__ jobs, int64_t n_jobs,
                                                     int32_t *__restrict__ out, uint32_t bound,
                                                     uint32_t *__restrict__ err) {
-    const int lane = threadIdx.x & 63;
-    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    __shared__ uint32_t xbuf[4][256];  // a wave's slice, for coalesced stores
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
     if (j >= n_jobs) return;
     const RowJob r = jobs[j];
     const uint32_t fb = r.meta & 0xffu;
@@ -113,60 +131,92 @@ __global__ __launch_bounds__(256) void k_forbp_rows(const uint32_t *__restrict__
     const uint64_t tail_bit = (r.src + (uint64_t)(full / kSimd) * fb * 16u) * 8u;
     uint64_t carry = r.fmin;
     uint32_t bad = 0;
-    for (uint32_t base = 0; base < r.nnz; base += 64) {
-        const uint32_t i = base + lane;
-        uint64_t d = 0;
-        if (i < r.nnz && fb) {
-            if (i < full) {
-                // value p of a 128-chunk: lane p & 3, slot p >> 2 of that lane's
-                // little-endian u32 stream at 16-byte stride
-                const uint32_t c = i / kSimd, p = i % kSimd, l = p & 3u, slot = p >> 2;
-                const uint32_t bitpos = slot * fb, w0 = bitpos >> 5, bin = bitpos & 31u;
-                const uint64_t byte = r.src + (uint64_t)c * fb * 16u + w0 * 16u + l * 4u;
-                uint64_t v = (uint32_t)window(stage, byte * 8u) >> bin;
-                if (bin + fb > 32u) v |= (uint64_t)(uint32_t)window(stage, (byte + 16u) * 8u) << (32u - bin);
-                d = v & mask;
-            } else {
-                d = window(stage, tail_bit + (uint64_t)(i - full) * fb) & mask;
-            }
+    uint32_t *xb = xbuf[wave];
+    for (uint32_t base = 0; base < r.nnz; base += 256) {
+        const uint32_t i0 = base + (uint32_t)lane * 4u;
+        uint32_t d[4];
+        uint64_t sum = 0;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            d[e] = (i0 + e < r.nnz && fb) ? forbp_delta(stage, r, i0 + e, fb, mask, full, tail_bit) : 0u;
+            sum += d[e];
         }
-        uint64_t s = d;
+        uint64_t s = sum;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
             const uint64_t t = __shfl_up(s, off, 64);
             if (lane >= off) s += t;
         }
-        const uint64_t idx = carry + s;
-        if (i < r.nnz) {
-            if (idx > 0xffffffffull) bad |= ERR_U32;
-            else if (idx > 0x7fffffffull) bad |= ERR_I32;
-            else if (bound && idx >= bound) bad |= ERR_BOUND;
-            out[r.out + i] = (int32_t)(uint32_t)idx;
+        uint64_t idx = carry + s - sum;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t i = i0 + e;
+            idx += d[e];
+            if (i < r.nnz) {
+                if (idx > 0xffffffffull) bad |= ERR_U32;
+                else if (idx > 0x7fffffffull) bad |= ERR_I32;
+                else if (bound && idx >= bound) bad |= ERR_BOUND;
+            }
+            xb[lane * 4 + e] = (uint32_t)idx;
         }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const uint32_t i = base + (uint32_t)e * 64u + (uint32_t)lane;
+            if (i < r.nnz) out[r.out + i] = (int32_t)xb[e * 64 + lane];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         carry += __shfl(s, 63, 64);
     }
     if (bad) atomicOr(err, bad);
 }
 
-// Rice blocks: a lane per block of up to 256 values. The host walk already
-// checked every code of the block, so no read leaves it.
+// Rice blocks: a lane per block of up to 256 values, 64 consecutive blocks
+// per wave. The blocks of a wave are adjacent in the staging buffer, so the
+// wave first copies their bytes into LDS with coalesced loads and every lane
+// then walks its codes out of LDS (a serial chain of LDS reads instead of
+// global-memory round trips); a span above the LDS budget is walked in place.
+// The host walk already checked every code, so no read leaves the span.
 // (rice_decode_kernel semantics, scx-gpu/kernels/rice_decode.cu:14-88:
 // value = ((q << k) | r) + 1, then the u32 -> f32 conversion of
 // decode_shard_scipy, scx-codec/src/dispatch.rs:252.)
 __global__ __launch_bounds__(256) void k_rice_blocks(const uint32_t *__restrict__ stage,
                                                      const RiceJob *__restrict__ jobs, int64_t n_jobs,
                                                      float *__restrict__ out) {
-    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n_jobs) return;
-    const RiceJob b = jobs[j];
-    uint64_t p = b.bit;
-    const uint32_t k = b.k;
+    __shared__ uint32_t lds[4][kRiceLdsWords + 2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t j0 = ((int64_t)blockIdx.x * 4 + wave) * 64;
+    const int64_t jn = j0 < n_jobs ? (n_jobs - j0 < 64 ? n_jobs : j0 + 64) : j0;
+    const int64_t j = j0 + lane;
+    RiceJob b{0, 0, 0, 0};
+    if (j < jn) b = jobs[j];
+    bool in_lds = false;
+    uint64_t w0 = 0;
+    if (jn > j0) {
+        const uint64_t first = __shfl(b.bit, 0, 64);
+        const int last = (int)(jn - j0 - 1);
+        const uint64_t end = __shfl(b.bit + b.nbits, last, 64);
+        w0 = first >> 5;
+        const uint64_t nw = ((end + 31) >> 5) - w0 + 2;
+        in_lds = nw <= (uint64_t)kRiceLdsWords;
+        if (in_lds)
+            for (uint64_t w = lane; w < nw; w += 64) lds[wave][w] = stage[w0 + w];
+    }
+    __syncthreads();
+    if (j >= jn) return;
+    const uint32_t n = b.nk & 0xffffu, k = b.nk >> 16;
     const uint32_t mask = (1u << k) - 1u;
+    const uint32_t *src = in_lds ? lds[wave] : stage;
+    uint64_t p = in_lds ? b.bit - w0 * 32 : b.bit;
     float *o = out + b.out;
-    for (uint32_t i = 0; i < b.n; ++i) {
+    for (uint32_t i = 0; i < n; ++i) {
         uint32_t q = 0;
         for (;;) {
-            const uint64_t w = window(stage, p);
+            const uint64_t w = window(src, p);
             const uint32_t valid = 64u - (uint32_t)(p & 31);
             const uint64_t inv = ~w;
             const uint32_t t = inv ? (uint32_t)__builtin_ctzll(inv) : 64u;
@@ -180,11 +230,70 @@ __global__ __launch_bounds__(256) void k_rice_blocks(const uint32_t *__restrict_
         }
         uint32_t r = 0;
         if (k) {
-            r = (uint32_t)window(stage, p) & mask;
+            r = (uint32_t)window(src, p) & mask;
             p += k;
         }
         o[i] = (float)(((q << k) | r) + 1u);
     }
+}
+
+// Rice blocks with k = 0, a wavefront per block: every code is a run of ones
+// closed by a zero, so value i is the distance between the i-th and the
+// (i-1)-th zero of the body. Lane l takes bits [32 l, 32 l + 32) of each
+// 2,048-bit slice; a wave prefix sum of the zero counts numbers the codes, a
+// wave max-scan of the last zero positions links the slices, the values go
+// to LDS and leave in coalesced stores.
+__global__ __launch_bounds__(256) void k_rice_k0(const uint32_t *__restrict__ stage,
+                                                 const RiceJob *__restrict__ jobs, int64_t n_jobs,
+                                                 float *__restrict__ out) {
+    __shared__ float obuf[4][kBVal];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t j = (int64_t)blockIdx.x * 4 + wave;
+    if (j >= n_jobs) return;
+    const RiceJob b = jobs[j];
+    const uint32_t n = b.nk & 0xffffu, nbits = b.nbits;
+    uint32_t base = 0;
+    int32_t last = -1;  // bit of the previous code's closing zero
+    for (uint32_t c0 = 0; c0 < nbits; c0 += 2048) {
+        const uint32_t at = c0 + (uint32_t)lane * 32u;
+        uint32_t z = 0;
+        if (at < nbits) {
+            z = ~(uint32_t)window(stage, b.bit + at);
+            const uint32_t valid = nbits - at;
+            if (valid < 32) z &= (1u << valid) - 1u;
+        }
+        const uint32_t cnt = (uint32_t)__builtin_popcount(z);
+        uint32_t incl = cnt;
+        int32_t mx = z ? (int32_t)(at + 31u - (uint32_t)__builtin_clz(z)) : -1;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t t = __shfl_up(incl, off, 64);
+            const int32_t m = __shfl_up(mx, off, 64);
+            if (lane >= off) {
+                incl += t;
+                mx = m > mx ? m : mx;
+            }
+        }
+        // exclusive forms: the codes before this lane, the last zero before it
+        uint32_t idx = base + incl - cnt;
+        int32_t prev = __shfl_up(mx, 1, 64);
+        if (lane == 0) prev = -1;
+        prev = prev > last ? prev : last;
+        while (z) {
+            const int32_t zp = (int32_t)(at + (uint32_t)__builtin_ctz(z));
+            if (idx < (uint32_t)kBVal) obuf[wave][idx] = (float)(uint32_t)(zp - prev);
+            prev = zp;
+            ++idx;
+            z &= z - 1u;
+        }
+        base += __shfl(incl, 63, 64);
+        const int32_t lm = __shfl(mx, 63, 64);
+        last = lm > last ? lm : last;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t i = (uint32_t)lane; i < n; i += 64) out[b.out + i] = obuf[wave][i];
 }
 
 // Packed-byte add: four independent mod-256 sums in one u32.
@@ -393,7 +502,7 @@ struct Group {
     uint64_t st_ix, st_vv;       // staging offsets
     uint64_t sz_ix, sz_vv;       // staged bytes
     std::vector<RowJob> rows;
-    std::vector<RiceJob> rice;
+    std::vector<RiceJob> rice, rice0;  // general blocks, k = 0 blocks
     std::vector<PlaneJob> planes;
     std::string err;
 };
@@ -600,8 +709,12 @@ inline bool host_skip_zeros(const uint8_t *s, uint64_t len, uint64_t &p, uint32_
             p += valid;
             continue;
         }
-        for (uint32_t i = 1; i < m; ++i) z &= z - 1;  // drop the first m - 1 zeros
-        p += (uint64_t)__builtin_ctzll(z) + 1;
+        // the m-th set bit of z: skip whole bytes by popcount, then bits
+        uint32_t pos = 0;
+        for (uint32_t c8; (c8 = (uint32_t)__builtin_popcountll(z & 0xffu)) < m; z >>= 8, pos += 8) m -= c8;
+        for (;; z >>= 1, ++pos)
+            if ((z & 1u) && --m == 0) break;
+        p += (uint64_t)pos + 1;
         return true;
     }
     return m == 0;
@@ -743,7 +856,7 @@ inline std::string rice_walk(Group &g, uint64_t stage_base) {
         }
         const uint32_t k = hdr & 0x0f;
         p += 8;
-        g.rice.push_back(RiceJob{stage_base * 8 + p, (uint64_t)(g.nnz0 + done), m, k});
+        RiceJob job{stage_base * 8 + p, (uint64_t)(g.nnz0 + done), m | (k << 16), 0};
         const uint64_t body0 = p;
         if (k == 0 && len < (1ull << 29)) {
             // a value is q + 1 with q < 2^32 - 1 whenever the stream holds fewer than 2^32 bits
@@ -763,7 +876,8 @@ inline std::string rice_walk(Group &g, uint64_t stage_base) {
                     return "Rice value overflows u32 (corrupt stream)";
             }
         }
-        (void)body0;
+        job.nbits = (uint32_t)std::min<uint64_t>(p - body0, 0xffffffffull);
+        (k == 0 && job.nbits < (1u << 30) ? g.rice0 : g.rice).push_back(job);
         p = (p + 7) & ~7ull;
         done += m;
     }
@@ -880,112 +994,60 @@ extern "C" scxpca_status scxshard_decode_csr(const uint8_t *const *shards, const
                 g.sz_vv = h.codec == C_SCX1 ? g.nvv : (uint64_t)g.nnz * vw;
                 g.st_ix = stage;
                 stage = align16(stage + g.sz_ix + kStagePad);
+            }
+            // all value payloads after all index payloads: consecutive Rice
+            // blocks stay adjacent across row groups and shards
+            for (auto &g : groups) {
                 g.st_vv = stage;
                 stage = align16(stage + g.sz_vv + kStagePad);
             }
         }
+        // 3. batches of groups (about a sixth of the payload each, at least
+        //    8 MiB): the host work of batch b + 1 runs while batch b is copied
+        //    and decoded on the stream.
+        std::vector<size_t> cuts{0};
+        {
+            const uint64_t target = std::max<uint64_t>(stage / 6, 8ull << 20);
+            uint64_t acc = 0;
+            for (size_t i = 0; i < groups.size(); ++i) {
+                acc += groups[i].sz_ix + groups[i].sz_vv;
+                if (acc >= target && i + 1 < groups.size()) {
+                    cuts.push_back(i + 1);
+                    acc = 0;
+                }
+            }
+            cuts.push_back(groups.size());
+        }
+        // work-list capacity per batch (upper bounds: a row job per row, a
+        // Rice job per 256 values, two plane jobs and their chunks per group)
+        struct Cap {
+            uint64_t rows = 0, rice = 0, planes = 0, chunks = 0;
+        };
+        std::vector<Cap> cap(cuts.size() - 1);
+        Cap tot;
+        for (size_t b = 0; b + 1 < cuts.size(); ++b) {
+            for (size_t i = cuts[b]; i < cuts[b + 1]; ++i) {
+                const Group &g = groups[i];
+                cap[b].rows += (uint64_t)g.n_rows;
+                cap[b].rice += ((uint64_t)g.nnz + kBVal - 1) / kBVal;
+                cap[b].planes += 2;
+                cap[b].chunks += 2 * (((uint64_t)g.nnz + kPlaneChunk - 1) / kPlaneChunk);
+            }
+            tot.rows += cap[b].rows, tot.rice += cap[b].rice, tot.planes += cap[b].planes, tot.chunks += cap[b].chunks;
+        }
+        const uint64_t o_rows = align16(stage + 64), o_rice = align16(o_rows + tot.rows * sizeof(RowJob)),
+                       o_pl = align16(o_rice + tot.rice * sizeof(RiceJob)),
+                       o_ch = align16(o_pl + tot.planes * sizeof(PlaneJob)),
+                       o_ip = align16(o_ch + tot.chunks * sizeof(ChunkJob)),
+                       host_bytes = align16(o_ip + ((uint64_t)n_rows + 1) * 8),
+                       o_sums = host_bytes, o_carry = align16(o_sums + tot.chunks * 4),
+                       o_err = align16(o_carry + tot.chunks * 4), dev_bytes = o_err + 16;
         Staging &st = Staging::get();
         std::lock_guard<std::mutex> lock(st.mu);
-        st.ensure(stage + 64);
+        st.ensure(host_bytes);
         uint8_t *hs = static_cast<uint8_t *>(st.host);
-        std::vector<int64_t> h_indptr((size_t)n_rows + 1);
+        int64_t *h_indptr = reinterpret_cast<int64_t *>(hs + o_ip);
         h_indptr[0] = 0;
-        // 3. per-group host work
-        parallel_for((int64_t)groups.size(), [&](int64_t gi) {
-            Group &g = groups[(size_t)gi];
-            const Hdr &h = *g.h;
-            const int iw = h.u16 ? 2 : 4, vw = venc_width(h.venc);
-            try {
-                // indptr
-                std::vector<uint64_t> ip((size_t)g.n_rows + 1);
-                std::string e;
-                const uint64_t ipb = (uint64_t)(g.n_rows + 1) * 8;
-                if (h.codec == C_SCX1) {
-                    e = dg_decode(g.ip, g.nip, g.n_rows + 1, ip.data());
-                } else if (h.codec == C_NONE) {
-                    if (g.nip != ipb) e = "indptr: " + std::to_string(g.nip) + " bytes, expected " + std::to_string(ipb);
-                    else for (int64_t r = 0; r <= g.n_rows; ++r) ip[(size_t)r] = rd64(g.ip + 8 * r);
-                } else {
-                    std::vector<uint8_t> raw(ipb);
-                    if (h.codec == C_ZSTD) {
-                        e = zstd_exact(g.ip, g.nip, raw.data(), ipb, "indptr");
-                        if (e.empty()) for (int64_t r = 0; r <= g.n_rows; ++r) ip[(size_t)r] = rd64(raw.data() + 8 * r);
-                    } else {
-                        e = h.codec == C_LZ4 ? lz4_exact(g.ip, g.nip, raw.data(), ipb, "indptr")
-                                             : zstd_exact(g.ip, g.nip, raw.data(), ipb, "indptr");
-                        if (e.empty()) planes_to_u64(raw.data(), g.n_rows + 1, h.codec == C_SHUF, ip.data());
-                    }
-                }
-                if (e.empty()) {
-                    // check_indptr_shape (scx-codec/src/guards.rs:258-305)
-                    if (ip[0] != 0) e = "indptr must start at 0, got " + std::to_string(ip[0]);
-                    for (int64_t r = 1; e.empty() && r <= g.n_rows; ++r)
-                        if (ip[(size_t)r] < ip[(size_t)r - 1]) e = "indptr not monotone at row " + std::to_string(r - 1);
-                    if (e.empty() && ip[(size_t)g.n_rows] != (uint64_t)g.nnz)
-                        e = "indptr ends at " + std::to_string(ip[(size_t)g.n_rows]) + ", nnz is " + std::to_string(g.nnz);
-                }
-                if (e.empty())
-                    for (int64_t r = 1; r <= g.n_rows; ++r) h_indptr[(size_t)(g.row0 + r)] = g.nnz0 + (int64_t)ip[(size_t)r];
-                // payloads
-                if (e.empty() && g.nnz > 0) {
-                    uint8_t *dix = hs + g.st_ix, *dvv = hs + g.st_vv;
-                    if (h.codec == C_SCX1) {
-                        if ((uint64_t)g.nnz > g.nix * 8 || (uint64_t)g.nnz > g.nvv * 8)
-                            e = "Scx1: declared nnz exceeds the sub-streams' capacity";
-                        if (e.empty()) {
-                            std::memcpy(dix, g.ix, g.nix);
-                            std::memcpy(dvv, g.vv, g.nvv);
-                            e = forbp_walk(g, g.st_ix);
-                        }
-                        if (e.empty()) e = rice_walk(g, g.st_vv);
-                    } else {
-                        const bool shuffled = h.codec == C_LZ4 || h.codec == C_SHUF;
-                        if (h.codec == C_NONE) {
-                            if (g.nix != g.sz_ix || g.nvv != g.sz_vv) e = "None codec: sub-stream lengths disagree with nnz";
-                            else {
-                                std::memcpy(dix, g.ix, g.nix);
-                                std::memcpy(dvv, g.vv, g.nvv);
-                            }
-                        } else if (h.codec == C_LZ4) {
-                            e = lz4_exact(g.ix, g.nix, dix, g.sz_ix, "indices");
-                            if (e.empty()) e = lz4_exact(g.vv, g.nvv, dvv, g.sz_vv, "values");
-                        } else {
-                            e = zstd_exact(g.ix, g.nix, dix, g.sz_ix, "indices");
-                            if (e.empty()) e = zstd_exact(g.vv, g.nvv, dvv, g.sz_vv, "values");
-                        }
-                        PlaneJob ji{g.st_ix, (uint64_t)g.nnz0, (uint64_t)g.nnz, (uint32_t)iw,
-                                    kIndex | (shuffled && iw > 1 ? kPlaneMajor : 0u) | (h.codec == C_SHUF ? kDelta : 0u), 0};
-                        // float values of ShufDeltaZstd are zstd only (no shuffle)
-                        const bool vshuf = h.codec == C_LZ4 || (h.codec == C_SHUF && venc_integer(h.venc));
-                        PlaneJob jv{g.st_vv, (uint64_t)g.nnz0, (uint64_t)g.nnz, (uint32_t)vw,
-                                    (vshuf && vw > 1 ? kPlaneMajor : 0u) | ((uint32_t)h.venc << 8), 0};
-                        g.planes.push_back(ji);
-                        g.planes.push_back(jv);
-                    }
-                }
-                if (!e.empty()) g.err = "shard " + std::to_string(g.shard) + ": " + e;
-            } catch (const std::exception &ex) {
-                g.err = "shard " + std::to_string(g.shard) + ": " + ex.what();
-            }
-        });
-        for (auto &g : groups)
-            if (!g.err.empty()) throw Error(SCXPCA_ERR_INVALID, g.err);
-        // 4. work lists
-        std::vector<RowJob> rows;
-        std::vector<RiceJob> rice;
-        std::vector<PlaneJob> planes;
-        std::vector<ChunkJob> chunks;
-        for (auto &g : groups) {
-            rows.insert(rows.end(), g.rows.begin(), g.rows.end());
-            rice.insert(rice.end(), g.rice.begin(), g.rice.end());
-            for (auto pj : g.planes) {
-                pj.chunk0 = chunks.size();
-                for (uint64_t e0 = 0; e0 < pj.n; e0 += kPlaneChunk) chunks.push_back(ChunkJob{(uint32_t)planes.size(), 0, e0});
-                planes.push_back(pj);
-            }
-        }
-        const auto t_host = clk::now();
-        // 5. device buffers and uploads
         hipStream_t sm = static_cast<hipStream_t>(stream);
         hipStream_t own = nullptr;
         if (!sm) {
@@ -998,70 +1060,231 @@ extern "C" scxpca_status scxshard_decode_csr(const uint8_t *const *shards, const
                 if (s) (void)hipStreamDestroy(s);
             }
         } so{own};
-        const uint64_t tbl_rows = rows.size() * sizeof(RowJob), tbl_rice = rice.size() * sizeof(RiceJob),
-                       tbl_pl = planes.size() * sizeof(PlaneJob), tbl_ch = chunks.size() * sizeof(ChunkJob);
-        const uint64_t o_rows = align16(stage + 64), o_rice = align16(o_rows + tbl_rows),
-                       o_pl = align16(o_rice + tbl_rice), o_ch = align16(o_pl + tbl_pl),
-                       o_sums = align16(o_ch + tbl_ch), o_carry = align16(o_sums + chunks.size() * 4),
-                       o_err = align16(o_carry + chunks.size() * 4), dev_bytes = o_err + 16;
         DevBuf dbuf;
         dbuf.ensure(dev_bytes);
         uint8_t *d = dbuf.as<uint8_t>();
-        hipEvent_t ev[3];
-        for (auto &e : ev) HIPC(hipEventCreate(&e));
+        const size_t nb = cuts.size() - 1;
+        std::vector<hipEvent_t> ev(3 * nb, nullptr);  // per batch: copies start, copies end, kernels end
         struct EvOwner {
-            hipEvent_t *e;
+            std::vector<hipEvent_t> &e;
             ~EvOwner() {
-                for (int i = 0; i < 3; ++i) (void)hipEventDestroy(e[i]);
+                for (auto x : e)
+                    if (x) (void)hipEventDestroy(x);
             }
         } eo{ev};
-        std::memset(hs + stage, 0, 64);
-        HIPC(hipEventRecord(ev[0], sm));
-        HIPC(hipMemcpyAsync(d, hs, stage + 64, hipMemcpyHostToDevice, sm));
-        if (tbl_rows) HIPC(hipMemcpyAsync(d + o_rows, rows.data(), tbl_rows, hipMemcpyHostToDevice, sm));
-        if (tbl_rice) HIPC(hipMemcpyAsync(d + o_rice, rice.data(), tbl_rice, hipMemcpyHostToDevice, sm));
-        if (tbl_pl) HIPC(hipMemcpyAsync(d + o_pl, planes.data(), tbl_pl, hipMemcpyHostToDevice, sm));
-        if (tbl_ch) HIPC(hipMemcpyAsync(d + o_ch, chunks.data(), tbl_ch, hipMemcpyHostToDevice, sm));
-        HIPC(hipMemcpyAsync(indptr, h_indptr.data(), h_indptr.size() * 8, hipMemcpyHostToDevice, sm));
+        for (auto &e : ev) HIPC(hipEventCreate(&e));
         uint32_t *err = reinterpret_cast<uint32_t *>(d + o_err);
-        k_zero_words<<<1, 64, 0, sm>>>(err, 4);
-        HIPC(hipGetLastError());
         const uint32_t bound = (uint32_t)std::min<int64_t>(n_cols, 0xffffffffll);
         const uint32_t *st32 = reinterpret_cast<const uint32_t *>(d);
-        HIPC(hipEventRecord(ev[1], sm));
-        if (!rows.empty())
-            k_forbp_rows<<<(unsigned)((rows.size() + 3) / 4), 256, 0, sm>>>(
-                st32, reinterpret_cast<const RowJob *>(d + o_rows), (int64_t)rows.size(), indices, bound, err);
-        if (!rice.empty())
-            k_rice_blocks<<<blocks_for((int64_t)rice.size()), 256, 0, sm>>>(
-                st32, reinterpret_cast<const RiceJob *>(d + o_rice), (int64_t)rice.size(), data);
-        if (!chunks.empty()) {
-            const auto *pj = reinterpret_cast<const PlaneJob *>(d + o_pl);
-            const auto *cj = reinterpret_cast<const ChunkJob *>(d + o_ch);
-            auto *sums = reinterpret_cast<uint32_t *>(d + o_sums);
-            auto *carry = reinterpret_cast<uint32_t *>(d + o_carry);
-            k_zero_words<<<blocks_for((int64_t)chunks.size()), 256, 0, sm>>>(carry, (int64_t)chunks.size());
-            k_plane_sums<<<(unsigned)chunks.size(), 256, 0, sm>>>(d, pj, cj, (int64_t)chunks.size(), sums);
-            k_plane_carry<<<blocks_for((int64_t)planes.size()), 256, 0, sm>>>(pj, (int64_t)planes.size(), sums, carry);
-            k_plane_emit<<<(unsigned)chunks.size(), 256, 0, sm>>>(d, pj, cj, (int64_t)chunks.size(), carry, indices, data,
-                                                                  bound, err);
-        }
-        HIPC(hipGetLastError());
-        HIPC(hipEventRecord(ev[2], sm));
+        uint64_t up_bytes = 0;
+        double host_s = std::chrono::duration<double>(clk::now() - t0).count();
+        bool launched = false;
         uint32_t herr = 0;
-        HIPC(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, sm));
-        HIPC(hipStreamSynchronize(sm));
+        try {
+            Cap cur;  // table cursors
+            for (size_t b = 0; b < nb; ++b) {
+                const auto th0 = clk::now();
+                // 3a. host work of the batch's groups
+                parallel_for((int64_t)(cuts[b + 1] - cuts[b]), [&](int64_t li) {
+                    Group &g = groups[cuts[b] + (size_t)li];
+                    const Hdr &h = *g.h;
+                    const int iw = h.u16 ? 2 : 4, vw = venc_width(h.venc);
+                    try {
+                        // indptr
+                        std::vector<uint64_t> ip((size_t)g.n_rows + 1);
+                        std::string e;
+                        const uint64_t ipb = (uint64_t)(g.n_rows + 1) * 8;
+                        if (h.codec == C_SCX1) {
+                            e = dg_decode(g.ip, g.nip, g.n_rows + 1, ip.data());
+                        } else if (h.codec == C_NONE) {
+                            if (g.nip != ipb)
+                                e = "indptr: " + std::to_string(g.nip) + " bytes, expected " + std::to_string(ipb);
+                            else
+                                for (int64_t r = 0; r <= g.n_rows; ++r) ip[(size_t)r] = rd64(g.ip + 8 * r);
+                        } else {
+                            std::vector<uint8_t> raw(ipb);
+                            if (h.codec == C_ZSTD) {
+                                e = zstd_exact(g.ip, g.nip, raw.data(), ipb, "indptr");
+                                if (e.empty())
+                                    for (int64_t r = 0; r <= g.n_rows; ++r) ip[(size_t)r] = rd64(raw.data() + 8 * r);
+                            } else {
+                                e = h.codec == C_LZ4 ? lz4_exact(g.ip, g.nip, raw.data(), ipb, "indptr")
+                                                     : zstd_exact(g.ip, g.nip, raw.data(), ipb, "indptr");
+                                if (e.empty()) planes_to_u64(raw.data(), g.n_rows + 1, h.codec == C_SHUF, ip.data());
+                            }
+                        }
+                        if (e.empty()) {
+                            // check_indptr_shape (scx-codec/src/guards.rs:258-305)
+                            if (ip[0] != 0) e = "indptr must start at 0, got " + std::to_string(ip[0]);
+                            for (int64_t r = 1; e.empty() && r <= g.n_rows; ++r)
+                                if (ip[(size_t)r] < ip[(size_t)r - 1])
+                                    e = "indptr not monotone at row " + std::to_string(r - 1);
+                            if (e.empty() && ip[(size_t)g.n_rows] != (uint64_t)g.nnz)
+                                e = "indptr ends at " + std::to_string(ip[(size_t)g.n_rows]) + ", nnz is " +
+                                    std::to_string(g.nnz);
+                        }
+                        if (e.empty())
+                            for (int64_t r = 1; r <= g.n_rows; ++r)
+                                h_indptr[g.row0 + r] = g.nnz0 + (int64_t)ip[(size_t)r];
+                        // payloads
+                        if (e.empty() && g.nnz > 0) {
+                            uint8_t *dix = hs + g.st_ix, *dvv = hs + g.st_vv;
+                            if (h.codec == C_SCX1) {
+                                if ((uint64_t)g.nnz > g.nix * 8 || (uint64_t)g.nnz > g.nvv * 8)
+                                    e = "Scx1: declared nnz exceeds the sub-streams' capacity";
+                                if (e.empty()) {
+                                    std::memcpy(dix, g.ix, g.nix);
+                                    std::memcpy(dvv, g.vv, g.nvv);
+                                    e = forbp_walk(g, g.st_ix);
+                                }
+                                if (e.empty()) e = rice_walk(g, g.st_vv);
+                            } else {
+                                const bool shuffled = h.codec == C_LZ4 || h.codec == C_SHUF;
+                                if (h.codec == C_NONE) {
+                                    if (g.nix != g.sz_ix || g.nvv != g.sz_vv)
+                                        e = "None codec: sub-stream lengths disagree with nnz";
+                                    else {
+                                        std::memcpy(dix, g.ix, g.nix);
+                                        std::memcpy(dvv, g.vv, g.nvv);
+                                    }
+                                } else if (h.codec == C_LZ4) {
+                                    e = lz4_exact(g.ix, g.nix, dix, g.sz_ix, "indices");
+                                    if (e.empty()) e = lz4_exact(g.vv, g.nvv, dvv, g.sz_vv, "values");
+                                } else {
+                                    e = zstd_exact(g.ix, g.nix, dix, g.sz_ix, "indices");
+                                    if (e.empty()) e = zstd_exact(g.vv, g.nvv, dvv, g.sz_vv, "values");
+                                }
+                                PlaneJob ji{g.st_ix, (uint64_t)g.nnz0, (uint64_t)g.nnz, (uint32_t)iw,
+                                            kIndex | (shuffled && iw > 1 ? kPlaneMajor : 0u) |
+                                                (h.codec == C_SHUF ? kDelta : 0u),
+                                            0};
+                                // float values of ShufDeltaZstd are zstd only (no shuffle)
+                                const bool vshuf = h.codec == C_LZ4 || (h.codec == C_SHUF && venc_integer(h.venc));
+                                PlaneJob jv{g.st_vv, (uint64_t)g.nnz0, (uint64_t)g.nnz, (uint32_t)vw,
+                                            (vshuf && vw > 1 ? kPlaneMajor : 0u) | ((uint32_t)h.venc << 8), 0};
+                                g.planes.push_back(ji);
+                                g.planes.push_back(jv);
+                            }
+                        }
+                        if (!e.empty()) g.err = "shard " + std::to_string(g.shard) + ": " + e;
+                    } catch (const std::exception &ex) {
+                        g.err = "shard " + std::to_string(g.shard) + ": " + ex.what();
+                    }
+                });
+                for (size_t i = cuts[b]; i < cuts[b + 1]; ++i)
+                    if (!groups[i].err.empty()) throw Error(SCXPCA_ERR_INVALID, groups[i].err);
+                // 3b. the batch's work lists, in the pinned buffer (group
+                //     offsets by a prefix sum, the copies in parallel)
+                RowJob *rows = reinterpret_cast<RowJob *>(hs + o_rows) + cur.rows;
+                RiceJob *rice = reinterpret_cast<RiceJob *>(hs + o_rice) + cur.rice;
+                PlaneJob *planes = reinterpret_cast<PlaneJob *>(hs + o_pl) + cur.planes;
+                ChunkJob *chunks = reinterpret_cast<ChunkJob *>(hs + o_ch) + cur.chunks;
+                const size_t ng_b = cuts[b + 1] - cuts[b];
+                std::vector<uint64_t> orow(ng_b + 1, 0), orc(ng_b + 1, 0), or0(ng_b + 1, 0), opl(ng_b + 1, 0),
+                    och(ng_b + 1, 0);
+                for (size_t li = 0; li < ng_b; ++li) {
+                    const Group &g = groups[cuts[b] + li];
+                    uint64_t nch = 0;
+                    for (const PlaneJob &pj : g.planes) nch += (pj.n + kPlaneChunk - 1) / kPlaneChunk;
+                    orow[li + 1] = orow[li] + g.rows.size();
+                    orc[li + 1] = orc[li] + g.rice.size();
+                    or0[li + 1] = or0[li] + g.rice0.size();
+                    opl[li + 1] = opl[li] + g.planes.size();
+                    och[li + 1] = och[li] + nch;
+                }
+                const uint64_t n_rows_b = orow[ng_b], n_rice = orc[ng_b], n_rice0 = or0[ng_b], n_pl = opl[ng_b],
+                               n_ch = och[ng_b];
+                parallel_for((int64_t)ng_b, [&](int64_t li) {
+                    Group &g = groups[cuts[b] + (size_t)li];
+                    if (!g.rows.empty()) std::memcpy(rows + orow[li], g.rows.data(), g.rows.size() * sizeof(RowJob));
+                    if (!g.rice.empty()) std::memcpy(rice + orc[li], g.rice.data(), g.rice.size() * sizeof(RiceJob));
+                    if (!g.rice0.empty())
+                        std::memcpy(rice + n_rice + or0[li], g.rice0.data(), g.rice0.size() * sizeof(RiceJob));
+                    uint64_t c = och[li], pl = opl[li];
+                    for (PlaneJob pj : g.planes) {
+                        pj.chunk0 = c;
+                        for (uint64_t e0 = 0; e0 < pj.n; e0 += kPlaneChunk) chunks[c++] = ChunkJob{(uint32_t)pl, 0, e0};
+                        planes[pl++] = pj;
+                    }
+                    std::vector<RowJob>().swap(g.rows);
+                    std::vector<RiceJob>().swap(g.rice);
+                    std::vector<RiceJob>().swap(g.rice0);
+                });
+                host_s += std::chrono::duration<double>(clk::now() - th0).count();
+                // 3c. copies and kernels of the batch
+                const Group &ga = groups[cuts[b]], &gz = groups[cuts[b + 1] - 1];
+                const uint64_t ix0 = ga.st_ix, ix1 = gz.st_ix + gz.sz_ix + kStagePad;
+                const uint64_t vv0 = ga.st_vv, vv1 = b + 1 == nb ? stage + 64 : gz.st_vv + gz.sz_vv + kStagePad;
+                if (b + 1 == nb) std::memset(hs + stage, 0, 64);
+                HIPC(hipEventRecord(ev[3 * b], sm));
+                auto h2d = [&](uint64_t off, uint64_t bytes) {
+                    if (!bytes) return;
+                    HIPC(hipMemcpyAsync(d + off, hs + off, bytes, hipMemcpyHostToDevice, sm));
+                    up_bytes += bytes;
+                };
+                h2d(ix0, ix1 - ix0);
+                h2d(vv0, vv1 - vv0);
+                h2d(o_rows + cur.rows * sizeof(RowJob), n_rows_b * sizeof(RowJob));
+                h2d(o_rice + cur.rice * sizeof(RiceJob), (n_rice + n_rice0) * sizeof(RiceJob));
+                h2d(o_pl + cur.planes * sizeof(PlaneJob), n_pl * sizeof(PlaneJob));
+                h2d(o_ch + cur.chunks * sizeof(ChunkJob), n_ch * sizeof(ChunkJob));
+                if (b == 0) {
+                    k_zero_words<<<1, 64, 0, sm>>>(err, 4);
+                    HIPC(hipGetLastError());
+                }
+                HIPC(hipEventRecord(ev[3 * b + 1], sm));
+                launched = true;
+                const auto *drows = reinterpret_cast<const RowJob *>(d + o_rows) + cur.rows;
+                const auto *drice = reinterpret_cast<const RiceJob *>(d + o_rice) + cur.rice;
+                if (n_rows_b)
+                    k_forbp_rows<<<(unsigned)((n_rows_b + 3) / 4), 256, 0, sm>>>(st32, drows, (int64_t)n_rows_b,
+                                                                              indices, bound, err);
+                if (n_rice)
+                    k_rice_blocks<<<(unsigned)((n_rice + 255) / 256), 256, 0, sm>>>(st32, drice, (int64_t)n_rice, data);
+                if (n_rice0)
+                    k_rice_k0<<<(unsigned)((n_rice0 + 3) / 4), 256, 0, sm>>>(st32, drice + n_rice, (int64_t)n_rice0,
+                                                                          data);
+                if (n_ch) {
+                    const auto *pj = reinterpret_cast<const PlaneJob *>(d + o_pl) + cur.planes;
+                    const auto *cj = reinterpret_cast<const ChunkJob *>(d + o_ch) + cur.chunks;
+                    auto *sums = reinterpret_cast<uint32_t *>(d + o_sums) + cur.chunks;
+                    auto *carry = reinterpret_cast<uint32_t *>(d + o_carry) + cur.chunks;
+                    k_zero_words<<<blocks_for((int64_t)n_ch), 256, 0, sm>>>(carry, (int64_t)n_ch);
+                    k_plane_sums<<<(unsigned)n_ch, 256, 0, sm>>>(d, pj, cj, (int64_t)n_ch, sums);
+                    k_plane_carry<<<blocks_for((int64_t)n_pl), 256, 0, sm>>>(pj, (int64_t)n_pl, sums, carry);
+                    k_plane_emit<<<(unsigned)n_ch, 256, 0, sm>>>(d, pj, cj, (int64_t)n_ch, carry, indices, data, bound,
+                                                                 err);
+                }
+                HIPC(hipGetLastError());
+                HIPC(hipEventRecord(ev[3 * b + 2], sm));
+                cur.rows += cap[b].rows, cur.rice += cap[b].rice, cur.planes += cap[b].planes,
+                    cur.chunks += cap[b].chunks;
+            }
+            HIPC(hipMemcpyAsync(indptr, h_indptr, ((size_t)n_rows + 1) * 8, hipMemcpyHostToDevice, sm));
+            up_bytes += ((uint64_t)n_rows + 1) * 8;
+            HIPC(hipMemcpyAsync(&herr, err, 4, hipMemcpyDeviceToHost, sm));
+            HIPC(hipStreamSynchronize(sm));
+        } catch (...) {
+            // no DMA may still read the shared pinned buffer when the lock drops
+            if (launched) (void)hipStreamSynchronize(sm);
+            throw;
+        }
         const auto t_end = clk::now();
         if (herr & ERR_U32) throw Error(SCXPCA_ERR_INVALID, "FOR-BP: a decoded index overflows u32 (corrupt stream)");
         if (herr & ERR_I32) throw Error(SCXPCA_ERR_INVALID, "a decoded index does not fit i32 (>= 2^31)");
         if (herr & ERR_BOUND)
             throw Error(SCXPCA_ERR_INVALID, "a decoded index is out of range for n_minor " + std::to_string(n_cols));
         if (stats) {
-            float up_ms = 0, dev_ms = 0;
-            HIPC(hipEventElapsedTime(&up_ms, ev[0], ev[1]));
-            HIPC(hipEventElapsedTime(&dev_ms, ev[1], ev[2]));
+            double up_s = 0, dev_s = 0;
+            for (size_t b = 0; b < nb; ++b) {
+                float a = 0, c = 0;
+                HIPC(hipEventElapsedTime(&a, ev[3 * b], ev[3 * b + 1]));
+                HIPC(hipEventElapsedTime(&c, ev[3 * b + 1], ev[3 * b + 2]));
+                up_s += a * 1e-3;
+                dev_s += c * 1e-3;
+            }
             std::memset(stats, 0, sizeof *stats);
-            stats->host_uploaded_bytes = stage + 64 + tbl_rows + tbl_rice + tbl_pl + tbl_ch + h_indptr.size() * 8;
+            stats->host_uploaded_bytes = up_bytes;
             stats->device_decoded_bytes = (uint64_t)nnz * 8;
             stats->fully_device_decoded = 1;
             for (int64_t s = 0; s < n_shards; ++s) {
@@ -1072,9 +1295,9 @@ extern "C" scxpca_status scxshard_decode_csr(const uint8_t *const *shards, const
                 if (c != C_SCX1 && c != C_NONE) stats->fully_device_decoded = 0;
             }
             stats->n_groups = (uint32_t)groups.size();
-            stats->host_seconds = std::chrono::duration<double>(t_host - t0).count();
-            stats->device_seconds = dev_ms * 1e-3;
-            stats->upload_seconds = up_ms * 1e-3;
+            stats->host_seconds = host_s;
+            stats->device_seconds = dev_s;
+            stats->upload_seconds = up_s;
             stats->total_seconds = std::chrono::duration<double>(t_end - t0).count();
         }
     });

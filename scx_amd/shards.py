@@ -112,6 +112,48 @@ def decode_shard_gpu(shard, device: int = 0, stream=None) -> GpuCsr:
     return decode_csr_shards_to_device([shard], device=device, stream=stream)
 
 
+_VENC = {np.dtype(np.uint8): 0, np.dtype(np.uint16): 1, np.dtype(np.uint32): 2, np.dtype(np.float32): 3,
+         np.dtype(np.float16): 4}
+
+
+def encode_shard(indptr, indices, values, codec: str = "scx1", n_minor: int = 0, global_offset: int = 0,
+                 row_group_rows: int = 256, index_dtype: int | None = None) -> bytes:
+    """One CSR shard section (scxshard_encode): Scx1 (integer values) or None.
+    ``row_group_rows`` > 0 frames it (shard format 2), as the reference
+    writer does by default at 256 rows."""
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(indices, dtype=np.uint32)
+    vals = np.ascontiguousarray(values)
+    if vals.dtype not in _VENC:
+        raise ValueError(f"values must be uint8/16/32, float32 or float16, not {vals.dtype}")
+    if index_dtype is None:
+        index_dtype = 0 if (n_minor and n_minor <= 65536) or (not n_minor and (ix.size == 0 or ix.max() <= 65535)) else 1
+    cid = {"none": 0, "scx1": 1}[codec]
+    args = [_lib.ptr(ip), len(ip) - 1, _lib.ptr(ix) if ix.size else None, _lib.ptr(vals) if vals.size else None,
+            _VENC[vals.dtype], cid, index_dtype, n_minor, global_offset, row_group_rows]
+    n = ctypes.c_uint64()
+    _lib.check(_lib.lib().scxshard_encode(*args, None, 0, ctypes.byref(n)))
+    out = np.empty(n.value, np.uint8)
+    _lib.check(_lib.lib().scxshard_encode(*args, _lib.ptr(out), n.value, ctypes.byref(n)))
+    return out.tobytes()
+
+
+def encode_csr_shards(indptr, indices, values, n_cols: int, rows_per_shard: int = 16384, codec: str = "scx1",
+                      row_group_rows: int = 256) -> list:
+    """A CSR split into shards of ``rows_per_shard`` rows (the reference
+    writer's DEFAULT_SHARD_TARGET_ROWS is 16,384), each framed at
+    ``row_group_rows``."""
+    ip = np.asarray(indptr, np.int64)
+    n = len(ip) - 1
+    out = []
+    for a in range(0, max(n, 1), rows_per_shard):
+        b = min(n, a + rows_per_shard)
+        lo, hi = int(ip[a]), int(ip[b])
+        out.append(encode_shard(ip[a:b + 1] - lo, indices[lo:hi], values[lo:hi], codec, n_cols, a,
+                                row_group_rows if b > a else 0))
+    return out
+
+
 def read_catalog(path_or_bytes) -> tuple[dict, list[Section]]:
     """File header fields and the active full catalog's entries of a ``.scx``
     file (docs/format.md sections 2-3)."""

@@ -221,3 +221,35 @@ def test_malformed_shards_are_rejected():
     struct.pack_into("<I", fr, bo + 4 + 18, struct.unpack_from("<I", fr, bo + 4 + 18)[0] + 1)
     with pytest.raises(S.ShardOracleError):
         S.decode_shard(bytes(fr))
+
+
+# ---- the library's shard writer (host code of libscxpca.so) ----
+@pytest.mark.parametrize("codec", ["scx1", "none"])
+@pytest.mark.parametrize("n_cols", [4000, 70000])
+@pytest.mark.parametrize("framing", [0, 4, 256])
+@pytest.mark.parametrize("venc", ["u8", "u16", "u32"])
+def test_library_writer_matches_oracle_encoder(codec, n_cols, framing, venc):
+    rng = np.random.default_rng(5)
+    rows = [int(x) for x in rng.integers(0, 700, 30)] + ROWS
+    ip, ix, vv = dense_rows(rows, n_cols, seed=9)
+    vv = ((vv % 250) + 1).astype(_VD[venc]) if venc == "u8" else vv.astype(_VD[venc])
+    u16 = n_cols <= 65536
+    lib = host.encode_shard(ip.astype(np.int64), ix, vv, codec, n_minor=n_cols, global_offset=12345,
+                            row_group_rows=framing, index_dtype=0 if u16 else 1)
+    ref = S.encode_shard(ip, ix, vv, codec, venc, u16=u16, n_minor=n_cols, global_offset=12345,
+                         row_group_rows=framing)
+    assert lib == ref
+
+
+def test_library_writer_golden_and_errors():
+    buf, exp = _golden("golden_scx1_u16")
+    sh = bytes(host.open_csr_shards(buf)[0])
+    lib = host.encode_shard(exp["indptr"], exp["indices"], exp["data"].astype(np.uint16), "scx1", n_minor=50,
+                            row_group_rows=0)
+    assert lib[:68] == sh[:68] and lib[76:] == sh[76:]
+    with pytest.raises(ValueError, match="float"):
+        host.encode_shard([0, 1], [0], np.ones(1, np.float32), "scx1", n_minor=4)
+    with pytest.raises(ValueError, match="non-zero"):
+        host.encode_shard([0, 1], [0], np.zeros(1, np.uint16), "scx1", n_minor=4)
+    with pytest.raises(ValueError, match="sorted"):
+        host.encode_shard([0, 2], [3, 1], np.ones(2, np.uint16), "scx1", n_minor=4)
