@@ -436,7 +436,8 @@ def cpu_baseline_rpca(args, x, w, pflog=False):
                                              n_power_iterations=w.n_power_iterations, seed=0,
                                              target_sum=w.target_sum, log1p=True, nthreads=threads)
 
-    rows = min(n, 250_000, max(n // 4, 1)) if args.cpu_rows < 0 else min(args.cpu_rows, n)
+    # the whole matrix up to 250,000 rows, else a quarter of it (at most 250,000 rows)
+    rows = (n if n <= 250_000 else min(250_000, n // 4)) if args.cpu_rows < 0 else min(args.cpu_rows, n)
     med, ts = _median_runs(run(rows, cores), args.cpu_runs)
     what = ("alpha estimate, delta, baseline and pflog_pca" if pflog else "randomized_pca_inmemory")
     out = {"value": round(med * n / rows, 3), "unit": "s", "cores": cores, "kind": "port",
@@ -444,7 +445,7 @@ def cpu_baseline_rpca(args, x, w, pflog=False):
                      f"linearly to {n:,} rows; the C/OpenMP restatement of {what} (f64), {cores} threads",
            "runs_s": [round(t, 3) for t in ts], "cpu_model": _cpu_model()}
     if args.cpu_rows_1t != 0:
-        rows1 = min(n, 25_000, max(n // 40, 1)) if args.cpu_rows_1t < 0 else min(args.cpu_rows_1t, n)
+        rows1 = (n if n <= 25_000 else min(25_000, n // 40)) if args.cpu_rows_1t < 0 else min(args.cpu_rows_1t, n)
         med1, ts1 = _median_runs(run(rows1, 1), args.cpu_runs)
         out["one_thread"] = {"value": round(med1 * n / rows1, 3), "unit": "s", "cores": 1,
                              "sample": f"first {rows1:,} rows; 1 warm-up + median of {len(ts1)} runs ({med1:.3f} s), "
@@ -517,11 +518,11 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--qr", choices=["cholesky", "householder"], default="cholesky")
     ap.add_argument("--cpu-rows", type=int, default=-1,
-                    help="rows of the all-cores CPU baseline sample: -1 a quarter of the matrix (at most "
-                         "250,000 rows), 0 skip the CPU baseline, N the first N rows")
+                    help="rows of the all-cores CPU baseline sample: -1 the whole matrix up to 250,000 rows, "
+                         "else a quarter of it (at most 250,000 rows); 0 skip the CPU baseline; N the first N rows")
     ap.add_argument("--cpu-rows-1t", type=int, default=-1,
-                    help="rows of the one-thread CPU baseline sample: -1 a fortieth of the matrix (at most "
-                         "25,000 rows), 0 skip")
+                    help="rows of the one-thread CPU baseline sample: -1 the whole matrix up to 25,000 rows, "
+                         "else a fortieth of it (at most 25,000 rows); 0 skip")
     ap.add_argument("--cpu-runs", type=int, default=5,
                     help="timed CPU baseline runs after one warm-up (the median is reported)")
     ap.add_argument("--legs", type=int, default=5, metavar="STEPS",

@@ -52,7 +52,9 @@ typedef enum {
 
 /* CSR matrix: int64 indptr (n_obs+1), uint32 indices (nnz, non-decreasing per
  * row on device input; host input is sorted if needed), float32 values.
- * Raw counts; the normalize/log1p transform is applied on the fly. */
+ * Raw counts; the normalize/log1p transform runs once per prepare (matrix
+ * creation and each refresh), and X~ is kept in HBM in the layouts the
+ * products read; the caller's arrays are never modified. */
 typedef struct {
     int64_t n_obs;
     int64_t n_vars;

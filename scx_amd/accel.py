@@ -14,8 +14,10 @@ Mirrors the reference binding so ``device="gpu"`` is a drop-in:
 * ``normalize_total(adata, target_sum=None)`` / ``log1p(adata)`` record the
   transform lazily (pyscx/src/accel/preprocessing.rs:95-342,
   pyscx/src/lazy_transform/transforms.rs:15-88): ``adata.X`` becomes a
-  :class:`LazyCsr` and the GPU PCA applies normalize -> log1p inside its SpMM
-  kernels, so X~ is never materialized.
+  :class:`LazyCsr`; the GPU PCA applies normalize -> log1p once per prepared
+  matrix (the reference's transform, bit for bit) and keeps X~ in HBM in the
+  two layouts its products read (CSR order and the row-tiled CSC). The host
+  CSR is never rewritten.
 
 Only the GPU route is implemented here. ``device="cpu"`` (and ``"auto"`` with no
 GPU) raises: the CPU path of the reference is not part of this package.

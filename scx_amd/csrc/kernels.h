@@ -4,11 +4,14 @@
 //   CSR          indptr i64[n_obs+1], indices u32[nnz], values f32[nnz] (raw counts)
 //   rowf         f64[n_obs]   normalize_total factor target/rowsum (1 where rowsum <= 0)
 //   xt           f32[nnz]     X~ = log1p(f32(x * rowf)) in CSR order, computed once per
-//                             prepared matrix; xbf = bf16(xt) for the BF16 sketch form
+//                             prepared matrix (read by the FP32 forward product)
 //   spl          u32[n_obs][nsplit+1]  offset of the first nonzero of each CB-gene
 //                chunk inside each row (sorted rows)
 //   tile-CSC     per 256-row tile, gene-major runs: tptr u32[n_tiles][n_vars+1],
-//                trow u8[nnz] (row in tile), tval f32[nnz] (X~), txbf bf16[nnz]
+//                trow u8[nnz] (row in tile), tval f32[nnz] (X~); tpk u32[nnz] packed
+//                bf16 entries for the BF16 sketch form (kernels_mfma.h)
+//   (the BF16 template arguments of k_spmm_fwd / k_spmm_fwd3 / k_spmm_t6 and their
+//    xbf / txbf operands are the earlier bf16-operand form; no route launches them)
 //   panels       f32 row-major, KP columns (k' padded to 32/64/128, pad columns 0):
 //                M (n_vars x KP), Y and Q (n_obs x KP)
 //   B            f64 n_vars x KP
