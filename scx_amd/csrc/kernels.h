@@ -10,8 +10,6 @@
 //   tile-CSC     per 256-row tile, gene-major runs: tptr u32[n_tiles][n_vars+1],
 //                trow u8[nnz] (row in tile), tval f32[nnz] (X~); tpk u32[nnz] packed
 //                bf16 entries for the BF16 sketch form (kernels_mfma.h)
-//   (the BF16 template arguments of k_spmm_fwd / k_spmm_fwd3 / k_spmm_t6 and their
-//    xbf / txbf operands are the earlier bf16-operand form; no route launches them)
 //   panels       f32 row-major, KP columns (k' padded to 32/64/128, pad columns 0):
 //                M (n_vars x KP), Y and Q (n_obs x KP)
 //   B            f64 n_vars x KP
@@ -775,11 +773,10 @@ __global__ void __launch_bounds__(256) k_colsum_reduce(const double *__restrict_
 // summed with cross-lane shuffles.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool BF16, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j] (PFlog)
+template <int KP, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j] (PFlog)
 __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ idx,
                                                   const float *__restrict__ xt,
-                                                  const uint16_t *__restrict__ xbf,
                                                   const float *__restrict__ M, const float *__restrict__ mc,
                                                   float *__restrict__ Y, int64_t n_obs,
                                                   const double *__restrict__ rowb, const float *__restrict__ cs) {
@@ -798,7 +795,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
             float x = 0.f;
             if (lane < cnt) {
                 c = idx[base + lane];
-                x = BF16 ? bf16_to_f32(xbf[base + lane]) : xt[base + lane];
+                x = xt[base + lane];
             }
             stage[wave][lane] = make_uint2(c, __float_as_uint(x));
             wave_lds_sync();
@@ -1153,11 +1150,10 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
 // the four group partials of a row are summed and the row is written once.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool BF16, int R, int NW, int CG, int PF, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j]
+template <int KP, int R, int NW, int CG, int PF, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j]
 __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict__ indptr,
                                                        const uint16_t *__restrict__ lg,
                                                        const float *__restrict__ xt,
-                                                       const uint16_t *__restrict__ xbf,
                                                        const float *__restrict__ M, const float *__restrict__ mc,
                                                        float *__restrict__ Y, const uint32_t *__restrict__ spl,
                                                        int nsplit, int cpc, int nch, int64_t n_obs,
@@ -1211,7 +1207,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
         int64_t p = base_r + lo_r + esel;
         p = p < p_max ? p : p_max;
         pi = lg[p];
-        pv = BF16 ? bf16_to_f32(xbf[p]) : xt[p];
+        pv = xt[p];
     };
     const float4 *M4 = reinterpret_cast<const float4 *>(M);
     uint32_t wlo, whi, nlo = 0u, nhi = 0u;
@@ -1276,7 +1272,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
                     int64_t p = base_r + o + esel;
                     p = p < p_max ? p : p_max;
                     const uint32_t c = lg[p];
-                    const float v = BF16 ? bf16_to_f32(xbf[p]) : xt[p];
+                    const float v = xt[p];
                     const bool ok = esel < cnt;
                     const uint32_t ofs = ok ? (c & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
                     const float x = ok ? v : 0.f;
@@ -1725,12 +1721,11 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
 // ---------------------------------------------------------------------------
 constexpr int64_t kEntryPad = 256;
 
-template <int KP, int GW, int NB0, bool BF16, int NWT>
+template <int KP, int GW, int NB0, int NWT>
 __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ tptr,
                                                   const uint8_t *__restrict__ trow,
                                                   const float *__restrict__ tval,
-                                                  const uint16_t *__restrict__ txbf,
                                                   const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
                                                   int nch, int ntr, int tiles_per_tr, int64_t n_obs,
                                                   int64_t n_vars, int n_tiles, float *__restrict__ part,
@@ -1819,7 +1814,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     float pval0[NB0], pval[GW];
     auto issue = [&](int64_t p0, uint8_t &pr, float &pv) {  // p0 wave-uniform
         pr = (trow + p0)[esel];
-        pv = BF16 ? bf16_to_f32((txbf + p0)[esel]) : (tval + p0)[esel];
+        pv = (tval + p0)[esel];
     };
     auto issue_tile = [&](int64_t tb_, const uint32_t (&lo_)[GW], int from, int to) {
         for (int gi = from; gi < to; ++gi) {
@@ -2608,29 +2603,28 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
 // final diagonal: per-step compares would cost scalar registers).
 // ---------------------------------------------------------------------------
 template <bool FLOOR>
-__global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int k, double shift,
-                                              double *__restrict__ R64, int *__restrict__ status,
-                                              double floor_tau) {
+__global__ void __launch_bounds__(256) k_chol_w(const double *__restrict__ G, int k, double shift,
+                                               double *__restrict__ R64, int *__restrict__ status,
+                                               double floor_tau) {
     constexpr int KP = 64;
     __shared__ double rowb[2][KP];
     __shared__ double Rs[KP * KP];
+    // the masked Gram staged row-major in LDS by all four waves (four times
+    // the loads in flight of one wave); wave 0 then factors it alone
+    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, cc = t % KP;
+        Rs[t] = (i < k && cc < k) ? G[t] : (i == cc ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (threadIdx.x >= KP) return;
     const int c = threadIdx.x;
-    const double tr = wave_sum_dpp(c < k ? G[c * KP + c] : 0.0);
+    const double tr = wave_sum_dpp(c < k ? Rs[c * KP + c] : 0.0);
     const double sh = (shift > 0.0 && c < k) ? shift * tr : 0.0;
     const double mdiag = k > 0 ? (tr + (shift > 0.0 ? shift * tr * k : 0.0)) / k : 0.0;
     const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // FLOOR: the pivot floor
-    // masked, shifted Gram staged row-major in LDS (eight rows' loads in
-    // flight: the registers of a[] are not live yet), then one unconditional
-    // read per register
-#pragma unroll 8
-    for (int i = 0; i < KP; ++i) {
-        const double g = (i < k && c < k) ? G[i * KP + c] : (i == c ? 1.0 : 0.0);
-        Rs[i * KP + c] = (i == c) ? g + sh : g;
-    }
-    wave_lds_sync();
     double a[KP];
 #pragma unroll
-    for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c];
+    for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c] + ((i == c) ? sh : 0.0);
     wave_lds_sync();
     bool bad = false;
     double rdiag = 1.0;  // FLOOR: R[c][c], kept by lane c
@@ -2680,15 +2674,17 @@ __global__ void __launch_bounds__(64) k_chol_w(const double *__restrict__ G, int
 // R[i][j] read from LDS as broadcasts. Pivots j >= k are taken as 1 (the
 // identity block), so padded columns never reach columns c < k. A separate
 // kernel: inlined into the factorization the two 64-register columns spill.
-__global__ void __launch_bounds__(64) k_trinv_w(const double *__restrict__ R64, int k, double *__restrict__ W64,
-                                               float *__restrict__ W32) {
+__global__ void __launch_bounds__(256) k_trinv_w(const double *__restrict__ R64, int k, double *__restrict__ W64,
+                                                float *__restrict__ W32) {
     constexpr int KP = 64;
     __shared__ double Rs[KP * KP];
     __shared__ double rinv[KP];
+    // R staged by all four waves; wave 0 inverts it alone
+    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) Rs[t] = R64[t];
+    __syncthreads();
+    if (threadIdx.x >= KP) return;
     const int c = threadIdx.x;
-#pragma unroll 8
-    for (int i = 0; i < KP; ++i) Rs[i * KP + c] = R64[i * KP + c];  // eight loads in flight
-    rinv[c] = c < k ? 1.0 / R64[c * KP + c] : 1.0;
+    rinv[c] = c < k ? 1.0 / Rs[c * KP + c] : 1.0;
     wave_lds_sync();
     double x[KP];
 #pragma unroll

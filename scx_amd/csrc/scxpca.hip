@@ -214,7 +214,7 @@ struct scxpca_matrix {
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
     DevBuf lg;    // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
-    bool xbf_ready = false;     // tpk holds the packed bf16 entries of the current prepare
+    bool tpk_ready = false;     // tpk holds the packed bf16 entries of the current prepare
     bool tpk_in_build = false;  // the BF16 form ran on this matrix: the tile-CSC build writes tpk too
     DevBuf qpk, mpk;         // the sketch products' panels, hi / lo bf16 in the MFMA B-operand layout
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
@@ -290,7 +290,7 @@ struct StageScope {
 // later stage may run on this matrix.
 int prepare(scxpca_matrix *m) {
     m->stats_ready = false;
-    m->xbf_ready = false;
+    m->tpk_ready = false;
     if (m->n_obs == 0) return 0;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
     // gcnt holds the per-gene entry counts, at [n_vars] the validation bits of
@@ -326,7 +326,7 @@ int prepare(scxpca_matrix *m) {
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
             m->tsq.as<double>(), vflag, m->tpk_in_build ? m->tpk.as<uint32_t>() : nullptr);
-        m->xbf_ready = m->tpk_in_build;
+        m->tpk_ready = m->tpk_in_build;
     }
     else
         k_build_tcsc<<<m->n_tiles, 512, 0, m->stream>>>(m->indptr, m->idx, m->xt.as<float>(), m->n_obs,
@@ -584,8 +584,8 @@ void build_matrix(scxpca_matrix *m, const scxpca_csr *x, double target, int flag
 // The BF16 sketch passes' X~: the tile-CSC entries packed as bf16(x~) | row
 // << 16 | (gene mod 64) << 24 (one 4-byte load per entry, the gene without
 // a search over the run starts)
-void ensure_xbf(scxpca_matrix *m) {
-    if (m->xbf_ready) return;
+void ensure_tpk(scxpca_matrix *m) {
+    if (m->tpk_ready) return;
     m->tpk.ensure(sizeof(uint32_t) * (m->nnz + kEntryPad));
     if (m->n_obs > 0 && m->n_vars > 0) {
         const int64_t groups = static_cast<int64_t>(m->n_tiles) * ((m->n_vars + 63) / 64);
@@ -596,7 +596,7 @@ void ensure_xbf(scxpca_matrix *m) {
     // later prepares of this matrix (the split layout) write the packed entries in the tile-CSC build
     m->tpk_in_build = true;
     HIPC(hipGetLastError());
-    m->xbf_ready = true;
+    m->tpk_ready = true;
 }
 
 // Column statistics of X~ (sum, sum_sq, nnz per gene) from the tile-CSC,
@@ -824,12 +824,12 @@ struct Engine {
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
-            k_spmm_fwd3<KP, false, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
-                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), nullptr, panel, mcp, Y,
+            k_spmm_fwd3<KP, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
+                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y,
                 m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
-            k_spmm_fwd<KP, false, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), nullptr, panel,
+            k_spmm_fwd<KP, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), panel,
                                                        mcp, Y, m->n_obs, rowb, cs32);
         }
     }
@@ -881,9 +881,9 @@ struct Engine {
         if (mm->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb * nch);
             StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T);
-            k_spmm_t6<KP, GW, NB0, false, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
+            k_spmm_t6<KP, GW, NB0, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
                                                                  mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                                                                 nullptr, Q, gp, nch, nrb, tpr, mm->n_obs, nv,
+                                                                 Q, gp, nch, nrb, tpr, mm->n_obs, nv,
                                                                  mm->n_tiles, sc.tpart, qsp);
             HIPC(hipGetLastError());
         } else {  // no rows: zero partials (a fill kernel, no memset node)
@@ -954,11 +954,11 @@ struct Engine {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
         if constexpr (KP == 64) {
             if (floor_tau > 0.0)
-                k_chol_w<true><<<1, 64, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
+                k_chol_w<true><<<1, 256, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
             else
-                k_chol_w<false><<<1, 64, 0, s>>>(G, k, shift, Rdst, status, 0.0);
+                k_chol_w<false><<<1, 256, 0, s>>>(G, k, shift, Rdst, status, 0.0);
             check();
-            k_trinv_w<<<1, 64, 0, s>>>(Rdst, k, W64, W32);
+            k_trinv_w<<<1, 256, 0, s>>>(Rdst, k, W64, W32);
         } else {
             k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status, floor_tau);
         }
@@ -1154,7 +1154,7 @@ struct Engine {
         const bool sketch_bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
-        if (sketch_bf16) ensure_xbf(m);
+        if (sketch_bf16) ensure_tpk(m);
         ops.stats();
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), baseline_mean(), mu);
         check();
@@ -1574,7 +1574,7 @@ struct WideEngine {
         const bool bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(eg.status, 0, sizeof(int), s));
-        if (bf16) ensure_xbf(m);
+        if (bf16) ensure_tpk(m);
         ops.stats();
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), eg.baseline_mean(), eg.mu);
         check();
@@ -1678,7 +1678,7 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     validate(m, p);
     const int64_t k = pca_k(m, p);
     DeviceGuard dg(m->device);
-    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_xbf(m);
+    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_tpk(m);
     ensure_stats(m);
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
@@ -1743,7 +1743,7 @@ void stage_spmm(scxpca_matrix *m, int transpose, int64_t k, const double *in, co
     Engine<KP> eng(m, static_cast<int>(k));
     const int64_t n = m->n_obs, nv = m->n_vars;
     hipStream_t s = m->stream;
-    if (bf16) ensure_xbf(m);
+    if (bf16) ensure_tpk(m);
     const bool centered = means != nullptr;
     if (centered) HIPC(hipMemcpyAsync(eng.mu, means, sizeof(double) * nv, hipMemcpyHostToDevice, s));
     if (!transpose) {
