@@ -696,7 +696,7 @@ struct Engine {
     static constexpr int TW = 16;                   // waves per k_spmm_t6 block
     static constexpr int CG = TW * GW;              // genes per k_spmm_t6 block
     static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
-    static constexpr int FPF = 8;                   // first-batch prefetch ring depth (rows)
+    static constexpr int FPF = 4;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
     hipStream_t s;
     int k = 0;
