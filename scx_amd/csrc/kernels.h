@@ -1137,6 +1137,49 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
 #undef SCX_STEP
 }
 
+// Wave priority for the rest of a block-synchronized segment (a tile, a gene
+// chunk), by the quarter of the wave's work in it already done: waves that
+// are behind issue first. A SIMD otherwise arbitrates among its waves
+// oldest-first, so the youngest wave of each SIMD ends its segment last
+// (measured 1.26x the mean time in the transposed product) while the others
+// wait at the barrier. `quarter` is a constant after unrolling.
+__device__ __forceinline__ void segment_priority(int quarter) {
+    if (quarter <= 0) __builtin_amdgcn_s_setprio(3);
+    else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
+    else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
+// Two batches at once (two accumulators, one step count: the longer batch's;
+// the shorter one's slots past its count carry x = 0 and offset 0): per step
+// two LDS reads one step ahead and 2E FMAs, so one read latency at the start
+// serves both batches.
+template <int E>
+__device__ __forceinline__ void gather_batch2(float (&accA)[E], uint32_t ofsA, float xA, float (&accB)[E],
+                                              uint32_t ofsB, float xB, int cnt, const char *lds,
+                                              uint32_t lane_ofs) {
+    const int steps = __builtin_amdgcn_readfirstlane((cnt + 3) >> 2);
+    float4 a0[E / 4], a1[E / 4], b0[E / 4], b1[E / 4];
+    gather_load<0, E>(a0, ofsA, lds, lane_ofs);
+    gather_load<0, E>(b0, ofsB, lds, lane_ofs);
+#define SCX_STEP2(T, CA, NA, CB, NB)                                   \
+    if ((T) >= steps) return;                                          \
+    if ((T) + 1 < 16) {                                                \
+        gather_load<((T) + 1) & 15, E>(NA, ofsA, lds, lane_ofs);       \
+        gather_load<((T) + 1) & 15, E>(NB, ofsB, lds, lane_ofs);       \
+    }                                                                  \
+    __builtin_amdgcn_sched_barrier(0);                                 \
+    gather_fma<T, E>(accA, CA, xA);                                    \
+    gather_fma<T, E>(accB, CB, xB);
+    SCX_STEP2(0, a0, a1, b0, b1) SCX_STEP2(1, a1, a0, b1, b0) SCX_STEP2(2, a0, a1, b0, b1)
+    SCX_STEP2(3, a1, a0, b1, b0) SCX_STEP2(4, a0, a1, b0, b1) SCX_STEP2(5, a1, a0, b1, b0)
+    SCX_STEP2(6, a0, a1, b0, b1) SCX_STEP2(7, a1, a0, b1, b0) SCX_STEP2(8, a0, a1, b0, b1)
+    SCX_STEP2(9, a1, a0, b1, b0) SCX_STEP2(10, a0, a1, b0, b1) SCX_STEP2(11, a1, a0, b1, b0)
+    SCX_STEP2(12, a0, a1, b0, b1) SCX_STEP2(13, a1, a0, b1, b0) SCX_STEP2(14, a0, a1, b0, b1)
+    SCX_STEP2(15, a1, a0, b1, b0)
+#undef SCX_STEP2
+}
+
 // ---------------------------------------------------------------------------
 // Forward SpMM, gene-chunked panel in LDS: Y[r][:] = sum x~ M[c][:] - mc.
 // A block (NW waves) owns NW*R consecutive rows; each wave keeps R rows'
@@ -1146,8 +1189,10 @@ __device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, floa
 // split offsets), 64 nonzeros per batch: lane-parallel load + transform, then
 // broadcast steps that feed the four groups with entries 4t+g. The first batch
 // of each (row, chunk) window is loaded PF rows ahead, across chunk
-// boundaries, so HBM latency overlaps the gather loop. After the last chunk
-// the four group partials of a row are summed and the row is written once.
+// boundaries, so HBM latency overlaps the gather loop. A wave lowers its
+// priority as it gets through its rows (segment_priority). After the last
+// chunk the four group partials of a row are summed and the row is written
+// once.
 // ---------------------------------------------------------------------------
 
 template <int KP, int R, int NW, int CG, int PF, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j]
@@ -1241,8 +1286,10 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict
         // code fed by the ring (no loads inside loops, so the waits on the
         // ring stay partial).
         bool any_long = false;
+        segment_priority(0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
+            if (r > 0 && (r * 4) / R != ((r - 1) * 4) / R) segment_priority((r * 4) / R);
             const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
             const uint32_t len = hi_r - lo_r;
             any_long |= len > static_cast<uint32_t>(kWave);
@@ -1711,9 +1758,13 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
 // address is a wave-uniform run start plus the lane's fixed slot, so a batch
 // costs two loads and no per-lane index arithmetic, and no entry list passes
 // through LDS. Entries beyond the prefetched batches (rare outside slot 0)
-// are read directly. Loads may run up to NB0 * 64 - 1 entries past the last
-// run: the tile-CSC arrays carry kEntryPad entries of padding, and lanes past
-// a run's end are masked at use. The next tile's base is this tile's base
+// are read directly. With KP = 64 the genes after slot 0 run in pairs
+// through one gather loop (gather_batch2: one LDS read latency per pair), and
+// each wave lowers its priority as it gets through its genes
+// (segment_priority), so the waves of a SIMD reach the next tile's barrier
+// together. Loads may run up to NB0 * 64 - 1 entries past the last run: the
+// tile-CSC arrays carry kEntryPad entries of padding, and lanes past a run's
+// end are masked at use. The next tile's base is this tile's base
 // plus its entry count (tptr[t][n_vars], read by the spare lane GW with the
 // run bounds), so the tile loop issues no scalar loads. Blocks are XCD-aware
 // (blockIdx = slot * 8 + xcd, the gene chunks of one tile range on one XCD,
@@ -1736,7 +1787,11 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     constexpr int QC = KP / kWave;  // Q columns per lane in the column sums
     static_assert(GW < 16, "one gene per lane of a group, plus the spare lane GW");
     static_assert(NB0 >= 1 && NB0 * kWave <= kEntryPad, "slot-0 look-ahead must stay inside the padding");
+    // KP = 64: genes 1 + 2i and 2 + 2i share one gather loop (registers allow
+    // the second set of read buffers only there)
+    constexpr bool kPair = KP == 64;
     __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
+    const char *qb = reinterpret_cast<const char *>(qs);
     const int tid = threadIdx.x;
     const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
     const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
@@ -1829,8 +1884,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     auto gather = [&](float (&acc)[E], uint32_t rr, float v, uint32_t left) {
         const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
         const bool ok = esel < cnt;
-        gather_batch<E>(acc, ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt, reinterpret_cast<const char *>(qs),
-                        lane_ofs);
+        gather_batch<E>(acc, ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt, qb, lane_ofs);
     };
     // A prefetched batch is turned into (LDS offset, value) before the next
     // tile's load into the same registers is issued (a scheduling barrier
@@ -1848,7 +1902,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     };
     auto gather_prepped = [&](float (&acc)[E], uint32_t ofs, float x, uint32_t left) {
         const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
-        gather_batch<E>(acc, ofs, x, cnt, reinterpret_cast<const char *>(qs), lane_ofs);
+        gather_batch<E>(acc, ofs, x, cnt, qb, lane_ofs);
     };
 
     // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
@@ -1878,7 +1932,7 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
         __syncthreads();
         if (do_qsum) {
-            const float *qf = reinterpret_cast<const float *>(qs);
+            const float *qf = reinterpret_cast<const float *>(qb);
             for (int rr = gc + nch * wave; rr < kTileRows; rr += nch * NW)
 #pragma unroll
                 for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
@@ -1891,9 +1945,11 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
         layout(wlo, whi, nlo, nln, ntot);
         bounds(tnn, wlo, whi);
         uint32_t longest = 0u;
+        segment_priority(0);
 #pragma unroll
         for (int gi = 0; gi < GW; ++gi) {
             const uint32_t len = ln[gi];
+            if (gi > 0 && (gi * 4) / GW != ((gi - 1) * 4) / GW) segment_priority((gi * 4) / GW);
             if (gi == 0) {
                 uint32_t ofs[NB0];
                 float x[NB0];
@@ -1908,6 +1964,23 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
                 for (int b = 0; b < NB0; ++b)
                     if (len > static_cast<uint32_t>(b * kWave)) gather_prepped(z[0], ofs[b], x[b], len - b * kWave);
                 if (kIssueLate) issue_tile(ntb, nlo, 0, 1);
+            } else if (kPair && (gi & 1) && gi + 1 < GW) {  // genes gi and gi + 1 together
+                const uint32_t lb = ln[gi + 1];
+                uint32_t ofs, ofsb;
+                float x, xb;
+                prep(prow[gi], pval[gi], len, ofs, x);
+                prep(prow[gi + 1], pval[gi + 1], lb, ofsb, xb);
+                __builtin_amdgcn_sched_barrier(0);
+                issue_tile(ntb, nlo, gi, gi + 2);
+                longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
+                longest = lb > static_cast<uint32_t>(kWave) && lb > longest ? lb : longest;
+                const uint32_t lm = len > lb ? len : lb;
+                if (lm > 0u)
+                    gather_batch2<E>(z[gi], ofs, x, z[gi + 1], ofsb, xb,
+                                     lm < static_cast<uint32_t>(kWave) ? static_cast<int>(lm) : kWave, qb,
+                                     lane_ofs);
+            } else if (kPair && !(gi & 1)) {
+                // gene gi ran with gene gi - 1
             } else {
                 uint32_t ofs;
                 float x;
