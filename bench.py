@@ -253,6 +253,23 @@ def run_decode(args, x, w):
     ok = (np.array_equal(csr.indptr.cpu().numpy(), x.indptr) and np.array_equal(csr.indices.cpu().numpy(),
                                                                                 x.indices.view(np.int32))
           and np.array_equal(csr.data.cpu().numpy(), x.data))
+    # the path's consumer: the headline rPCA on the decoded device CSR, and the
+    # whole chain from shard bytes (decode + resident matrix + rPCA)
+    from scx_amd import _lib
+
+    kw = dict(n_comps=w.n_comps, n_oversamples=w.n_oversamples, n_power_iterations=w.n_power_iterations, seed=0,
+              qr_method=_lib.QR_CHOLESKY)
+    chain = []
+    for it in range(3):
+        torch.cuda.synchronize()
+        tc = time.perf_counter()
+        c2 = host.decode_csr_shards_to_device(shards)
+        mm = _lib.Matrix(c2.indptr, c2.indices, c2.data, c2.shape, target_sum=w.target_sum, log1p=True)
+        mm.pca(**kw)
+        torch.cuda.synchronize()
+        chain.append(time.perf_counter() - tc)
+        mm.close()
+        del c2
     med = lambda k: float(np.median([s[k] for s in stats]))  # noqa: E731
     dev_s = med("device_seconds")
     n, nnz = x.shape[0], x.nnz
@@ -301,6 +318,9 @@ def run_decode(args, x, w):
         "decoded_gbs_end_to_end": round(out_bytes / (ms * 1e-3) / 1e9, 2),
         "stage_ms": {"host": round(med("host_seconds") * 1e3, 3), "upload": round(med("upload_seconds") * 1e3, 3),
                      "device": round(dev_s * 1e3, 3), "total": round(med("total_seconds") * 1e3, 3)},
+        "shards_to_pca_ms": round(float(np.median(chain)) * 1e3, 3),
+        "shards_to_pca_note": "decode + resident matrix (prepare) + rPCA (k=50, p=10, q=3, FP32, cholesky), "
+                              "sigma / embeddings / components copied to host; median of 3",
         "host_uploaded_bytes": stats[-1]["host_uploaded_bytes"],
         "round_trip_exact": bool(ok),
         "cpu_baseline": cpu,
