@@ -194,8 +194,9 @@ scxpca_status scxpca_matrix_pflog_dense(scxpca_matrix *m, float *z, int32_t on_d
  * scx-accel/src/pca/cpu.rs:815-866, streams its delta source): alpha pinned or
  * estimated by a streamed raw pass, every shard's delta and baseline built
  * from its own rows. fit (may be NULL) receives the alpha fit, baseline (may
- * be NULL) the n_obs baseline (host f64). The transform fields of p are
- * ignored. FP32 only. */
+ * be NULL) the n_obs baseline (host f64). out == NULL skips the PCA: the fit
+ * and the baseline only (pflog's store="baseline" / "dense" on a matrix that
+ * does not stay resident). The transform fields of p are ignored. FP32 only. */
 scxpca_status scxpca_pflog_pca_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, const scxpca_params *p,
                                         int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit,
                                         double *baseline, int device);

@@ -431,10 +431,12 @@ def randomized_pca_streamed(indptr, indices, data, shape, shard_rows, n_comps=50
 
 def pflog_pca_streamed(indptr, indices, data, shape, shard_rows, alpha=None, delta_form=PFLOG_DELTA_LAZY,
                        n_comps=50, n_oversamples=10, n_power_iterations=2, zero_center=True, seed=0,
-                       qr_method=QR_HOUSEHOLDER, mu_min=ALPHA_MU_MIN, fallback_alpha=ALPHA_FALLBACK, device=0):
+                       qr_method=QR_HOUSEHOLDER, mu_min=ALPHA_MU_MIN, fallback_alpha=ALPHA_FALLBACK, device=0,
+                       pca=True):
     """PFlog PCA of a host CSR streamed to the GPU in shards on every pass;
     returns (the result dict of ``Matrix.pca``, the fit dict of
-    ``Matrix.pflog_fit`` with the baseline)."""
+    ``Matrix.pflog_fit`` with the baseline). ``pca=False``: the fit and the
+    baseline only (one streamed pass), returned as (None, fit)."""
     indptr = np.ascontiguousarray(indptr, dtype=np.int64)
     indices = np.ascontiguousarray(indices, dtype=np.uint32)
     data = np.ascontiguousarray(data, dtype=np.float32)
@@ -443,23 +445,26 @@ def pflog_pca_streamed(indptr, indices, data, shape, shard_rows, alpha=None, del
     pp = PflogParams(0.0 if alpha is None else float(alpha), float(mu_min), float(fallback_alpha), int(delta_form))
     prm = Params(n_comps, n_oversamples, n_power_iterations, int(bool(zero_center)), qr_method, seed, 0.0, 0,
                  PRECISION_FP32)
-    k = max(1, min(n_comps + n_oversamples, n, nv))
-    var = np.zeros(max(n_comps, 1))
-    ratio = np.zeros(max(n_comps, 1))
-    mean = np.zeros(max(nv, 1))
-    sigma = np.zeros(k)
-    emb = np.empty((n, n_comps), dtype=np.float32)
-    comps = np.empty((n_comps, nv), dtype=np.float32)
     base = np.zeros(n)
-    res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0, 0)
     f = PflogFit()
+    result, res = None, None
+    if pca:
+        k = max(1, min(n_comps + n_oversamples, n, nv))
+        var = np.zeros(max(n_comps, 1))
+        ratio = np.zeros(max(n_comps, 1))
+        mean = np.zeros(max(nv, 1))
+        sigma = np.zeros(k)
+        emb = np.empty((n, n_comps), dtype=np.float32)
+        comps = np.empty((n_comps, nv), dtype=np.float32)
+        res = Result(ptr(emb), ptr(comps), ptr(var), ptr(ratio), ptr(mean), ptr(sigma), 0, 0, 0.0, 0, 0.0, 0)
     check(lib().scxpca_pflog_pca_streamed(ctypes.byref(csr), ctypes.byref(pp), ctypes.byref(prm),
-                                          _i64(int(shard_rows)), ctypes.byref(res), ctypes.byref(f), ptr(base),
-                                          device))
-    result = {"embeddings": emb, "components": comps, "variance": var[:n_comps], "variance_ratio": ratio[:n_comps],
-              "mean": mean[:nv] if zero_center else None, "sigma": sigma[: res.k], "k": int(res.k),
-              "total_variance": float(res.total_variance), "tv_recomputed": bool(res.tv_recomputed),
-              "seconds_device": float(res.seconds_device)}
+                                          _i64(int(shard_rows)), ctypes.byref(res) if pca else None,
+                                          ctypes.byref(f), ptr(base), device))
+    if pca:
+        result = {"embeddings": emb, "components": comps, "variance": var[:n_comps],
+                  "variance_ratio": ratio[:n_comps], "mean": mean[:nv] if zero_center else None,
+                  "sigma": sigma[: res.k], "k": int(res.k), "total_variance": float(res.total_variance),
+                  "tv_recomputed": bool(res.tv_recomputed), "seconds_device": float(res.seconds_device)}
     fit = {"alpha": float(f.alpha), "pseudocount": float(f.pseudocount),
            "alpha_source": "estimated" if f.estimated else "pinned", "baseline": base}
     if f.estimated:

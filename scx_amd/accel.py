@@ -446,6 +446,26 @@ def pca(adata, n_comps: int = 50, zero_center: bool = True, random_state: int = 
 DENSE_MAX_ELEMS = 200_000_000  # pyscx/src/accel/pflog.rs:60
 
 
+def _pflog_dense_by_shards(ip, ix, dv, shape, alpha, form, rows, device_id):
+    """The dense PFlog layer Z = f32(b_i) + delta (n_obs x n_vars f32) of a
+    matrix that does not stay resident: each shard of ``rows`` rows is built
+    as its own PFlog matrix with the fitted alpha pinned (its baseline and
+    delta depend on its own rows and alpha only) and fills its rows of Z."""
+    n, nv = int(shape[0]), int(shape[1])
+    z = np.empty((n, nv), dtype=np.float32)
+    for a in range(0, n, rows):
+        b = min(n, a + rows)
+        lo, hi = int(ip[a]), int(ip[b])
+        m = _lib.Matrix(ip[a:b + 1] - lo, ix[lo:hi], dv[lo:hi], (b - a, nv), device=device_id,
+                        pflog={"alpha": alpha, "delta_form": form})
+        try:
+            m.pflog_fit(baseline=True)
+            z[a:b] = m.pflog_dense()
+        finally:
+            m.close()
+    return z
+
+
 def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: int = 50, n_oversamples: int = 10,
           n_power_iterations: int = 2, zero_center: bool = True, random_state: int = 0,
           obsm_key: str = "X_pflog_pca", baseline_key: str = "pflog_baseline", layer_out=None, out=None,
@@ -521,17 +541,22 @@ def pflog(adata, *, alpha=None, layer=None, store: str = "pca", n_components: in
     k = max(1, min(n_components + n_oversamples, *raw.shape))
     seed = int(random_state) & ((1 << 64) - 1)
     res = dense = None
-    if want_pca and not resident_fits(raw.shape[0], raw.shape[1], raw.nnz, k, device_id):
-        # the matrix does not stay in HBM: PFlog over streamed shards
-        if want_dense:
-            raise RuntimeError("pflog: the dense transform needs the matrix resident in HBM; use store='pca'")
+    if not resident_fits(raw.shape[0], raw.shape[1], raw.nnz, k, device_id):
+        # the matrix does not stay in HBM (every store mode): the fit, the
+        # baseline and the PCA over streamed shards; the dense layer shard by
+        # shard under the same dense_max_elems guard
+        rows = stream_shard_rows(raw.shape[0], raw.shape[1], raw.nnz, k, device_id)
         try:
-            res, fit = _lib.pflog_pca_streamed(ip, ix, dv, raw.shape,
-                                               stream_shard_rows(raw.shape[0], raw.shape[1], raw.nnz, k, device_id),
-                                               alpha=alpha, delta_form=form, n_comps=n_components,
-                                               n_oversamples=n_oversamples, n_power_iterations=n_power_iterations,
-                                               zero_center=zero_center, seed=seed, qr_method=_lib.QR_HOUSEHOLDER,
-                                               device=device_id)
+            if want_pca:
+                res, fit = _lib.pflog_pca_streamed(ip, ix, dv, raw.shape, rows, alpha=alpha, delta_form=form,
+                                                   n_comps=n_components, n_oversamples=n_oversamples,
+                                                   n_power_iterations=n_power_iterations, zero_center=zero_center,
+                                                   seed=seed, qr_method=_lib.QR_HOUSEHOLDER, device=device_id)
+            else:
+                _, fit = _lib.pflog_pca_streamed(ip, ix, dv, raw.shape, rows, alpha=alpha, delta_form=form,
+                                                 n_comps=1, n_oversamples=0, device=device_id, pca=False)
+            if want_dense and n_elems <= dense_max_elems:
+                dense = _pflog_dense_by_shards(ip, ix, dv, raw.shape, fit["alpha"], form, rows, device_id)
         except ValueError as e:
             raise runtime(e) from None
         resident = False

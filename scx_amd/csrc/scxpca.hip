@@ -2399,7 +2399,8 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
             ops.pflog = true;
             ops.baseline_out = baseline;
         }
-        eg.run_with(ops, p, out);
+        if (out) eg.run_with(ops, p, out);
+        else ops.stats();  // fit and baseline only: one streamed pass over the PFlog shards
     };
     if (k <= 64) {
         Engine<64> eg(&shell, static_cast<int>(k));
@@ -2422,8 +2423,10 @@ void run_pflog_streamed(const scxpca_csr *x, const scxpca_pflog_params *pp, cons
     HIPC(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    out->seconds_device = ms * 1e-3;
-    out->resident = 0;
+    if (out) {
+        out->seconds_device = ms * 1e-3;
+        out->resident = 0;
+    }
 }
 
 // HBM a resident matrix and its PCA workspace take in this build: the CSR
@@ -2516,7 +2519,7 @@ scxpca_status scxpca_pflog_pca_streamed(const scxpca_csr *x, const scxpca_pflog_
                                         int64_t shard_rows, scxpca_result *out, scxpca_pflog_fit *fit,
                                         double *baseline, int device) {
     return guarded([&] {
-        if (!x || !pp || !p || !out) throw Error(SCXPCA_ERR_INVALID, "null argument");
+        if (!x || !pp || !p) throw Error(SCXPCA_ERR_INVALID, "null argument");
         run_pflog_streamed(x, pp, p, shard_rows, out, fit, baseline, device);
     });
 }

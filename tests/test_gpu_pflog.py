@@ -412,3 +412,42 @@ def test_accel_pflog_streams_when_not_resident(gpu, monkeypatch):
     assert s_.uns["pflog"] == r.uns["pflog"]
     np.testing.assert_array_equal(np.asarray(s_.obs["pflog_baseline"]), np.asarray(r.obs["pflog_baseline"]))
     np.testing.assert_allclose(s_.uns["X_pflog_pca_singular_values"], r.uns["X_pflog_pca_singular_values"], rtol=1e-5)
+
+
+@pytest.mark.parametrize("store", ["baseline", "dense", "all"])
+def test_accel_pflog_store_modes_stream_when_not_resident(gpu, monkeypatch, store):
+    """Every store mode takes the streamed route when the residency pre-flight
+    declines: the fit and baseline from a streamed pass, the dense layer built
+    shard by shard (shards of 256 rows here, so several) with the fitted alpha
+    pinned, bit-identical to the resident route's, and the PCA spectrum of
+    store="all" within the streamed-vs-resident tolerance."""
+    from scx_amd import accel
+    from scx_amd.adata import AnnDataLite
+
+    x = cfg1_head(700)
+    out = {}
+    for env in ("1", "0"):
+        monkeypatch.setenv("SCX_GPU_PCA_RESIDENT", env)
+        monkeypatch.setattr(accel, "stream_shard_rows", lambda *a, **k: 256)
+        a = AnnDataLite(x.copy())
+        say(f"case: accel.pflog store={store} SCX_GPU_PCA_RESIDENT={env}")
+        accel.pflog(a, store=store, n_components=10, device="gpu", layer_out="Z")
+        out[env] = a
+    r, s_ = out["1"], out["0"]
+    assert s_.uns["scx_accel"]["pflog"]["resident_csr"] is False and r.uns["scx_accel"]["pflog"]["resident_csr"] is True
+    assert s_.uns["pflog"] == r.uns["pflog"]
+    np.testing.assert_array_equal(np.asarray(s_.obs["pflog_baseline"]), np.asarray(r.obs["pflog_baseline"]))
+    if store in ("dense", "all"):
+        assert s_.layers["Z"].dtype == np.float32
+        np.testing.assert_array_equal(s_.layers["Z"], r.layers["Z"])
+    else:
+        assert "Z" not in s_.layers
+    if store == "all":
+        np.testing.assert_allclose(s_.uns["X_pflog_pca_singular_values"], r.uns["X_pflog_pca_singular_values"],
+                                   rtol=1e-5)
+    else:
+        assert "X_pflog_pca" not in s_.obsm
+    if store == "dense":  # the in-memory guard holds on the streamed route too
+        g = AnnDataLite(x.copy())
+        with pytest.raises(RuntimeError, match="over the dense_max_elems=1000 guard"):
+            accel.pflog(g, store="dense", n_components=10, device="gpu", dense_max_elems=1000)
