@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace scxpca {
 
 constexpr int kWave = 64;
@@ -854,8 +856,17 @@ __global__ void __launch_bounds__(256) k_spmm_t_reduce(const float *__restrict__
     if (t >= n_vars * KP) return;
     const int64_t v = t / KP;
     const int j = static_cast<int>(t % KP);
+    const int64_t stride = n_vars * KP;
     double s = 0.0;
-    for (int rb = 0; rb < nrb; ++rb) s += static_cast<double>(part[static_cast<int64_t>(rb) * n_vars * KP + t]);
+    int rb = 0;
+    for (; rb + 8 <= nrb; rb += 8) {  // eight loads in flight, summed in partial order
+        float p[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) p[u] = part[static_cast<int64_t>(rb + u) * stride + t];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += static_cast<double>(p[u]);
+    }
+    for (; rb < nrb; ++rb) s += static_cast<double>(part[static_cast<int64_t>(rb) * stride + t]);
     if (mu) s -= mu[v] * sumq[j];
     if (bq) s += bq[j];
     if (B) B[t] = s;
@@ -2034,6 +2045,363 @@ __global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict_
     if (do_qsum) {  // the waves' column sums, added in wave order
         __syncthreads();
         double *red = reinterpret_cast<double *>(qs);  // NW x KP f64 fits in the Q tile buffer
+#pragma unroll
+        for (int e = 0; e < QC; ++e) red[wave * KP + lane + kWave * e] = qsum[e];
+        __syncthreads();
+        if (tid < KP) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += red[w * KP + tid];
+            qsum_part[(static_cast<int64_t>(tr) * nch + gc) * KP + tid] = t;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Transposed SpMM with genes owned per 16-lane group (cpu.rs:1282-1389).
+//
+// k_spmm_t6 has the four groups of a wave share each gene (entries 4t+g of a
+// batch) and sums their partials at the end, so a wave covers GW genes and a
+// 1024-thread block 16 GW: every Q tile is staged once per 128 genes, and a
+// wave's share of a tile is ~160 entries between two block barriers. Here
+// each group owns its own GW genes: lane 16g+t holds entry t of group g's
+// current run, and the row broadcast (DPP row_newbcast:t) hands every group
+// its own entry, so the same gather steps serve four genes at once. A wave
+// covers 4 GW genes, a block 64 GW (512 at KP = 64), each Q tile is staged
+// once per 512 genes, and a wave does ~650 entries per tile. The four genes
+// of a slot (gperm deals genes to groups by entry count, snake order, so
+// they have neighbouring counts) run together for the longest of their four
+// runs; the shorter runs' lanes carry x = 0 and offset 0. Each group's Z
+// rows stay whole in its own lanes (no cross-group sum), accumulated in run
+// order: rows ascending inside a tile, tiles ascending.
+//
+// Entry loads: a run's first 16 entries per group (P0 x 16 for slot 0, the
+// densest genes) are loaded one tile ahead into the gather layout; longer
+// runs stream through a 64-entry ring per group, the next chunk issued
+// before the current one is gathered. Every load index is clamped into the
+// group's run (or the run start), so no load leaves the tile-CSC and the
+// loads are unconditional. Per tile the Q tile is staged between two block
+// barriers; its loads are spread over the slots of the previous tile.
+// ---------------------------------------------------------------------------
+
+// Values computed into registers at this point: loads that follow stay after
+// it, so a prefetch register read here can take its next load in place
+// (no copy at the loop edge, where a copy of a register with a load in
+// flight would wait for every outstanding load).
+__device__ __forceinline__ void pin_regs(uint32_t &a, float &b) { asm volatile("" : "+v"(a), "+v"(b)::"memory"); }
+
+// Wave priority by the share of the tile's gather steps already done (done,
+// total: wave-uniform): waves that are behind issue first, against the SIMD's
+// oldest-first arbitration that otherwise leaves the youngest waves of each
+// SIMD last to reach the tile barrier.
+__device__ __forceinline__ void progress_priority(int done, int total) {
+    if (4 * done < total) __builtin_amdgcn_s_setprio(3);
+    else if (2 * done < total) __builtin_amdgcn_s_setprio(2);
+    else if (4 * done < 3 * total) __builtin_amdgcn_s_setprio(1);
+    else __builtin_amdgcn_s_setprio(0);
+}
+
+// lane T of each 16-lane row copied to the whole row
+template <int T>
+__device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
+    uint32_t r;
+    asm volatile("s_nop 1\n\tv_mov_b32_dpp %0, %1 row_newbcast:%2 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+                 : "=v"(r)
+                 : "v"(v), "i"(T));
+    return r;
+}
+
+// Gather steps S..N-1 of a run held as NC components of 16 entries per group
+// (step s: component s >> 4, lane s & 15 of each group); LDS reads two steps
+// ahead through three rotating buffers, as gather_batch.
+template <int S, int N, int E, int NC>
+struct GatherRun {
+    static __device__ __forceinline__ void run(float (&acc)[E], const uint32_t (&ofs)[NC], const float (&x)[NC],
+                                               int steps, const char *lds, uint32_t lane_ofs, float4 (&cur)[E / 4],
+                                               float4 (&nx1)[E / 4], float4 (&nx2)[E / 4]) {
+        if (S >= steps) return;
+        if constexpr (S + 2 < N) gather_load<(S + 2) & 15, E>(nx2, ofs[(S + 2) >> 4], lds, lane_ofs);
+        __builtin_amdgcn_sched_barrier(0);
+        gather_fma<S & 15, E>(acc, cur, x[S >> 4]);
+        GatherRun<S + 1, N, E, NC>::run(acc, ofs, x, steps, lds, lane_ofs, nx1, nx2, cur);
+    }
+};
+template <int N, int E, int NC>
+struct GatherRun<N, N, E, NC> {
+    static __device__ __forceinline__ void run(float (&)[E], const uint32_t (&)[NC], const float (&)[NC], int,
+                                               const char *, uint32_t, float4 (&)[E / 4], float4 (&)[E / 4],
+                                               float4 (&)[E / 4]) {}
+};
+
+// steps: wave-uniform, 1..16 NC
+template <int E, int NC>
+__device__ __forceinline__ void gather_run(float (&acc)[E], const uint32_t (&ofs)[NC], const float (&x)[NC],
+                                           int steps, const char *lds, uint32_t lane_ofs) {
+    float4 b0[E / 4], b1[E / 4], b2[E / 4];
+    gather_load<0, E>(b0, ofs[0], lds, lane_ofs);
+    gather_load<1, E>(b1, ofs[0], lds, lane_ofs);
+    GatherRun<0, 16 * NC, E, NC>::run(acc, ofs, x, steps, lds, lane_ofs, b0, b1, b2);
+}
+
+// f(std::integral_constant<int, I>) for I = B .. N-1, unrolled
+template <int I, int N, typename F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// word 3 of a raw buffer descriptor (gfx9 family: 32-bit data format, no swizzle)
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
+// n_entries: entries in trow / tval including their padding
+template <int KP, int GW, int P0, int P1, int RC, int NW>
+__global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__ indptr,
+                                                 const uint32_t *__restrict__ tptr,
+                                                 const uint8_t *__restrict__ trow,
+                                                 const float *__restrict__ tval, int64_t n_entries,
+                                                 const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
+                                                 int nch, int ntr, int tiles_per_tr, int64_t n_obs,
+                                                 int64_t n_vars, int n_tiles, float *__restrict__ part,
+                                                 double *__restrict__ qsum_part) {
+    constexpr int E = KP / 16;
+    constexpr int NT = NW * kWave;
+    constexpr int QC = KP / kWave;                 // Q columns per lane in the column sums
+    constexpr int TQ = kTileRows * KP / 4;         // float4s per Q tile
+    constexpr int QPT = TQ / NT;                   // direct loads per lane per Q tile
+    constexpr int kAfter = 2 * P0;  // entry loads issued after a Q tile's loads, always
+    static_assert(GW >= 2 && GW < 16, "one slot per lane of a group, plus the spare lane GW");
+    static_assert(P0 >= 1 && P0 <= 4 && P1 >= 1 && P1 <= 4 && RC >= 1 && RC <= 4, "16 to 64 entries per load");
+    static_assert(TQ % NT == 0 && kAfter < 64, "whole Q tiles per block; vmcnt range");
+    __shared__ float4 qs[2][TQ];  // two Q tiles: 128 KB (KP=64)
+    const int tid = threadIdx.x;
+    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
+    const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int lane = tid & 63, g = lane >> 4, l = lane & 15;
+    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
+    // lane (g, l < GW) holds the gene of slot l of group g
+    const int64_t gs = ((static_cast<int64_t>(gc) * NW + wave) * 4 + g) * GW + l;
+    const uint32_t gene = (l < GW) ? gperm[gs] : static_cast<uint32_t>(n_vars);
+    const bool gvalid = gene < static_cast<uint64_t>(n_vars);
+    const bool spare = l == GW;  // reads the tile's entry count
+    const uint32_t ia = gvalid ? gene : 0u;
+    const uint32_t ib = gvalid ? gene + 1u : static_cast<uint32_t>(n_vars);
+    float z[GW][E];
+#pragma unroll
+    for (int gi = 0; gi < GW; ++gi)
+#pragma unroll
+        for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
+    const int t_begin = tr * tiles_per_tr;
+    const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
+    const int t_end = (tr < ntr) ? t_end0 : t_begin;
+
+    auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
+        const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
+        const uint32_t a0 = tp[ia], b0 = tp[ib];
+        a = gvalid ? a0 : 0u;
+        b = (gvalid || spare) ? b0 : 0u;
+    };
+    // Q tile tt into buffer bi by direct global -> LDS loads: wave w's load j
+    // fills float4s (j NW + w) * 64 + lane of the tile, rows past n_obs read
+    // row 0 (never gathered: no entry refers to them; the column sums stop at
+    // the last row). Written as asm so that the compiler does not count these
+    // loads against the LDS reads of the other buffer (it would wait for them
+    // before every gather); their completion is awaited explicitly (vmcnt,
+    // then the block barrier) before the buffer is read.
+    const int64_t qlim = n_obs * (KP / 4);
+    auto load_q = [&](int tt, int bi) {
+        const int64_t q0 = static_cast<int64_t>(tt) * TQ;
+#pragma unroll
+        for (int j = 0; j < QPT; ++j) {
+            const int i = (j * NW + wave) * kWave;
+            const int64_t src = q0 + i + lane;
+            const float4 *gp = reinterpret_cast<const float4 *>(Q) + (src < qlim ? src : 0);
+            const uint32_t dst = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&qs[bi][i]));
+            asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(gp), "{m0}"(dst) : "memory");
+        }
+    };
+    // Entry loads through buffer descriptors of a tile's trow / tval (base:
+    // the tile's first entry, scalar; records: up to the end of the padded
+    // arrays): entry j = l + 16 c of a group's run at lo is the lane offset
+    // lo + l plus the immediate 16 c (x4 for tval), so a load costs no
+    // address arithmetic. Past a run the lanes read the next runs' entries
+    // (masked at use); past the arrays the descriptor's range check returns 0.
+    struct TileRs {
+        __amdgpu_buffer_rsrc_t row, val;
+    };
+    auto tile_rs = [&](int64_t tb_) {
+        const int64_t left = n_entries - tb_;
+        const int64_t rb = left < 0x7fffffff ? left : 0x7fffffff;
+        const int64_t vb = 4 * left < 0x7fffffff ? 4 * left : 0x7fffffff;
+        return TileRs{__builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(trow + tb_), 0, static_cast<int>(rb),
+                                                        kBufferRsrcWord3),
+                      __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(tval + tb_), 0, static_cast<int>(vb),
+                                                        kBufferRsrcWord3)};
+    };
+    auto issue = [&](const TileRs &rs, uint32_t lo_, int c, uint8_t &pr, float &pv) {
+        const uint32_t o = lo_ + static_cast<uint32_t>(l);
+        pr = __builtin_amdgcn_raw_buffer_load_b8(rs.row, static_cast<int>(o + 16 * c), 0, 0);
+        pv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs.val, static_cast<int>(4 * o + 64 * c), 0, 0));
+    };
+    auto prep = [&](uint8_t pr, float pv, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
+        const bool ok = j0 + static_cast<uint32_t>(l) < len_;
+        ofs = ok ? static_cast<uint32_t>(pr) * (KP * 4) : 0u;
+        x = ok ? pv : 0.f;
+    };
+    // prefetched components: P0 for slot 0, P1 for slot 1, one for the others
+    constexpr int PM = P0 > P1 ? P0 : P1;
+    auto ncomp = [](int gi) { return gi == 0 ? P0 : (gi == 1 ? P1 : 1); };
+    uint8_t prow[GW][PM];  // the loaded type: no extension on the loop edge
+    float pval[GW][PM];
+    auto issue_slot = [&](int gi, const TileRs &rs, uint32_t lo_) {
+#pragma unroll
+        for (int c = 0; c < PM; ++c)
+            if (c < ncomp(gi)) issue(rs, lo_, c, prow[gi][c], pval[gi][c]);
+    };
+    auto slot_bounds = [&](auto gic, uint32_t wl_, uint32_t wh_, uint32_t &lo_, uint32_t &len_) {
+        constexpr int GI = decltype(gic)::value;
+        lo_ = row_bcast<GI>(wl_);
+        len_ = row_bcast<GI>(wh_) - lo_;
+    };
+
+    // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
+    // with r mod nch == gc (wave w takes the slice's rows w, w + 16, ...)
+    const bool do_qsum = qsum_part != nullptr;
+    double qsum[QC];
+#pragma unroll
+    for (int e = 0; e < QC; ++e) qsum[e] = 0.0;
+    // run bounds of this tile (cl, ch), the next (nl, nh) and the one after (wl, wh)
+    uint32_t cl = 0u, ch = 0u, nl = 0u, nh = 0u, wl = 0u, wh = 0u;
+    int64_t tb = 0;
+    if (t_begin < t_end) {
+        bounds(t_begin, cl, ch);
+        tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
+        load_q(t_begin, 0);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");  // the first Q tile, once per block
+        const TileRs rs0 = tile_rs(tb);
+        static_for<0, GW>([&](auto ic) {
+            uint32_t lo_, len_;
+            slot_bounds(ic, cl, ch, lo_, len_);
+            issue_slot(decltype(ic)::value, rs0, lo_);
+        });
+        const int tn = t_begin + 1 < t_end ? t_begin + 1 : t_begin;
+        bounds(tn, nl, nh);
+    }
+    for (int t = t_begin; t < t_end; ++t) {
+        const int bi = (t - t_begin) & 1;
+        const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
+        const int tnn = tn + 1 < t_end ? tn + 1 : tn;
+        const char *qb = reinterpret_cast<const char *>(qs[bi]);
+        // this tile's Q loads were followed by at least kAfter entry loads:
+        // with at most kAfter loads outstanding they have landed in LDS
+        asm volatile("s_waitcnt vmcnt(%0)" ::"i"(kAfter) : "memory");
+        __syncthreads();  // every wave's share of tile t is in qs[bi]; tile t - 1 (qs[bi ^ 1]) is done
+        bounds(tnn, wl, wh);
+        if (do_qsum) {  // rows gc + nch w + nch NW i, four reads in flight, summed in row order
+            const float *qf = reinterpret_cast<const float *>(qb);
+            const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
+            const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
+            const int step = nch * NW;
+            for (int rr = gc + nch * wave; rr < nr; rr += 4 * step) {
+                float v[4][QC];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int e = 0; e < QC; ++e) {
+                        const int r = rr + u * step;
+                        v[u][e] = qf[(r < nr ? r : 0) * KP + lane + kWave * e];
+                    }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int e = 0; e < QC; ++e)
+                        if (rr + u * step < nr) qsum[e] += static_cast<double>(v[u][e]);
+            }
+        }
+        const uint32_t tot = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(ch), GW));
+        const int64_t ntb = tn != t ? tb + tot : tb;
+        const TileRs rs = tile_rs(tb), nrs = tile_rs(ntb);
+        // longest run of each slot over the four groups (scalars, by readlane)
+        const int rl_len = static_cast<int>(ch - cl);
+        int smx[GW];
+        int total = 0, done = 0;  // gather steps of the tile
+#pragma unroll
+        for (int gi = 0; gi < GW; ++gi) {
+            const int a0 = __builtin_amdgcn_readlane(rl_len, gi), a1 = __builtin_amdgcn_readlane(rl_len, 16 + gi);
+            const int a2 = __builtin_amdgcn_readlane(rl_len, 32 + gi), a3 = __builtin_amdgcn_readlane(rl_len, 48 + gi);
+            const int m01 = a0 > a1 ? a0 : a1, m23 = a2 > a3 ? a2 : a3;
+            smx[gi] = m01 > m23 ? m01 : m23;
+            total += smx[gi];
+        }
+        // slots from the sparsest to the densest: each slot's prefetch was
+        // issued a whole tile earlier, and the next Q tile goes out at the
+        // densest slot, whose long gathers cover its latency
+        static_for<0, GW>([&](auto ic) {
+            constexpr int gi = GW - 1 - decltype(ic)::value;
+            constexpr int NC = gi == 0 ? P0 : (gi == 1 ? P1 : 1);
+            const std::integral_constant<int, gi> gic{};
+            uint32_t lo_, len_, nlo_, nlen_;
+            slot_bounds(gic, cl, ch, lo_, len_);
+            slot_bounds(gic, nl, nh, nlo_, nlen_);
+            const int sm = smx[gi];
+            progress_priority(done, total);
+            uint32_t ofs[NC];
+            float x[NC];
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                prep(prow[gi][c], pval[gi][c], len_, 16u * c, ofs[c], x[c]);
+                pin_regs(ofs[c], x[c]);  // read before the same registers are reloaded
+            }
+            if (gi == 0 && tn != t) load_q(tn, bi ^ 1);
+            issue_slot(gi, nrs, nlo_);
+            // the ring: entries from 16 NC on, 16 RC per group per chunk
+            uint8_t rr[RC];
+            float rv[RC];
+            if (sm > 16 * NC) {
+#pragma unroll
+                for (int c = 0; c < RC; ++c) issue(rs, lo_, NC + c, rr[c], rv[c]);
+            }
+            if (sm > 0) gather_run<E, NC>(z[gi], ofs, x, sm < 16 * NC ? sm : 16 * NC, qb, lane_ofs);
+            done += sm < 16 * NC ? sm : 16 * NC;
+            for (int o = 16 * NC; o < sm; o += 16 * RC) {
+                progress_priority(done, total);
+                done += sm - o < 16 * RC ? sm - o : 16 * RC;
+                uint32_t ofr[RC];
+                float xr[RC];
+#pragma unroll
+                for (int c = 0; c < RC; ++c) {
+                    prep(rr[c], rv[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
+                    pin_regs(ofr[c], xr[c]);
+                }
+                if (o + 16 * RC < sm) {
+#pragma unroll
+                    for (int c = 0; c < RC; ++c) issue(rs, lo_ + static_cast<uint32_t>(o + 16 * RC), c, rr[c], rv[c]);
+                }
+                const int left = sm - o;
+                gather_run<E, RC>(z[gi], ofr, xr, left < 16 * RC ? left : 16 * RC, qb, lane_ofs);
+            }
+        });
+        cl = nl;
+        ch = nh;
+        nl = wl;
+        nh = wh;
+        tb = ntb;
+    }
+    // every group writes its own genes' rows (16 lanes x E columns)
+    static_for<0, GW>([&](auto ic) {
+        constexpr int I = decltype(ic)::value;
+        const uint32_t c = row_bcast<I>(gene);
+        if (tr < ntr && c < static_cast<uint64_t>(n_vars)) {
+            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
+#pragma unroll
+            for (int e = 0; e < E / 4; ++e)
+                dst[e] = make_float4(z[I][4 * e], z[I][4 * e + 1], z[I][4 * e + 2], z[I][4 * e + 3]);
+        }
+    });
+    if (do_qsum) {  // the waves' column sums, added in wave order
+        __syncthreads();
+        double *red = reinterpret_cast<double *>(qs[0]);  // NW x KP f64 fits in a Q tile buffer
 #pragma unroll
         for (int e = 0; e < QC; ++e) red[wave * KP + lane + kWave * e] = qsum[e];
         __syncthreads();

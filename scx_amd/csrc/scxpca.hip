@@ -34,6 +34,7 @@
 #include <cstring>
 #include <memory>
 #include <mutex>
+#include <queue>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -427,28 +428,42 @@ void throw_invalid(const scxpca_matrix *m, int h, int64_t row_base = 0, int64_t 
     }
 }
 
-// Gene order for the transposed SpMM: genes sorted by entry count (descending,
-// ties by index) are dealt to the nch*tw waves in snake order (0..W-1, W-1..0,
-// ...), round r filling slot r of each wave, so per-wave entry totals differ by
-// at most about one gene's count. Unused slots hold n_vars.
-// The dealt order into h[0, max(S, 1)), S = nch * tw * gw slots.
-void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, uint32_t *h) {
+// Gene order for the transposed SpMM. Genes sorted by entry count
+// (descending, ties by index) are cut into units of `quad` consecutive genes
+// (quad = 4: the four 16-lane groups of a k_spmm_t7 wave, which run one gene
+// each in lockstep, so a unit's genes have neighbouring counts; quad = 1: a
+// k_spmm_t6 wave's gene). Units go, largest first (cost: the unit's largest
+// count), to the wave with the least total so far among the nch * tw / quad
+// waves with a free slot (ties: the lowest wave), a wave's k-th unit filling
+// its slot k: per-wave totals, and so the blocks' totals, come out within
+// about one unit of each other, and slot 0 holds each wave's densest genes.
+// Owner o = wave * quad + group takes h[o * gw + slot]; unused slots hold
+// n_vars. The dealt order into h[0, max(S, 1)), S = nch * tw * gw slots.
+void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, int quad, uint32_t *h) {
     const int64_t nv = m->n_vars;
-    const int64_t W = static_cast<int64_t>(nch) * tw, S = W * gw;
-    // genes by entry count, descending, ties by gene index: one 64-bit key
-    // each (a matrix shell that was never prepared has no counts: all zero)
+    const int64_t S = static_cast<int64_t>(nch) * tw * gw;
+    const int64_t waves = static_cast<int64_t>(nch) * tw / quad;
     std::vector<uint64_t> key(nv);
     const int64_t nc = static_cast<int64_t>(m->host_gcnt.size());
-    for (int64_t c = 0; c < nv; ++c) {
-        const uint32_t cnt = c < nc ? m->host_gcnt[c] : 0u;
-        key[c] = (static_cast<uint64_t>(~cnt) << 32) | static_cast<uint64_t>(c);
-    }
+    auto count = [&](int64_t c) { return c < nc ? m->host_gcnt[c] : 0u; };
+    for (int64_t c = 0; c < nv; ++c)
+        key[c] = (static_cast<uint64_t>(~count(c)) << 32) | static_cast<uint64_t>(c);
     std::sort(key.begin(), key.end());
     for (int64_t i = 0; i < std::max<int64_t>(S, 1); ++i) h[i] = static_cast<uint32_t>(nv);
-    for (int64_t i = 0; i < nv; ++i) {
-        const int64_t r = i / W, pos = i % W;
-        const int64_t w = (r % 2 == 0) ? pos : W - 1 - pos;
-        h[w * gw + r] = static_cast<uint32_t>(key[i]);
+    // min-heap of (load, wave): waves with a free slot
+    using Item = std::pair<uint64_t, int64_t>;
+    std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+    for (int64_t w = 0; w < waves; ++w) heap.push({0u, w});
+    std::vector<int> used(static_cast<size_t>(waves), 0);
+    for (int64_t u = 0; u * quad < nv && !heap.empty(); ++u) {
+        const Item top = heap.top();
+        heap.pop();
+        const int64_t w = top.second;
+        const int slot = used[static_cast<size_t>(w)]++;
+        const uint32_t first = static_cast<uint32_t>(key[u * quad]);
+        for (int g = 0; g < quad && u * quad + g < nv; ++g)
+            h[(w * quad + g) * gw + slot] = static_cast<uint32_t>(key[u * quad + g]);
+        if (used[static_cast<size_t>(w)] < gw) heap.push({top.first + count(first), w});
     }
 }
 
@@ -457,13 +472,13 @@ bool gene_order_valid(const scxpca_matrix *m, int gw, int nch, int tw) {
     return go.valid && go.gw == gw && go.nch == nch && go.tw == tw;
 }
 
-const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
+const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw, int quad) {
     auto &go = m->gorder[gw == 8 ? 0 : 1];
     if (gene_order_valid(m, gw, nch, tw)) return go.dev.as<uint32_t>();
     const int64_t S = static_cast<int64_t>(nch) * tw * gw;
     // the previous upload from go.host (if any) was drained by prepare's sync
     go.host.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
-    deal_gene_order(m, gw, nch, tw, go.host.as<uint32_t>());
+    deal_gene_order(m, gw, nch, tw, quad, go.host.as<uint32_t>());
     uint32_t *h = go.host.as<uint32_t>();
     go.dev.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
     HIPC(hipMemcpyAsync(go.dev.p, h, sizeof(uint32_t) * std::max<int64_t>(S, 1), hipMemcpyHostToDevice, m->stream));
@@ -691,10 +706,17 @@ struct Engine {
     // Cholesky: the f32-Gram tall passes, the f64 Rayleigh-Ritz passes
     static constexpr double kStableFloor = 1e-6;
     static constexpr double kSmallFloor = 1e-13;
-    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per wave in k_spmm_t6 (Z in registers)
-    static constexpr int NB0 = (KP == 64) ? 4 : 1;  // slot-0 prefetch batches in k_spmm_t6
-    static constexpr int TW = 16;                   // waves per k_spmm_t6 block
-    static constexpr int CG = TW * GW;              // genes per k_spmm_t6 block
+    // transposed SpMM: k_spmm_t7 (genes owned per 16-lane group) at KP = 64,
+    // k_spmm_t6 (genes owned per wave) at KP = 128
+    static constexpr bool kT7 = KP == 64;
+    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per owner (Z in registers)
+    static constexpr int P0 = 4;                    // k_spmm_t7: slot-0 prefetch, 16-entry components per group
+    static constexpr int P1 = 2;                    // k_spmm_t7: slot-1 prefetch
+    static constexpr int RC = 2;                    // k_spmm_t7: 16-entry components per ring load
+    static constexpr int NB0 = 1;                   // k_spmm_t6: slot-0 prefetch batches
+    static constexpr int TW = 16;                   // waves per block
+    static constexpr int TU = kT7 ? 4 * TW : TW;    // gene owners per block
+    static constexpr int CG = TU * GW;              // genes per block
     static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
     static constexpr int FPF = 4;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
@@ -724,7 +746,7 @@ struct Engine {
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         t_geometry(m, nch_t, nrb_t, tiles_per_tr);
-        gperm = ensure_gene_order(m, GW, nch_t, TW);
+        gperm = ensure_gene_order(m, GW, nch_t, TU, TU / TW);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
@@ -881,10 +903,14 @@ struct Engine {
         if (mm->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb * nch);
             StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T);
-            k_spmm_t6<KP, GW, NB0, TW><<<grid, TW * 64, 0, st>>>(mm->indptr, mm->tptr.as<uint32_t>(),
-                                                                 mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                                                                 Q, gp, nch, nrb, tpr, mm->n_obs, nv,
-                                                                 mm->n_tiles, sc.tpart, qsp);
+            if constexpr (kT7)
+                k_spmm_t7<KP, GW, P0, P1, RC, TW><<<grid, TW * 64, 0, st>>>(
+                    mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(),
+                    mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
+            else
+                k_spmm_t6<KP, GW, NB0, TW><<<grid, TW * 64, 0, st>>>(
+                    mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(), Q, gp, nch,
+                    nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
             HIPC(hipGetLastError());
         } else {  // no rows: zero partials (a fill kernel, no memset node)
             k_zero_words<<<blocks_for(static_cast<int64_t>(nrb) * nv * KP), 256, 0, st>>>(
@@ -1900,8 +1926,8 @@ void project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, cons
     int nch = 0, nrb = 0, tpr = 0;
     E::t_geometry(m, nch, nrb, tpr);
     const bool pack = k != KP;
-    const bool have_order = gene_order_valid(m, E::GW, nch, E::TW);
-    const int64_t slots = std::max<int64_t>(static_cast<int64_t>(nch) * E::TW * E::GW, 1);
+    const bool have_order = gene_order_valid(m, E::GW, nch, E::TU);
+    const int64_t slots = std::max<int64_t>(static_cast<int64_t>(nch) * E::TU * E::GW, 1);
     size_t off = 0;
     auto take = [&](size_t bytes) {
         const size_t o = off;
@@ -1927,7 +1953,7 @@ void project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, cons
         gp = m->gorder[E::GW == 8 ? 0 : 1].dev.template as<uint32_t>();
     } else {  // dealt on the host, staged through pageable memory (no allocation on the device)
         std::vector<uint32_t> h(slots);
-        deal_gene_order(m, E::GW, nch, E::TW, h.data());
+        deal_gene_order(m, E::GW, nch, E::TU, E::TU / E::TW, h.data());
         HIPC(hipMemcpyAsync(base + o_perm, h.data(), sizeof(uint32_t) * slots, hipMemcpyHostToDevice, s));
         HIPC(hipStreamSynchronize(s));
         gp = reinterpret_cast<const uint32_t *>(base + o_perm);
