@@ -2414,6 +2414,238 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
+// Forward SpMM with rows owned per 16-lane group (cpu.rs:1185-1229,1471-1523):
+// Y[r][:] = sum x~ M[c][:] - mc (+ b_r cs for a PFlog matrix).
+//
+// k_spmm_fwd3 has the four groups of a wave share each row and sums their
+// partials at the end, so a 16-wave block covers 256 rows and stages all of
+// M (four 128 KB gene chunks) once per 256 rows: 2 KB of M per row, more
+// than the row's own entries. Here each group owns R rows (lane 16g+t holds
+// entry t of the group's current window; the row broadcast hands every
+// group its own entry), so a wave covers 4R = 64 rows and a block 1024: M is
+// staged once per 1024 rows. A wave sorts its 64 rows by length at the start
+// and deals them so that the four rows of a slot have neighbouring lengths
+// (the slot runs for the longest of its four windows). Per 512-gene chunk the
+// M rows are copied into LDS by direct global -> LDS loads between two block
+// barriers; each row's window of the chunk (split offsets) is gathered in
+// 16-entry components: the first PC components of a slot's four windows are
+// loaded PF slots ahead (across chunk boundaries), longer windows stream the
+// rest through a ring. Entry loads go through buffer descriptors based at the
+// wave's first entry (32-bit lane offsets, range-checked).
+// ---------------------------------------------------------------------------
+template <int KP, int R, int NW, int CG, int PF, int PC, int RC, bool OFFSET = false>
+__global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict__ indptr,
+                                                       const uint16_t *__restrict__ lg,
+                                                       const float *__restrict__ xt,
+                                                       const float *__restrict__ M, const float *__restrict__ mc,
+                                                       float *__restrict__ Y, const uint32_t *__restrict__ spl,
+                                                       int nsplit, int cpc, int nch, int64_t n_obs,
+                                                       int64_t n_vars, int64_t nnz,
+                                                       const double *__restrict__ rowb,
+                                                       const float *__restrict__ cs) {
+    constexpr int E = KP / 16;
+    constexpr int NT = NW * kWave;
+    constexpr int TM = CG * KP / 4;  // float4s per M chunk
+    constexpr int MPT = TM / NT;     // direct loads per lane per chunk
+    static_assert(R == 16, "one row per lane: a wave's 64 rows are sorted across its lanes");
+    static_assert(R % PF == 0 && PF >= 1, "ring slots are static: slot = r % PF in every chunk");
+    static_assert(PC >= 1 && PC <= 4 && RC >= 1 && RC <= 4, "16 to 64 entries per group and load");
+    static_assert((CG & (CG - 1)) == 0 && CG <= static_cast<int>(kChunkMod), "lg holds gene mod kChunkMod");
+    static_assert(TM % NT == 0, "whole float4s per lane");
+    __shared__ float4 ms[TM];
+    const int tid = threadIdx.x;
+    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
+    const int lane = tid & 63, g = lane >> 4, l = lane & 15;
+    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
+    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * NW + wave) * (4 * R);
+    float acc[R][E];
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+#pragma unroll
+        for (int e = 0; e < E; ++e) acc[r][e] = 0.f;
+
+    // --- the wave's rows by length, descending (ties: row order); rank k
+    // goes to slot k / 4 of group k % 4, i.e. lane (g, s) takes rank 4 s + g
+    uint32_t key;
+    {
+        const int64_t rw = row0 + lane;
+        const int64_t rc = rw < n_obs ? rw : n_obs - 1;
+        const uint32_t len = rw < n_obs ? static_cast<uint32_t>(indptr[rc + 1] - indptr[rc]) : 0u;
+        key = (len << 6) | static_cast<uint32_t>(63 - lane);  // len < 2^26
+#pragma unroll
+        for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(key), j));
+                const bool desc = (lane & k) == 0;  // descending runs, so the whole sequence ends descending
+                const bool lower = (lane & j) == 0;
+                const uint32_t hi = key > other ? key : other, lo = key > other ? other : key;
+                key = (desc == lower) ? hi : lo;
+            }
+    }
+    // the lane's row: row0 + rid (rid < 64); its first entry relative to the wave's
+    const uint32_t rid = 63u - (static_cast<uint32_t>(__shfl(static_cast<int>(key), 4 * l + g)) & 63u);
+    const int64_t wbase = indptr[row0 < n_obs ? row0 : n_obs];  // the wave's first entry (scalar)
+    uint32_t rrel;  // ~0u: no row
+    {
+        const int64_t row = row0 + rid;
+        rrel = row < n_obs ? static_cast<uint32_t>(indptr[row] - wbase) : ~0u;
+    }
+    auto bounds = [&](int ch, uint32_t &a, uint32_t &b) {  // the row's window of chunk ch, wave-relative
+        const int s_lo = ch * cpc, s_hi = (ch + 1) * cpc < nsplit ? (ch + 1) * cpc : nsplit;
+        const bool ok = rrel != ~0u;
+        const uint32_t *splr = spl + (ok ? row0 + rid : 0) * (nsplit + 1);
+        const uint32_t a0 = splr[s_lo], b0 = splr[s_hi];
+        a = ok ? rrel + a0 : 0u;
+        b = ok ? rrel + b0 : 0u;
+    };
+    // buffer descriptors of lg / xt from the wave's first entry
+    const int64_t left = nnz - wbase;
+    const int64_t lb = 2 * left < 0x7fffffff ? 2 * left : 0x7fffffff;
+    const int64_t vb = 4 * left < 0x7fffffff ? 4 * left : 0x7fffffff;
+    const __amdgpu_buffer_rsrc_t rs_lg =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(lg + wbase), 0, static_cast<int>(lb), kBufferRsrcWord3);
+    const __amdgpu_buffer_rsrc_t rs_xt =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(xt + wbase), 0, static_cast<int>(vb), kBufferRsrcWord3);
+    auto issue = [&](uint32_t lo_, int c, uint16_t &pi, float &pv) {
+        const uint32_t o = lo_ + static_cast<uint32_t>(l);
+        pi = __builtin_amdgcn_raw_buffer_load_b16(rs_lg, static_cast<int>(2 * o + 32 * c), 0, 0);
+        pv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_xt, static_cast<int>(4 * o + 64 * c), 0, 0));
+    };
+    auto prep = [&](uint16_t pi, float pv, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
+        const bool ok = j0 + static_cast<uint32_t>(l) < len_;
+        ofs = ok ? (static_cast<uint32_t>(pi) & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+        x = ok ? pv : 0.f;
+    };
+    auto win = [&](auto rc_, uint32_t a, uint32_t b, uint32_t &lo_, uint32_t &len_) {
+        constexpr int RR = decltype(rc_)::value;
+        lo_ = row_bcast<RR>(a);
+        len_ = row_bcast<RR>(b) - lo_;
+    };
+    // the ring: PC components of the four windows of a slot, PF slots ahead
+    uint16_t pidx[PF][PC];  // the loaded type: no extension on the loop edge
+    float pval[PF][PC];
+    auto issue_win = [&](int slot, uint32_t lo_) {
+#pragma unroll
+        for (int c = 0; c < PC; ++c) issue(lo_, c, pidx[slot][c], pval[slot][c]);
+    };
+    const float4 *M4 = reinterpret_cast<const float4 *>(M);
+    const int64_t mlim = n_vars * (KP / 4);
+    auto load_m = [&](int ch) {  // wave w's load j fills float4s (j NW + w) * 64 + lane of the chunk
+        const int64_t m0 = static_cast<int64_t>(ch) * TM;
+#pragma unroll
+        for (int j = 0; j < MPT; ++j) {
+            const int i = (j * NW + wave) * kWave;
+            const int64_t src = m0 + i + lane;
+            const float4 *gp = M4 + (src < mlim ? src : 0);  // genes past n_vars: never gathered
+            const uint32_t dst = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&ms[i]));
+            asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(gp), "{m0}"(dst) : "memory");
+        }
+    };
+
+    uint32_t ca, cb, na = 0u, nb = 0u;  // windows of this chunk, of the next
+    bounds(0, ca, cb);
+    static_for<0, PF>([&](auto ic) {
+        uint32_t lo_, len_;
+        win(ic, ca, cb, lo_, len_);
+        issue_win(decltype(ic)::value, lo_);
+    });
+    for (int ch = 0; ch < nch; ++ch) {
+        bounds(ch + 1 < nch ? ch + 1 : ch, na, nb);
+        __syncthreads();  // every wave is done with the previous chunk's M rows
+        load_m(ch);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();  // the chunk's M rows are in LDS
+        const char *mb = reinterpret_cast<const char *>(ms);
+        // longest window of each slot over the four groups (lane l: slot l)
+        uint32_t mx = cb - ca;
+        {
+            const uint32_t m16 = static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), 16));
+            mx = m16 > mx ? m16 : mx;
+            const uint32_t m32 = static_cast<uint32_t>(__shfl_xor(static_cast<int>(mx), 32));
+            mx = m32 > mx ? m32 : mx;
+        }
+        int total = 0, done = 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) total += __builtin_amdgcn_readlane(static_cast<int>(mx), r);
+        static_for<0, R>([&](auto ic) {
+            constexpr int r = decltype(ic)::value;
+            uint32_t lo_, len_;
+            win(ic, ca, cb, lo_, len_);
+            const int sm = __builtin_amdgcn_readlane(static_cast<int>(mx), r);
+            progress_priority(done, total);
+            uint32_t ofs[PC];
+            float x[PC];
+#pragma unroll
+            for (int c = 0; c < PC; ++c) {
+                prep(pidx[r % PF][c], pval[r % PF][c], len_, 16u * c, ofs[c], x[c]);
+                pin_regs(ofs[c], x[c]);  // read before the same registers are reloaded
+            }
+            {  // the ring slot's next window: slot r + PF of this chunk, or of the next
+                uint32_t lo2, len2;
+                if constexpr (r + PF < R) {
+                    win(std::integral_constant<int, r + PF>{}, ca, cb, lo2, len2);
+                } else {
+                    win(std::integral_constant<int, r + PF - R>{}, na, nb, lo2, len2);
+                }
+                issue_win(r % PF, lo2);
+            }
+            uint16_t ri[RC];
+            float rv[RC];
+            if (sm > 16 * PC) {
+#pragma unroll
+                for (int c = 0; c < RC; ++c) issue(lo_ + 16 * PC, c, ri[c], rv[c]);
+            }
+            if (sm > 0) gather_run<E, PC>(acc[r], ofs, x, sm < 16 * PC ? sm : 16 * PC, mb, lane_ofs);
+            done += sm < 16 * PC ? sm : 16 * PC;
+            for (int o = 16 * PC; o < sm; o += 16 * RC) {
+                progress_priority(done, total);
+                done += sm - o < 16 * RC ? sm - o : 16 * RC;
+                uint32_t ofr[RC];
+                float xr[RC];
+#pragma unroll
+                for (int c = 0; c < RC; ++c) {
+                    prep(ri[c], rv[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
+                    pin_regs(ofr[c], xr[c]);
+                }
+                if (o + 16 * RC < sm) {
+#pragma unroll
+                    for (int c = 0; c < RC; ++c) issue(lo_ + static_cast<uint32_t>(o + 16 * RC), c, ri[c], rv[c]);
+                }
+                const int lft = sm - o;
+                gather_run<E, RC>(acc[r], ofr, xr, lft < 16 * RC ? lft : 16 * RC, mb, lane_ofs);
+            }
+        });
+        ca = na;
+        cb = nb;
+    }
+    // every group writes its own rows (16 lanes x E columns)
+    static_for<0, R>([&](auto ic) {
+        constexpr int r = decltype(ic)::value;
+        const int64_t row = row0 + row_bcast<r>(rid);
+        if (row < n_obs) {
+            float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
+            const float br = OFFSET ? static_cast<float>(rowb[row]) : 0.f;
+#pragma unroll
+            for (int e = 0; e < E / 4; ++e) {
+                float4 o;
+                o.x = acc[r][4 * e + 0] - (mc ? mc[l * E + 4 * e + 0] : 0.f);
+                o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
+                o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
+                o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
+                if constexpr (OFFSET) {
+                    o.x += br * cs[l * E + 4 * e + 0];
+                    o.y += br * cs[l * E + 4 * e + 1];
+                    o.z += br * cs[l * E + 4 * e + 2];
+                    o.w += br * cs[l * E + 4 * e + 3];
+                }
+                yr[e] = o;
+            }
+        }
+    });
+}
+
+// ---------------------------------------------------------------------------
 // Covariance route (cpu.rs:1663-1854): G = X~^T X~ from the tile-CSC.
 // Block (pair p = (I, J), I <= J, of 64-gene blocks; tile range tr) has 16
 // waves. Per 256-row tile the slices X~[tile, I] and X~[tile, J] are

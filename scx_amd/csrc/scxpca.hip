@@ -717,6 +717,12 @@ struct Engine {
     static constexpr int TW = 16;                   // waves per block
     static constexpr int TU = kT7 ? 4 * TW : TW;    // gene owners per block
     static constexpr int CG = TU * GW;              // genes per block
+    // forward SpMM: k_spmm_fwd4 (rows owned per 16-lane group) at KP = 64,
+    // k_spmm_fwd3 (rows owned per wave) at KP = 128
+    static constexpr bool kF4 = KP == 64;
+    static constexpr int F4PF = 2;                  // k_spmm_fwd4: ring depth, slots
+    static constexpr int F4PC = 2;                  // k_spmm_fwd4: prefetched 16-entry components per window
+    static constexpr int F4RC = 1;                  // k_spmm_fwd4: components per overflow load
     static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
     static constexpr int FPF = 4;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
@@ -845,10 +851,17 @@ struct Engine {
             constexpr int NWF = 16;
             constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
             const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
-            const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
-            k_spmm_fwd3<KP, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
-                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y,
-                m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
+            if constexpr (kF4) {  // rows owned per 16-lane group: 64 rows per wave
+                const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * 64 - 1) / (NWF * 64));
+                k_spmm_fwd4<KP, 16, NWF, CGF, F4PF, F4PC, F4RC, OFF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y, m->spl.as<uint32_t>(),
+                    m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
+            } else {
+                const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
+                k_spmm_fwd3<KP, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
+                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y,
+                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
+            }
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
             k_spmm_fwd<KP, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), panel,
