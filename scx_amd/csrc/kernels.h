@@ -1080,21 +1080,15 @@ __global__ void k_f64_to_f32(const double *__restrict__ s, float *__restrict__ d
 
 // Lane t of each 16-lane row broadcast to the whole row (DPP row_newbcast, gfx90a+).
 
-// Group-gather loop. The four 16-lane groups take the entries held by lane T
-// of their row (entries 4T+g of the batch, by the interleaved load), read E
-// floats of an LDS panel row at byte offset ofs + lane_ofs and accumulate
-// x * panel. The broadcast of lane T (DPP row_newbcast) is folded into the
-// consumers: one v_add_u32_dpp forms the LDS address and each FMA takes the
-// broadcast x as its DPP source, so a step costs E/4 LDS reads, E FMAs and
-// one add. The s_nop 1 in each block covers the VALU-write -> DPP-read hazard
-// (two wait states) for operands the compiler may have just moved. The LDS
-// reads run two steps ahead of the FMAs (three rotating buffers), and the step
-// count is wave-uniform so the exits are scalar branches. Slots past cnt carry
-// x = 0 and offset 0 (a valid address), so the look-ahead reads are
-// unconditional: up to two reads past the last step. A scheduling barrier
-// keeps each look-ahead read ahead of the current step's FMAs (otherwise the
-// scheduler sinks it next to its use to save registers and every step waits
-// for its own read).
+// Gather step primitives. Each 16-lane group holds a batch of its own
+// entries, entry T in lane T (an LDS byte offset of the panel row, and x~).
+// Step T of a batch: one v_add_u32_dpp (row_newbcast:T) forms every lane's
+// LDS address (lane T's row offset + the lane's column offset), ds_read_b128
+// loads E/4 x 4 floats of the panel row, and each v_fmac_f32_dpp takes the
+// broadcast x~ of lane T as its DPP source, so a step costs E/4 LDS reads,
+// E FMAs and one add for the four groups' four entries. The s_nop 1 in each
+// block covers the VALU-write -> DPP-read hazard (two wait states) for
+// operands the compiler may have just moved.
 template <int T>
 __device__ __forceinline__ uint32_t bcast_add(uint32_t ofs, uint32_t lane_ofs) {
     uint32_t a;
@@ -1128,257 +1122,6 @@ __device__ __forceinline__ void gather_fma(float (&acc)[E], const float4 (&m)[E 
     for (int e = 0; e < E / 4; ++e)
         bcast_fma4<T>(acc[4 * e + 0], acc[4 * e + 1], acc[4 * e + 2], acc[4 * e + 3], x, m[e]);
 }
-
-template <int E>
-__device__ __forceinline__ void gather_batch(float (&acc)[E], uint32_t ofs, float x, int cnt, const char *lds,
-                                             uint32_t lane_ofs) {
-    const int steps = __builtin_amdgcn_readfirstlane((cnt + 3) >> 2);
-    float4 b0[E / 4], b1[E / 4], b2[E / 4];
-    gather_load<0, E>(b0, ofs, lds, lane_ofs);
-    gather_load<1, E>(b1, ofs, lds, lane_ofs);
-#define SCX_STEP(T, CUR, NXT)                                                  \
-    if ((T) >= steps) return;                                                  \
-    if ((T) + 2 < 16) gather_load<((T) + 2) & 15, E>(NXT, ofs, lds, lane_ofs); \
-    __builtin_amdgcn_sched_barrier(0);                                         \
-    gather_fma<T, E>(acc, CUR, x);
-    SCX_STEP(0, b0, b2) SCX_STEP(1, b1, b0) SCX_STEP(2, b2, b1) SCX_STEP(3, b0, b2)
-    SCX_STEP(4, b1, b0) SCX_STEP(5, b2, b1) SCX_STEP(6, b0, b2) SCX_STEP(7, b1, b0)
-    SCX_STEP(8, b2, b1) SCX_STEP(9, b0, b2) SCX_STEP(10, b1, b0) SCX_STEP(11, b2, b1)
-    SCX_STEP(12, b0, b2) SCX_STEP(13, b1, b0) SCX_STEP(14, b2, b1) SCX_STEP(15, b0, b2)
-#undef SCX_STEP
-}
-
-// Wave priority for the rest of a block-synchronized segment (a tile, a gene
-// chunk), by the quarter of the wave's work in it already done: waves that
-// are behind issue first. A SIMD otherwise arbitrates among its waves
-// oldest-first, so the youngest wave of each SIMD ends its segment last
-// (measured 1.26x the mean time in the transposed product) while the others
-// wait at the barrier. `quarter` is a constant after unrolling.
-__device__ __forceinline__ void segment_priority(int quarter) {
-    if (quarter <= 0) __builtin_amdgcn_s_setprio(3);
-    else if (quarter == 1) __builtin_amdgcn_s_setprio(2);
-    else if (quarter == 2) __builtin_amdgcn_s_setprio(1);
-    else __builtin_amdgcn_s_setprio(0);
-}
-
-// Two batches at once (two accumulators, one step count: the longer batch's;
-// the shorter one's slots past its count carry x = 0 and offset 0): per step
-// two LDS reads one step ahead and 2E FMAs, so one read latency at the start
-// serves both batches.
-template <int E>
-__device__ __forceinline__ void gather_batch2(float (&accA)[E], uint32_t ofsA, float xA, float (&accB)[E],
-                                              uint32_t ofsB, float xB, int cnt, const char *lds,
-                                              uint32_t lane_ofs) {
-    const int steps = __builtin_amdgcn_readfirstlane((cnt + 3) >> 2);
-    float4 a0[E / 4], a1[E / 4], b0[E / 4], b1[E / 4];
-    gather_load<0, E>(a0, ofsA, lds, lane_ofs);
-    gather_load<0, E>(b0, ofsB, lds, lane_ofs);
-#define SCX_STEP2(T, CA, NA, CB, NB)                                   \
-    if ((T) >= steps) return;                                          \
-    if ((T) + 1 < 16) {                                                \
-        gather_load<((T) + 1) & 15, E>(NA, ofsA, lds, lane_ofs);       \
-        gather_load<((T) + 1) & 15, E>(NB, ofsB, lds, lane_ofs);       \
-    }                                                                  \
-    __builtin_amdgcn_sched_barrier(0);                                 \
-    gather_fma<T, E>(accA, CA, xA);                                    \
-    gather_fma<T, E>(accB, CB, xB);
-    SCX_STEP2(0, a0, a1, b0, b1) SCX_STEP2(1, a1, a0, b1, b0) SCX_STEP2(2, a0, a1, b0, b1)
-    SCX_STEP2(3, a1, a0, b1, b0) SCX_STEP2(4, a0, a1, b0, b1) SCX_STEP2(5, a1, a0, b1, b0)
-    SCX_STEP2(6, a0, a1, b0, b1) SCX_STEP2(7, a1, a0, b1, b0) SCX_STEP2(8, a0, a1, b0, b1)
-    SCX_STEP2(9, a1, a0, b1, b0) SCX_STEP2(10, a0, a1, b0, b1) SCX_STEP2(11, a1, a0, b1, b0)
-    SCX_STEP2(12, a0, a1, b0, b1) SCX_STEP2(13, a1, a0, b1, b0) SCX_STEP2(14, a0, a1, b0, b1)
-    SCX_STEP2(15, a1, a0, b1, b0)
-#undef SCX_STEP2
-}
-
-// ---------------------------------------------------------------------------
-// Forward SpMM, gene-chunked panel in LDS: Y[r][:] = sum x~ M[c][:] - mc.
-// A block (NW waves) owns NW*R consecutive rows; each wave keeps R rows'
-// partial outputs in registers (per 16-lane group, E columns per lane). For
-// each chunk of CG genes the block stages M[chunk] (CG*KP*4 = 128 KB) into
-// LDS, then every wave walks its rows' windows of that chunk (precomputed
-// split offsets), 64 nonzeros per batch: lane-parallel load + transform, then
-// broadcast steps that feed the four groups with entries 4t+g. The first batch
-// of each (row, chunk) window is loaded PF rows ahead, across chunk
-// boundaries, so HBM latency overlaps the gather loop. A wave lowers its
-// priority as it gets through its rows (segment_priority). After the last
-// chunk the four group partials of a row are summed and the row is written
-// once.
-// ---------------------------------------------------------------------------
-
-template <int KP, int R, int NW, int CG, int PF, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j]
-__global__ void __launch_bounds__(NW * 64) k_spmm_fwd3(const int64_t *__restrict__ indptr,
-                                                       const uint16_t *__restrict__ lg,
-                                                       const float *__restrict__ xt,
-                                                       const float *__restrict__ M, const float *__restrict__ mc,
-                                                       float *__restrict__ Y, const uint32_t *__restrict__ spl,
-                                                       int nsplit, int cpc, int nch, int64_t n_obs,
-                                                       int64_t n_vars, int64_t nnz,
-                                                       const double *__restrict__ rowb,
-                                                       const float *__restrict__ cs) {
-    constexpr int E = KP / 16;
-    static_assert(R <= 16, "row metadata is held one row per lane of a 16-lane group");
-    static_assert(R % PF == 0, "ring slots are static: slot = row % PF in every chunk");
-    static_assert((CG & (CG - 1)) == 0 && CG <= static_cast<int>(kChunkMod), "lg holds gene mod kChunkMod");
-    // Entry loads are unconditional at an index clamped into [0, max(nnz,1)),
-    // with validity applied when the batch is consumed: a load under a
-    // data-dependent branch would make every later wait drain all loads.
-    const int64_t p_max = nnz > 0 ? nnz - 1 : 0;
-    __shared__ float4 ms[CG * KP / 4];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x & 63, g = lane >> 4, l = lane & 15;
-    const int64_t row0 = (static_cast<int64_t>(blockIdx.x) * NW + wave) * R;
-    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
-    const int esel = 4 * l + g;  // entry of a 64-batch this lane loads
-    float acc[R][E];
-#pragma unroll
-    for (int r = 0; r < R; ++r)
-#pragma unroll
-        for (int e = 0; e < E; ++e) acc[r][e] = 0.f;
-    // per-row metadata, lane l of every group holds row row0 + l; read out
-    // with v_readlane (scalar registers), not through LDS
-    const int64_t mrow = row0 + l;
-    const bool mvalid = (l < R) && (mrow < n_obs);
-    const int64_t mrow_c = mrow < n_obs ? mrow : n_obs - 1;
-    const int64_t rbase = indptr[mrow_c];
-    const uint32_t *splr = spl + mrow_c * (nsplit + 1);
-    auto bounds = [&](int ch, uint32_t &a, uint32_t &b) {
-        const int s_lo = ch * cpc, s_hi = (ch + 1) * cpc < nsplit ? (ch + 1) * cpc : nsplit;
-        const uint32_t a0 = splr[s_lo], b0 = splr[s_hi];
-        a = mvalid ? a0 : 0u;
-        b = mvalid ? b0 : 0u;
-    };
-    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
-    auto rl64 = [](int64_t v, int r) {
-        const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v & 0xffffffffll), r));
-        const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v >> 32), r));
-        return static_cast<int64_t>((static_cast<uint64_t>(hi) << 32) | lo);
-    };
-    // First 64-entry batch of each (row, chunk) window, PF rows ahead; the
-    // ring runs on across chunk boundaries (row r of chunk ch is slot r % PF).
-    uint16_t pidx[PF];  // the loaded type: no extension on the loop edge
-    float pval[PF];
-    auto issue = [&](int r, uint32_t wl, uint32_t wh, uint16_t &pi, float &pv) {
-        const int64_t base_r = rl64(rbase, r);
-        const uint32_t lo_r = rl32(wl, r);
-        int64_t p = base_r + lo_r + esel;
-        p = p < p_max ? p : p_max;
-        pi = lg[p];
-        pv = xt[p];
-    };
-    const float4 *M4 = reinterpret_cast<const float4 *>(M);
-    uint32_t wlo, whi, nlo = 0u, nhi = 0u;
-    bounds(0, wlo, whi);
-#pragma unroll
-    for (int r = 0; r < PF; ++r) issue(r, wlo, whi, pidx[r], pval[r]);
-    for (int ch = 0; ch < nch; ++ch) {
-        const int64_t g0 = static_cast<int64_t>(ch) * CG;
-        bounds(ch + 1 < nch ? ch + 1 : ch, nlo, nhi);  // unconditional: see the note on loads
-        __syncthreads();
-        // the chunk's panel rows: unconditional loads at clamped rows (genes past
-        // n_vars zeroed by a select), MG in flight per wait
-        constexpr int MPT = CG * KP / 4 / (NW * kWave), MG = MPT < 2 ? MPT : 2;
-        static_assert(MPT * NW * kWave == CG * KP / 4 && MPT % MG == 0, "whole float4s per thread");
-#pragma unroll
-        for (int h = 0; h < MPT; h += MG) {
-            float4 mv[MG];
-#pragma unroll
-            for (int j = 0; j < MG; ++j) {
-                const int i = threadIdx.x + (h + j) * NW * kWave;
-                const int64_t gene = g0 + i / (KP / 4);
-                const bool ok = gene < n_vars;
-                const float4 v = M4[(ok ? gene : 0) * (KP / 4) + i % (KP / 4)];
-                mv[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-#pragma unroll
-            for (int j = 0; j < MG; ++j) ms[threadIdx.x + (h + j) * NW * kWave] = mv[j];
-        }
-        __syncthreads();
-        // Main flow: the first batch of every row's window, straight-line
-        // code fed by the ring (no loads inside loops, so the waits on the
-        // ring stay partial).
-        bool any_long = false;
-        segment_priority(0);
-#pragma unroll
-        for (int r = 0; r < R; ++r) {
-            if (r > 0 && (r * 4) / R != ((r - 1) * 4) / R) segment_priority((r * 4) / R);
-            const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
-            const uint32_t len = hi_r - lo_r;
-            any_long |= len > static_cast<uint32_t>(kWave);
-            const int cnt = len < static_cast<uint32_t>(kWave) ? static_cast<int>(len) : kWave;
-            const bool ok = esel < cnt;
-            // the slot is read before its next load is issued (a scheduling
-            // barrier keeps the order), so the old and new values share a
-            // register and the loop carries it without a copy
-            const uint32_t ofs = ok ? (static_cast<uint32_t>(pidx[r % PF]) & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
-            const float x = ok ? pval[r % PF] : 0.f;
-            __builtin_amdgcn_sched_barrier(0);
-            if (r + PF < R)
-                issue(r + PF, wlo, whi, pidx[r % PF], pval[r % PF]);
-            else  // next chunk's rows (after the last chunk: harmless reads)
-                issue(r + PF - R, nlo, nhi, pidx[r % PF], pval[r % PF]);
-            gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
-        }
-        // Windows longer than one batch (rare): the remaining batches, read
-        // directly.
-        if (any_long) {
-#pragma unroll
-            for (int r = 0; r < R; ++r) {
-                const int64_t base_r = rl64(rbase, r);
-                const uint32_t lo_r = rl32(wlo, r), hi_r = rl32(whi, r);
-                for (uint32_t o = lo_r + kWave; o < hi_r; o += kWave) {
-                    const int cnt = (hi_r - o) < static_cast<uint32_t>(kWave) ? static_cast<int>(hi_r - o) : kWave;
-                    int64_t p = base_r + o + esel;
-                    p = p < p_max ? p : p_max;
-                    const uint32_t c = lg[p];
-                    const float v = xt[p];
-                    const bool ok = esel < cnt;
-                    const uint32_t ofs = ok ? (c & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
-                    const float x = ok ? v : 0.f;
-                    gather_batch<E>(acc[r], ofs, x, cnt, reinterpret_cast<const char *>(ms), lane_ofs);
-                }
-            }
-        }
-        wlo = nlo;
-        whi = nhi;
-    }
-#pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int64_t row = row0 + r;
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            acc[r][e] += __shfl_xor(acc[r][e], 16);
-            acc[r][e] += __shfl_xor(acc[r][e], 32);
-        }
-        if (row < n_obs && g == 0) {
-            float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
-            const float br = OFFSET ? static_cast<float>(rowb[row]) : 0.f;
-#pragma unroll
-            for (int e = 0; e < E / 4; ++e) {
-                float4 o;
-                o.x = acc[r][4 * e + 0] - (mc ? mc[l * E + 4 * e + 0] : 0.f);
-                o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
-                o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
-                o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
-                if constexpr (OFFSET) {
-                    o.x += br * cs[l * E + 4 * e + 0];
-                    o.y += br * cs[l * E + 4 * e + 1];
-                    o.z += br * cs[l * E + 4 * e + 2];
-                    o.w += br * cs[l * E + 4 * e + 3];
-                }
-                yr[e] = o;
-            }
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// Row-tiled CSC ("tile-CSC"): for each tile of RT = 256 consecutive rows, the
-// tile's nonzeros ordered by gene, then by row. tptr[t][c] = offset of gene c's
-// run inside tile t (relative to indptr[t*RT]); trow = row within the tile
-// (u8); tval = raw value. Built once per matrix; the transposed SpMM and the
-// column statistics read it, so each gene's contributions in a tile are one
-// contiguous run. Rows are scattered in order, so runs are row-ascending.
-// ---------------------------------------------------------------------------
 
 constexpr int kTileRows = 256;
 
@@ -1757,317 +1500,21 @@ __global__ void __launch_bounds__(256) k_gene_counts_tcsc(const uint32_t *__rest
     if (s) atomicAdd(&cnt[c], s);
 }
 
-// ---------------------------------------------------------------------------
-// Transposed SpMM over the tile-CSC, entries in registers. Block (tile range
-// tr, gene chunk gc) has NWT waves; wave w owns GW genes (its gperm slots:
-// genes dealt to waves by entry count, so slot 0 holds the densest genes) and
-// keeps their Z rows in registers (E columns per lane, one partial per 16-lane
-// group) across the range. Per tile the Q tile (256 rows x KP) is staged in
-// LDS between two block barriers. Each gene's run in the next tile is loaded
-// one tile ahead straight into the gather layout (lane 4l+g holds entry 4l+g
-// of a 64-entry batch): NB0 batches for slot 0, one for the other slots. The
-// address is a wave-uniform run start plus the lane's fixed slot, so a batch
-// costs two loads and no per-lane index arithmetic, and no entry list passes
-// through LDS. Entries beyond the prefetched batches (rare outside slot 0)
-// are read directly. With KP = 64 the genes after slot 0 run in pairs
-// through one gather loop (gather_batch2: one LDS read latency per pair), and
-// each wave lowers its priority as it gets through its genes
-// (segment_priority), so the waves of a SIMD reach the next tile's barrier
-// together. Loads may run up to NB0 * 64 - 1 entries past the last run: the
-// tile-CSC arrays carry kEntryPad entries of padding, and lanes past a run's
-// end are masked at use. The next tile's base is this tile's base
-// plus its entry count (tptr[t][n_vars], read by the spare lane GW with the
-// run bounds), so the tile loop issues no scalar loads. Blocks are XCD-aware
-// (blockIdx = slot * 8 + xcd, the gene chunks of one tile range on one XCD,
-// sharing its Q tiles in L2).
-// ---------------------------------------------------------------------------
+// entries of padding after trow / tval / tpk (read-ahead of the bf16 kernels)
 constexpr int64_t kEntryPad = 256;
-
-template <int KP, int GW, int NB0, int NWT>
-__global__ void __launch_bounds__(NWT * 64) k_spmm_t6(const int64_t *__restrict__ indptr,
-                                                  const uint32_t *__restrict__ tptr,
-                                                  const uint8_t *__restrict__ trow,
-                                                  const float *__restrict__ tval,
-                                                  const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
-                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
-                                                  int64_t n_vars, int n_tiles, float *__restrict__ part,
-                                                  double *__restrict__ qsum_part) {
-    constexpr int E = KP / 16;
-    constexpr int NW = NWT;  // waves per block
-    constexpr int NT = NW * kWave;
-    constexpr int QC = KP / kWave;  // Q columns per lane in the column sums
-    static_assert(GW < 16, "one gene per lane of a group, plus the spare lane GW");
-    static_assert(NB0 >= 1 && NB0 * kWave <= kEntryPad, "slot-0 look-ahead must stay inside the padding");
-    // KP = 64: genes 1 + 2i and 2 + 2i share one gather loop (registers allow
-    // the second set of read buffers only there)
-    constexpr bool kPair = KP == 64;
-    __shared__ float4 qs[kTileRows * KP / 4];  // 64 KB (KP=64) / 128 KB (KP=128)
-    const char *qb = reinterpret_cast<const char *>(qs);
-    const int tid = threadIdx.x;
-    const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-    const int tr = (slot / nch) * 8 + xcd, gc = slot % nch;
-    const int wave = __builtin_amdgcn_readfirstlane(tid / kWave);
-    const int lane = tid & 63, g = lane >> 4, l = lane & 15;
-    const uint32_t lane_ofs = static_cast<uint32_t>(l * E * 4);
-    const int esel = 4 * l + g;
-    const int64_t gslot = (static_cast<int64_t>(gc) * NW + wave) * GW + l;
-    const uint32_t gene = (l < GW) ? gperm[gslot] : static_cast<uint32_t>(n_vars);
-    const bool gvalid = gene < static_cast<uint64_t>(n_vars);
-    const bool spare = l == GW;  // reads the tile's entry count
-    const uint32_t ia = gvalid ? gene : 0u;
-    const uint32_t ib = gvalid ? gene + 1u : static_cast<uint32_t>(n_vars);
-    float z[GW][E];
-#pragma unroll
-    for (int gi = 0; gi < GW; ++gi)
-#pragma unroll
-        for (int e = 0; e < E; ++e) z[gi][e] = 0.f;
-    const int t_begin = tr * tiles_per_tr;
-    const int t_end0 = (t_begin + tiles_per_tr) < n_tiles ? (t_begin + tiles_per_tr) : n_tiles;
-    const int t_end = (tr < ntr) ? t_end0 : t_begin;
-
-    auto rl32 = [](uint32_t v, int r) { return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), r)); };
-    auto bounds = [&](int tt, uint32_t &a, uint32_t &b) {
-        const uint32_t *tp = tptr + static_cast<int64_t>(tt) * (n_vars + 1);
-        const uint32_t a0 = tp[ia], b0 = tp[ib];
-        a = gvalid ? a0 : 0u;
-        b = (gvalid || spare) ? b0 : 0u;
-    };
-    constexpr int QPT = kTileRows * KP / 4 / NT;
-    constexpr bool kSpreadQ = KP == 64;
-    constexpr bool kIssueLate = KP == 128;
-    float4 qpre[QPT];
-    auto load_q = [&](int tt) {
-        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
-        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
-        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
-        if (nr == kTileRows) {  // every tile but the last: no row checks
-#pragma unroll
-            for (int j = 0; j < QPT; ++j) qpre[j] = Q4[tid + j * NT];
-        } else {
-#pragma unroll
-            for (int j = 0; j < QPT; ++j) {
-                const int i = tid + j * NT;
-                const bool ok = i / (KP / 4) < nr;
-                const float4 v = Q4[ok ? i : 0];
-                qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
-            }
-        }
-    };
-    // one of the QPT loads of a Q tile (issued spread over the gene loop, so
-    // the tile's 64-128 KB do not arrive as one burst ahead of the entry loads)
-    auto load_q_j = [&](int tt, int j) {
-        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
-        const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
-        const float4 *Q4 = reinterpret_cast<const float4 *>(Q + r0 * KP);
-        const int i = tid + j * NT;
-        const bool ok = i / (KP / 4) < nr;
-        const float4 v = Q4[ok ? i : 0];
-        qpre[j] = ok ? v : make_float4(0.f, 0.f, 0.f, 0.f);
-    };
-    // run start (tile-relative) and length of each gene in a tile, and the
-    // tile's entry count: scalars
-    uint32_t lo[GW], ln[GW], nlo[GW], nln[GW], tot = 0u, ntot = 0u;
-    auto layout = [&](uint32_t wl, uint32_t wh, uint32_t (&lo_)[GW], uint32_t (&ln_)[GW], uint32_t &tot_) {
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) {
-            lo_[gi] = rl32(wl, gi);
-            ln_[gi] = rl32(wh, gi) - lo_[gi];
-        }
-        tot_ = rl32(wh, GW);
-    };
-    // prefetched batches in the gather layout: NB0 for slot 0, one per other slot
-    uint8_t prow0[NB0], prow[GW];  // the loaded type: no extension on the loop edge
-    float pval0[NB0], pval[GW];
-    auto issue = [&](int64_t p0, uint8_t &pr, float &pv) {  // p0 wave-uniform
-        pr = (trow + p0)[esel];
-        pv = (tval + p0)[esel];
-    };
-    auto issue_tile = [&](int64_t tb_, const uint32_t (&lo_)[GW], int from, int to) {
-        for (int gi = from; gi < to; ++gi) {
-            if (gi == 0) {
-#pragma unroll
-                for (int b = 0; b < NB0; ++b) issue(tb_ + lo_[0] + b * kWave, prow0[b], pval0[b]);
-            } else {
-                issue(tb_ + lo_[gi], prow[gi], pval[gi]);
-            }
-        }
-    };
-    auto gather = [&](float (&acc)[E], uint32_t rr, float v, uint32_t left) {
-        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
-        const bool ok = esel < cnt;
-        gather_batch<E>(acc, ok ? rr * (KP * 4) : 0u, ok ? v : 0.f, cnt, qb, lane_ofs);
-    };
-    // A prefetched batch is turned into (LDS offset, value) before the next
-    // tile's load into the same registers is issued (a scheduling barrier
-    // keeps that order), so the old and new values need not overlap. With
-    // KP = 128 the load goes out after the gene's gathers: only then does
-    // the register allocator keep each gene in one register, so the loop
-    // edge has no copy (a copy of a register with a load in flight waits for
-    // every outstanding load). With KP = 64 the edge keeps its copies but the
-    // earlier issue (a longer look-ahead) measured faster.
-    auto prep = [&](uint32_t rr, float v, uint32_t left, uint32_t &ofs, float &x) {
-        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
-        const bool ok = esel < cnt;
-        ofs = ok ? rr * (KP * 4) : 0u;
-        x = ok ? v : 0.f;
-    };
-    auto gather_prepped = [&](float (&acc)[E], uint32_t ofs, float x, uint32_t left) {
-        const int cnt = left < static_cast<uint32_t>(kWave) ? static_cast<int>(left) : kWave;
-        gather_batch<E>(acc, ofs, x, cnt, qb, lane_ofs);
-    };
-
-    // 1^T Q (centering term): block gc sums, in f64, the staged Q tile rows r
-    // with r mod nch == gc (wave w takes the slice's rows w, w + 16, ...), so
-    // the extra work is spread over all blocks of the tile range
-    const bool do_qsum = qsum_part != nullptr;
-    double qsum[QC];
-#pragma unroll
-    for (int e = 0; e < QC; ++e) qsum[e] = 0.0;
-    uint32_t wlo = 0u, whi = 0u;
-    int64_t tb = 0;
-    if (t_begin < t_end) {
-        bounds(t_begin, wlo, whi);
-        tb = indptr[static_cast<int64_t>(t_begin) * kTileRows];
-        layout(wlo, whi, lo, ln, tot);
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) issue_tile(tb, lo, gi, gi + 1);
-        load_q(t_begin);
-        const int tn = t_begin + 1 < t_end ? t_begin + 1 : t_begin;
-        bounds(tn, wlo, whi);
-    }
-    for (int t = t_begin; t < t_end; ++t) {
-        const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
-        const int tnn = tn + 1 < t_end ? tn + 1 : tn;
-        __syncthreads();  // previous tile's gathers are done with qs
-#pragma unroll
-        for (int j = 0; j < QPT; ++j) qs[tid + j * NT] = qpre[j];
-        __syncthreads();
-        if (do_qsum) {
-            const float *qf = reinterpret_cast<const float *>(qb);
-            for (int rr = gc + nch * wave; rr < kTileRows; rr += nch * NW)
-#pragma unroll
-                for (int e = 0; e < QC; ++e) qsum[e] += static_cast<double>(qf[rr * KP + lane + kWave * e]);
-        }
-        // next tile: base, run layout (bounds loaded one tile ago); its Q tile
-        // is loaded during the gene loop (KP = 64) or here in one go (KP = 128:
-        // measured faster at 2M x 33,538)
-        if (!kSpreadQ) load_q(tn);
-        const int64_t ntb = tn != t ? tb + tot : tb;
-        layout(wlo, whi, nlo, nln, ntot);
-        bounds(tnn, wlo, whi);
-        uint32_t longest = 0u;
-        segment_priority(0);
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) {
-            const uint32_t len = ln[gi];
-            if (gi > 0 && (gi * 4) / GW != ((gi - 1) * 4) / GW) segment_priority((gi * 4) / GW);
-            if (gi == 0) {
-                uint32_t ofs[NB0];
-                float x[NB0];
-#pragma unroll
-                for (int b = 0; b < NB0; ++b)
-                    prep(prow0[b], pval0[b], len > static_cast<uint32_t>(b * kWave) ? len - b * kWave : 0u, ofs[b],
-                         x[b]);
-                __builtin_amdgcn_sched_barrier(0);
-                if (!kIssueLate) issue_tile(ntb, nlo, 0, 1);
-                longest = len > static_cast<uint32_t>(NB0 * kWave) ? len : 0u;
-#pragma unroll
-                for (int b = 0; b < NB0; ++b)
-                    if (len > static_cast<uint32_t>(b * kWave)) gather_prepped(z[0], ofs[b], x[b], len - b * kWave);
-                if (kIssueLate) issue_tile(ntb, nlo, 0, 1);
-            } else if (kPair && (gi & 1) && gi + 1 < GW) {  // genes gi and gi + 1 together
-                const uint32_t lb = ln[gi + 1];
-                uint32_t ofs, ofsb;
-                float x, xb;
-                prep(prow[gi], pval[gi], len, ofs, x);
-                prep(prow[gi + 1], pval[gi + 1], lb, ofsb, xb);
-                __builtin_amdgcn_sched_barrier(0);
-                issue_tile(ntb, nlo, gi, gi + 2);
-                longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
-                longest = lb > static_cast<uint32_t>(kWave) && lb > longest ? lb : longest;
-                const uint32_t lm = len > lb ? len : lb;
-                if (lm > 0u)
-                    gather_batch2<E>(z[gi], ofs, x, z[gi + 1], ofsb, xb,
-                                     lm < static_cast<uint32_t>(kWave) ? static_cast<int>(lm) : kWave, qb,
-                                     lane_ofs);
-            } else if (kPair && !(gi & 1)) {
-                // gene gi ran with gene gi - 1
-            } else {
-                uint32_t ofs;
-                float x;
-                prep(prow[gi], pval[gi], len, ofs, x);
-                __builtin_amdgcn_sched_barrier(0);
-                if (!kIssueLate) issue_tile(ntb, nlo, gi, gi + 1);
-                longest = len > static_cast<uint32_t>(kWave) && len > longest ? len : longest;
-                if (len > 0u) gather_prepped(z[gi], ofs, x, len);
-                if (kIssueLate) issue_tile(ntb, nlo, gi, gi + 1);
-            }
-#pragma unroll
-            for (int j = 0; j < QPT; ++j)
-                if (kSpreadQ && (j * GW) / QPT == gi) load_q_j(tn, j);
-        }
-        // entries beyond the prefetched batches: read directly
-        if (longest) {
-#pragma unroll
-            for (int gi = 0; gi < GW; ++gi) {
-                const uint32_t len = ln[gi];
-                for (uint32_t o = (gi == 0 ? NB0 : 1) * kWave; o < len; o += kWave) {
-                    uint8_t rr;
-                    float v;
-                    issue(tb + lo[gi] + o, rr, v);
-                    gather(z[gi], rr, v, len - o);
-                }
-            }
-        }
-#pragma unroll
-        for (int gi = 0; gi < GW; ++gi) {
-            lo[gi] = nlo[gi];
-            ln[gi] = nln[gi];
-        }
-        tot = ntot;
-        tb = ntb;
-    }
-#pragma unroll
-    for (int gi = 0; gi < GW; ++gi) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            z[gi][e] += __shfl_xor(z[gi][e], 16);
-            z[gi][e] += __shfl_xor(z[gi][e], 32);
-        }
-        const uint32_t c = rl32(gene, gi);
-        if (tr < ntr && c < static_cast<uint64_t>(n_vars) && g == 0) {
-            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
-#pragma unroll
-            for (int e = 0; e < E / 4; ++e)
-                dst[e] = make_float4(z[gi][4 * e], z[gi][4 * e + 1], z[gi][4 * e + 2], z[gi][4 * e + 3]);
-        }
-    }
-    if (do_qsum) {  // the waves' column sums, added in wave order
-        __syncthreads();
-        double *red = reinterpret_cast<double *>(qs);  // NW x KP f64 fits in the Q tile buffer
-#pragma unroll
-        for (int e = 0; e < QC; ++e) red[wave * KP + lane + kWave * e] = qsum[e];
-        __syncthreads();
-        if (tid < KP) {
-            double t = 0.0;
-            for (int w = 0; w < NW; ++w) t += red[w * KP + tid];
-            qsum_part[(static_cast<int64_t>(tr) * nch + gc) * KP + tid] = t;
-        }
-    }
-}
 
 // ---------------------------------------------------------------------------
 // Transposed SpMM with genes owned per 16-lane group (cpu.rs:1282-1389).
 //
-// k_spmm_t6 has the four groups of a wave share each gene (entries 4t+g of a
-// batch) and sums their partials at the end, so a wave covers GW genes and a
-// 1024-thread block 16 GW: every Q tile is staged once per 128 genes, and a
-// wave's share of a tile is ~160 entries between two block barriers. Here
-// each group owns its own GW genes: lane 16g+t holds entry t of group g's
-// current run, and the row broadcast (DPP row_newbcast:t) hands every group
-// its own entry, so the same gather steps serve four genes at once. A wave
-// covers 4 GW genes, a block 64 GW (512 at KP = 64), each Q tile is staged
-// once per 512 genes, and a wave does ~650 entries per tile. The four genes
+// Z rows stay in registers, so a wave can only hold a few genes; if the four
+// groups of a wave shared each gene, a 1024-thread block would cover 16 GW
+// genes and every Q tile would be staged once per 128 genes, with a wave's
+// share of a tile ~160 entries between two block barriers. Instead each
+// group owns its own GW genes: lane 16g+t holds entry t of group g's current
+// run, and the row broadcast (DPP row_newbcast:t) hands every group its own
+// entry, so the same gather steps serve four genes at once. A wave covers
+// 4 GW genes, a block 64 GW (512 at KP = 64), each Q tile is staged once per
+// 512 genes, and a wave does ~650 entries per tile. The four genes
 // of a slot (gperm deals genes to groups by entry count, snake order, so
 // they have neighbouring counts) run together for the longest of their four
 // runs; the shorter runs' lanes carry x = 0 and offset 0. Each group's Z
@@ -2111,8 +1558,13 @@ __device__ __forceinline__ uint32_t row_bcast(uint32_t v) {
 }
 
 // Gather steps S..N-1 of a run held as NC components of 16 entries per group
-// (step s: component s >> 4, lane s & 15 of each group); LDS reads two steps
-// ahead through three rotating buffers, as gather_batch.
+// (step s: component s >> 4, lane s & 15 of each group). The LDS reads run
+// two steps ahead of the FMAs through three rotating buffers, and the step
+// count is wave-uniform so the exits are scalar branches; lanes past a run
+// carry x = 0 and offset 0 (a valid address), so the look-ahead reads are
+// unconditional. A scheduling barrier keeps each look-ahead read ahead of the
+// current step's FMAs (otherwise the scheduler sinks it next to its use and
+// every step waits for its own read).
 template <int S, int N, int E, int NC>
 struct GatherRun {
     static __device__ __forceinline__ void run(float (&acc)[E], const uint32_t (&ofs)[NC], const float (&x)[NC],
@@ -2154,13 +1606,17 @@ __device__ __forceinline__ void static_for(F &&f) {
 // word 3 of a raw buffer descriptor (gfx9 family: 32-bit data format, no swizzle)
 constexpr int kBufferRsrcWord3 = 0x00020000;
 
-// n_entries: entries in trow / tval including their padding
-template <int KP, int GW, int P0, int P1, int RC, int NW>
+// n_entries: entries in trow / tval including their padding. The kernel works
+// on KP columns: columns KP * blockIdx.y .. of panels whose rows hold LD
+// floats (Q, the partials and their column sums), so a 128-column panel runs
+// as two 64-column halves (gridDim.y = 2).
+template <int KP, int LD, int GW, int P0, int P1, int RC, int NW>
 __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__ indptr,
                                                  const uint32_t *__restrict__ tptr,
                                                  const uint8_t *__restrict__ trow,
                                                  const float *__restrict__ tval, int64_t n_entries,
-                                                 const float *__restrict__ Q, const uint32_t *__restrict__ gperm,
+                                                 const float *__restrict__ Q,
+                                                 const uint32_t *__restrict__ gperm,
                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
                                                  int64_t n_vars, int n_tiles, float *__restrict__ part,
                                                  double *__restrict__ qsum_part) {
@@ -2209,14 +1665,18 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
     // loads against the LDS reads of the other buffer (it would wait for them
     // before every gather); their completion is awaited explicitly (vmcnt,
     // then the block barrier) before the buffer is read.
-    const int64_t qlim = n_obs * (KP / 4);
+    static_assert(LD % KP == 0, "whole column halves");
+    const int col0 = KP * static_cast<int>(blockIdx.y);
+    constexpr int ld4 = LD / 4;
+    const float4 *Qh = reinterpret_cast<const float4 *>(Q + col0);
     auto load_q = [&](int tt, int bi) {
-        const int64_t q0 = static_cast<int64_t>(tt) * TQ;
+        const int64_t r0 = static_cast<int64_t>(tt) * kTileRows;
+        static_assert(KP == 64, "a 64-lane load covers four 16-float4 panel rows");
 #pragma unroll
         for (int j = 0; j < QPT; ++j) {
             const int i = (j * NW + wave) * kWave;
-            const int64_t src = q0 + i + lane;
-            const float4 *gp = reinterpret_cast<const float4 *>(Q) + (src < qlim ? src : 0);
+            const int64_t row = r0 + 4 * (j * NW + wave) + g;  // tile row (i + lane) / 16
+            const float4 *gp = Qh + (row < n_obs ? row : 0) * ld4 + l;
             const uint32_t dst = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&qs[bi][i]));
             asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(gp), "{m0}"(dst) : "memory");
         }
@@ -2393,7 +1853,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
         constexpr int I = decltype(ic)::value;
         const uint32_t c = row_bcast<I>(gene);
         if (tr < ntr && c < static_cast<uint64_t>(n_vars)) {
-            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * KP + l * E);
+            float4 *dst = reinterpret_cast<float4 *>(part + (static_cast<int64_t>(tr) * n_vars + c) * LD + col0 + l * E);
 #pragma unroll
             for (int e = 0; e < E / 4; ++e)
                 dst[e] = make_float4(z[I][4 * e], z[I][4 * e + 1], z[I][4 * e + 2], z[I][4 * e + 3]);
@@ -2408,7 +1868,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
         if (tid < KP) {
             double t = 0.0;
             for (int w = 0; w < NW; ++w) t += red[w * KP + tid];
-            qsum_part[(static_cast<int64_t>(tr) * nch + gc) * KP + tid] = t;
+            qsum_part[(static_cast<int64_t>(tr) * nch + gc) * LD + col0 + tid] = t;
         }
     }
 }
@@ -2417,13 +1877,13 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
 // Forward SpMM with rows owned per 16-lane group (cpu.rs:1185-1229,1471-1523):
 // Y[r][:] = sum x~ M[c][:] - mc (+ b_r cs for a PFlog matrix).
 //
-// k_spmm_fwd3 has the four groups of a wave share each row and sums their
-// partials at the end, so a 16-wave block covers 256 rows and stages all of
-// M (four 128 KB gene chunks) once per 256 rows: 2 KB of M per row, more
-// than the row's own entries. Here each group owns R rows (lane 16g+t holds
-// entry t of the group's current window; the row broadcast hands every
-// group its own entry), so a wave covers 4R = 64 rows and a block 1024: M is
-// staged once per 1024 rows. A wave sorts its 64 rows by length at the start
+// Y rows stay in registers; if the four groups of a wave shared each row, a
+// 16-wave block would cover 256 rows and stage all of M (four 128 KB gene
+// chunks) once per 256 rows: 2 KB of M per row, more than the row's own
+// entries. Instead each group owns R rows (lane 16g+t holds entry t of the
+// group's current window; the row broadcast hands every group its own
+// entry), so a wave covers 4R = 64 rows and a block 1024: M is staged once
+// per 1024 rows. A wave sorts its 64 rows by length at the start
 // and deals them so that the four rows of a slot have neighbouring lengths
 // (the slot runs for the longest of its four windows). Per 512-gene chunk the
 // M rows are copied into LDS by direct global -> LDS loads between two block
@@ -2433,11 +1893,15 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
 // rest through a ring. Entry loads go through buffer descriptors based at the
 // wave's first entry (32-bit lane offsets, range-checked).
 // ---------------------------------------------------------------------------
-template <int KP, int R, int NW, int CG, int PF, int PC, int RC, bool OFFSET = false>
+// The kernel works on KP columns: columns KP * blockIdx.y .. of panels whose
+// rows hold LD floats (M, Y, mc, cs), so a 128-column panel runs as two
+// 64-column halves (gridDim.y = 2).
+template <int KP, int LD, int R, int NW, int CG, int PF, int PC, int RC, bool OFFSET = false>
 __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict__ indptr,
                                                        const uint16_t *__restrict__ lg,
                                                        const float *__restrict__ xt,
-                                                       const float *__restrict__ M, const float *__restrict__ mc,
+                                                       const float *__restrict__ M,
+                                                       const float *__restrict__ mc,
                                                        float *__restrict__ Y, const uint32_t *__restrict__ spl,
                                                        int nsplit, int cpc, int nch, int64_t n_obs,
                                                        int64_t n_vars, int64_t nnz,
@@ -2529,15 +1993,18 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
 #pragma unroll
         for (int c = 0; c < PC; ++c) issue(lo_, c, pidx[slot][c], pval[slot][c]);
     };
-    const float4 *M4 = reinterpret_cast<const float4 *>(M);
-    const int64_t mlim = n_vars * (KP / 4);
+    static_assert(LD % KP == 0, "whole column halves");
+    const int col0 = KP * static_cast<int>(blockIdx.y);
+    constexpr int ld4 = LD / 4;
+    const float4 *Mh = reinterpret_cast<const float4 *>(M + col0);
+    static_assert(KP == 64, "a 64-lane load covers four 16-float4 panel rows");
     auto load_m = [&](int ch) {  // wave w's load j fills float4s (j NW + w) * 64 + lane of the chunk
-        const int64_t m0 = static_cast<int64_t>(ch) * TM;
+        const int64_t g0 = static_cast<int64_t>(ch) * CG;
 #pragma unroll
         for (int j = 0; j < MPT; ++j) {
             const int i = (j * NW + wave) * kWave;
-            const int64_t src = m0 + i + lane;
-            const float4 *gp = M4 + (src < mlim ? src : 0);  // genes past n_vars: never gathered
+            const int64_t gene = g0 + 4 * (j * NW + wave) + g;  // chunk row (i + lane) / 16
+            const float4 *gp = Mh + (gene < n_vars ? gene : 0) * ld4 + l;  // genes past n_vars: never gathered
             const uint32_t dst = static_cast<uint32_t>(reinterpret_cast<uintptr_t>(&ms[i]));
             asm volatile("global_load_lds_dwordx4 %0, off" ::"v"(gp), "{m0}"(dst) : "memory");
         }
@@ -2624,20 +2091,22 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
         constexpr int r = decltype(ic)::value;
         const int64_t row = row0 + row_bcast<r>(rid);
         if (row < n_obs) {
-            float4 *yr = reinterpret_cast<float4 *>(Y + row * KP + l * E);
+            float4 *yr = reinterpret_cast<float4 *>(Y + row * LD + col0 + l * E);
             const float br = OFFSET ? static_cast<float>(rowb[row]) : 0.f;
+            const float *mch = mc ? mc + col0 : nullptr;
+            const float *csh = OFFSET ? cs + col0 : nullptr;
 #pragma unroll
             for (int e = 0; e < E / 4; ++e) {
                 float4 o;
-                o.x = acc[r][4 * e + 0] - (mc ? mc[l * E + 4 * e + 0] : 0.f);
-                o.y = acc[r][4 * e + 1] - (mc ? mc[l * E + 4 * e + 1] : 0.f);
-                o.z = acc[r][4 * e + 2] - (mc ? mc[l * E + 4 * e + 2] : 0.f);
-                o.w = acc[r][4 * e + 3] - (mc ? mc[l * E + 4 * e + 3] : 0.f);
+                o.x = acc[r][4 * e + 0] - (mch ? mch[l * E + 4 * e + 0] : 0.f);
+                o.y = acc[r][4 * e + 1] - (mch ? mch[l * E + 4 * e + 1] : 0.f);
+                o.z = acc[r][4 * e + 2] - (mch ? mch[l * E + 4 * e + 2] : 0.f);
+                o.w = acc[r][4 * e + 3] - (mch ? mch[l * E + 4 * e + 3] : 0.f);
                 if constexpr (OFFSET) {
-                    o.x += br * cs[l * E + 4 * e + 0];
-                    o.y += br * cs[l * E + 4 * e + 1];
-                    o.z += br * cs[l * E + 4 * e + 2];
-                    o.w += br * cs[l * E + 4 * e + 3];
+                    o.x += br * csh[l * E + 4 * e + 0];
+                    o.y += br * csh[l * E + 4 * e + 1];
+                    o.z += br * csh[l * E + 4 * e + 2];
+                    o.w += br * csh[l * E + 4 * e + 3];
                 }
                 yr[e] = o;
             }

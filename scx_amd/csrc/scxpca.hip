@@ -222,7 +222,7 @@ struct scxpca_matrix {
     DevBuf tptr, trow, tval, tpk;
     int n_tiles = 0;
     // entries per gene (host copy) and the load-balanced gene order of the
-    // transposed SpMM for one wave-slot layout (GW genes x 16 waves x nch)
+    // transposed SpMM (GW genes x 64 16-lane groups x nch blocks)
     DevBuf gcnt;
     std::vector<uint32_t> host_gcnt;
     struct GeneOrder {
@@ -230,8 +230,8 @@ struct scxpca_matrix {
         bool valid = false;
         PinnedBuf host;  // kept alive for the asynchronous upload
         DevBuf dev;
-    } gorder[2];  // one per wave-slot layout (KP = 64 and KP = 128)
-    bool use_split = false;  // split offsets, k_build_tcsc4 (+ per-tile column partials), k_spmm_fwd3
+    } gorder;
+    bool use_split = false;  // split offsets, k_build_tcsc4 (+ per-tile column partials), k_spmm_fwd4
     // cached column statistics of X~
     bool stats_ready = false;
     DevBuf tsum, tsq;  // per-tile column-statistics partials (written by the tile-CSC build)
@@ -357,7 +357,7 @@ int prepare(scxpca_matrix *m) {
     HIPC(hipStreamSynchronize(m->stream));
     const uint32_t *hg = m->hgcnt.as<uint32_t>();
     m->host_gcnt.assign(hg, hg + m->n_vars);
-    for (auto &g : m->gorder) g.valid = false;
+    m->gorder.valid = false;
     if (want_b) {
         m->pf_bsum = m->hpf.as<double>()[0];
         m->pf_b2 = m->hpf.as<double>()[1];
@@ -428,18 +428,19 @@ void throw_invalid(const scxpca_matrix *m, int h, int64_t row_base = 0, int64_t 
     }
 }
 
-// Gene order for the transposed SpMM. Genes sorted by entry count
-// (descending, ties by index) are cut into units of `quad` consecutive genes
-// (quad = 4: the four 16-lane groups of a k_spmm_t7 wave, which run one gene
-// each in lockstep, so a unit's genes have neighbouring counts; quad = 1: a
-// k_spmm_t6 wave's gene). Units go, largest first (cost: the unit's largest
-// count), to the wave with the least total so far among the nch * tw / quad
-// waves with a free slot (ties: the lowest wave), a wave's k-th unit filling
-// its slot k: per-wave totals, and so the blocks' totals, come out within
-// about one unit of each other, and slot 0 holds each wave's densest genes.
-// Owner o = wave * quad + group takes h[o * gw + slot]; unused slots hold
-// n_vars. The dealt order into h[0, max(S, 1)), S = nch * tw * gw slots.
-void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, int quad, uint32_t *h) {
+// Gene order for the transposed SpMM (k_spmm_t7: tw gene owners per block,
+// the 16-lane groups of its waves, gw slots each). Genes sorted by entry
+// count (descending, ties by index) are cut into units of four consecutive
+// genes, one per group of a wave (the four run in lockstep, so a unit's genes
+// have neighbouring counts). Units go, largest first (cost: the unit's
+// largest count), to the wave with the least total so far among the
+// nch * tw / 4 waves with a free slot (ties: the lowest wave), a wave's k-th
+// unit filling its slot k: per-wave totals, and so the blocks' totals, come
+// out within about one unit of each other, and slot 0 holds each wave's
+// densest genes. Owner o = wave * 4 + group takes h[o * gw + slot]; unused
+// slots hold n_vars. The dealt order into h[0, max(S, 1)), S = nch * tw * gw.
+void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, uint32_t *h) {
+    constexpr int quad = 4;
     const int64_t nv = m->n_vars;
     const int64_t S = static_cast<int64_t>(nch) * tw * gw;
     const int64_t waves = static_cast<int64_t>(nch) * tw / quad;
@@ -468,17 +469,17 @@ void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, int quad, 
 }
 
 bool gene_order_valid(const scxpca_matrix *m, int gw, int nch, int tw) {
-    const auto &go = m->gorder[gw == 8 ? 0 : 1];
+    const auto &go = m->gorder;
     return go.valid && go.gw == gw && go.nch == nch && go.tw == tw;
 }
 
-const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw, int quad) {
-    auto &go = m->gorder[gw == 8 ? 0 : 1];
+const uint32_t *ensure_gene_order(scxpca_matrix *m, int gw, int nch, int tw) {
+    auto &go = m->gorder;
     if (gene_order_valid(m, gw, nch, tw)) return go.dev.as<uint32_t>();
     const int64_t S = static_cast<int64_t>(nch) * tw * gw;
     // the previous upload from go.host (if any) was drained by prepare's sync
     go.host.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
-    deal_gene_order(m, gw, nch, tw, quad, go.host.as<uint32_t>());
+    deal_gene_order(m, gw, nch, tw, go.host.as<uint32_t>());
     uint32_t *h = go.host.as<uint32_t>();
     go.dev.ensure(sizeof(uint32_t) * std::max<int64_t>(S, 1));
     HIPC(hipMemcpyAsync(go.dev.p, h, sizeof(uint32_t) * std::max<int64_t>(S, 1), hipMemcpyHostToDevice, m->stream));
@@ -706,25 +707,19 @@ struct Engine {
     // Cholesky: the f32-Gram tall passes, the f64 Rayleigh-Ritz passes
     static constexpr double kStableFloor = 1e-6;
     static constexpr double kSmallFloor = 1e-13;
-    // transposed SpMM: k_spmm_t7 (genes owned per 16-lane group) at KP = 64,
-    // k_spmm_t6 (genes owned per wave) at KP = 128
-    static constexpr bool kT7 = KP == 64;
-    static constexpr int GW = (KP == 64) ? 8 : 4;   // genes per owner (Z in registers)
+    // transposed SpMM k_spmm_t7 (genes owned per 16-lane group) and forward
+    // SpMM k_spmm_fwd4 (rows owned per 16-lane group), over the KP / 64
+    // 64-column halves of the panel
+    static constexpr int GW = 8;                    // k_spmm_t7: genes per owner (Z in registers)
     static constexpr int P0 = 4;                    // k_spmm_t7: slot-0 prefetch, 16-entry components per group
     static constexpr int P1 = 2;                    // k_spmm_t7: slot-1 prefetch
     static constexpr int RC = 2;                    // k_spmm_t7: 16-entry components per ring load
-    static constexpr int NB0 = 1;                   // k_spmm_t6: slot-0 prefetch batches
-    static constexpr int TW = 16;                   // waves per block
-    static constexpr int TU = kT7 ? 4 * TW : TW;    // gene owners per block
-    static constexpr int CG = TU * GW;              // genes per block
-    // forward SpMM: k_spmm_fwd4 (rows owned per 16-lane group) at KP = 64,
-    // k_spmm_fwd3 (rows owned per wave) at KP = 128
-    static constexpr bool kF4 = KP == 64;
+    static constexpr int TW = 16;                   // k_spmm_t7: waves per block
+    static constexpr int TU = 4 * TW;               // k_spmm_t7: gene owners (16-lane groups) per block
+    static constexpr int CG = TU * GW;              // k_spmm_t7: genes per block
     static constexpr int F4PF = 2;                  // k_spmm_fwd4: ring depth, slots
     static constexpr int F4PC = 2;                  // k_spmm_fwd4: prefetched 16-entry components per window
     static constexpr int F4RC = 1;                  // k_spmm_fwd4: components per overflow load
-    static constexpr int FR = (KP == 64) ? 16 : 8;  // rows per wave in k_spmm_fwd3
-    static constexpr int FPF = 4;                   // first-batch prefetch ring depth (rows)
     scxpca_matrix *m;
     hipStream_t s;
     int k = 0;
@@ -752,7 +747,7 @@ struct Engine {
     explicit Engine(scxpca_matrix *mm, int kk) : m(mm), s(mm->stream), k(kk) {
         const int64_t n = m->n_obs, nv = m->n_vars;
         t_geometry(m, nch_t, nrb_t, tiles_per_tr);
-        gperm = ensure_gene_order(m, GW, nch_t, TU, TU / TW);
+        gperm = ensure_gene_order(m, GW, nch_t, TU);
         gram_waves = static_cast<int>(std::max<int64_t>(4, std::min<int64_t>((n + 255) / 256, KP == 64 ? 1024 : 512)));
         gram_waves = (gram_waves + 3) / 4 * 4;
         rows_per_wave = (n + gram_waves - 1) / gram_waves;
@@ -847,21 +842,14 @@ struct Engine {
                                                                      cs32, Y);
             return;
         }
-        if (m->use_split) {
+        if (m->use_split) {  // rows owned per 16-lane group: 64 rows per wave, 1024 per block
             constexpr int NWF = 16;
-            constexpr int CGF = 32768 / KP;  // 128 KB panel chunk in LDS
-            const int nchf = static_cast<int>((m->n_vars + CGF - 1) / CGF);
-            if constexpr (kF4) {  // rows owned per 16-lane group: 64 rows per wave
-                const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * 64 - 1) / (NWF * 64));
-                k_spmm_fwd4<KP, 16, NWF, CGF, F4PF, F4PC, F4RC, OFF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y, m->spl.as<uint32_t>(),
-                    m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
-            } else {
-                const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * FR - 1) / (NWF * FR));
-                k_spmm_fwd3<KP, FR, NWF, CGF, FPF, OFF><<<grid, NWF * 64, 0, s>>>(
-                    m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y,
-                    m->spl.as<uint32_t>(), m->nsplit, CGF / kCB, nchf, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
-            }
+            constexpr int CG4 = 512;  // genes per 128 KB chunk of a 64-column half
+            const int nch4 = static_cast<int>((m->n_vars + CG4 - 1) / CG4);
+            const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * 64 - 1) / (NWF * 64));
+            k_spmm_fwd4<64, KP, 16, NWF, CG4, F4PF, F4PC, F4RC, OFF><<<dim3(grid, KP / 64), NWF * 64, 0, s>>>(
+                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y, m->spl.as<uint32_t>(),
+                m->nsplit, CG4 / kCB, nch4, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
             k_spmm_fwd<KP, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), panel,
@@ -916,14 +904,9 @@ struct Engine {
         if (mm->n_obs > 0) {
             const unsigned grid = static_cast<unsigned>(nrb * nch);
             StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T);
-            if constexpr (kT7)
-                k_spmm_t7<KP, GW, P0, P1, RC, TW><<<grid, TW * 64, 0, st>>>(
-                    mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                    mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
-            else
-                k_spmm_t6<KP, GW, NB0, TW><<<grid, TW * 64, 0, st>>>(
-                    mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(), Q, gp, nch,
-                    nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
+            k_spmm_t7<64, KP, GW, P0, P1, RC, TW><<<dim3(grid, KP / 64), TW * 64, 0, st>>>(
+                mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(),
+                mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
             HIPC(hipGetLastError());
         } else {  // no rows: zero partials (a fill kernel, no memset node)
             k_zero_words<<<blocks_for(static_cast<int64_t>(nrb) * nv * KP), 256, 0, st>>>(
@@ -1963,10 +1946,10 @@ void project_backward_noalloc(scxpca_matrix *m, const float *dz, int64_t k, cons
     char *base = static_cast<char *>(noalloc_scratch(m->device));
     const uint32_t *gp = nullptr;
     if (have_order) {
-        gp = m->gorder[E::GW == 8 ? 0 : 1].dev.template as<uint32_t>();
+        gp = m->gorder.dev.template as<uint32_t>();
     } else {  // dealt on the host, staged through pageable memory (no allocation on the device)
         std::vector<uint32_t> h(slots);
-        deal_gene_order(m, E::GW, nch, E::TU, E::TU / E::TW, h.data());
+        deal_gene_order(m, E::GW, nch, E::TU, h.data());
         HIPC(hipMemcpyAsync(base + o_perm, h.data(), sizeof(uint32_t) * slots, hipMemcpyHostToDevice, s));
         HIPC(hipStreamSynchronize(s));
         gp = reinterpret_cast<const uint32_t *>(base + o_perm);
