@@ -660,18 +660,6 @@ def main():
     dom_ms = prof[dom][0] / max(prof[dom][1], 1)
     bytes_per = spmm_bytes(x.nnz, n, nv, k, bf16=dom.endswith("bf16"))
     achieved = bytes_per / (dom_ms * 1e-3) / 1e9
-    mfma_roof = None
-    if dom.endswith("bf16"):
-        # the BF16 form's products run densified tiles on the bf16 matrix cores:
-        # dense-equivalent flops 2 n nv_pad KP x 2 (the panel's hi and lo terms)
-        kp = 64 if k <= 64 else 128
-        nv_pad = (nv + 63) // 64 * 64
-        fl = 2.0 * n * nv_pad * kp * 2
-        mfma_roof = {"bound": "mfma", "kernel": dom, "achieved": round(fl / (dom_ms * 1e-3) / 1e12, 1),
-                     "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(fl / (dom_ms * 1e-3) / 1e12 / BF16_MFMA_PEAK_TFLOPS, 4),
-                     "flops_per_launch": fl, "avg_launch_ms": round(dom_ms, 4)}
-
     traffic = None
     tf = os.path.join(ROOT, "profiles", "spmm_traffic.json")
     if os.path.exists(tf) and not pflog:
@@ -759,7 +747,7 @@ def main():
             "n_obs": n, "n_vars": nv, "nnz": x.nnz, "k": k,
             "parallelism": "single GPU",
         },
-        "roofline": mfma_roof or {
+        "roofline": {
             "bound": "hbm",
             "kernel": dom,
             "achieved": round(achieved, 1),

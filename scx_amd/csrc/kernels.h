@@ -9,7 +9,8 @@
 //                chunk inside each row (sorted rows)
 //   tile-CSC     per 256-row tile, gene-major runs: tptr u32[n_tiles][n_vars+1],
 //                trow u8[nnz] (row in tile), tval f32[nnz] (X~); tpk u32[nnz] packed
-//                bf16 entries for the BF16 sketch form (kernels_mfma.h)
+//                bf16 entries for the BF16 sketch form
+//   xpk          u32[nnz] packed bf16 entries in CSR order (BF16 sketch form)
 //   panels       f32 row-major, KP columns (k' padded to 32/64/128, pad columns 0):
 //                M (n_vars x KP), Y and Q (n_obs x KP)
 //   B            f64 n_vars x KP
@@ -100,6 +101,15 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f) {
 __device__ __forceinline__ uint32_t pack_bf16_entry(float x, uint32_t row, uint32_t gene) {
     return static_cast<uint32_t>(f32_to_bf16_rne(x)) | ((row & 255u) << 16) | ((gene & 63u) << 24);
 }
+
+// The BF16 sketch form's packed CSR-order entry: bf16(x~) (bits 16-31), the
+// gene mod kChunkMod (bits 0-15): x~ is the entry with its low half cleared.
+__device__ __forceinline__ uint32_t pack_bf16_row_entry(float x, uint32_t lgv) {
+    return (static_cast<uint32_t>(f32_to_bf16_rne(x)) << 16) | (lgv & 0xffffu);
+}
+
+// x rounded to bf16 (round to nearest even) and back to f32
+__device__ __forceinline__ float bf16_round(float x) { return bf16_to_f32(f32_to_bf16_rne(x)); }
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
@@ -202,7 +212,8 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                                                       const float *__restrict__ val, int64_t n_obs,
                                                       double target, int flags, int nsplit,
                                                       double *__restrict__ rowf, float *__restrict__ xt,
-                                                      uint16_t *__restrict__ lg, uint32_t *__restrict__ spl,
+                                                      uint16_t *__restrict__ lg, uint32_t *__restrict__ xpk,
+                                                      uint32_t *__restrict__ spl,
                                                       int64_t n_vars, int *__restrict__ flag,
                                                       double *__restrict__ rowb,
                                                       unsigned long long *__restrict__ errpos) {
@@ -296,6 +307,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 const float d = xtilde<LOGF64>(v[u], f, flags);
                 xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
+                if (xpk) xpk[p] = pack_bf16_row_entry(d, c[u] & (kChunkMod - 1));
                 if constexpr (PFLOG) {
                     dsum += static_cast<double>(d);
                     if (flags & kXfCheck) check_value(v[u], d, p);
@@ -324,6 +336,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 const float d = xtilde<LOGF64>(v, f, flags);
                 xt[p] = d;
                 lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
+                if (xpk) xpk[p] = pack_bf16_row_entry(d, c & (kChunkMod - 1));
                 if constexpr (PFLOG) {
                     dsum += static_cast<double>(d);
                     if (flags & kXfCheck) check_value(v, d, p);
@@ -573,6 +586,47 @@ __global__ void __launch_bounds__(256) k_zero_words(uint32_t *__restrict__ p, in
 }
 
 // bf16 (RNE) copy of an f32 array: the sketch form's X~ values.
+// Packed bf16 tile entries (pack_bf16_entry) from the tile-CSC, for a matrix
+// whose build did not write them: one wave per (tile, 64-gene group), whose
+// runs are one contiguous range; lane j holds gene g0 + j's run start and
+// each entry's gene comes from a 6-step search over them. Grid: 4 waves per
+// block over n_tiles * ceil(n_vars / 64) groups.
+__global__ void __launch_bounds__(256) k_pack_tile_entries(const int64_t *__restrict__ indptr,
+                                                           const uint32_t *__restrict__ tptr,
+                                                           const uint8_t *__restrict__ trow,
+                                                           const float *__restrict__ tval, int64_t n_vars,
+                                                           int n_tiles, uint32_t *__restrict__ tpk) {
+    const int lane = threadIdx.x & 63;
+    const int64_t ng = (n_vars + 63) / 64;
+    const int64_t wid = static_cast<int64_t>(blockIdx.x) * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (wid >= static_cast<int64_t>(n_tiles) * ng) return;  // wave-uniform
+    const int64_t t = wid / ng, g0 = (wid % ng) * 64;
+    const uint32_t *tp = tptr + t * (n_vars + 1);
+    const int64_t gl = g0 + lane;
+    const uint32_t st = tp[gl < n_vars ? gl : n_vars];
+    const uint32_t end = tp[(g0 + 64) < n_vars ? g0 + 64 : n_vars];
+    const uint32_t s0 = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(st), 0));
+    const int64_t tb = indptr[t * 256];
+    for (uint32_t p0 = s0; p0 < end; p0 += 64) {
+        const uint32_t p = p0 + static_cast<uint32_t>(lane);
+        int lo = 0;
+#pragma unroll
+        for (int step = 32; step >= 1; step >>= 1) {
+            const uint32_t v = static_cast<uint32_t>(__shfl(static_cast<int>(st), lo + step < 64 ? lo + step : 63));
+            lo = (lo + step < 64 && v <= p) ? lo + step : lo;
+        }
+        if (p < end) tpk[tb + p] = pack_bf16_entry(tval[tb + p], trow[tb + p], static_cast<uint32_t>(g0 + lo));
+    }
+}
+
+// xpk[p] = pack_bf16_row_entry(xt[p], lg[p])
+__global__ void __launch_bounds__(256) k_pack_row_entries(const float *__restrict__ xt,
+                                                          const uint16_t *__restrict__ lg, int64_t nnz,
+                                                          uint32_t *__restrict__ xpk) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < nnz) xpk[p] = pack_bf16_row_entry(xt[p], lg[p]);
+}
+
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ src, int64_t n,
                                                  uint16_t *__restrict__ out) {
     const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
@@ -775,7 +829,8 @@ __global__ void __launch_bounds__(256) k_colsum_reduce(const double *__restrict_
 // summed with cross-lane shuffles.
 // ---------------------------------------------------------------------------
 
-template <int KP, bool OFFSET = false>  // OFFSET: + rowb[r] cs[j] (PFlog)
+// BF16: x~ rounded to bf16 (the BF16 sketch form on the general layout)
+template <int KP, bool OFFSET = false, bool BF16 = false>  // OFFSET: + rowb[r] cs[j] (PFlog)
 __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ indptr,
                                                   const uint32_t *__restrict__ idx,
                                                   const float *__restrict__ xt,
@@ -804,7 +859,7 @@ __global__ void __launch_bounds__(256) k_spmm_fwd(const int64_t *__restrict__ in
             const int steps = (cnt + 3) >> 2;
             for (int t = 0; t < steps; ++t) {
                 const uint2 ent = stage[wave][4 * t + g];
-                const float xv = __uint_as_float(ent.y);
+                const float xv = BF16 ? bf16_round(__uint_as_float(ent.y)) : __uint_as_float(ent.y);
                 const float4 *mr = reinterpret_cast<const float4 *>(M + static_cast<int64_t>(ent.x) * KP + l * E);
 #pragma unroll
                 for (int e = 0; e < E / 4; ++e) {
@@ -1606,15 +1661,30 @@ __device__ __forceinline__ void static_for(F &&f) {
 // word 3 of a raw buffer descriptor (gfx9 family: 32-bit data format, no swizzle)
 constexpr int kBufferRsrcWord3 = 0x00020000;
 
-// n_entries: entries in trow / tval including their padding. The kernel works
+// n_entries: entries in trow / tval / tpk including their padding. BFE: the
+// BF16 sketch form, entries from tpk (x~ rounded to bf16). The kernel works
 // on KP columns: columns KP * blockIdx.y .. of panels whose rows hold LD
 // floats (Q, the partials and their column sums), so a 128-column panel runs
 // as two 64-column halves (gridDim.y = 2).
-template <int KP, int LD, int GW, int P0, int P1, int RC, int NW>
+// A prefetched transposed entry: the row in the tile and x~ (FP32 form), or
+// the packed bf16 entry of tpk (BF16 sketch form: bf16(x~) in bits 0-15, the
+// row in bits 16-23).
+template <bool BFE>
+struct TEnt {
+    uint8_t r;  // the loaded type: no extension on the loop edge
+    float v;
+};
+template <>
+struct TEnt<true> {
+    uint32_t e;
+};
+
+template <int KP, int LD, int GW, int P0, int P1, int RC, int NW, bool BFE = false>
 __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__ indptr,
                                                  const uint32_t *__restrict__ tptr,
                                                  const uint8_t *__restrict__ trow,
-                                                 const float *__restrict__ tval, int64_t n_entries,
+                                                 const float *__restrict__ tval,
+                                                 const uint32_t *__restrict__ tpk, int64_t n_entries,
                                                  const float *__restrict__ Q,
                                                  const uint32_t *__restrict__ gperm,
                                                  int nch, int ntr, int tiles_per_tr, int64_t n_obs,
@@ -1694,30 +1764,45 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
         const int64_t left = n_entries - tb_;
         const int64_t rb = left < 0x7fffffff ? left : 0x7fffffff;
         const int64_t vb = 4 * left < 0x7fffffff ? 4 * left : 0x7fffffff;
-        return TileRs{__builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(trow + tb_), 0, static_cast<int>(rb),
-                                                        kBufferRsrcWord3),
-                      __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(tval + tb_), 0, static_cast<int>(vb),
-                                                        kBufferRsrcWord3)};
+        if constexpr (BFE) {  // one descriptor: tpk
+            const __amdgpu_buffer_rsrc_t pk = __builtin_amdgcn_make_buffer_rsrc(
+                const_cast<uint32_t *>(tpk + tb_), 0, static_cast<int>(vb), kBufferRsrcWord3);
+            return TileRs{pk, pk};
+        } else {
+            return TileRs{__builtin_amdgcn_make_buffer_rsrc(const_cast<uint8_t *>(trow + tb_), 0,
+                                                            static_cast<int>(rb), kBufferRsrcWord3),
+                          __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(tval + tb_), 0, static_cast<int>(vb),
+                                                            kBufferRsrcWord3)};
+        }
     };
-    auto issue = [&](const TileRs &rs, uint32_t lo_, int c, uint8_t &pr, float &pv) {
+    auto issue = [&](const TileRs &rs, uint32_t lo_, int c, TEnt<BFE> &pe) {
         const uint32_t o = lo_ + static_cast<uint32_t>(l);
-        pr = __builtin_amdgcn_raw_buffer_load_b8(rs.row, static_cast<int>(o + 16 * c), 0, 0);
-        pv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs.val, static_cast<int>(4 * o + 64 * c), 0, 0));
+        if constexpr (BFE) {
+            pe.e = __builtin_amdgcn_raw_buffer_load_b32(rs.row, static_cast<int>(4 * o + 64 * c), 0, 0);
+        } else {
+            pe.r = __builtin_amdgcn_raw_buffer_load_b8(rs.row, static_cast<int>(o + 16 * c), 0, 0);
+            pe.v = __builtin_bit_cast(float,
+                                      __builtin_amdgcn_raw_buffer_load_b32(rs.val, static_cast<int>(4 * o + 64 * c), 0, 0));
+        }
     };
-    auto prep = [&](uint8_t pr, float pv, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
+    auto prep = [&](const TEnt<BFE> &pe, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
         const bool ok = j0 + static_cast<uint32_t>(l) < len_;
-        ofs = ok ? static_cast<uint32_t>(pr) * (KP * 4) : 0u;
-        x = ok ? pv : 0.f;
+        if constexpr (BFE) {
+            ofs = ok ? ((pe.e >> 16) & 255u) * (KP * 4) : 0u;
+            x = ok ? __builtin_bit_cast(float, pe.e << 16) : 0.f;
+        } else {
+            ofs = ok ? static_cast<uint32_t>(pe.r) * (KP * 4) : 0u;
+            x = ok ? pe.v : 0.f;
+        }
     };
     // prefetched components: P0 for slot 0, P1 for slot 1, one for the others
     constexpr int PM = P0 > P1 ? P0 : P1;
     auto ncomp = [](int gi) { return gi == 0 ? P0 : (gi == 1 ? P1 : 1); };
-    uint8_t prow[GW][PM];  // the loaded type: no extension on the loop edge
-    float pval[GW][PM];
+    TEnt<BFE> pent[GW][PM];
     auto issue_slot = [&](int gi, const TileRs &rs, uint32_t lo_) {
 #pragma unroll
         for (int c = 0; c < PM; ++c)
-            if (c < ncomp(gi)) issue(rs, lo_, c, prow[gi][c], pval[gi][c]);
+            if (c < ncomp(gi)) issue(rs, lo_, c, pent[gi][c]);
     };
     auto slot_bounds = [&](auto gic, uint32_t wl_, uint32_t wh_, uint32_t &lo_, uint32_t &len_) {
         constexpr int GI = decltype(gic)::value;
@@ -1810,17 +1895,16 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
             float x[NC];
 #pragma unroll
             for (int c = 0; c < NC; ++c) {
-                prep(prow[gi][c], pval[gi][c], len_, 16u * c, ofs[c], x[c]);
+                prep(pent[gi][c], len_, 16u * c, ofs[c], x[c]);
                 pin_regs(ofs[c], x[c]);  // read before the same registers are reloaded
             }
             if (gi == 0 && tn != t) load_q(tn, bi ^ 1);
             issue_slot(gi, nrs, nlo_);
             // the ring: entries from 16 NC on, 16 RC per group per chunk
-            uint8_t rr[RC];
-            float rv[RC];
+            TEnt<BFE> rent[RC];
             if (sm > 16 * NC) {
 #pragma unroll
-                for (int c = 0; c < RC; ++c) issue(rs, lo_, NC + c, rr[c], rv[c]);
+                for (int c = 0; c < RC; ++c) issue(rs, lo_, NC + c, rent[c]);
             }
             if (sm > 0) gather_run<E, NC>(z[gi], ofs, x, sm < 16 * NC ? sm : 16 * NC, qb, lane_ofs);
             done += sm < 16 * NC ? sm : 16 * NC;
@@ -1831,12 +1915,12 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
                 float xr[RC];
 #pragma unroll
                 for (int c = 0; c < RC; ++c) {
-                    prep(rr[c], rv[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
+                    prep(rent[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
                     pin_regs(ofr[c], xr[c]);
                 }
                 if (o + 16 * RC < sm) {
 #pragma unroll
-                    for (int c = 0; c < RC; ++c) issue(rs, lo_ + static_cast<uint32_t>(o + 16 * RC), c, rr[c], rv[c]);
+                    for (int c = 0; c < RC; ++c) issue(rs, lo_ + static_cast<uint32_t>(o + 16 * RC), c, rent[c]);
                 }
                 const int left = sm - o;
                 gather_run<E, RC>(z[gi], ofr, xr, left < 16 * RC ? left : 16 * RC, qb, lane_ofs);
@@ -1896,10 +1980,25 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
 // The kernel works on KP columns: columns KP * blockIdx.y .. of panels whose
 // rows hold LD floats (M, Y, mc, cs), so a 128-column panel runs as two
 // 64-column halves (gridDim.y = 2).
-template <int KP, int LD, int R, int NW, int CG, int PF, int PC, int RC, bool OFFSET = false>
+// A prefetched forward entry: the gene mod kChunkMod and x~ (FP32 form), or
+// the packed bf16 entry of xpk (BF16 sketch form: gene mod kChunkMod in bits
+// 0-15, bf16(x~) in bits 16-31).
+template <bool BFE>
+struct FEnt {
+    uint16_t i;  // the loaded type: no extension on the loop edge
+    float v;
+};
+template <>
+struct FEnt<true> {
+    uint32_t e;
+};
+
+// BFE: the BF16 sketch form, entries from xpk.
+template <int KP, int LD, int R, int NW, int CG, int PF, int PC, int RC, bool OFFSET = false, bool BFE = false>
 __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict__ indptr,
                                                        const uint16_t *__restrict__ lg,
                                                        const float *__restrict__ xt,
+                                                       const uint32_t *__restrict__ xpk,
                                                        const float *__restrict__ M,
                                                        const float *__restrict__ mc,
                                                        float *__restrict__ Y, const uint32_t *__restrict__ spl,
@@ -1968,18 +2067,31 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
     const int64_t lb = 2 * left < 0x7fffffff ? 2 * left : 0x7fffffff;
     const int64_t vb = 4 * left < 0x7fffffff ? 4 * left : 0x7fffffff;
     const __amdgpu_buffer_rsrc_t rs_lg =
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(lg + wbase), 0, static_cast<int>(lb), kBufferRsrcWord3);
+        BFE ? __builtin_amdgcn_make_buffer_rsrc(const_cast<uint32_t *>(xpk + wbase), 0, static_cast<int>(vb),
+                                                kBufferRsrcWord3)
+            : __builtin_amdgcn_make_buffer_rsrc(const_cast<uint16_t *>(lg + wbase), 0, static_cast<int>(lb),
+                                                kBufferRsrcWord3);
     const __amdgpu_buffer_rsrc_t rs_xt =
         __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(xt + wbase), 0, static_cast<int>(vb), kBufferRsrcWord3);
-    auto issue = [&](uint32_t lo_, int c, uint16_t &pi, float &pv) {
+    auto issue = [&](uint32_t lo_, int c, FEnt<BFE> &pe) {
         const uint32_t o = lo_ + static_cast<uint32_t>(l);
-        pi = __builtin_amdgcn_raw_buffer_load_b16(rs_lg, static_cast<int>(2 * o + 32 * c), 0, 0);
-        pv = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs_xt, static_cast<int>(4 * o + 64 * c), 0, 0));
+        if constexpr (BFE) {
+            pe.e = __builtin_amdgcn_raw_buffer_load_b32(rs_lg, static_cast<int>(4 * o + 64 * c), 0, 0);
+        } else {
+            pe.i = __builtin_amdgcn_raw_buffer_load_b16(rs_lg, static_cast<int>(2 * o + 32 * c), 0, 0);
+            pe.v = __builtin_bit_cast(float,
+                                      __builtin_amdgcn_raw_buffer_load_b32(rs_xt, static_cast<int>(4 * o + 64 * c), 0, 0));
+        }
     };
-    auto prep = [&](uint16_t pi, float pv, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
+    auto prep = [&](const FEnt<BFE> &pe, uint32_t len_, uint32_t j0, uint32_t &ofs, float &x) {
         const bool ok = j0 + static_cast<uint32_t>(l) < len_;
-        ofs = ok ? (static_cast<uint32_t>(pi) & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
-        x = ok ? pv : 0.f;
+        if constexpr (BFE) {
+            ofs = ok ? (pe.e & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+            x = ok ? __builtin_bit_cast(float, pe.e & 0xffff0000u) : 0.f;
+        } else {
+            ofs = ok ? (static_cast<uint32_t>(pe.i) & static_cast<uint32_t>(CG - 1)) * (KP * 4) : 0u;
+            x = ok ? pe.v : 0.f;
+        }
     };
     auto win = [&](auto rc_, uint32_t a, uint32_t b, uint32_t &lo_, uint32_t &len_) {
         constexpr int RR = decltype(rc_)::value;
@@ -1987,11 +2099,10 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
         len_ = row_bcast<RR>(b) - lo_;
     };
     // the ring: PC components of the four windows of a slot, PF slots ahead
-    uint16_t pidx[PF][PC];  // the loaded type: no extension on the loop edge
-    float pval[PF][PC];
+    FEnt<BFE> pent[PF][PC];
     auto issue_win = [&](int slot, uint32_t lo_) {
 #pragma unroll
-        for (int c = 0; c < PC; ++c) issue(lo_, c, pidx[slot][c], pval[slot][c]);
+        for (int c = 0; c < PC; ++c) issue(lo_, c, pent[slot][c]);
     };
     static_assert(LD % KP == 0, "whole column halves");
     const int col0 = KP * static_cast<int>(blockIdx.y);
@@ -2045,7 +2156,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
             float x[PC];
 #pragma unroll
             for (int c = 0; c < PC; ++c) {
-                prep(pidx[r % PF][c], pval[r % PF][c], len_, 16u * c, ofs[c], x[c]);
+                prep(pent[r % PF][c], len_, 16u * c, ofs[c], x[c]);
                 pin_regs(ofs[c], x[c]);  // read before the same registers are reloaded
             }
             {  // the ring slot's next window: slot r + PF of this chunk, or of the next
@@ -2057,11 +2168,10 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
                 }
                 issue_win(r % PF, lo2);
             }
-            uint16_t ri[RC];
-            float rv[RC];
+            FEnt<BFE> rent[RC];
             if (sm > 16 * PC) {
 #pragma unroll
-                for (int c = 0; c < RC; ++c) issue(lo_ + 16 * PC, c, ri[c], rv[c]);
+                for (int c = 0; c < RC; ++c) issue(lo_ + 16 * PC, c, rent[c]);
             }
             if (sm > 0) gather_run<E, PC>(acc[r], ofs, x, sm < 16 * PC ? sm : 16 * PC, mb, lane_ofs);
             done += sm < 16 * PC ? sm : 16 * PC;
@@ -2072,12 +2182,12 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_fwd4(const int64_t *__restrict
                 float xr[RC];
 #pragma unroll
                 for (int c = 0; c < RC; ++c) {
-                    prep(ri[c], rv[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
+                    prep(rent[c], len_, static_cast<uint32_t>(o + 16 * c), ofr[c], xr[c]);
                     pin_regs(ofr[c], xr[c]);
                 }
                 if (o + 16 * RC < sm) {
 #pragma unroll
-                    for (int c = 0; c < RC; ++c) issue(lo_ + static_cast<uint32_t>(o + 16 * RC), c, ri[c], rv[c]);
+                    for (int c = 0; c < RC; ++c) issue(lo_ + static_cast<uint32_t>(o + 16 * RC), c, rent[c]);
                 }
                 const int lft = sm - o;
                 gather_run<E, RC>(acc[r], ofr, xr, lft < 16 * RC ? lft : 16 * RC, mb, lane_ofs);

@@ -42,7 +42,6 @@
 
 #include "../../include/scxpca.h"
 #include "kernels.h"
-#include "kernels_mfma.h"
 
 namespace scxpca {
 void gaussian_matrix(uint64_t rows, uint64_t cols, uint64_t seed, double *out);
@@ -215,9 +214,12 @@ struct scxpca_matrix {
     int nsplit = 0;
     DevBuf xt;  // X~ values in CSR order (f32), computed by prepare()
     DevBuf lg;    // gene mod kChunkMod per entry (u16), the forward SpMM's LDS chunk index
+    // the BF16 sketch form's packed entries: xpk (CSR order, bf16(x~) | gene
+    // mod kChunkMod; split layout only) and tpk (tile-CSC order)
+    DevBuf xpk;
+    bool xpk_ready = false, xpk_in_prepare = false;
     bool tpk_ready = false;     // tpk holds the packed bf16 entries of the current prepare
     bool tpk_in_build = false;  // the BF16 form ran on this matrix: the tile-CSC build writes tpk too
-    DevBuf qpk, mpk;         // the sketch products' panels, hi / lo bf16 in the MFMA B-operand layout
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
     DevBuf tptr, trow, tval, tpk;
     int n_tiles = 0;
@@ -292,6 +294,7 @@ struct StageScope {
 int prepare(scxpca_matrix *m) {
     m->stats_ready = false;
     m->tpk_ready = false;
+    m->xpk_ready = false;
     if (m->n_obs == 0) return 0;
     StageScope prof(m, m->stream, SCXPCA_STAGE_PREPARE);
     // gcnt holds the per-gene entry counts, at [n_vars] the validation bits of
@@ -310,6 +313,7 @@ int prepare(scxpca_matrix *m) {
         m->hpf.ensure(sizeof(double) * 2);
     }
     const unsigned row_blocks = std::min(blocks_for(m->n_obs, 4), 65536u);  // one row per wave
+    const bool want_xpk = m->xpk_in_prepare && m->use_split;
     static_assert(kCB == 128, "k_prepare_rows counts 128-gene split boundaries");
     auto prep = (m->flags & kXfLogF64) ? k_prepare_rows<true, true>
                 : (m->flags & (kXfCheck | kXfScale)) ? k_prepare_rows<true, false> : k_prepare_rows<false, false>;
@@ -317,9 +321,11 @@ int prepare(scxpca_matrix *m) {
                                             m->use_split ? m->nsplit : 0,
                                             (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
                                             m->xt.as<float>(), m->lg.as<uint16_t>(),
+                                            want_xpk ? m->xpk.as<uint32_t>() : nullptr,
                                             m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
                                             vflag, want_b ? m->pfb.as<double>() : nullptr, errpos);
     HIPC(hipGetLastError());
+    m->xpk_ready = want_xpk;
     if (m->use_split)
     {
         const int nck = static_cast<int>((m->n_vars + kB4Genes - 1) / kB4Genes);
@@ -504,6 +510,7 @@ void size_derived(scxpca_matrix *m) {
     m->trow.ensure(m->nnz + kEntryPad);
     m->tval.ensure(sizeof(float) * (m->nnz + kEntryPad));
     if (m->tpk_in_build) m->tpk.ensure(sizeof(uint32_t) * (m->nnz + kEntryPad));
+    if (m->xpk_in_prepare && m->use_split) m->xpk.ensure(sizeof(uint32_t) * std::max<int64_t>(m->nnz, 1));
     m->xt.ensure(sizeof(float) * std::max<int64_t>(m->nnz, 1));
     m->lg.ensure(sizeof(uint16_t) * std::max<int64_t>(m->nnz, 1));
     if (m->use_split) {
@@ -613,6 +620,27 @@ void ensure_tpk(scxpca_matrix *m) {
     m->tpk_in_build = true;
     HIPC(hipGetLastError());
     m->tpk_ready = true;
+}
+
+// The BF16 sketch form's forward entries on the split layout: xpk =
+// bf16(x~) << 16 | gene mod kChunkMod, packed from xt / lg once, written by
+// later prepares directly
+void ensure_xpk(scxpca_matrix *m) {
+    if (!m->use_split || m->xpk_ready) return;
+    m->xpk.ensure(sizeof(uint32_t) * std::max<int64_t>(m->nnz, 1));
+    if (m->nnz > 0) {
+        k_pack_row_entries<<<blocks_for(m->nnz), 256, 0, m->stream>>>(m->xt.as<float>(), m->lg.as<uint16_t>(),
+                                                                      m->nnz, m->xpk.as<uint32_t>());
+        HIPC(hipGetLastError());
+    }
+    m->xpk_in_prepare = true;
+    m->xpk_ready = true;
+}
+
+// both packed entry forms of the BF16 sketch passes
+void ensure_bf16_entries(scxpca_matrix *m) {
+    ensure_tpk(m);
+    ensure_xpk(m);
 }
 
 // Column statistics of X~ (sum, sum_sq, nnz per gene) from the tile-CSC,
@@ -767,11 +795,7 @@ struct Engine {
         m->b64.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64a.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
         m->t64b.ensure(sizeof(double) * std::max<int64_t>(nv, 1) * KP);
-        {
-            int ngg = 0, nrb_m = 0, tpr_m = 0;  // the BF16 form's transposed partials share tpart
-            t_mfma_geometry(m, ngg, nrb_m, tpr_m);
-            m->tpart.ensure(sizeof(float) * static_cast<size_t>(std::max(nrb_t, nrb_m)) * std::max<int64_t>(nv, 1) * KP);
-        }
+        m->tpart.ensure(sizeof(float) * static_cast<size_t>(nrb_t) * std::max<int64_t>(nv, 1) * KP);
         const int nb64 = gram64_blocks();
         m->gpart.ensure(sizeof(double) * static_cast<size_t>(std::max(std::max(gram_waves, gram_blocks), nb64)) * KP * KP);
         m->cpart.ensure(sizeof(double) * static_cast<size_t>(std::max(kMaxColBlocks, nrb_t * nch_t)) * KP);
@@ -827,50 +851,23 @@ struct Engine {
         check();
     }
 
-    template <bool OFF>
-    void launch_fwd(const float *panel, const float *mcp, bool bf16, float *Y) {
-        if (bf16) {  // the BF16 sketch form: densified tiles on the bf16 matrix cores
-            const int64_t kpad = static_cast<int64_t>(mfma_nch()) * kFG;
-            m->mpk.ensure(sizeof(uint16_t) * 2 * static_cast<size_t>(std::max<int64_t>(kpad, 8)) * KP);
-            uint16_t *mh = m->mpk.as<uint16_t>(), *ml = mh + static_cast<size_t>(kpad) * KP;
-            k_pack_hilo<KP><<<blocks_for(kpad / 8 * KP), 256, 0, s>>>(panel, m->n_vars, kpad, mh, ml);
-            check();
-            const unsigned grid = static_cast<unsigned>((m->n_tiles + 1) / 2 * (KP / 64));
-            k_spmm_fwd_mfma<OFF, kMfmaPrefetch><<<grid, 512, 0, s>>>(m->indptr, m->tptr.as<uint32_t>(),
-                                                                     m->tpk.as<uint32_t>(), mh, ml, KP, KP / 64,
-                                                                     m->n_obs, m->n_vars, m->n_tiles, mcp, rowb,
-                                                                     cs32, Y);
-            return;
-        }
+    // bf16: the BF16 sketch form, x~ rounded to bf16 (the packed entries xpk
+    // on the split layout)
+    template <bool OFF, bool BF>
+    void launch_fwd(const float *panel, const float *mcp, float *Y) {
         if (m->use_split) {  // rows owned per 16-lane group: 64 rows per wave, 1024 per block
             constexpr int NWF = 16;
             constexpr int CG4 = 512;  // genes per 128 KB chunk of a 64-column half
             const int nch4 = static_cast<int>((m->n_vars + CG4 - 1) / CG4);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * 64 - 1) / (NWF * 64));
-            k_spmm_fwd4<64, KP, 16, NWF, CG4, F4PF, F4PC, F4RC, OFF><<<dim3(grid, KP / 64), NWF * 64, 0, s>>>(
-                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), panel, mcp, Y, m->spl.as<uint32_t>(),
-                m->nsplit, CG4 / kCB, nch4, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
+            k_spmm_fwd4<64, KP, 16, NWF, CG4, F4PF, F4PC, F4RC, OFF, BF><<<dim3(grid, KP / 64), NWF * 64, 0, s>>>(
+                m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), BF ? m->xpk.as<uint32_t>() : nullptr, panel,
+                mcp, Y, m->spl.as<uint32_t>(), m->nsplit, CG4 / kCB, nch4, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
             const unsigned grid = std::min<unsigned>(blocks_for(m->n_obs, 4), 16384u);
-            k_spmm_fwd<KP, OFF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), panel,
-                                                       mcp, Y, m->n_obs, rowb, cs32);
+            k_spmm_fwd<KP, OFF, BF><<<grid, 256, 0, s>>>(m->indptr, m->idx, m->xt.as<float>(), panel, mcp, Y,
+                                                           m->n_obs, rowb, cs32);
         }
-    }
-
-    // 64-gene chunks of the bf16 MFMA forward product; entry batches each wave
-    // of the bf16 MFMA kernels loads one step ahead
-    static constexpr int kMfmaPrefetch = 28;
-    static constexpr int kTWaves = 4;  // waves (64 genes each) per block of the bf16 transposed product
-    int mfma_nch() const { return static_cast<int>((m->n_vars + kFG - 1) / kFG); }
-    // grid of the bf16 MFMA transposed product: tile ranges (a multiple of 8,
-    // enough blocks to fill 256 CUs about twice) x 256-gene groups x column halves
-    static void t_mfma_geometry(const scxpca_matrix *mm, int &ngg, int &nrb, int &tpr) {
-        ngg = static_cast<int>((mm->n_vars + kTWaves * kTG - 1) / (kTWaves * kTG));
-        const int per = std::max(1, ngg * (KP / 64));
-        int ntr = std::max(1, (512 + per - 1) / per);
-        ntr = std::min(ntr, std::max(1, mm->n_tiles));
-        nrb = (ntr + 7) / 8 * 8;
-        tpr = (mm->n_tiles + nrb - 1) / nrb;
     }
 
     // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
@@ -879,8 +876,13 @@ struct Engine {
         if (rowb) weighted_colsum(panel, nullptr, m->n_vars, cs64, cs32);
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
         const float *mcp = centered ? mc32 : nullptr;
-        if (rowb) launch_fwd<true>(panel, mcp, bf16, Y);
-        else launch_fwd<false>(panel, mcp, bf16, Y);
+        if (rowb) {
+            if (bf16) launch_fwd<true, true>(panel, mcp, Y);
+            else launch_fwd<true, false>(panel, mcp, Y);
+        } else {
+            if (bf16) launch_fwd<false, true>(panel, mcp, Y);
+            else launch_fwd<false, false>(panel, mcp, Y);
+        }
         check();
     }
 
@@ -895,18 +897,24 @@ struct Engine {
     // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix,
     // rowb its baseline; cpu.rs:653-687) over the buffers in sc. mu null: no
     // centering. Every partial is written by the launch (the grid covers all
-    // nrb x nch slots), so sc needs no clearing between calls.
+    // nrb x nch slots), so sc needs no clearing between calls. bf16: the BF16
+    // sketch form (x~ rounded to bf16, the packed entries tpk).
     static void transposed(scxpca_matrix *mm, hipStream_t st, const uint32_t *gp, int nch, int nrb, int tpr,
                            const float *Q, const double *mu, const double *rowb, const TScratch &sc,
-                           double *B64, float *B32) {
+                           double *B64, float *B32, bool bf16 = false) {
         const int64_t nv = mm->n_vars;
         double *qsp = mu ? sc.qsp : nullptr;
         if (mm->n_obs > 0) {
-            const unsigned grid = static_cast<unsigned>(nrb * nch);
-            StageScope prof(mm, st, SCXPCA_STAGE_SPMM_T);
-            k_spmm_t7<64, KP, GW, P0, P1, RC, TW><<<dim3(grid, KP / 64), TW * 64, 0, st>>>(
-                mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(),
-                mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
+            const dim3 grid(static_cast<unsigned>(nrb * nch), KP / 64);
+            StageScope prof(mm, st, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
+            if (bf16)
+                k_spmm_t7<64, KP, GW, P0, P1, RC, TW, true><<<grid, TW * 64, 0, st>>>(
+                    mm->indptr, mm->tptr.as<uint32_t>(), nullptr, nullptr, mm->tpk.as<uint32_t>(),
+                    mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
+            else
+                k_spmm_t7<64, KP, GW, P0, P1, RC, TW, false><<<grid, TW * 64, 0, st>>>(
+                    mm->indptr, mm->tptr.as<uint32_t>(), mm->trow.as<uint8_t>(), mm->tval.as<float>(), nullptr,
+                    mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
             HIPC(hipGetLastError());
         } else {  // no rows: zero partials (a fill kernel, no memset node)
             k_zero_words<<<blocks_for(static_cast<int64_t>(nrb) * nv * KP), 256, 0, st>>>(
@@ -932,42 +940,10 @@ struct Engine {
     // B64 (and/or B32) = X~^T Q - mu sumq^T (+ 1 (b^T Q) for a PFlog matrix)
     // on the matrix's own scratch
     void spmm_t(const float *Q, bool centered, bool bf16, double *B64, float *B32) {
-        if (bf16 && m->n_obs > 0) {
-            spmm_t_mfma(Q, centered, B64, B32);
-            return;
-        }
         // 1^T Q partials in cpart (nrb_t x nch_t x KP), then b^T Q's in the
         // same buffer: colsum_into runs after the 1^T Q reduction
         const TScratch sc{m->tpart.as<float>(), m->cpart.as<double>(), m->cpart.as<double>(), sumq, bq64};
-        transposed(m, s, gperm, nch_t, nrb_t, tiles_per_tr, Q, centered ? mu : nullptr, rowb, sc, B64, B32);
-    }
-
-    // The BF16 sketch form's transposed product: Q split into hi / lo bf16
-    // (packed per 256-row tile), densified X~ tiles on the bf16 matrix cores
-    // into tile-range partials, 1^T Q (and b^T Q) in f64 from the f32 Q, then
-    // the same reduction and centering as the FP32 form.
-    void spmm_t_mfma(const float *Q, bool centered, double *B64, float *B32) {
-        const int64_t nv = m->n_vars;
-        int ngg = 0, nrb = 0, tpr = 0;
-        t_mfma_geometry(m, ngg, nrb, tpr);
-        const int64_t kpad = static_cast<int64_t>(m->n_tiles) * kTileRows;
-        m->qpk.ensure(sizeof(uint16_t) * 2 * static_cast<size_t>(kpad) * KP);
-        uint16_t *qh = m->qpk.as<uint16_t>(), *ql = qh + static_cast<size_t>(kpad) * KP;
-        {
-            StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_BF16);
-            k_pack_hilo<KP><<<blocks_for(kpad / 8 * KP), 256, 0, s>>>(Q, m->n_obs, kpad, qh, ql);
-            check();
-            k_spmm_t_mfma<kTWaves, kMfmaPrefetch><<<static_cast<unsigned>(nrb * ngg * (KP / 64)), kTWaves * 64, 0, s>>>(
-                m->indptr, m->tptr.as<uint32_t>(), m->tpk.as<uint32_t>(), qh, ql, KP, ngg, KP / 64, nrb, tpr, nv,
-                m->n_tiles, m->tpart.as<float>());
-            check();
-        }
-        if (centered) weighted_colsum(Q, nullptr, m->n_obs, sumq, nullptr);
-        if (rowb) weighted_colsum(Q, rowb, m->n_obs, bq64, nullptr);
-        StageScope prof(m, s, SCXPCA_STAGE_SPMM_T_REDUCE);
-        k_spmm_t_reduce<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->tpart.as<float>(), nrb, nv, centered ? mu : nullptr,
-                                                                 sumq, B64, B32, rowb ? bq64 : nullptr);
-        check();
+        transposed(m, s, gperm, nch_t, nrb_t, tiles_per_tr, Q, centered ? mu : nullptr, rowb, sc, B64, B32, bf16);
     }
 
     // One Cholesky step on the current Gram G: W32/W64 = R^-1, R -> Rdst.
@@ -1176,7 +1152,7 @@ struct Engine {
         const bool sketch_bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(status, 0, sizeof(int), s));
-        if (sketch_bf16) ensure_tpk(m);
+        if (sketch_bf16) ensure_bf16_entries(m);
         ops.stats();
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), baseline_mean(), mu);
         check();
@@ -1596,7 +1572,7 @@ struct WideEngine {
         const bool bf16 = p->precision == SCXPCA_PRECISION_BF16_SKETCH;
         const int nc = static_cast<int>(p->n_comps);
         HIPC(hipMemsetAsync(eg.status, 0, sizeof(int), s));
-        if (bf16) ensure_tpk(m);
+        if (bf16) ensure_bf16_entries(m);
         ops.stats();
         k_mean<<<blocks_for(nv), 256, 0, s>>>(m->sum.as<double>(), nv, static_cast<double>(n), eg.baseline_mean(), eg.mu);
         check();
@@ -1700,7 +1676,7 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     validate(m, p);
     const int64_t k = pca_k(m, p);
     DeviceGuard dg(m->device);
-    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_tpk(m);
+    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_bf16_entries(m);
     ensure_stats(m);
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
@@ -1765,7 +1741,7 @@ void stage_spmm(scxpca_matrix *m, int transpose, int64_t k, const double *in, co
     Engine<KP> eng(m, static_cast<int>(k));
     const int64_t n = m->n_obs, nv = m->n_vars;
     hipStream_t s = m->stream;
-    if (bf16) ensure_tpk(m);
+    if (bf16) ensure_bf16_entries(m);
     const bool centered = means != nullptr;
     if (centered) HIPC(hipMemcpyAsync(eng.mu, means, sizeof(double) * nv, hipMemcpyHostToDevice, s));
     if (!transpose) {
