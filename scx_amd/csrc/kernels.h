@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             if (ok) {
                 const float d = xtilde<LOGF64>(v[u], f, flags);
                 xt[p] = d;
-                lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
+                if (lg) lg[p] = static_cast<uint16_t>(c[u] & (kChunkMod - 1));
                 if (xpk) xpk[p] = pack_bf16_row_entry(d, c[u] & (kChunkMod - 1));
                 if constexpr (PFLOG) {
                     dsum += static_cast<double>(d);
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
                 const float v = val[p];
                 const float d = xtilde<LOGF64>(v, f, flags);
                 xt[p] = d;
-                lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
+                if (lg) lg[p] = static_cast<uint16_t>(c & (kChunkMod - 1));
                 if (xpk) xpk[p] = pack_bf16_row_entry(d, c & (kChunkMod - 1));
                 if constexpr (PFLOG) {
                     dsum += static_cast<double>(d);
@@ -617,6 +617,13 @@ __global__ void __launch_bounds__(256) k_pack_tile_entries(const int64_t *__rest
         }
         if (p < end) tpk[tb + p] = pack_bf16_entry(tval[tb + p], trow[tb + p], static_cast<uint32_t>(g0 + lo));
     }
+}
+
+// lg[p] = the gene field of xpk[p]
+__global__ void __launch_bounds__(256) k_unpack_row_genes(const uint32_t *__restrict__ xpk, int64_t nnz,
+                                                          uint16_t *__restrict__ lg) {
+    const int64_t p = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p < nnz) lg[p] = static_cast<uint16_t>(xpk[p]);
 }
 
 // xpk[p] = pack_bf16_row_entry(xt[p], lg[p])

@@ -218,6 +218,9 @@ struct scxpca_matrix {
     // mod kChunkMod; split layout only) and tpk (tile-CSC order)
     DevBuf xpk;
     bool xpk_ready = false, xpk_in_prepare = false;
+    // lg of the current prepare is written (a prepare that writes xpk leaves
+    // it out; the FP32 forward unpacks it from xpk when it runs)
+    bool lg_ready = false;
     bool tpk_ready = false;     // tpk holds the packed bf16 entries of the current prepare
     bool tpk_in_build = false;  // the BF16 form ran on this matrix: the tile-CSC build writes tpk too
     // row-tiled CSC (tiles of kTileRows rows, gene-major inside a tile)
@@ -320,12 +323,13 @@ int prepare(scxpca_matrix *m) {
     prep<<<row_blocks, 256, 0, m->stream>>>(m->indptr, m->idx, m->val, m->n_obs, m->target, m->flags,
                                             m->use_split ? m->nsplit : 0,
                                             (m->flags & kXfNorm) ? m->rowf.as<double>() : nullptr,
-                                            m->xt.as<float>(), m->lg.as<uint16_t>(),
+                                            m->xt.as<float>(), want_xpk ? nullptr : m->lg.as<uint16_t>(),
                                             want_xpk ? m->xpk.as<uint32_t>() : nullptr,
                                             m->use_split ? m->spl.as<uint32_t>() : nullptr, m->n_vars,
                                             vflag, want_b ? m->pfb.as<double>() : nullptr, errpos);
     HIPC(hipGetLastError());
     m->xpk_ready = want_xpk;
+    m->lg_ready = !want_xpk;
     if (m->use_split)
     {
         const int nck = static_cast<int>((m->n_vars + kB4Genes - 1) / kB4Genes);
@@ -637,6 +641,17 @@ void ensure_xpk(scxpca_matrix *m) {
     m->xpk_ready = true;
 }
 
+// lg for the FP32 forward after a prepare that wrote xpk instead
+void ensure_lg(scxpca_matrix *m) {
+    if (m->lg_ready || !m->use_split) return;
+    if (m->nnz > 0) {
+        k_unpack_row_genes<<<blocks_for(m->nnz), 256, 0, m->stream>>>(m->xpk.as<uint32_t>(), m->nnz,
+                                                                      m->lg.as<uint16_t>());
+        HIPC(hipGetLastError());
+    }
+    m->lg_ready = true;
+}
+
 // both packed entry forms of the BF16 sketch passes
 void ensure_bf16_entries(scxpca_matrix *m) {
     ensure_tpk(m);
@@ -748,6 +763,10 @@ struct Engine {
     static constexpr int F4PF = 2;                  // k_spmm_fwd4: ring depth, slots
     static constexpr int F4PC = 2;                  // k_spmm_fwd4: prefetched 16-entry components per window
     static constexpr int F4RC = 1;                  // k_spmm_fwd4: components per overflow load
+    // the BF16 form's packed entries take one register per prefetched
+    // component instead of two: deeper prefetch in the same budget
+    static constexpr int P1B = 4, RCB = 4;          // k_spmm_t7<..., BFE>
+    static constexpr int F4PCB = 3, F4RCB = 2;      // k_spmm_fwd4<..., BFE>
     scxpca_matrix *m;
     hipStream_t s;
     int k = 0;
@@ -860,7 +879,8 @@ struct Engine {
             constexpr int CG4 = 512;  // genes per 128 KB chunk of a 64-column half
             const int nch4 = static_cast<int>((m->n_vars + CG4 - 1) / CG4);
             const unsigned grid = static_cast<unsigned>((m->n_obs + NWF * 64 - 1) / (NWF * 64));
-            k_spmm_fwd4<64, KP, 16, NWF, CG4, F4PF, F4PC, F4RC, OFF, BF><<<dim3(grid, KP / 64), NWF * 64, 0, s>>>(
+            k_spmm_fwd4<64, KP, 16, NWF, CG4, F4PF, BF ? F4PCB : F4PC, BF ? F4RCB : F4RC, OFF, BF>
+                <<<dim3(grid, KP / 64), NWF * 64, 0, s>>>(
                 m->indptr, m->lg.as<uint16_t>(), m->xt.as<float>(), BF ? m->xpk.as<uint32_t>() : nullptr, panel,
                 mcp, Y, m->spl.as<uint32_t>(), m->nsplit, CG4 / kCB, nch4, m->n_obs, m->n_vars, m->nnz, rowb, cs32);
         } else {
@@ -873,6 +893,7 @@ struct Engine {
     // Y = X~ panel - 1 mc^T (+ b (1^T panel)^T for a PFlog matrix, cpu.rs:617-647)
     void spmm_fwd(const float *panel, bool centered, bool bf16, float *Y) {
         if (m->n_obs == 0) return;
+        if (!bf16) ensure_lg(m);
         if (rowb) weighted_colsum(panel, nullptr, m->n_vars, cs64, cs32);
         StageScope prof(m, s, bf16 ? SCXPCA_STAGE_SPMM_FWD_BF16 : SCXPCA_STAGE_SPMM_FWD);
         const float *mcp = centered ? mc32 : nullptr;
@@ -908,7 +929,7 @@ struct Engine {
             const dim3 grid(static_cast<unsigned>(nrb * nch), KP / 64);
             StageScope prof(mm, st, bf16 ? SCXPCA_STAGE_SPMM_T_BF16 : SCXPCA_STAGE_SPMM_T);
             if (bf16)
-                k_spmm_t7<64, KP, GW, P0, P1, RC, TW, true><<<grid, TW * 64, 0, st>>>(
+                k_spmm_t7<64, KP, GW, P0, P1B, RCB, TW, true><<<grid, TW * 64, 0, st>>>(
                     mm->indptr, mm->tptr.as<uint32_t>(), nullptr, nullptr, mm->tpk.as<uint32_t>(),
                     mm->nnz + kEntryPad, Q, gp, nch, nrb, tpr, mm->n_obs, nv, mm->n_tiles, sc.tpart, qsp);
             else

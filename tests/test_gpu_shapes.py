@@ -173,21 +173,15 @@ def _bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-def _hilo(a):
-    """The panel as the bf16 products see it: bf16(a) + bf16(a - bf16(a))."""
-    a32 = np.asarray(a, dtype=np.float32)
-    hi = _bf16(a32)
-    return hi + _bf16((a32.astype(np.float64) - hi).astype(np.float32))
-
-
 @pytest.mark.parametrize("k", [60, 100, 5])
 def test_spmm_bf16(case, k):
-    """The BF16 sketch form's products (densified bf16 tiles on the matrix
-    cores): both directions against the f64 product of bf16-rounded X~ and the
-    panel's hi + lo bf16 split, 2e-5 of the f64 magnitude (f32 accumulation
-    and the centering's cancellation, the FP32 form's stage bar), and against
-    the FP32 form's product to the bf16 rounding of X~ (2^-9 per value; 1e-2
-    of the magnitude after the centering's cancellation)."""
+    """The BF16 sketch form's products: x~ rounded to bf16 (the packed
+    entries), times the f32 panel with f32 accumulation. Both directions
+    against the f64 product of bf16-rounded X~ and the panel, 2e-5 of the f64
+    magnitude (f32 accumulation and the centering's cancellation, the FP32
+    form's stage bar), and against the FP32 form's product to the bf16
+    rounding of X~ (2^-9 per value; 1e-2 of the magnitude after the
+    centering's cancellation)."""
     name, csr, mat, xt, mu = case
     xb = xt.with_data(_bf16(xt.data).astype(np.float32))
     rng = np.random.default_rng(200 + k)
@@ -195,9 +189,11 @@ def test_spmm_bf16(case, k):
     q = rng.standard_normal((csr.shape[0], k))
     say(f"case: {name} bf16 forward + transpose k={k}")
     got_f = mat.spmm(m, transpose=False, means=mu, bf16=True)
-    ref_f = oracle.spmm_forward(xb, _hilo(m), mu)
     got_t = mat.spmm(q, transpose=True, means=mu, bf16=True)
-    ref_t = oracle.spmm_transpose(xb, _hilo(q), mu)
+    m32 = m.astype(np.float32).astype(np.float64)
+    q32 = q.astype(np.float32).astype(np.float64)
+    ref_f = oracle.spmm_forward(xb, m32, mu)
+    ref_t = oracle.spmm_transpose(xb, q32, mu)
     ef = np.abs(got_f - ref_f).max() / np.abs(ref_f).max()
     et = np.abs(got_t - ref_t).max() / np.abs(ref_t).max()
     f32_f = oracle.spmm_forward(xt, m, mu)
