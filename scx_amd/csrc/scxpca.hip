@@ -760,6 +760,7 @@ struct Engine {
     static constexpr int TW = 16;                   // k_spmm_t7: waves per block
     static constexpr int TU = 4 * TW;               // k_spmm_t7: gene owners (16-lane groups) per block
     static constexpr int CG = TU * GW;              // k_spmm_t7: genes per block
+    static constexpr int64_t kFwd4MinRows = 256 * 1024;  // k_spmm_fwd4: one 1024-row block per CU
     static constexpr int F4PF = 2;                  // k_spmm_fwd4: ring depth, slots
     static constexpr int F4PC = 2;                  // k_spmm_fwd4: prefetched 16-entry components per window
     static constexpr int F4RC = 1;                  // k_spmm_fwd4: components per overflow load
@@ -874,7 +875,10 @@ struct Engine {
     // on the split layout)
     template <bool OFF, bool BF>
     void launch_fwd(const float *panel, const float *mcp, float *Y) {
-        if (m->use_split) {  // rows owned per 16-lane group: 64 rows per wave, 1024 per block
+        // rows owned per 16-lane group (64 rows per wave, 1024 per block) once
+        // the grid fills the 256 CUs; below that the row-parallel kernel (one
+        // wave per row, the panel from L2) has the parallelism
+        if (m->use_split && m->n_obs >= kFwd4MinRows) {
             constexpr int NWF = 16;
             constexpr int CG4 = 512;  // genes per 128 KB chunk of a 64-column half
             const int nch4 = static_cast<int>((m->n_vars + CG4 - 1) / CG4);
