@@ -1054,13 +1054,19 @@ __global__ void __launch_bounds__(256) k_small_gemm64(const double *__restrict__
     }
 }
 
-// Embedding weights W_emb = (V Sigma)[:, :nc] as f32 KP x KP (zero padded).
+// Embedding weights W_emb = Rinv (V Sigma)[:, :nc] as f32 KP x KP (zero
+// padded), Rinv upper triangular (f64 products, summed in index order): the
+// embeddings Q W_emb of a basis Q = Q1 Rinv come from Q1 directly.
 template <int KP>
-__global__ void k_emb_weights(const double *__restrict__ V, const double *__restrict__ sigma, int nc,
-                              float *__restrict__ W) {
+__global__ void __launch_bounds__(256) k_emb_weights(const double *__restrict__ Rinv, const double *__restrict__ V,
+                                                     const double *__restrict__ sigma, int nc,
+                                                     float *__restrict__ W) {
     for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
-        const int j = t % KP;
-        W[t] = j < nc ? static_cast<float>(V[t] * sigma[j]) : 0.f;
+        const int i = t / KP, j = t % KP;
+        double acc = 0.0;
+        if (j < nc)
+            for (int l = i; l < KP; ++l) acc += Rinv[i * KP + l] * V[l * KP + j];
+        W[t] = j < nc ? static_cast<float>(acc * sigma[j]) : 0.f;
     }
 }
 

@@ -772,7 +772,7 @@ struct Engine {
     hipStream_t s;
     int k = 0;
     // small KP x KP f64 scratch layout inside m->small
-    double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *partc, *cs64, *bq64;
+    double *G, *R1, *R2, *R3, *Rt, *W64, *U, *V, *Acol, *Vcol, *sig, *mu, *sumq, *mc64, *Rfin, *cs64, *bq64;
     float *W32, *mc32, *cs32;
     // PFlog operator delta + b 1^T: the rank-1 baseline terms of both SpMMs
     const double *rowb = nullptr;
@@ -823,8 +823,8 @@ struct Engine {
         m->small.ensure(sizeof(double) * (KK * 11 + 6 * KP + std::max<int64_t>(nv, 1)) + sizeof(float) * (KK + 2 * KP));
         double *d = m->small.as<double>();
         G = d; R1 = G + KK; R2 = R1 + KK; R3 = R2 + KK; Rt = R3 + KK; W64 = Rt + KK; U = W64 + KK; V = U + KK;
-        Acol = V + KK; Vcol = Acol + KK; partc = Vcol + KK;  // partc: KK doubles spare
-        sig = partc + KK; sumq = sig + KP; mc64 = sumq + KP; cs64 = mc64 + KP + KP; bq64 = cs64 + KP;
+        Acol = V + KK; Vcol = Acol + KK; Rfin = Vcol + KK;  // Rfin: R^-1 of the final basis's last pass
+        sig = Rfin + KK; sumq = sig + KP; mc64 = sumq + KP; cs64 = mc64 + KP + KP; bq64 = cs64 + KP;
         mu = bq64 + KP;
         W32 = reinterpret_cast<float *>(mu + std::max<int64_t>(nv, 1));
         mc32 = W32 + KK;
@@ -1014,7 +1014,10 @@ struct Engine {
     // above the rank, where the reference's Householder QR still returns an
     // orthonormal basis) gets one: the floored directions are normalised by
     // the following passes. CholeskyQR2 keeps the strict non-SPD error.
-    float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true) {
+    // With apply_last == false the last pass forms no Q: W32 holds its R^-1
+    // and src is returned unchanged, for a caller that applies R^-1 to a
+    // product linear in src.
+    float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true, bool apply_last = true) {
         if (n < 0) n = m->n_obs;
         const int passes = !final ? 1 : (qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2);
         const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, gram_blocks)));
@@ -1036,6 +1039,7 @@ struct Engine {
             check();
             const double shift = !final ? 1e-7 : ((qr_method == SCXPCA_QR_HOUSEHOLDER && pass == 0) ? 1e-5 : 0.0);
             chol(shift, R1, qr_method == SCXPCA_QR_HOUSEHOLDER ? kStableFloor : 0.0);
+            if (!apply_last && pass + 1 == passes) break;
             {
                 StageScope prof(m, s, SCXPCA_STAGE_APPLY);
                 apply(src, tmp, n, KP, KP, true);
@@ -1187,22 +1191,41 @@ struct Engine {
         if (centered) mean_correction(m->panel.as<float>());
         ops.fwd(m->panel.as<float>(), centered, sketch_bf16, Y);
         const int64_t n_it = p->n_power_iterations;
-        float *Q = cqr_tall(Y, T, p->qr_method, -1, n_it == 0);
+        // The last CholeskyQR pass of every tall basis is never applied: the
+        // panel stays as it is with R^-1 in W32 / W64, and since the
+        // transposed product, its centering term and the PFlog term are
+        // linear in the panel, X~^T (Y R^-1) = (X~^T Y) R^-1 is applied to the
+        // n_vars x KP result instead of the n_obs x KP panel (for the final
+        // basis also to the embedding weights).
+        float *Q = cqr_tall(Y, T, p->qr_method, -1, n_it == 0, false);
         float *Qother = (Q == Y) ? T : Y;
         for (int64_t it = 0; it < n_it; ++it) {
             // B (f32 n_vars x KP) -> basis by the tall MFMA CholeskyQR
-            ops.t(Q, centered, sketch_bf16, nullptr, m->panel.as<float>());
+            ops.t(Q, centered, sketch_bf16, nullptr, m->panel2.as<float>());
+            {
+                StageScope prof(m, s, SCXPCA_STAGE_APPLY);
+                apply(m->panel2.as<float>(), m->panel.as<float>(), nv, KP, KP, true);
+            }
             float *qb = cqr_tall(m->panel.as<float>(), m->panel2.as<float>(), p->qr_method, nv, false);
             if (centered) mean_correction(qb);
             ops.fwd(qb, centered, sketch_bf16, Qother);
-            float *Qn = cqr_tall(Qother, Q, p->qr_method, -1, it + 1 == n_it);
+            const bool last = it + 1 == n_it;
+            float *Qn = cqr_tall(Qother, Q, p->qr_method, -1, last, false);
             Qother = (Qn == Q) ? Qother : Q;
             Q = Qn;
         }
-        // Final B pass in FP32 (also in the BF16 sketch form).
+        // Final B pass in FP32 (also in the BF16 sketch form), then B R^-1 in
+        // f64; R^-1 kept (Rfin) for the embedding weights
         ops.t(Q, centered, false, m->b64.as<double>(), nullptr);
+        {
+            StageScope prof(m, s, SCXPCA_STAGE_APPLY);
+            k_apply64<KP><<<blocks_for(nv * KP), 256, 0, s>>>(m->b64.as<double>(), W64, m->t64b.as<double>(), nullptr,
+                                                           nv);
+            check();
+            HIPC(hipMemcpyAsync(Rfin, W64, sizeof(double) * KP * KP, hipMemcpyDeviceToDevice, s));
+        }
         // Rayleigh-Ritz: B = Q_B Rt, Rt = U_R S V^T
-        cqr_small64(m->b64.as<double>(), m->panel64.as<double>());
+        cqr_small64(m->t64b.as<double>(), m->panel64.as<double>());
         {
             StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
             if constexpr (KP == 64)
@@ -1224,7 +1247,7 @@ struct Engine {
         }
         k_components<KP><<<blocks_for(nv * nc), 256, 0, s>>>(m->t64a.as<double>(), nc, nv, comps_dev);
         check();
-        k_emb_weights<KP><<<1, 256, 0, s>>>(V, sig, nc, W32);
+        k_emb_weights<KP><<<1, 256, 0, s>>>(Rfin, V, sig, nc, W32);
         check();
         float *emb_dev = nullptr;
         if (out->on_device) {
