@@ -1328,9 +1328,12 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
 // entries, directly otherwise. Longer waves process their remaining batches
 // from memory in both passes. With psum, the block also writes the tile's
 // per-gene sum and sum of squares of X~ (f64, run order).
+// PK: also writes the packed bf16 entries (tpk; the staged gene bytes cost
+// 8 KB of LDS). Without them the block's LDS (50 KB: a 7,680-entry stage,
+// 16-bit wave offsets) leaves room for three blocks per CU.
 constexpr int kB4Genes = 256;
-constexpr int kB4Stage = 8192;
 constexpr int kB4Batches = 12;
+template <bool PK>
 __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__ indptr,
                                                      const uint32_t *__restrict__ idx,
                                                      const float *__restrict__ xt,
@@ -1340,14 +1343,15 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
                                                      double *__restrict__ psum, double *__restrict__ psq,
                                                      const int *__restrict__ flag, uint32_t *__restrict__ tpk) {
     constexpr int NG = kB4Genes, NW = 8, U = kB4Batches;
+    constexpr int kB4Stage = PK ? 8192 : 7680;
     if (*flag) return;  // the matrix failed validation: nothing here may trust its indices
     constexpr int SPC = NG / 128;  // 128-gene split offsets per chunk
     __shared__ uint32_t mask[NW][NG];
-    __shared__ uint32_t woff[NW][NG];
+    __shared__ uint16_t woff[NW][NG];  // placement offsets < entries of the chunk <= 256 * 256
     __shared__ uint32_t start[NG + 1];
     __shared__ float sval[kB4Stage];
     __shared__ uint8_t srow[kB4Stage];
-    __shared__ uint8_t sgene[kB4Stage];  // gene in the chunk, for the packed bf16 entries
+    __shared__ uint8_t sgene[PK ? kB4Stage : 1];  // gene in the chunk, for the packed bf16 entries
     __shared__ uint32_t wlo[NW], gsum[4];
     const int slot = static_cast<int>(blockIdx.x >> 3), xcd = static_cast<int>(blockIdx.x & 7);
     const int t = (slot / nck) * 8 + xcd, ck = slot % nck;
@@ -1442,7 +1446,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     if (tid < NG) {
 #pragma unroll
         for (int ww = 0; ww < NW; ++ww) {
-            woff[ww][tid] = tot;
+            woff[ww][tid] = static_cast<uint16_t>(tot);
             tot += __popc(mask[ww][tid]);
         }
     }
@@ -1463,7 +1467,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
         for (int ww = 0; ww < w; ++ww) st += gsum[ww];
         start[tid] = st;
 #pragma unroll
-        for (int ww = 0; ww < NW; ++ww) woff[ww][tid] += st;
+        for (int ww = 0; ww < NW; ++ww) woff[ww][tid] = static_cast<uint16_t>(woff[ww][tid] + st);
         if (tid < ng) tptr[static_cast<int64_t>(t) * (n_vars + 1) + g0 + tid] = cbase + st;
     }
     if (tid == 0) {
@@ -1476,16 +1480,16 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     auto place = [&](int row, uint32_t g, float x) {
         const uint32_t gl = g - g0;
         const uint32_t below = row ? (mask[w][gl] & ((1u << row) - 1u)) : 0u;
-        const uint32_t pos = woff[w][gl] + __popc(below);
+        const uint32_t pos = static_cast<uint32_t>(woff[w][gl]) + __popc(below);
         const uint8_t rt = static_cast<uint8_t>(32 * w + row);
         if (staged) {
             srow[pos] = rt;
             sval[pos] = x;
-            sgene[pos] = static_cast<uint8_t>(gl);
+            if constexpr (PK) sgene[pos] = static_cast<uint8_t>(gl);
         } else {
             trow[base + cbase + pos] = rt;
             tval[base + cbase + pos] = x;
-            if (tpk) tpk[base + cbase + pos] = pack_bf16_entry(x, rt, gl);
+            if constexpr (PK) tpk[base + cbase + pos] = pack_bf16_entry(x, rt, gl);
         }
     };
 #pragma unroll
@@ -1504,7 +1508,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
         for (uint32_t q = tid; q < ctotal; q += blockDim.x) {
             trow[base + cbase + q] = srow[q];
             tval[base + cbase + q] = sval[q];
-            if (tpk) tpk[base + cbase + q] = pack_bf16_entry(sval[q], srow[q], sgene[q]);
+            if constexpr (PK) tpk[base + cbase + q] = pack_bf16_entry(sval[q], srow[q], sgene[q]);
         }
     if (psum && tid < ng) {
         const uint32_t a = start[tid], e = start[tid + 1];

@@ -333,7 +333,8 @@ int prepare(scxpca_matrix *m) {
     if (m->use_split)
     {
         const int nck = static_cast<int>((m->n_vars + kB4Genes - 1) / kB4Genes);
-        k_build_tcsc4<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
+        auto build = m->tpk_in_build ? k_build_tcsc4<true> : k_build_tcsc4<false>;
+        build<<<static_cast<unsigned>((m->n_tiles + 7) / 8 * 8) * nck, 512, 0, m->stream>>>(
             m->indptr, m->idx, m->xt.as<float>(), m->spl.as<uint32_t>(), m->nsplit, nck, m->n_obs, m->n_vars,
             m->tptr.as<uint32_t>(), m->trow.as<uint8_t>(), m->tval.as<float>(), m->tsum.as<double>(),
             m->tsq.as<double>(), vflag, m->tpk_in_build ? m->tpk.as<uint32_t>() : nullptr);
