@@ -1855,6 +1855,11 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
         const int tn = t + 1 < t_end ? t + 1 : t;  // next tile (the last one re-reads itself)
         const int tnn = tn + 1 < t_end ? tn + 1 : tn;
         const char *qb = reinterpret_cast<const char *>(qs[bi]);
+        // gathers address both buffers from the array's base: the buffer
+        // offset rides in the lane's column offset (one add per tile, none per
+        // gather step)
+        const char *q0 = reinterpret_cast<const char *>(&qs[0][0]);
+        const uint32_t lane_ofs_t = lane_ofs + static_cast<uint32_t>(bi) * static_cast<uint32_t>(sizeof(qs[0]));
         // this tile's Q loads were followed by at least kAfter entry loads:
         // with at most kAfter loads outstanding they have landed in LDS
         asm volatile("s_waitcnt vmcnt(%0)" ::"i"(kAfter) : "memory");
@@ -1923,7 +1928,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
 #pragma unroll
                 for (int c = 0; c < RC; ++c) issue(rs, lo_, NC + c, rent[c]);
             }
-            if (sm > 0) gather_run<E, NC>(z[gi], ofs, x, sm < 16 * NC ? sm : 16 * NC, qb, lane_ofs);
+            if (sm > 0) gather_run<E, NC>(z[gi], ofs, x, sm < 16 * NC ? sm : 16 * NC, q0, lane_ofs_t);
             done += sm < 16 * NC ? sm : 16 * NC;
             for (int o = 16 * NC; o < sm; o += 16 * RC) {
                 progress_priority(done, total);
@@ -1940,7 +1945,7 @@ __global__ void __launch_bounds__(NW * 64) k_spmm_t7(const int64_t *__restrict__
                     for (int c = 0; c < RC; ++c) issue(rs, lo_ + static_cast<uint32_t>(o + 16 * RC), c, rent[c]);
                 }
                 const int left = sm - o;
-                gather_run<E, RC>(z[gi], ofr, xr, left < 16 * RC ? left : 16 * RC, qb, lane_ofs);
+                gather_run<E, RC>(z[gi], ofr, xr, left < 16 * RC ? left : 16 * RC, q0, lane_ofs_t);
             }
         });
         cl = nl;
