@@ -2871,18 +2871,21 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
 // (double buffered), and every lane updates a[i][c] -= R[j][i] R[j][c] for
 // i > j with R[j][i] read back as a broadcast (measured faster than 63
 // readlanes per step). Lanes c < i keep updating their lower-triangle
-// registers (never read), which keeps the code branch-free. k_trinv_w then
-// inverts R. The pivot rule is k_chol2's; FLOOR selects the floor (per step
+// registers (never read), which keeps the code branch-free. After a block
+// barrier the same wave inverts R (W64 / W32 = R^-1) from its LDS copy, in the
+// same launch. The pivot rule is k_chol2's; FLOOR selects the floor (per step
 // a max with the floor, non-finite values checked on the input and on the
 // final diagonal: per-step compares would cost scalar registers).
 // ---------------------------------------------------------------------------
 template <bool FLOOR>
 __global__ void __launch_bounds__(256) k_chol_w(const double *__restrict__ G, int k, double shift,
-                                               double *__restrict__ R64, int *__restrict__ status,
+                                               double *__restrict__ R64, double *__restrict__ W64,
+                                               float *__restrict__ W32, int *__restrict__ status,
                                                double floor_tau) {
     constexpr int KP = 64;
     __shared__ double rowb[2][KP];
     __shared__ double Rs[KP * KP];
+    __shared__ double rinv[KP];
     // the masked Gram staged row-major in LDS by all four waves (four times
     // the loads in flight of one wave); wave 0 then factors it alone
     for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
@@ -2890,74 +2893,68 @@ __global__ void __launch_bounds__(256) k_chol_w(const double *__restrict__ G, in
         Rs[t] = (i < k && cc < k) ? G[t] : (i == cc ? 1.0 : 0.0);
     }
     __syncthreads();
-    if (threadIdx.x >= KP) return;
     const int c = threadIdx.x;
-    const double tr = wave_sum_dpp(c < k ? Rs[c * KP + c] : 0.0);
-    const double sh = (shift > 0.0 && c < k) ? shift * tr : 0.0;
-    const double mdiag = k > 0 ? (tr + (shift > 0.0 ? shift * tr * k : 0.0)) / k : 0.0;
-    const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // FLOOR: the pivot floor
-    double a[KP];
+    if (c < KP) {
+        const double tr = wave_sum_dpp(c < k ? Rs[c * KP + c] : 0.0);
+        const double sh = (shift > 0.0 && c < k) ? shift * tr : 0.0;
+        const double mdiag = k > 0 ? (tr + (shift > 0.0 ? shift * tr * k : 0.0)) / k : 0.0;
+        const double fl = floor_tau * mdiag > 0.0 ? floor_tau * mdiag : 1.0;  // FLOOR: the pivot floor
+        double a[KP];
 #pragma unroll
-    for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c] + ((i == c) ? sh : 0.0);
-    wave_lds_sync();
-    bool bad = false;
-    double rdiag = 1.0;  // FLOOR: R[c][c], kept by lane c
-    if constexpr (FLOOR) {
-        // non-finite input is checked once here and on the final diagonal, so
-        // the step needs no compare (per-step masks cost scalar registers)
-#pragma unroll
-        for (int i = 0; i < KP; ++i) bad |= !isfinite(a[i]);
-    }
-#pragma unroll
-    for (int j = 0; j < KP; ++j) {
-        const double d = readlane_f64(a[j], j);
-        double dd;  // select the input, not the result: no branch
-        if constexpr (FLOOR) {
-            dd = fmax(d, fl);
-        } else {
-            const bool ok = (d > 0.0) && isfinite(d);
-            bad |= !ok;
-            dd = ok ? d : 1.0;
-        }
-        const double ir = rsq_nr(dd);
-        const double r = dd * ir;
-        const double t = (c == j) ? r : a[j] * ir;
-        if constexpr (FLOOR) rdiag = (c == j) ? r : rdiag;
-        a[j] = t;
-        rowb[j & 1][c] = t;
+        for (int i = 0; i < KP; ++i) a[i] = Rs[i * KP + c] + ((i == c) ? sh : 0.0);
         wave_lds_sync();
+        bool bad = false;
+        double rdiag = 1.0;  // FLOOR: R[c][c], kept by lane c
+        if constexpr (FLOOR) {
+            // non-finite input is checked once here and on the final diagonal, so
+            // the step needs no compare (per-step masks cost scalar registers)
 #pragma unroll
-        for (int i = j + 1; i < KP; ++i) a[i] -= rowb[j & 1][i] * t;
-    }
-    if constexpr (FLOOR) {
-        // R[c][c] = sqrt(max(pivot, floor)): equal to sqrt(floor) where it floored
-        const bool mine = c < k;
-        bad |= mine && !isfinite(rdiag);
-        const bool floored = mine && rdiag * rdiag <= fl * (1.0 + 1e-12);
-        if (__ballot(floored) != 0 && c == 0) atomicOr(status, 4);
-        bad = __ballot(bad) != 0;
-    }
-    if (bad && c == 0) atomicOr(status, 1);
+            for (int i = 0; i < KP; ++i) bad |= !isfinite(a[i]);
+        }
 #pragma unroll
-    for (int i = 0; i < KP; ++i) R64[i * KP + c] = (i <= c && c < k) ? a[i] : 0.0;
-}
-
-// W = R^-1 for the upper-triangular k x k R of k_chol_w (KP = 64, zeros
-// outside), one wave, right-looking: lane c holds column c of X; X = I; for
-// j = 63 .. 0, X[j][:] /= R[j][j], X[i][:] -= R[i][j] X[j][:] for i < j, with
-// R[i][j] read from LDS as broadcasts. Pivots j >= k are taken as 1 (the
-// identity block), so padded columns never reach columns c < k. A separate
-// kernel: inlined into the factorization the two 64-register columns spill.
-__global__ void __launch_bounds__(256) k_trinv_w(const double *__restrict__ R64, int k, double *__restrict__ W64,
-                                                float *__restrict__ W32) {
-    constexpr int KP = 64;
-    __shared__ double Rs[KP * KP];
-    __shared__ double rinv[KP];
-    // R staged by all four waves; wave 0 inverts it alone
-    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) Rs[t] = R64[t];
+        for (int j = 0; j < KP; ++j) {
+            const double d = readlane_f64(a[j], j);
+            double dd;  // select the input, not the result: no branch
+            if constexpr (FLOOR) {
+                dd = fmax(d, fl);
+            } else {
+                const bool ok = (d > 0.0) && isfinite(d);
+                bad |= !ok;
+                dd = ok ? d : 1.0;
+            }
+            const double ir = rsq_nr(dd);
+            const double r = dd * ir;
+            const double t = (c == j) ? r : a[j] * ir;
+            if constexpr (FLOOR) rdiag = (c == j) ? r : rdiag;
+            a[j] = t;
+            rowb[j & 1][c] = t;
+            wave_lds_sync();
+#pragma unroll
+            for (int i = j + 1; i < KP; ++i) a[i] -= rowb[j & 1][i] * t;
+        }
+        if constexpr (FLOOR) {
+            // R[c][c] = sqrt(max(pivot, floor)): equal to sqrt(floor) where it floored
+            const bool mine = c < k;
+            bad |= mine && !isfinite(rdiag);
+            const bool floored = mine && rdiag * rdiag <= fl * (1.0 + 1e-12);
+            if (__ballot(floored) != 0 && c == 0) atomicOr(status, 4);
+            bad = __ballot(bad) != 0;
+        }
+        if (bad && c == 0) atomicOr(status, 1);
+        // R out, and into LDS (the Gram's reads above are done) for the inverse
+#pragma unroll
+        for (int i = 0; i < KP; ++i) {
+            const double v = (i <= c && c < k) ? a[i] : 0.0;
+            R64[i * KP + c] = v;
+            Rs[i * KP + c] = v;
+        }
+    }
     __syncthreads();
-    if (threadIdx.x >= KP) return;
-    const int c = threadIdx.x;
+    if (c >= KP) return;
+    // W = R^-1, wave 0, right-looking: lane c holds column c of X; X = I; for
+    // j = 63 .. 0, X[j][:] /= R[j][j], X[i][:] -= R[i][j] X[j][:] for i < j,
+    // R[i][j] read from LDS as broadcasts. Pivots j >= k are taken as 1, so
+    // padded columns never reach columns c < k.
     rinv[c] = c < k ? 1.0 / Rs[c * KP + c] : 1.0;
     wave_lds_sync();
     double x[KP];

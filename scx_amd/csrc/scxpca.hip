@@ -756,7 +756,7 @@ struct Engine {
     // 64-column halves of the panel
     static constexpr int GW = 8;                    // k_spmm_t7: genes per owner (Z in registers)
     static constexpr int P0 = 4;                    // k_spmm_t7: slot-0 prefetch, 16-entry components per group
-    static constexpr int P1 = 2;                    // k_spmm_t7: slot-1 prefetch
+    static constexpr int P1 = 3;                    // k_spmm_t7: slot-1 prefetch
     static constexpr int RC = 2;                    // k_spmm_t7: 16-entry components per ring load
     static constexpr int TW = 16;                   // k_spmm_t7: waves per block
     static constexpr int TU = 4 * TW;               // k_spmm_t7: gene owners (16-lane groups) per block
@@ -978,11 +978,9 @@ struct Engine {
         StageScope prof(m, s, SCXPCA_STAGE_SOLVE);
         if constexpr (KP == 64) {
             if (floor_tau > 0.0)
-                k_chol_w<true><<<1, 256, 0, s>>>(G, k, shift, Rdst, status, floor_tau);
+                k_chol_w<true><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status, floor_tau);
             else
-                k_chol_w<false><<<1, 256, 0, s>>>(G, k, shift, Rdst, status, 0.0);
-            check();
-            k_trinv_w<<<1, 256, 0, s>>>(Rdst, k, W64, W32);
+                k_chol_w<false><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status, 0.0);
         } else {
             k_chol2<KP><<<1, 256, 0, s>>>(G, k, shift, Rdst, W64, W32, status, floor_tau);
         }
