@@ -1328,9 +1328,10 @@ __global__ void __launch_bounds__(512) k_build_tcsc(const int64_t *__restrict__ 
 // entries, directly otherwise. Longer waves process their remaining batches
 // from memory in both passes. With psum, the block also writes the tile's
 // per-gene sum and sum of squares of X~ (f64, run order).
-// PK: also writes the packed bf16 entries (tpk; the staged gene bytes cost
-// 8 KB of LDS). Without them the block's LDS (50 KB: a 7,680-entry stage,
-// 16-bit wave offsets) leaves room for three blocks per CU.
+// PK: also writes the packed bf16 entries (tpk), which stage a gene byte per
+// entry: a 6,400-entry stage then, 7,680 without (16-bit wave offsets; either
+// way 51 KB of LDS, three blocks per CU; about 1 % of the 1M x 2k matrix's
+// (tile, chunk) pairs hold more than 6,400 entries and are written directly).
 constexpr int kB4Genes = 256;
 constexpr int kB4Batches = 12;
 template <bool PK>
@@ -1343,7 +1344,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
                                                      double *__restrict__ psum, double *__restrict__ psq,
                                                      const int *__restrict__ flag, uint32_t *__restrict__ tpk) {
     constexpr int NG = kB4Genes, NW = 8, U = kB4Batches;
-    constexpr int kB4Stage = PK ? 8192 : 7680;
+    constexpr int kB4Stage = PK ? 6400 : 7680;
     if (*flag) return;  // the matrix failed validation: nothing here may trust its indices
     constexpr int SPC = NG / 128;  // 128-gene split offsets per chunk
     __shared__ uint32_t mask[NW][NG];
