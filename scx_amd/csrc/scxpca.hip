@@ -189,6 +189,18 @@ struct OmegaScratch {
     PinnedBuf hc, hstart;  // draw outcomes in, draw starts out
     std::vector<uint32_t> hpos;
     std::vector<double> htab, hval;
+    // a first attempt issued ahead of other work (omega_begin): its
+    // parameters, and an event after the outcomes' copy to the host
+    hipEvent_t ev = nullptr;
+    bool started = false;
+    uint64_t s_rows = 0, s_seed = 0, s_words = 0;
+    int s_k = 0;
+    OmegaScratch() = default;
+    OmegaScratch(const OmegaScratch &) = delete;
+    OmegaScratch &operator=(const OmegaScratch &) = delete;
+    ~OmegaScratch() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
 };
 
 struct scxpca_matrix {
@@ -455,12 +467,26 @@ void deal_gene_order(const scxpca_matrix *m, int gw, int nch, int tw, uint32_t *
     const int64_t nv = m->n_vars;
     const int64_t S = static_cast<int64_t>(nch) * tw * gw;
     const int64_t waves = static_cast<int64_t>(nch) * tw / quad;
-    std::vector<uint64_t> key(nv);
     const int64_t nc = static_cast<int64_t>(m->host_gcnt.size());
     auto count = [&](int64_t c) { return c < nc ? m->host_gcnt[c] : 0u; };
-    for (int64_t c = 0; c < nv; ++c)
-        key[c] = (static_cast<uint64_t>(~count(c)) << 32) | static_cast<uint64_t>(c);
-    std::sort(key.begin(), key.end());
+    // genes by count descending, ties by index: a stable LSD radix sort of the
+    // indices on the complemented counts (11-bit digits, only as many passes
+    // as the largest count needs)
+    std::vector<uint32_t> key(nv), tmp(nv);
+    uint32_t cmax = 0;
+    for (int64_t c = 0; c < nv; ++c) {
+        key[c] = static_cast<uint32_t>(c);
+        cmax = std::max(cmax, count(c));
+    }
+    constexpr int kDigit = 11;
+    for (int shift = 0; shift < 32 && (shift == 0 || (cmax >> shift) != 0); shift += kDigit) {
+        uint32_t hist[(1 << kDigit) + 1] = {};
+        auto digit = [&](uint32_t g) { return ((~count(g)) >> shift) & ((1u << kDigit) - 1u); };
+        for (int64_t i = 0; i < nv; ++i) ++hist[digit(key[i]) + 1];
+        for (int d = 0; d < (1 << kDigit); ++d) hist[d + 1] += hist[d];
+        for (int64_t i = 0; i < nv; ++i) tmp[hist[digit(key[i])]++] = key[i];
+        key.swap(tmp);
+    }
     for (int64_t i = 0; i < std::max<int64_t>(S, 1); ++i) h[i] = static_cast<uint32_t>(nv);
     // min-heap of (load, wave): waves with a free slot
     using Item = std::pair<uint64_t, int64_t>;
@@ -1448,6 +1474,48 @@ int64_t pca_k(const scxpca_matrix *m, const scxpca_params *p) {
     return std::min<int64_t>(std::min<int64_t>(p->n_comps + p->n_oversamples, m->n_vars), m->n_obs);
 }
 
+// One attempt's keystream and draw outcomes for n_words draw positions, and
+// the outcomes' copy to the host, on s.
+void omega_launch(hipStream_t s, uint64_t n_words, uint64_t seed, OmegaScratch &sc) {
+    sc.w.ensure(sizeof(uint64_t) * n_words);
+    sc.c.ensure(n_words);
+    sc.v.ensure(sizeof(double) * n_words);
+    if (!sc.tab.p) {  // the ziggurat tables are constants: built and uploaded once
+        sc.tab.ensure(sizeof(double) * 2 * 257);
+        sc.htab.resize(2 * 257);
+        omega_tables(sc.htab.data(), sc.htab.data() + 257);
+        HIPC(hipMemcpyAsync(sc.tab.p, sc.htab.data(), sizeof(double) * sc.htab.size(), hipMemcpyHostToDevice, s));
+    }
+    ChaChaKey key;
+    omega_key(seed, key.k);
+    k_chacha8_u64<<<blocks_for((n_words + 7) / 8), 256, 0, s>>>(key, n_words, sc.w.as<uint64_t>());
+    HIPC(hipGetLastError());
+    k_zig_outcome<<<std::min(blocks_for(n_words), 65536u), 256, 0, s>>>(
+        sc.w.as<uint64_t>(), n_words, sc.tab.as<double>(), sc.tab.as<double>() + 257, sc.c.as<uint8_t>(),
+        sc.v.as<double>());
+    HIPC(hipGetLastError());
+    sc.hc.ensure(n_words);
+    HIPC(hipMemcpyAsync(sc.hc.p, sc.c.p, n_words, hipMemcpyDeviceToHost, s));
+}
+
+// device_omega's first attempt issued ahead: the keystream and outcomes run
+// (and come back to the host) while the caller queues other work on s, and
+// device_omega with the same rows, k and seed then waits only for the copy.
+void omega_begin(hipStream_t s, uint64_t rows, int k, uint64_t seed, OmegaScratch &sc) {
+    sc.started = false;
+    const uint64_t n = rows * static_cast<uint64_t>(k);
+    const uint64_t n_words = n + n / 16 + 4096;
+    if (n == 0 || n_words >= (1ull << 32)) return;
+    omega_launch(s, n_words, seed, sc);
+    if (!sc.ev) HIPC(hipEventCreateWithFlags(&sc.ev, hipEventDisableTiming));
+    HIPC(hipEventRecord(sc.ev, s));
+    sc.started = true;
+    sc.s_rows = rows;
+    sc.s_k = k;
+    sc.s_seed = seed;
+    sc.s_words = n_words;
+}
+
 // Omega (rows x k, the reference stream) on the device: keystream and
 // per-position draw outcomes in parallel, the chain of draw starts walked on
 // the host, values gathered into the f32 panel (KP columns) and/or f64 out64.
@@ -1468,30 +1536,18 @@ void device_omega(hipStream_t s, uint64_t rows, int k, uint64_t seed, float *pan
     const uint64_t n = rows * static_cast<uint64_t>(k);
     if (n == 0) return;
     uint64_t n_words = n + n / 16 + 4096;
+    const bool pre = sc.started && sc.s_rows == rows && sc.s_k == k && sc.s_seed == seed && sc.s_words == n_words;
+    sc.started = false;
     // the previous call's uploads from the host vectors were drained by the
-    // synchronization that precedes every walk (same stream)
+    // wait that precedes every walk (same stream, issued before the copy)
     for (int attempt = 0; attempt < 3; ++attempt, n_words *= 2) {
         if (n_words >= (1ull << 32)) break;
-        sc.w.ensure(sizeof(uint64_t) * n_words);
-        sc.c.ensure(n_words);
-        sc.v.ensure(sizeof(double) * n_words);
-        if (!sc.tab.p) {  // the ziggurat tables are constants: built and uploaded once
-            sc.tab.ensure(sizeof(double) * 2 * 257);
-            sc.htab.resize(2 * 257);
-            omega_tables(sc.htab.data(), sc.htab.data() + 257);
-            HIPC(hipMemcpyAsync(sc.tab.p, sc.htab.data(), sizeof(double) * sc.htab.size(), hipMemcpyHostToDevice, s));
+        if (attempt == 0 && pre) {
+            HIPC(hipEventSynchronize(sc.ev));  // only the outcomes' copy: later work on s keeps running
+        } else {
+            omega_launch(s, n_words, seed, sc);
+            HIPC(hipStreamSynchronize(s));
         }
-        ChaChaKey key;
-        omega_key(seed, key.k);
-        k_chacha8_u64<<<blocks_for((n_words + 7) / 8), 256, 0, s>>>(key, n_words, sc.w.as<uint64_t>());
-        HIPC(hipGetLastError());
-        k_zig_outcome<<<std::min(blocks_for(n_words), 65536u), 256, 0, s>>>(
-            sc.w.as<uint64_t>(), n_words, sc.tab.as<double>(), sc.tab.as<double>() + 257, sc.c.as<uint8_t>(),
-            sc.v.as<double>());
-        HIPC(hipGetLastError());
-        sc.hc.ensure(n_words);
-        HIPC(hipMemcpyAsync(sc.hc.p, sc.c.p, n_words, hipMemcpyDeviceToHost, s));
-        HIPC(hipStreamSynchronize(s));
         sc.hstart.ensure(sizeof(uint32_t) * n);
         std::vector<std::pair<uint32_t, double>> ov;
         if (!omega_walk(sc.hc.as<uint8_t>(), n_words, n, seed, sc.hstart.as<uint32_t>(), &ov)) continue;  // more words
@@ -1723,12 +1779,15 @@ void run_pca(scxpca_matrix *m, const scxpca_params *p, scxpca_result *out) {
     validate(m, p);
     const int64_t k = pca_k(m, p);
     DeviceGuard dg(m->device);
-    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_bf16_entries(m);
-    ensure_stats(m);
     hipEvent_t e0, e1;
     HIPC(hipEventCreate(&e0));
     HIPC(hipEventCreate(&e1));
     HIPC(hipEventRecord(e0, m->stream));
+    // Omega's keystream and outcomes first: the host walk of its draw chain
+    // then overlaps the statistics and set-up work queued below
+    if (k <= 128) omega_begin(m->stream, static_cast<uint64_t>(m->n_vars), static_cast<int>(k), p->seed, m->om);
+    if (p->precision == SCXPCA_PRECISION_BF16_SKETCH) ensure_bf16_entries(m);
+    ensure_stats(m);
     if (k <= 64) {
         Engine<64> eng(m, static_cast<int>(k));
         eng.run(p, out);
