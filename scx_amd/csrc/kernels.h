@@ -227,8 +227,11 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
     // first), and the split offsets: an entry whose 128-gene chunk differs from
     // its predecessor's opens chunks (prev, ch]; their offsets are its position.
     auto scan = [&](uint32_t *sr, int64_t a, int64_t b, int64_t p0, bool ok, uint32_t c, int64_t &prev_last) {
-        int64_t prev = static_cast<int64_t>(static_cast<uint32_t>(__shfl_up(static_cast<int>(c), 1)));
-        if (lane == 0) prev = prev_last;
+        // lane - 1's index by a whole-wave DPP shift (wave_shr:1); lane 0
+        // takes the previous batch's last
+        const uint32_t up =
+            static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(c), 0x138, 0xf, 0xf, false));
+        const int64_t prev = lane == 0 ? prev_last : static_cast<int64_t>(up);
         if (ok) {
             bad |= (static_cast<int64_t>(c) >= n_vars) ? 2 : 0;
             bad |= (prev >= 0 && static_cast<int64_t>(c) < prev) ? 1 : 0;
@@ -239,8 +242,8 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
             if (ch > chp)
                 for (int g = chp + 1; g <= ch && g < nsplit; ++g) sr[g] = static_cast<uint32_t>(p0 + lane - a);
         }
-        const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;
-        prev_last = static_cast<int64_t>(static_cast<uint32_t>(__shfl(static_cast<int>(c), cnt - 1)));
+        const int cnt = (b - p0) < kWave ? static_cast<int>(b - p0) : kWave;  // wave-uniform
+        prev_last = static_cast<int64_t>(static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(c), cnt - 1)));
     };
     auto splits_tail = [&](uint32_t *sr, int64_t a, int64_t b, int64_t prev_last) {
         // chunks after the row's last entry start at the row's end

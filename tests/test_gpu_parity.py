@@ -140,6 +140,54 @@ def test_repeated_column_in_a_row_is_rejected(gpu, on_device):
             gpu.Matrix(indptr, indices, data, (3, 5), target_sum=1e4, log1p=True)
 
 
+def _device_csr(rows, n_vars):
+    """Device tensors of a CSR whose rows hold the given column lists."""
+    import torch
+
+    indptr = np.zeros(len(rows) + 1, dtype=np.int64)
+    indptr[1:] = np.cumsum([len(r) for r in rows])
+    indices = np.concatenate([np.asarray(r, dtype=np.uint32) for r in rows])
+    data = np.ones(len(indices), dtype=np.float32)
+    dev = torch.device("cuda:0")
+    t = (torch.from_numpy(indptr).to(dev), torch.from_numpy(indices.view(np.int32)).to(dev),
+         torch.from_numpy(data).to(dev))
+    torch.cuda.synchronize()
+    return t, (len(rows), n_vars)
+
+
+@pytest.mark.parametrize("pos", [10, 64, 128, 256])
+@pytest.mark.parametrize("kind", ["decrease", "repeat"])
+def test_row_order_checked_across_batches(gpu, kind, pos):
+    # the device check compares each entry with its predecessor: inside a
+    # 64-entry batch (pos 10) and across batches, where the predecessor is the
+    # previous batch's last entry (pos 64, 128; pos 256 in a row of 300
+    # entries, the long-row path); every other row is valid
+    cols = list(range(0, 2 * 300, 2))  # 300 sorted entries, n_vars 600
+    bad = list(cols)
+    bad[pos] = bad[pos - 1] - (1 if kind == "decrease" else 0)
+    rows = [cols[:130], bad[: max(pos + 2, 130)], cols[:5]]
+    t, shape = _device_csr(rows, 600)
+    say(f"case: row order {kind} at {pos}")
+    msg = "sorted column indices" if kind == "decrease" else "repeats a column index"
+    with pytest.raises(ValueError, match=msg):
+        gpu.Matrix(*t, shape, target_sum=1e4, log1p=True, device=0)
+
+
+def test_valid_long_rows_accepted(gpu):
+    # rows of 63, 64, 65, 130, 256, 257 and 300 sorted entries (batch and
+    # long-row boundaries) pass the device check, and their column counts come back
+    rows = [list(range(0, 2 * n, 2)) for n in (63, 64, 65, 130, 256, 257, 300)]
+    t, shape = _device_csr(rows, 600)
+    say("case: valid long rows")
+    m = gpu.Matrix(*t, shape, target_sum=1e4, log1p=True, device=0)
+    try:
+        _, _, cnt, _ = m.stats()
+        want = np.bincount(np.concatenate([np.asarray(r) for r in rows]), minlength=600)
+        np.testing.assert_array_equal(cnt, want)
+    finally:
+        m.close()
+
+
 @pytest.mark.parametrize("value", ["lots", "-1", "1e9x"])
 def test_invalid_split_cap_is_rejected(gpu, small, monkeypatch, value):
     # SCXPCA_SPLIT_MAX_BYTES is read when a matrix is created; a value that is
