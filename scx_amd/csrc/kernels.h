@@ -2547,7 +2547,10 @@ __global__ void __launch_bounds__(256) k_sum_parts16(const double *__restrict__ 
 }
 
 // out = Y W (Y: nrows x 64 f32, W: 64 x 64 f32), the first out_cols columns
-// stored with leading dimension out_ld. A wave takes 16-row tiles: lane l
+// stored with leading dimension out_ld. When out_ld == out_cols (and out is
+// 16-byte aligned) a tile's 16 output rows are one contiguous block: it is
+// staged in LDS and written as float4s (4 wave-wide stores for 64 columns,
+// instead of 16 stores of 64-byte pieces). A wave takes 16-row tiles: lane l
 // loads 4 float4 of row r0 + l%16 (columns 16m + 4(l/16) .. +3), so component
 // e of load m is A[i][k] for inner index 16m + 4k + e; the matching B values
 // W[16m + 4(l/16) + e][16jt + l%16] stay in 64 registers per lane. 64 MFMAs
@@ -2558,8 +2561,11 @@ __global__ void __launch_bounds__(256) k_apply4(const float *__restrict__ Y, con
                                                 float *__restrict__ out, int64_t nrows, int out_cols,
                                                 int64_t out_ld) {
     static_assert(KP == 64, "four float4 per lane per row");
+    __shared__ __align__(16) float tile_out[4][16 * KP];  // per wave: 16 rows x out_cols, compact
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i16 = lane & 15, k4 = lane >> 4;
+    const bool packed = out_ld == out_cols && (reinterpret_cast<uintptr_t>(out) & 15u) == 0;
+    float *tw = tile_out[wave];
     float wr[4][4][4];
 #pragma unroll
     for (int m = 0; m < 4; ++m)
@@ -2592,15 +2598,34 @@ __global__ void __launch_bounds__(256) k_apply4(const float *__restrict__ Y, con
                 for (int jt = UPPER ? m : 0; jt < 4; ++jt)
                     acc[jt] = __builtin_amdgcn_mfma_f32_16x16x4f32(c[e], wr[m][e][jt], acc[jt], 0, 0, 0);
         }
+        if (packed) {
 #pragma unroll
-        for (int jt = 0; jt < 4; ++jt) {
-            const int col = 16 * jt + i16;
-            if (col < out_cols)
+            for (int jt = 0; jt < 4; ++jt) {
+                const int col = 16 * jt + i16;
+                if (col < out_cols)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int64_t orow = r0 + 4 * k4 + q;
-                    if (orow < nrows) out[orow * out_ld + col] = acc[jt][q];
-                }
+                    for (int q = 0; q < 4; ++q) tw[(4 * k4 + q) * out_cols + col] = acc[jt][q];
+            }
+            wave_lds_sync();
+            const int64_t left = nrows - r0;
+            const int nv = static_cast<int>(left < 16 ? left : 16) * out_cols;  // floats of the tile
+            float *dst = out + r0 * out_ld;
+            const int n4 = nv >> 2;
+            for (int i = lane; i < n4; i += kWave)
+                reinterpret_cast<float4 *>(dst)[i] = reinterpret_cast<const float4 *>(tw)[i];
+            for (int i = 4 * n4 + lane; i < nv; i += kWave) dst[i] = tw[i];
+            wave_lds_sync();  // the tile is read before the next one is staged
+        } else {
+#pragma unroll
+            for (int jt = 0; jt < 4; ++jt) {
+                const int col = 16 * jt + i16;
+                if (col < out_cols)
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        const int64_t orow = r0 + 4 * k4 + q;
+                        if (orow < nrows) out[orow * out_ld + col] = acc[jt][q];
+                    }
+            }
         }
     }
 }
