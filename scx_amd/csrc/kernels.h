@@ -2526,16 +2526,26 @@ __global__ void __launch_bounds__(256) k_gram4(const float *__restrict__ Y, int6
 }
 
 // G = sum of nparts KP x KP f64 partials, in a fixed order: block b sums
-// entries [16b, 16b + 16), 16 lanes per entry each taking every 16th partial,
-// then the 16 lane sums are added in order.
+// entries [16b, 16b + 16), 16 lanes per entry each taking every 16th partial
+// (eight loads in flight, added as a fixed tree), then the 16 lane sums are
+// added in order.
 template <int KP>
 __global__ void __launch_bounds__(256) k_sum_parts16(const double *__restrict__ part, int nparts,
                                                      double *__restrict__ G) {
     __shared__ double red[16][17];
     const int e = blockIdx.x * 16 + (threadIdx.x & 15), g = threadIdx.x >> 4;
+    constexpr int64_t KK = static_cast<int64_t>(KP) * KP;
     double acc = 0.0;
-    if (e < KP * KP)
-        for (int p = g; p < nparts; p += 16) acc += part[static_cast<int64_t>(p) * KP * KP + e];
+    if (e < KP * KP) {
+        int p = g;
+        for (; p + 7 * 16 < nparts; p += 8 * 16) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = part[static_cast<int64_t>(p + 16 * u) * KK + e];
+            acc += ((v[0] + v[1]) + (v[2] + v[3])) + ((v[4] + v[5]) + (v[6] + v[7]));
+        }
+        for (; p < nparts; p += 16) acc += part[static_cast<int64_t>(p) * KK + e];
+    }
     red[g][threadIdx.x & 15] = acc;
     __syncthreads();
     if (g == 0 && e < KP * KP) {
