@@ -1045,7 +1045,7 @@ struct Engine {
     float *cqr_tall(float *src, float *tmp, int qr_method, int64_t n = -1, bool final = true, bool apply_last = true) {
         if (n < 0) n = m->n_obs;
         const int passes = !final ? 1 : (qr_method == SCXPCA_QR_HOUSEHOLDER ? 3 : 2);
-        const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 511) / 512, gram_blocks)));
+        const int gblocks = static_cast<int>(std::max<int64_t>(1, std::min<int64_t>((n + 127) / 128, gram_blocks)));
         int64_t rpg = (n + gblocks - 1) / gblocks;
         rpg = (rpg + 15) / 16 * 16;
         for (int pass = 0; pass < passes; ++pass) {
