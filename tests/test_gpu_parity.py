@@ -102,7 +102,7 @@ def _oracle_pca(csr, w, n_comps, n_over, n_iter, seed=0):
                                  target_sum=w.target_sum, log1p=True)
 
 
-@pytest.mark.parametrize("n_iter,qr_method", [(2, 0), (3, 1), (3, 0)])
+@pytest.mark.parametrize("n_iter,qr_method", [(2, 0), (3, 1), (3, 0), (0, 0), (0, 1), (1, 1)])
 def test_pca_parity_small(small, small_mat, n_iter, qr_method):
     say(f"case: pca small n_iter={n_iter} qr={qr_method}")
     ref = _oracle_pca(small, synth.CFG1, 50, 10, n_iter)
