@@ -1028,48 +1028,44 @@ __global__ void __launch_bounds__(256) k_apply64(const double *__restrict__ A, c
     if (out32) out32[t] = static_cast<float>(sm);
 }
 
-// C = A B for KP x KP f64 matrices: A staged in LDS by row tiles, B whole
-// (KP = 64: 32 KB each); one workgroup, KP * KP / 256 outputs per thread.
+// C = A B for KP x KP f64 matrices; block b computes rows 16b .. 16b + 15
+// (grid KP / 16): those rows of A staged in LDS, B staged whole for KP = 64
+// (32 KB) and read from global memory for KP = 128; each product summed over
+// the inner index as two interleaved partial sums (even, odd), then added.
 template <int KP>
 __global__ void __launch_bounds__(256) k_small_gemm64(const double *__restrict__ A, const double *__restrict__ B,
                                                       double *__restrict__ C) {
     static_assert(KP == 64 || KP == 128, "panel widths");
-    constexpr int RT = 64 * 64 / KP;  // rows of A per LDS tile (KP * RT doubles = 32 KB)
+    constexpr int RT = 16;  // rows of C per block
     __shared__ double bs[KP == 64 ? KP * KP : 1];
     __shared__ double at[RT * KP];
+    const int i0 = static_cast<int>(blockIdx.x) * RT;
     if constexpr (KP == 64)
         for (int i = threadIdx.x; i < KP * KP; i += blockDim.x) bs[i] = B[i];
-    for (int i0 = 0; i0 < KP; i0 += RT) {
-        __syncthreads();
-        for (int i = threadIdx.x; i < RT * KP; i += blockDim.x) at[i] = A[i0 * KP + i];
-        __syncthreads();
-        for (int t = threadIdx.x; t < RT * KP; t += blockDim.x) {
-            const int i = t / KP, j = t % KP;
-            double s0 = 0.0, s1 = 0.0;
-            for (int q = 0; q < KP; q += 2) {
-                const double b0 = KP == 64 ? bs[q * KP + j] : B[q * KP + j];
-                const double b1 = KP == 64 ? bs[(q + 1) * KP + j] : B[(q + 1) * KP + j];
-                s0 += at[i * KP + q] * b0;
-                s1 += at[i * KP + q + 1] * b1;
-            }
-            C[(i0 + i) * KP + j] = s0 + s1;
+    for (int i = threadIdx.x; i < RT * KP; i += blockDim.x) at[i] = A[i0 * KP + i];
+    __syncthreads();
+    for (int t = threadIdx.x; t < RT * KP; t += blockDim.x) {
+        const int i = t / KP, j = t % KP;
+        double s0 = 0.0, s1 = 0.0;
+        for (int q = 0; q < KP; q += 2) {
+            const double b0 = KP == 64 ? bs[q * KP + j] : B[q * KP + j];
+            const double b1 = KP == 64 ? bs[(q + 1) * KP + j] : B[(q + 1) * KP + j];
+            s0 += at[i * KP + q] * b0;
+            s1 += at[i * KP + q + 1] * b1;
         }
+        C[(i0 + i) * KP + j] = s0 + s1;
     }
 }
 
-// Embedding weights W_emb = Rinv (V Sigma)[:, :nc] as f32 KP x KP (zero
-// padded), Rinv upper triangular (f64 products, summed in index order): the
-// embeddings Q W_emb of a basis Q = Q1 Rinv come from Q1 directly.
+// Embedding weights W_emb = P[:, :nc] Sigma as f32 KP x KP (zero padded),
+// from P = Rinv V (k_small_gemm64), Rinv the last CholeskyQR pass's inverse
+// of the final basis: the embeddings Q W_emb of Q = Q1 Rinv come from Q1.
 template <int KP>
-__global__ void __launch_bounds__(256) k_emb_weights(const double *__restrict__ Rinv, const double *__restrict__ V,
-                                                     const double *__restrict__ sigma, int nc,
-                                                     float *__restrict__ W) {
+__global__ void __launch_bounds__(256) k_emb_weights(const double *__restrict__ P, const double *__restrict__ sigma,
+                                                     int nc, float *__restrict__ W) {
     for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
-        const int i = t / KP, j = t % KP;
-        double acc = 0.0;
-        if (j < nc)
-            for (int l = i; l < KP; ++l) acc += Rinv[i * KP + l] * V[l * KP + j];
-        W[t] = j < nc ? static_cast<float>(acc * sigma[j]) : 0.f;
+        const int j = t % KP;
+        W[t] = j < nc ? static_cast<float>(P[t] * sigma[j]) : 0.f;
     }
 }
 

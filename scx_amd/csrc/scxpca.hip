@@ -1098,9 +1098,9 @@ struct Engine {
             src = bufs[pass];
         }
         // Rt = R3 * (R2 * R1)
-        k_small_gemm64<KP><<<1, 256, 0, s>>>(R2, R1, Acol);
+        k_small_gemm64<KP><<<KP / 16, 256, 0, s>>>(R2, R1, Acol);
         check();
-        k_small_gemm64<KP><<<1, 256, 0, s>>>(R3, Acol, Rt);
+        k_small_gemm64<KP><<<KP / 16, 256, 0, s>>>(R3, Acol, Rt);
         check();
     }
 
@@ -1272,7 +1272,9 @@ struct Engine {
         }
         k_components<KP><<<blocks_for(nv * nc), 256, 0, s>>>(m->t64a.as<double>(), nc, nv, comps_dev);
         check();
-        k_emb_weights<KP><<<1, 256, 0, s>>>(Rfin, V, sig, nc, W32);
+        k_small_gemm64<KP><<<KP / 16, 256, 0, s>>>(Rfin, V, Acol);  // Acol is free after the Rayleigh-Ritz QR
+        check();
+        k_emb_weights<KP><<<1, 256, 0, s>>>(Acol, sig, nc, W32);
         check();
         float *emb_dev = nullptr;
         if (out->on_device) {
