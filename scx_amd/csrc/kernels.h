@@ -353,8 +353,10 @@ __global__ void __launch_bounds__(256) k_prepare_rows(const int64_t *__restrict_
     };
     // two rows per wave at a time (r and r + wstride): both rows' loads are in
     // flight together, so each wave keeps twice the bytes outstanding
-    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + threadIdx.x / kWave; r < n_obs;
-         r += 2 * wstride) {
+    // the wave's index made wave-uniform: the row bounds and the per-row
+    // bookkeeping then live in scalar registers (scalar loads of indptr)
+    const int wib = __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kWave));
+    for (int64_t r = static_cast<int64_t>(blockIdx.x) * (blockDim.x / kWave) + wib; r < n_obs; r += 2 * wstride) {
         const int64_t r2 = r + wstride;
         const bool has2 = r2 < n_obs;
         const int64_t a = indptr[r], b = indptr[r + 1];
@@ -1358,7 +1360,7 @@ __global__ void __launch_bounds__(512) k_build_tcsc4(const int64_t *__restrict__
     const int64_t r0 = static_cast<int64_t>(t) * kTileRows;
     if (r0 >= n_obs) return;  // padding blocks of the last XCD round
     const int nr = (n_obs - r0) < kTileRows ? static_cast<int>(n_obs - r0) : kTileRows;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint32_t g0 = static_cast<uint32_t>(ck) * NG;
     const int ng = (n_vars - g0) < NG ? static_cast<int>(n_vars - g0) : NG;
     const int cs = ck * SPC, ce = (cs + SPC) < nsplit ? cs + SPC : nsplit;
