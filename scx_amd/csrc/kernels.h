@@ -2910,8 +2910,9 @@ __global__ void __launch_bounds__(256) k_chol2(const double *__restrict__ G, int
 // i > j with R[j][i] read back as a broadcast (measured faster than 63
 // readlanes per step). Lanes c < i keep updating their lower-triangle
 // registers (never read), which keeps the code branch-free. After a block
-// barrier the same wave inverts R (W64 / W32 = R^-1) from its LDS copy, in the
-// same launch. The pivot rule is k_chol2's; FLOOR selects the floor (per step
+// barrier R is inverted (W64 / W32 = R^-1) from its LDS copy in the same
+// launch, by 2 x 2 blocks: the two diagonal blocks in 32 steps on one wave,
+// the off-diagonal block by two 32 x 32 products on all four waves. The pivot rule is k_chol2's; FLOOR selects the floor (per step
 // a max with the floor, non-finite values checked on the input and on the
 // final diagonal: per-step compares would cost scalar registers).
 // ---------------------------------------------------------------------------
@@ -2920,10 +2921,12 @@ __global__ void __launch_bounds__(256) k_chol_w(const double *__restrict__ G, in
                                                double *__restrict__ R64, double *__restrict__ W64,
                                                float *__restrict__ W32, int *__restrict__ status,
                                                double floor_tau) {
-    constexpr int KP = 64;
+    constexpr int KP = 64, H = KP / 2;
     __shared__ double rowb[2][KP];
     __shared__ double Rs[KP * KP];
     __shared__ double rinv[KP];
+    __shared__ double Wl[KP * KP];     // R^-1, row-major
+    __shared__ double Tl[H * (H + 1)];  // B C^-1 (padded rows)
     // the masked Gram staged row-major in LDS by all four waves (four times
     // the loads in flight of one wave); wave 0 then factors it alone
     for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
@@ -2988,31 +2991,55 @@ __global__ void __launch_bounds__(256) k_chol_w(const double *__restrict__ G, in
         }
     }
     __syncthreads();
-    if (c >= KP) return;
-    // W = R^-1, wave 0, right-looking: lane c holds column c of X; X = I; for
-    // j = 63 .. 0, X[j][:] /= R[j][j], X[i][:] -= R[i][j] X[j][:] for i < j,
-    // R[i][j] read from LDS as broadcasts. Pivots j >= k are taken as 1, so
-    // padded columns never reach columns c < k.
-    rinv[c] = c < k ? 1.0 / Rs[c * KP + c] : 1.0;
-    wave_lds_sync();
-    double x[KP];
-#pragma unroll
-    for (int i = 0; i < KP; ++i) x[i] = (i == c) ? 1.0 : 0.0;
-#pragma unroll
-    for (int j = KP - 1; j >= 0; --j) {
-        const double w = x[j] * rinv[j];
-        x[j] = w;
-#pragma unroll
-        for (int i = 0; i < j; ++i) x[i] -= Rs[i * KP + j] * w;
-        // a fence per step keeps every step's broadcast reads from being hoisted
-        // to the top (which would need ~2000 registers)
+    // W = R^-1 by 2 x 2 blocks of H = 32: R = [A B; 0 C], W = [A^-1, -A^-1 B C^-1;
+    // 0, C^-1]. Wave 0 inverts A (lanes 0-31) and C (lanes 32-63) together,
+    // right-looking: lane c holds column c of diag(A, C)^-1; X = I; for
+    // j = H-1 .. 0, X[j][:] /= R[j][j], X[i][:] -= R[i][j] X[j][:] for i < j
+    // (block-local indices), R[i][j] read from LDS (one address per half-wave).
+    // Pivots j >= k are taken as 1, so padded columns never reach columns c < k.
+    // Then all four waves form T = B C^-1 and the upper-right block -A^-1 T.
+    if (c < KP) {
+        rinv[c] = c < k ? 1.0 / Rs[c * KP + c] : 1.0;
         wave_lds_sync();
-    }
+        const int off = c < H ? 0 : H, cl = c - off;
+        double x[H];
 #pragma unroll
-    for (int i = 0; i < KP; ++i) {
-        const double v = (i <= c && c < k) ? x[i] : 0.0;
-        W64[i * KP + c] = v;
-        W32[i * KP + c] = static_cast<float>(v);
+        for (int i = 0; i < H; ++i) x[i] = (i == cl) ? 1.0 : 0.0;
+#pragma unroll
+        for (int j = H - 1; j >= 0; --j) {
+            const double w = x[j] * rinv[off + j];
+            x[j] = w;
+#pragma unroll
+            for (int i = 0; i < j; ++i) x[i] -= Rs[(off + i) * KP + off + j] * w;
+            // a fence per step keeps every step's reads from being hoisted to
+            // the top (which would need ~500 registers)
+            wave_lds_sync();
+        }
+#pragma unroll
+        for (int i = 0; i < H; ++i) Wl[(off + i) * KP + c] = x[i];
+    }
+    __syncthreads();
+    // T[i][cc] = sum_{q <= cc} B[i][q] C^-1[q][cc]  (C^-1 upper triangular)
+    for (int e = threadIdx.x; e < H * H; e += blockDim.x) {
+        const int i = e / H, cc = e % H;
+        double t = 0.0;
+        for (int q = 0; q <= cc; ++q) t = fma(Rs[i * KP + H + q], Wl[(H + q) * KP + H + cc], t);
+        Tl[i * (H + 1) + cc] = t;
+    }
+    __syncthreads();
+    // W[i][H + cc] = -sum_{q >= i} A^-1[i][q] T[q][cc]  (A^-1 upper triangular)
+    for (int e = threadIdx.x; e < H * H; e += blockDim.x) {
+        const int i = e / H, cc = e % H;
+        double t = 0.0;
+        for (int q = i; q < H; ++q) t = fma(Wl[i * KP + q], Tl[q * (H + 1) + cc], t);
+        Wl[i * KP + H + cc] = -t;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < KP * KP; t += blockDim.x) {
+        const int i = t / KP, cc = t % KP;
+        const double v = (i <= cc && cc < k) ? Wl[t] : 0.0;
+        W64[t] = v;
+        W32[t] = static_cast<float>(v);
     }
 }
 
